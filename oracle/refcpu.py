@@ -1,0 +1,246 @@
+"""ctypes binding of the CPU oracle (oracle/librefcpu.so).
+
+TEST INFRASTRUCTURE: the checker, never the product. Only tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline leg import this module; nothing under modlefork_amd/ does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "librefcpu.so")
+
+U64P = C.POINTER(C.c_uint64)
+SZP = C.POINTER(C.c_size_t)
+U8P = C.POINTER(C.c_uint8)
+DP = C.POINTER(C.c_double)
+
+COLLISION = 0x10
+CHROM_BOUNDARY = 0x08
+LEF_BAR = 0x04
+LEF_LEF_PRIMARY = 0x02
+LEF_LEF_SECONDARY = 0x01
+UNBOUND = (1 << 64) - 1
+
+
+class RcConfig(C.Structure):
+    _fields_ = [
+        ("bin_size", C.c_uint64),
+        ("diagonal_width", C.c_uint64),
+        ("rev_speed", C.c_uint64),
+        ("fwd_speed", C.c_uint64),
+        ("rev_speed_burnin", C.c_uint64),
+        ("fwd_speed_burnin", C.c_uint64),
+        ("rev_speed_std", C.c_double),
+        ("fwd_speed_std", C.c_double),
+        ("prob_release", C.c_double),
+        ("prob_release_burnin", C.c_double),
+        ("p_bypass", C.c_double),
+        ("pblock_major", C.c_double),
+        ("pblock_minor", C.c_double),
+        ("hard_stall_mult", C.c_double),
+        ("soft_stall_mult", C.c_double),
+        ("tad_to_loop_ratio", C.c_double),
+        ("gev_mu", C.c_double),
+        ("gev_sigma", C.c_double),
+        ("gev_xi", C.c_double),
+        ("noisify", C.c_int32),
+        ("skip_burnin", C.c_int32),
+        ("contact_sampling_interval", C.c_uint64),
+        ("burnin_history_length", C.c_uint64),
+        ("burnin_smoothing_window", C.c_uint64),
+        ("min_burnin_epochs", C.c_uint64),
+        ("max_burnin_epochs", C.c_uint64),
+        ("burnin_target_epochs_for_lef_activation", C.c_uint64),
+        ("target_contact_density", C.c_double),
+        ("target_simulation_epochs", C.c_uint64),
+        ("seed", C.c_uint64),
+    ]
+
+
+class RcChrom(C.Structure):
+    _fields_ = [
+        ("name", C.c_char_p),
+        ("name_len", C.c_uint64),
+        ("size", C.c_uint64),
+        ("start", C.c_uint64),
+        ("end", C.c_uint64),
+    ]
+
+
+class RcBarriers(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint64),
+        ("pos", U64P),
+        ("dir", U8P),
+        ("stp_active", DP),
+        ("stp_inactive", DP),
+    ]
+
+
+class RcCellStats(C.Structure):
+    _fields_ = [
+        ("epochs", C.c_uint64),
+        ("burnin_epochs", C.c_uint64),
+        ("contacts", C.c_uint64),
+        ("draws", C.c_uint64),
+    ]
+
+
+class RcBand(C.Structure):
+    _fields_ = [
+        ("nrows", C.c_uint64),
+        ("ncols", C.c_uint64),
+        ("data", C.POINTER(C.c_uint32)),
+        ("missed", C.c_uint64),
+    ]
+
+
+class RcRng(C.Structure):
+    _fields_ = [("s", C.c_uint64 * 4), ("draws", C.c_uint64)]
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "all"], cwd=_HERE, check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.rc_xxh3_64_withseed.restype = C.c_uint64
+        L.rc_xxh3_64_withseed.argtypes = [C.c_char_p, C.c_size_t, C.c_uint64]
+        L.rc_cell_seed.restype = C.c_uint64
+        L.rc_cell_seed.argtypes = [C.c_char_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64]
+        L.rc_draw_sequence.restype = None
+        L.rc_draw_sequence.argtypes = [C.c_uint64, C.c_int, C.c_double, C.c_double, C.c_double,
+                                       C.c_uint64, U64P]
+        L.rc_rng_seed.argtypes = [C.POINTER(RcRng), C.c_uint64]
+        L.rc_pdq_sort_idx.argtypes = [SZP, C.c_size_t, U64P]
+        L.rc_split_sort_idx.argtypes = [SZP, C.c_size_t, U64P]
+        L.rc_rank_lefs.argtypes = [U64P, U64P, U64P, C.c_size_t, SZP, SZP, C.c_int, C.c_int]
+        L.rc_adjust_and_clamp_moves.argtypes = [C.POINTER(RcChrom), U64P, U64P, U64P, C.c_size_t,
+                                                SZP, SZP, U64P, U64P, C.c_int]
+        L.rc_process_collisions.argtypes = [C.POINTER(RcConfig), C.POINTER(RcChrom), U64P, U64P,
+                                            U64P, C.c_size_t, SZP, SZP, U64P, U64P,
+                                            C.POINTER(RcBarriers), U8P, U64P, U64P,
+                                            C.POINTER(RcRng), C.c_int, U64P, U64P]
+        L.rc_generate_moves.argtypes = [C.POINTER(RcConfig), C.POINTER(RcChrom), U64P, U64P, U64P,
+                                        C.c_size_t, SZP, SZP, U64P, U64P, C.c_int,
+                                        C.POINTER(RcRng), C.c_int]
+        L.rc_bind_lefs.argtypes = [C.c_uint64, C.c_uint64, U64P, U64P, U64P, C.c_size_t, SZP, SZP,
+                                   U8P, C.POINTER(RcRng), C.c_uint64]
+        L.rc_occupancy_from_stp.restype = C.c_double
+        L.rc_occupancy_from_stp.argtypes = [C.c_double, C.c_double]
+        L.rc_stp_active_from_occupancy.restype = C.c_double
+        L.rc_stp_active_from_occupancy.argtypes = [C.c_double, C.c_double]
+        L.rc_band_increment.argtypes = [C.POINTER(RcBand), C.c_uint64, C.c_uint64]
+        L.rc_num_lefs.restype = C.c_uint64
+        L.rc_num_lefs.argtypes = [C.c_double, C.c_uint64]
+        L.rc_contacts_per_epoch.restype = C.c_uint64
+        L.rc_contacts_per_epoch.argtypes = [C.POINTER(RcConfig), C.c_uint64]
+        L.rc_plan_targets.restype = C.c_uint64
+        L.rc_plan_targets.argtypes = [C.POINTER(RcConfig), C.c_uint64, C.c_uint64, C.c_uint64,
+                                      U64P]
+        L.rc_simulate_cell.restype = C.c_int
+        L.rc_simulate_cell.argtypes = [C.POINTER(RcConfig), C.POINTER(RcChrom),
+                                       C.POINTER(RcBarriers), C.c_uint64, C.c_uint64, C.c_uint64,
+                                       C.c_uint64, C.POINTER(RcBand), C.POINTER(RcCellStats),
+                                       U64P, C.c_uint64]
+        L.rc_simulate_cells.restype = C.c_int
+        L.rc_simulate_cells.argtypes = [C.POINTER(RcConfig), C.POINTER(RcChrom),
+                                        C.POINTER(RcBarriers), C.c_uint64, C.c_uint64, C.c_uint64,
+                                        U64P, C.c_uint64, C.POINTER(RcBand), C.c_int,
+                                        C.POINTER(RcCellStats)]
+        _lib = L
+    return _lib
+
+
+def ptr(a: np.ndarray, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def make_config(d: dict) -> RcConfig:
+    c = RcConfig()
+    for name, _ in RcConfig._fields_:
+        setattr(c, name, d[name])
+    return c
+
+
+def make_chrom(name: str, size: int, start: int, end: int) -> RcChrom:
+    b = name.encode()
+    ch = RcChrom(b, len(b), size, start, end)
+    ch._keep = b
+    return ch
+
+
+class Barriers:
+    """Keeps numpy arrays alive for an RcBarriers view."""
+
+    def __init__(self, pos, dirs, stp_active, stp_inactive):
+        self.pos = np.ascontiguousarray(pos, dtype=np.uint64)
+        self.dir = np.ascontiguousarray(dirs, dtype=np.uint8)
+        self.stp_active = np.ascontiguousarray(stp_active, dtype=np.float64)
+        self.stp_inactive = np.ascontiguousarray(stp_inactive, dtype=np.float64)
+        self.c = RcBarriers(len(self.pos), ptr(self.pos, C.c_uint64), ptr(self.dir, C.c_uint8),
+                            ptr(self.stp_active, C.c_double), ptr(self.stp_inactive, C.c_double))
+
+
+def draw_sequence(seed: int, kind: int, n: int, a=0.0, b=0.0, c=0.0) -> np.ndarray:
+    out = np.zeros(n, dtype=np.uint64)
+    lib().rc_draw_sequence(seed, kind, a, b, c, n, ptr(out, C.c_uint64))
+    return out
+
+
+def rng(seed: int) -> RcRng:
+    g = RcRng()
+    lib().rc_rng_seed(C.byref(g), seed)
+    return g
+
+
+def simulate_cells(cfg: dict, chrom, barriers: Barriers, nlefs: int, cell_begin: int,
+                   cell_end: int, targets: np.ndarray, target_epochs: int, nrows: int,
+                   ncols: int, nthreads: int = 1, band: np.ndarray | None = None):
+    """Returns (band u32[nrows*ncols+1], missed, per-cell stats array)."""
+    if band is None:
+        band = np.zeros(nrows * ncols + 1, dtype=np.uint32)
+    rb = RcBand(nrows, ncols, ptr(band, C.c_uint32), 0)
+    ncell = cell_end - cell_begin
+    stats = (RcCellStats * max(ncell, 1))()
+    targets = np.ascontiguousarray(targets, dtype=np.uint64)
+    c = make_config(cfg)
+    rcode = lib().rc_simulate_cells(C.byref(c), C.byref(chrom), C.byref(barriers.c), nlefs,
+                                    cell_begin, cell_end, ptr(targets, C.c_uint64),
+                                    target_epochs, C.byref(rb), nthreads, stats)
+    if rcode != 0:
+        raise RuntimeError("oracle simulation failed")
+    st = np.array([(s.epochs, s.burnin_epochs, s.contacts, s.draws) for s in stats[:ncell]],
+                  dtype=np.uint64).reshape(-1, 4)
+    return band, int(rb.missed), st
+
+
+def simulate_cell_trace(cfg: dict, chrom, barriers: Barriers, nlefs: int, cell: int,
+                        target: int, target_epochs: int, nrows: int, ncols: int,
+                        max_trace: int = 100000):
+    band = np.zeros(nrows * ncols + 1, dtype=np.uint32)
+    rb = RcBand(nrows, ncols, ptr(band, C.c_uint32), 0)
+    trace = np.zeros(max_trace, dtype=np.uint64)
+    st = RcCellStats()
+    c = make_config(cfg)
+    rcode = lib().rc_simulate_cell(C.byref(c), C.byref(chrom), C.byref(barriers.c), nlefs, cell,
+                                   target, target_epochs, C.byref(rb), C.byref(st),
+                                   ptr(trace, C.c_uint64), max_trace)
+    if rcode != 0:
+        raise RuntimeError("oracle simulation failed")
+    return band, int(rb.missed), (st.epochs, st.burnin_epochs, st.contacts, st.draws), trace
