@@ -1,0 +1,204 @@
+"""Benchmark of the simulate hot path on one MI355X.
+
+A step is one pass of the hot path over one batch: every (chromosome, cell) task of the workload
+simulated to its contact target, with the inputs and the bands resident in HBM. Prints one JSON
+line (cells/s as `value`, contacts/s and epochs/s beside it), with the roofline of the simulation
+kernel and the CPU oracle timed on this host's cores on a bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload genome|small|stress]
+                    [--mode fast|det] [--cpu-seconds S]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_workload(name, ncells):
+    from modlefork_amd import workloads as W
+
+    if name == "genome":
+        return W.genome(ncells=ncells or 512)
+    if name == "small":
+        return W.small(ncells=ncells or 64)
+    if name == "stress":
+        return W.stress(ncells=ncells or 2048)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def algorithmic_bytes(plans, barriers, contacts, spill_bytes):
+    """Bytes one simulation launch must move (per cell-epoch 8 B per registered contact, a
+    read and a write of an int32 pixel, plus LEF state that spills out of LDS; per cell the
+    barrier table (pos 8 B, direction 1 B, two probabilities 16 B... counted as 21 B of
+    position/direction/probability data the kernel reads) and 32 B of engine state)."""
+    per_cell = sum(len(p.targets) * (21 * len(b) + 32) for p, b in zip(plans, barriers))
+    return 8 * contacts + per_cell + spill_bytes
+
+
+def cpu_baseline(wl, options, budget_s, nthreads):
+    """The oracle on this host: for every chromosome one batch of `nthreads` cells runs in
+    parallel; the workload's wall time is extrapolated linearly in cells per chromosome."""
+    from oracle import config as OC
+    from oracle import refcpu as R
+
+    R.build()
+    c = OC.transform(options)
+    rc = OC.to_rc(c)
+    cfg = R.make_config(rc)
+    ncells = options["num_cells"]
+    total_cells = 0
+    est_wall = 0.0
+    sampled = 0
+    spent = 0.0
+    chroms = list(wl.chroms)
+    for name, size in chroms:
+        bars = OC.prepare_barriers(wl.barriers_of(name), 0, size, c)
+        if len(bars.pos) == 0:
+            continue
+        total_cells += ncells
+        nl = R.lib().rc_num_lefs(c["number_of_lefs_per_mbp"], size)
+        ncols = R.lib().rc_band_ncols(c["bin_size"], size)
+        nrows = R.lib().rc_band_nrows(c["diagonal_width"], c["bin_size"], size)
+        t = np.zeros(ncells, np.uint64)
+        te = R.lib().rc_plan_targets(C.byref(cfg), ncells, nl, nrows * ncols,
+                                     t.ctypes.data_as(C.POINTER(C.c_uint64)))
+        k = min(nthreads, ncells)
+        ch = R.make_chrom(name, size, 0, size)
+        t0 = time.perf_counter()
+        R.simulate_cells(rc, ch, bars, nl, 0, k, t[:k], te, nrows, ncols, nthreads)
+        dt = time.perf_counter() - t0
+        spent += dt
+        sampled += k
+        est_wall += dt * (ncells / k)
+        if spent > budget_s:
+            # remaining chromosomes: extrapolate by LEF count (cost per cell ~ LEFs x epochs)
+            done = chroms.index((name, size)) + 1
+            if done < len(chroms):
+                per_lef = est_wall / sum(R.lib().rc_num_lefs(c["number_of_lefs_per_mbp"], s)
+                                         for _, s in chroms[:done])
+                for _, s in chroms[done:]:
+                    est_wall += per_lef * R.lib().rc_num_lefs(c["number_of_lefs_per_mbp"], s)
+                    total_cells += ncells
+            break
+    return {
+        "value": total_cells / est_wall,
+        "unit": "cells/s",
+        "cores": nthreads,
+        "kind": "port",
+        "sample": f"oracle (C restatement of the reference) on {sampled} cells: {min(nthreads, ncells)} "
+                  f"cells per chromosome run in parallel, workload wall time extrapolated "
+                  f"linearly in cells ({spent:.1f} s of CPU wall)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="genome")
+    ap.add_argument("--mode", default=os.environ.get("MGPU_BENCH_MODE", "det"))
+    ap.add_argument("--ncells", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the simulate path shards cells across GPUs with no "
+              "exchange until a final band reduction, and this build runs on a single device.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from modlefork_amd import _lib as L
+    from modlefork_amd import simulation as S
+
+    wl = build_workload(args.workload, args.ncells)
+    mode = L.MODE_FAST if args.mode == "fast" else L.MODE_DETERMINISTIC
+    sim = S.Simulation(wl.options, S.Context(0))
+    names, chroms, bars, plans = sim.prepare(wl)
+    log(f"workload {wl.name}: {len(chroms)} chromosomes, "
+        f"{sum(len(p.targets) for p in plans)} cells, mode {args.mode}")
+    job = S.Job(sim.ctx, sim.params, chroms, bars, plans, mode)
+
+    for i in range(args.warmup):
+        st = job.run()
+        log(f"warmup {i}: kernel {st.kernel_ms:.1f} ms")
+    kernel_ms = []
+    t0 = time.perf_counter()
+    stats = None
+    for i in range(args.steps):
+        stats = job.run()  # synchronizes the stream before and after
+        kernel_ms.append(stats.kernel_ms)
+        log(f"step {i}: kernel {stats.kernel_ms:.1f} ms, {stats.epochs} epochs")
+    wall = time.perf_counter() - t0
+    ms_per_step = 1000.0 * wall / args.steps
+    cells = int(stats.cells)
+    contacts = int(stats.contacts)
+    epochs = int(stats.epochs)
+    value = cells * args.steps / wall
+
+    avg_kernel_s = float(np.mean(kernel_ms)) / 1000.0
+    spill = 0
+    abytes = algorithmic_bytes(plans, bars, contacts, spill)
+    achieved = abytes / avg_kernel_s / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.mode}_{args.workload}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        nthreads = args.cpu_threads or min(os.cpu_count() or 1, 64)
+        log(f"cpu baseline: {nthreads} threads, ~{args.cpu_seconds:.0f} s")
+        cpu = cpu_baseline(wl, wl.options, args.cpu_seconds, nthreads)
+
+    line = {
+        "metric": "cells/sec + contacts/sec at 1 GPU",
+        "value": value,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64/f64" if mode == L.MODE_DETERMINISTIC else "u32/f32",
+        "data": "synthetic",
+        "config": {"workload": wl.name, "chromosomes": len(chroms), "cells": cells,
+                   "mode": "deterministic" if mode == L.MODE_DETERMINISTIC else "fast",
+                   "bin_size": int(sim.params.bin_size),
+                   "diagonal_width": int(sim.params.diagonal_width),
+                   "lefs_per_mbp": float(sim.params.lefs_per_mbp), "parallelism": "none"},
+        "contacts_per_sec": contacts * args.steps / wall,
+        "epochs_per_sec": epochs * args.steps / wall,
+        "kernel_ms": float(np.mean(kernel_ms)),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["speedup_vs_cpu"] = value / cpu["value"]
+    job.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

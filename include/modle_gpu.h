@@ -1,0 +1,231 @@
+/*
+ * modle_gpu.h: C-ABI of the MI355X engine for MoDLE's simulate hot path.
+ *
+ * The library replaces the per-cell epoch loop of libmodle's Simulation (the
+ * simulate_worker -> simulate_one_cell chain, src/libmodle/cpu/scheduler_simulate.cpp:186-284,
+ * src/libmodle/cpu/simulation.cpp:1056-1152) and the ContactMatrixDense<u32> it fills
+ * (src/contact_matrix/contact_matrix_dense_impl.hpp). The caller keeps everything above that
+ * line: CLI and Config parsing, genome/BED import and the Cooler writer.
+ *
+ * Conventions: plain C types, no exceptions across the boundary. Every function returns 0 on
+ * success and a negative MGPU_E* code on failure; mgpu_last_error() returns the message of the
+ * last failure on a context (or of the last context-free call on this thread when ctx is NULL).
+ * A context is not thread-safe: use one per driver thread.
+ */
+#ifndef MODLE_GPU_H
+#define MODLE_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MGPU_OK 0
+#define MGPU_EINVAL -1   /* invalid argument */
+#define MGPU_EDEVICE -2  /* HIP runtime error */
+#define MGPU_ENOMEM -3   /* device allocation failed */
+#define MGPU_ERANGE -4   /* input outside the supported range */
+
+#define MGPU_MODE_DETERMINISTIC 0 /* reference draw order, bit-exact bands */
+#define MGPU_MODE_FAST 1          /* counter-based RNG, statistically equivalent */
+
+/* Stopping criterion (simulation_config.hpp:31). */
+#define MGPU_STOP_CONTACT_DENSITY 0
+#define MGPU_STOP_EPOCHS 1
+
+/* User-facing parameters with the names, units and defaults of modle::Config
+ * (src/common/include/modle/common/simulation_config.hpp:24-135), limited to what the simulate
+ * hot path reads. Fill with mgpu_config_defaults(), set what the user passed, then call
+ * mgpu_config_transform() with the MGPU_SET_* bits of the options given explicitly. */
+typedef struct mgpu_config {
+  uint64_t bin_size;
+  uint64_t diagonal_width;
+  int32_t sample_loop_contacts; /* ContactSamplingStrategy::loop */
+  int32_t sample_tad_contacts;  /* ContactSamplingStrategy::tad */
+  int32_t noisify_contacts;     /* ContactSamplingStrategy::noisify */
+  int32_t stopping_criterion;   /* MGPU_STOP_* */
+  double tad_to_loop_contact_ratio;
+  double genextreme_mu, genextreme_sigma, genextreme_xi;
+  uint64_t rev_extrusion_speed, fwd_extrusion_speed;
+  double rev_extrusion_speed_std, fwd_extrusion_speed_std;
+  double number_of_lefs_per_mbp;
+  uint64_t avg_lef_processivity;
+  uint64_t contact_sampling_interval;
+  double extrusion_barrier_occupancy;
+  double barrier_occupied_stp, barrier_not_occupied_stp;
+  double hard_stall_lef_stability_multiplier, soft_stall_lef_stability_multiplier;
+  double probability_of_extrusion_unit_bypass;
+  double lef_bar_major_collision_pblock, lef_bar_minor_collision_pblock;
+  uint64_t target_simulation_epochs;
+  double target_contact_density;
+  uint64_t num_cells;
+  uint64_t seed;
+  uint64_t probability_normalization_factor;
+  int32_t normalize_probabilities;
+  int32_t skip_burnin;
+  uint64_t burnin_history_length, burnin_smoothing_window_size;
+  uint64_t min_burnin_epochs, max_burnin_epochs;
+  double burnin_speed_coefficient;
+} mgpu_config;
+
+/* Options given explicitly on the command line (the CLI's "option parsed" tests). */
+#define MGPU_SET_REV_SPEED (1u << 0)
+#define MGPU_SET_FWD_SPEED (1u << 1)
+#define MGPU_SET_BARRIER_OCCUPANCY (1u << 2)
+
+/* Derived parameters, i.e. modle::Config after Cli::transform_args (src/modle/cli.cpp:1192-1215).
+ * This is the block every simulate entry point consumes. */
+typedef struct mgpu_params {
+  uint64_t bin_size;
+  uint64_t diagonal_width;
+  uint64_t rev_speed, fwd_speed;               /* bp per epoch */
+  uint64_t rev_speed_burnin, fwd_speed_burnin; /* bp per epoch during burn-in */
+  double rev_speed_std, fwd_speed_std;         /* absolute, bp */
+  double prob_release, prob_release_burnin;
+  double p_bypass;
+  double pblock_major, pblock_minor;
+  double hard_stall_mult, soft_stall_mult;
+  double tad_to_loop_ratio; /* 0: loop only, +inf: TAD only */
+  double gev_mu, gev_sigma, gev_xi;
+  int32_t noisify;
+  int32_t skip_burnin;
+  uint64_t contact_sampling_interval;
+  uint64_t burnin_history_length;
+  uint64_t burnin_smoothing_window;
+  uint64_t min_burnin_epochs;
+  uint64_t max_burnin_epochs;
+  uint64_t burnin_target_epochs_for_lef_activation;
+  double target_contact_density; /* < 0 when the stopping criterion is epochs */
+  uint64_t target_simulation_epochs;
+  uint64_t seed;
+  double lefs_per_mbp;
+  double barrier_occupied_stp, barrier_not_occupied_stp; /* defaults for barriers w/o score */
+  int32_t override_barrier_occupancy; /* --extrusion-barrier-occupancy given */
+  int32_t reserved;
+} mgpu_params;
+
+/* One chromosome (Chromosome, src/libmodle/internal/include/modle/genome.hpp:32-120). */
+typedef struct mgpu_chrom {
+  const char* name;
+  uint64_t name_len;
+  uint64_t size;  /* chrom.sizes length; hashed into the per-cell seed */
+  uint64_t start; /* simulated range [start, end) */
+  uint64_t end;
+} mgpu_chrom;
+
+/* Barriers of one chromosome, sorted by position. dir is the blocking direction
+ * (dna::Direction: 1 = REV, 2 = FWD). */
+typedef struct mgpu_barriers {
+  uint64_t n;
+  const uint64_t* pos;
+  const uint8_t* dir;
+  const double* stp_active;   /* self-transition probability of the occupied state */
+  const double* stp_inactive; /* self-transition probability of the free state */
+} mgpu_barriers;
+
+/* A BED record as the genome importer sees it (bed::BED fields used at genome.cpp:194-210). */
+typedef struct mgpu_bed_record {
+  uint64_t chrom_start, chrom_end;
+  double score;
+  char strand; /* '+', '-' or '.' */
+} mgpu_bed_record;
+
+typedef struct mgpu_stats {
+  uint64_t cells;
+  uint64_t epochs;        /* sum over cells */
+  uint64_t burnin_epochs; /* sum over cells */
+  uint64_t contacts;      /* sum over cells (registered, incl. missed) */
+  uint64_t missed;        /* out-of-band updates */
+  double kernel_ms;       /* device time of the simulation launches (HIP events) */
+  double total_ms;        /* wall time of the call */
+} mgpu_stats;
+
+typedef struct mgpu_ctx mgpu_ctx;
+typedef struct mgpu_job mgpu_job;
+
+/* ---- context ---- */
+int mgpu_init(int device, mgpu_ctx** out);
+void mgpu_destroy(mgpu_ctx* ctx);
+const char* mgpu_last_error(const mgpu_ctx* ctx);
+const char* mgpu_version(void);
+
+/* ---- parameters (host only) ---- */
+/* modle::Config defaults, simulation_config.hpp:24-135. */
+int mgpu_config_defaults(mgpu_config* c);
+/* Cli::transform_args, cli.cpp:1085-1215; set_mask = MGPU_SET_* of explicit options. */
+int mgpu_config_transform(const mgpu_config* c, uint32_t set_mask, mgpu_params* out);
+/* ExtrusionBarrier::compute_stp_active_from_occupancy / compute_occupancy_from_stp,
+ * extrusion_barriers_impl.hpp:106-128. */
+double mgpu_stp_active_from_occupancy(double stp_inactive, double occupancy);
+double mgpu_occupancy_from_stp(double stp_active, double stp_inactive);
+/* Chromosome::add_extrusion_barrier for each record (genome.cpp:181-210), the occupancy override
+ * of Simulation::Simulation (simulation.cpp:68-78) and the position sort of ExtrusionBarriers
+ * (extrusion_barriers.cpp:234-259). Outputs hold n entries; *nout receives the number kept. */
+int mgpu_prepare_barriers(const mgpu_params* p, const mgpu_bed_record* recs, uint64_t n,
+                          uint64_t chrom_start, uint64_t chrom_end, uint64_t* pos, uint8_t* dir,
+                          double* stp_active, double* stp_inactive, uint64_t* nout);
+/* Per-chromosome plan of run_simulate (scheduler_simulate.cpp:128-153): number of LEFs, band shape
+ * and per-cell contact targets (target_contacts has ncells entries) and target epochs. */
+int mgpu_plan_chrom(const mgpu_params* p, const mgpu_chrom* chrom, uint64_t ncells,
+                    uint64_t* nlefs, uint64_t* nrows, uint64_t* ncols, uint64_t* target_contacts,
+                    uint64_t* target_epochs);
+/* Per-cell seed, Chromosome::hash (genome.cpp:317-336): XXH3-64 of name || size || cell_id. */
+uint64_t mgpu_cell_seed(const mgpu_chrom* chrom, uint64_t cell_id, uint64_t seed);
+
+/* ---- simulation ---- */
+/* Simulates cells [cell_begin, cell_begin + ncells) of one chromosome and adds their contacts to
+ * out_band (nrows*ncols + 1 u32, column-major band, data[col*nrows + (col-row)]; the layout of
+ * ContactMatrixDense<u32>::_contacts). out_band is a host pointer; *out_missed receives the
+ * out-of-band count. Replaces simulate_one_cell for every task of the chromosome. */
+int mgpu_simulate_chrom(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom* chrom,
+                        const mgpu_barriers* barriers, uint64_t nlefs, uint64_t cell_begin,
+                        uint64_t ncells, const uint64_t* target_contacts, uint64_t target_epochs,
+                        int mode, uint32_t* out_band, uint64_t* out_missed, mgpu_stats* stats);
+
+/* Device-resident batch over many chromosomes: inputs are uploaded once, bands live in HBM until
+ * read back. per_chrom arrays have nchroms entries; target_contacts[c] has ncells[c] entries. */
+int mgpu_job_create(mgpu_ctx* ctx, const mgpu_params* p, uint64_t nchroms,
+                    const mgpu_chrom* chroms, const mgpu_barriers* barriers, const uint64_t* nlefs,
+                    const uint64_t* ncells, const uint64_t* const* target_contacts,
+                    const uint64_t* target_epochs, int mode, mgpu_job** out);
+/* Zeroes the bands and simulates every cell of the job. */
+int mgpu_job_run(mgpu_job* job, mgpu_stats* stats);
+/* Copies chromosome c's band to host memory (nrows*ncols + 1 u32) and its missed count. */
+int mgpu_job_read_band(mgpu_job* job, uint64_t c, uint32_t* out_band, uint64_t* out_missed);
+/* Device pointer of chromosome c's band (valid until mgpu_job_destroy). */
+int mgpu_job_band_device_ptr(mgpu_job* job, uint64_t c, void** dptr, uint64_t* nelems);
+void mgpu_job_destroy(mgpu_job* job);
+
+/* ---- step-level entry points (the reference's ENABLE_TESTING hooks, simulation.hpp:499-651),
+ * run by the device step code on one LEF state ----
+ * rev/fwd/epoch: LEF arrays (UINT64_MAX when unbound); ranks and moves as in the reference;
+ * collisions use the reference's 64-bit encoding (index | event << 56).
+ * op: 0 adjust+clamp moves; 1 process_collisions without fix_secondary (test_process_collisions);
+ *     2 full process_collisions; 3 detect_primary + correct_primary + process_secondary with no
+ *     boundary units; 4 fix_secondary with no boundary units; 5 detect_lef_bar with no boundary
+ *     units; 6 detect_primary with no boundary units; 7 correct_moves_for_lef_bar;
+ *     8 detect_units_at_chrom_boundaries; 9 rank_lefs (pdq, init ranks); 10 rank_lefs (split);
+ *     11 generate_moves (burn-in speeds off). rng_state is the 4-word engine state (in/out). */
+int mgpu_debug_step(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom* chrom, int op,
+                    uint64_t n, uint64_t* rev, uint64_t* fwd, uint64_t* epoch,
+                    uint64_t* rev_ranks, uint64_t* fwd_ranks, uint64_t* rev_moves,
+                    uint64_t* fwd_moves, uint64_t* rev_coll, uint64_t* fwd_coll,
+                    const mgpu_barriers* barriers, const uint8_t* barrier_state,
+                    uint64_t* rng_state);
+/* n draws of one sampler from a seeded engine (kind as the oracle: 0 raw, 1 normal(a,b),
+ * 2 bernoulli(a), 3 canonical, 4 uniform[a,b], 5 poisson(a), 6 binomial(a,b), 7 gev(a,b,c),
+ * 8 exponential(a)); doubles are returned as bit patterns. Runs on the device. */
+int mgpu_debug_draws(mgpu_ctx* ctx, uint64_t seed, int kind, double a, double b, double c,
+                     uint64_t n, uint64_t* out);
+/* Per-epoch state hash of one cell in deterministic mode (for locating a first divergence). */
+int mgpu_debug_cell_trace(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom* chrom,
+                          const mgpu_barriers* barriers, uint64_t nlefs, uint64_t cell_id,
+                          uint64_t target_contacts, uint64_t target_epochs, uint64_t max_trace,
+                          uint64_t* trace, uint64_t* out_epochs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

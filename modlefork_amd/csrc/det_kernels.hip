@@ -1,0 +1,174 @@
+// Deterministic-mode kernels: the whole-cell epoch loop behind a device work queue, and the
+// step-level kernels behind the debug entry points. Built with -ffp-contract=off.
+#include <hip/hip_runtime.h>
+
+#include "det_sim.h"
+#include "kernels.h"
+
+namespace mgpu {
+
+// Each block (one wavefront) pulls (chromosome, cell) items from a device counter until the
+// queue is empty, so uneven chromosomes and cells with different epoch counts balance across CUs.
+// Only lane 0 runs the serial per-cell chain; the block's working set is carved from dynamic
+// LDS when use_lds is set, else from its slab of global scratch.
+__global__ void __launch_bounds__(64) det_cells_kernel(DetParams P, const DetChrom* chroms,
+                                                       const WorkItem* items, uint64_t nitems,
+                                                       unsigned long long* next_item,
+                                                       uint8_t* scratch, uint64_t stride,
+                                                       int use_lds, uint64_t* cell_stats) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  if (threadIdx.x != 0) return;
+  uint8_t* base = use_lds ? lds : scratch + (uint64_t)blockIdx.x * stride;
+  for (;;) {
+    const uint64_t it = atomicAdd(next_item, 1ULL);
+    if (it >= nitems) return;
+    const WorkItem w = items[it];
+    const DetChrom ch = chroms[w.chrom];
+    DetCell s;
+    det_carve(s, base, ch.nlefs, ch.nbar, P.hist_len);
+    det_simulate_cell(P, ch, s, w.seed, w.target, nullptr, 0);
+    if (cell_stats) {
+      cell_stats[4 * it + 0] = s.epoch;
+      cell_stats[4 * it + 1] = s.burnin_epochs;
+      cell_stats[4 * it + 2] = s.contacts;
+      cell_stats[4 * it + 3] = s.g.draws;
+    }
+  }
+}
+
+__global__ void det_trace_kernel(DetParams P, DetChrom ch, uint64_t seed, uint64_t target,
+                                 uint8_t* scratch, uint64_t* trace, uint64_t max_trace,
+                                 uint64_t* out_epochs) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  DetCell s;
+  det_carve(s, scratch, ch.nlefs, ch.nbar, P.hist_len);
+  det_simulate_cell(P, ch, s, seed, target, trace, max_trace);
+  out_epochs[0] = s.epoch;
+  out_epochs[1] = s.burnin_epochs;
+  out_epochs[2] = s.contacts;
+  out_epochs[3] = s.g.draws;
+}
+
+// One step of the epoch loop on a state held in global memory (debug / known-answer tests).
+__global__ void det_step_kernel(DetParams P, DetChrom ch, int op, uint64_t n, uint64_t* rev,
+                                uint64_t* fwd, uint64_t* ep, uint32_t* rr, uint32_t* fr,
+                                uint32_t* tmp, uint64_t* rm, uint64_t* fm, uint64_t* rc,
+                                uint64_t* fc, uint8_t* bst, uint64_t* rng) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  DetCell s;
+  s.rev = rev;
+  s.fwd = fwd;
+  s.ep = ep;
+  s.rr = rr;
+  s.fr = fr;
+  s.tmp = tmp;
+  s.rm = rm;
+  s.fm = fm;
+  s.rc = rc;
+  s.fc = fc;
+  s.bst = bst;
+  s.n = s.nact = n;
+  s.burnin_done = true;
+  s.g.s0 = rng[0];
+  s.g.s1 = rng[1];
+  s.g.s2 = rng[2];
+  s.g.s3 = rng[3];
+  s.g.draws = 0;
+  uint64_t n5 = 0, n3 = 0;
+  switch (op) {
+    case 0: det_adjust_clamp(ch, s, n, true); break;
+    case 1: det_process_collisions(P, ch, s, n, false); break;
+    case 2: det_process_collisions(P, ch, s, n, true); break;
+    case 3:
+      det_primary(P, ch, s, n, 0, 0);
+      det_correct_primary(s, n);
+      det_secondary(P, s, n, 0, 0);
+      break;
+    case 4: det_fix_secondary(ch, s, n, 0, 0); break;
+    case 5: det_lef_bar(P, ch, s, n, 0, 0); break;
+    case 6: det_primary(P, ch, s, n, 0, 0); break;
+    case 7: det_correct_lef_bar(ch, s, n); break;
+    case 8: det_boundaries(ch, s, n, n5, n3); break;
+    case 9:
+      for (uint64_t i = 0; i < n; ++i) rr[i] = fr[i] = (uint32_t)i;
+      det_rank(s, n, false);
+      break;
+    case 10: det_rank(s, n, true); break;
+    case 11: det_generate_moves(P, ch, s, n); break;
+    default: break;
+  }
+  rng[0] = s.g.s0;
+  rng[1] = s.g.s1;
+  rng[2] = s.g.s2;
+  rng[3] = s.g.s3;
+}
+
+__global__ void det_draws_kernel(uint64_t seed, int kind, double a, double b, double c, uint64_t n,
+                                 uint64_t* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  DetRng g;
+  det_seed(g, seed);
+  for (uint64_t i = 0; i < n; ++i) {
+    double d = 0.0;
+    uint64_t v = 0;
+    switch (kind) {
+      case 0: v = det_next(g); break;
+      case 1: d = det_normal(g, a, b); v = __double_as_longlong(d); break;
+      case 2: v = det_bernoulli(g, a) ? 1 : 0; break;
+      case 3: d = det_canonical(g); v = __double_as_longlong(d); break;
+      case 4: v = det_uniform(g, (uint64_t)a, (uint64_t)b); break;
+      case 5: v = det_poisson(g, a); break;
+      case 6: v = (uint64_t)det_binomial(g, (int64_t)a, b, -1.0); break;
+      case 7: d = det_gev(g, a, b, c); v = __double_as_longlong(d); break;
+      case 8: d = det_unit_exp(g) / a; v = __double_as_longlong(d); break;
+      default: break;
+    }
+    out[i] = v;
+  }
+}
+
+// ---- host-side launchers ----
+hipError_t launch_det_cells(hipStream_t stream, const DetParams& P, const DetChrom* d_chroms,
+                            const WorkItem* d_items, uint64_t nitems, unsigned long long* d_next,
+                            uint8_t* d_scratch, uint64_t stride, bool use_lds, uint32_t grid,
+                            uint64_t* d_cell_stats) {
+  const size_t shmem = use_lds ? stride : 0;
+  if (use_lds && shmem > 65536) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(det_cells_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(det_cells_kernel, dim3(grid), dim3(64), shmem, stream, P, d_chroms, d_items,
+                     nitems, d_next, d_scratch, stride, use_lds ? 1 : 0, d_cell_stats);
+  return hipGetLastError();
+}
+
+hipError_t launch_det_trace(hipStream_t stream, const DetParams& P, const DetChrom& ch,
+                            uint64_t seed, uint64_t target, uint8_t* d_scratch, uint64_t* d_trace,
+                            uint64_t max_trace, uint64_t* d_out) {
+  hipLaunchKernelGGL(det_trace_kernel, dim3(1), dim3(64), 0, stream, P, ch, seed, target,
+                     d_scratch, d_trace, max_trace, d_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_det_step(hipStream_t stream, const DetParams& P, const DetChrom& ch, int op,
+                           uint64_t n, uint64_t* rev, uint64_t* fwd, uint64_t* ep, uint32_t* rr,
+                           uint32_t* fr, uint32_t* tmp, uint64_t* rm, uint64_t* fm, uint64_t* rc,
+                           uint64_t* fc, uint8_t* bst, uint64_t* rng) {
+  hipLaunchKernelGGL(det_step_kernel, dim3(1), dim3(64), 0, stream, P, ch, op, n, rev, fwd, ep, rr,
+                     fr, tmp, rm, fm, rc, fc, bst, rng);
+  return hipGetLastError();
+}
+
+hipError_t launch_det_draws(hipStream_t stream, uint64_t seed, int kind, double a, double b,
+                            double c, uint64_t n, uint64_t* d_out) {
+  hipLaunchKernelGGL(det_draws_kernel, dim3(1), dim3(64), 0, stream, seed, kind, a, b, c, n, d_out);
+  return hipGetLastError();
+}
+
+uint64_t det_cell_bytes_host(uint64_t nlefs, uint64_t nbar, uint64_t hist) {
+  uint64_t b = nlefs * (7 * 8 + 3 * 4) + nbar + 2 * (hist + 1) * 8;
+  return (b + 255) & ~255ULL;
+}
+
+}  // namespace mgpu

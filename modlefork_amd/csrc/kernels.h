@@ -1,0 +1,65 @@
+// Launch-side declarations shared by the kernel translation units and the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mgpu {
+
+// One (chromosome, cell) task of run_simulate (scheduler_simulate.cpp:141-153).
+struct WorkItem {
+  uint32_t chrom;
+  uint32_t reserved;
+  uint64_t cell_id;
+  uint64_t seed;    // Chromosome::hash(seed, cell_id)
+  uint64_t target;  // contacts this cell must register
+};
+
+// Post-transform parameters in the form the deterministic kernels read them.
+struct DetParams {
+  uint64_t bin_size;
+  uint64_t rev_speed, fwd_speed, rev_speed_burnin, fwd_speed_burnin;
+  double rev_std, fwd_std;
+  double prob_release, prob_release_burnin;
+  double p_bypass, pblock_major, pblock_minor;
+  double inv_soft, inv_hard;  // 1/soft and 1/hard stall multipliers
+  double tad_to_loop_ratio;
+  double gev_mu, gev_sigma, gev_xi;
+  int32_t noisify, skip_burnin;
+  uint64_t hist_len, window;
+  uint64_t min_burnin_epochs, max_burnin_epochs;
+  double target_contact_density;
+};
+
+// Device view of one chromosome: barrier table (sorted), band and its counters.
+struct DetChrom {
+  uint64_t start, end;
+  uint64_t nlefs, nbar;
+  const uint64_t* bar_pos;
+  const uint8_t* bar_dir;
+  const double* bar_stp_active;
+  const double* bar_stp_inactive;
+  uint32_t* band;
+  unsigned long long* missed;
+  uint64_t nrows;
+  uint64_t cpe;            // contacts per epoch
+  uint64_t target_epochs;  // epochs criterion
+  double bind_rate;        // nlefs / burnin_target_epochs_for_lef_activation
+  const double* qn;        // pow(1 - p_loop, t) for t in [0, cpe], or null
+};
+
+hipError_t launch_det_cells(hipStream_t stream, const DetParams& P, const DetChrom* d_chroms,
+                            const WorkItem* d_items, uint64_t nitems, unsigned long long* d_next,
+                            uint8_t* d_scratch, uint64_t stride, bool use_lds, uint32_t grid,
+                            uint64_t* d_cell_stats);
+hipError_t launch_det_trace(hipStream_t stream, const DetParams& P, const DetChrom& ch,
+                            uint64_t seed, uint64_t target, uint8_t* d_scratch, uint64_t* d_trace,
+                            uint64_t max_trace, uint64_t* d_out);
+hipError_t launch_det_step(hipStream_t stream, const DetParams& P, const DetChrom& ch, int op,
+                           uint64_t n, uint64_t* rev, uint64_t* fwd, uint64_t* ep, uint32_t* rr,
+                           uint32_t* fr, uint32_t* tmp, uint64_t* rm, uint64_t* fm, uint64_t* rc,
+                           uint64_t* fc, uint8_t* bst, uint64_t* rng);
+hipError_t launch_det_draws(hipStream_t stream, uint64_t seed, int kind, double a, double b,
+                            double c, uint64_t n, uint64_t* d_out);
+uint64_t det_cell_bytes_host(uint64_t nlefs, uint64_t nbar, uint64_t hist);
+
+}  // namespace mgpu

@@ -1,0 +1,855 @@
+// C-ABI of the engine (include/modle_gpu.h): parameters, planning, device jobs and the debug
+// entry points. Host code only; kernels live in det_kernels.hip and fast_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/modle_gpu.h"
+#include "fast_kernels.h"
+#include "kernels.h"
+#include "xxh3_host.h"
+
+using namespace mgpu;
+
+struct mgpu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  int num_cus = 0;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const mgpu_ctx* ctx, int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (ctx) const_cast<mgpu_ctx*>(ctx)->err = buf;
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                      \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess) return fail(ctx, MGPU_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+double clamp01(double v) { return v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v); }
+
+uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+uint64_t contacts_per_epoch(const mgpu_params* p, uint64_t nlefs) {
+  const double speed = (double)(p->rev_speed + p->fwd_speed);
+  const double prob = speed / (double)p->contact_sampling_interval;
+  return (uint64_t)std::max(1.0, std::round((double)nlefs * prob));
+}
+
+DetParams det_params(const mgpu_params* p) {
+  DetParams P{};
+  P.bin_size = p->bin_size;
+  P.rev_speed = p->rev_speed;
+  P.fwd_speed = p->fwd_speed;
+  P.rev_speed_burnin = p->rev_speed_burnin;
+  P.fwd_speed_burnin = p->fwd_speed_burnin;
+  P.rev_std = p->rev_speed_std;
+  P.fwd_std = p->fwd_speed_std;
+  P.prob_release = p->prob_release;
+  P.prob_release_burnin = p->prob_release_burnin;
+  P.p_bypass = p->p_bypass;
+  P.pblock_major = p->pblock_major;
+  P.pblock_minor = p->pblock_minor;
+  P.inv_soft = 1.0 / p->soft_stall_mult;
+  P.inv_hard = 1.0 / p->hard_stall_mult;
+  P.tad_to_loop_ratio = p->tad_to_loop_ratio;
+  P.gev_mu = p->gev_mu;
+  P.gev_sigma = p->gev_sigma;
+  P.gev_xi = p->gev_xi;
+  P.noisify = p->noisify;
+  P.skip_burnin = p->skip_burnin;
+  P.hist_len = p->burnin_history_length;
+  P.window = p->burnin_smoothing_window;
+  P.min_burnin_epochs = p->min_burnin_epochs;
+  P.max_burnin_epochs = p->max_burnin_epochs;
+  P.target_contact_density = p->target_contact_density;
+  return P;
+}
+
+int check_params(const mgpu_params* p) {
+  if (!p) return fail(nullptr, MGPU_EINVAL, "null parameter block");
+  if (p->bin_size == 0) return fail(nullptr, MGPU_EINVAL, "bin_size must be > 0");
+  if (p->contact_sampling_interval == 0)
+    return fail(nullptr, MGPU_EINVAL, "contact_sampling_interval must be > 0");
+  if (p->burnin_history_length <= p->burnin_smoothing_window)
+    return fail(nullptr, MGPU_EINVAL, "burn-in history must be longer than its smoothing window");
+  if (p->burnin_target_epochs_for_lef_activation == 0)
+    return fail(nullptr, MGPU_EINVAL, "burn-in LEF activation epochs must be > 0");
+  return MGPU_OK;
+}
+
+int check_chrom(const mgpu_chrom* c) {
+  if (!c || !c->name) return fail(nullptr, MGPU_EINVAL, "null chromosome");
+  if (c->end <= c->start + 2)
+    return fail(nullptr, MGPU_EINVAL, "chromosome %.*s: simulated range is empty",
+                (int)c->name_len, c->name);
+  if (c->end > c->size) return fail(nullptr, MGPU_EINVAL, "chromosome range beyond its size");
+  if (c->name_len > 224) return fail(nullptr, MGPU_ERANGE, "chromosome name longer than 224 bytes");
+  return MGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mgpu_version(void) { return "modle-mi355x 0.1.0"; }
+
+const char* mgpu_last_error(const mgpu_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int mgpu_init(int device, mgpu_ctx** out) {
+  if (!out) return fail(nullptr, MGPU_EINVAL, "null output pointer");
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) return fail(nullptr, MGPU_EDEVICE, "no HIP device available");
+  if (device < 0 || device >= ndev) return fail(nullptr, MGPU_EINVAL, "device %d out of range", device);
+  auto* ctx = new mgpu_ctx();
+  ctx->device = device;
+  HIPCHK(ctx, hipSetDevice(device));
+  HIPCHK(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  HIPCHK(ctx, hipEventCreate(&ctx->ev0));
+  HIPCHK(ctx, hipEventCreate(&ctx->ev1));
+  hipDeviceProp_t prop;
+  HIPCHK(ctx, hipGetDeviceProperties(&prop, device));
+  ctx->num_cus = prop.multiProcessorCount;
+  *out = ctx;
+  return MGPU_OK;
+}
+
+void mgpu_destroy(mgpu_ctx* ctx) {
+  if (!ctx) return;
+  hipSetDevice(ctx->device);
+  if (ctx->ev0) hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) hipEventDestroy(ctx->ev1);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+// ---- parameters --------------------------------------------------------------------------------
+
+int mgpu_config_defaults(mgpu_config* c) {
+  if (!c) return fail(nullptr, MGPU_EINVAL, "null config");
+  *c = mgpu_config{};
+  c->bin_size = 5000;
+  c->diagonal_width = 3000000;
+  c->sample_loop_contacts = 1;
+  c->sample_tad_contacts = 1;
+  c->noisify_contacts = 1;
+  c->stopping_criterion = MGPU_STOP_CONTACT_DENSITY;
+  c->tad_to_loop_contact_ratio = 5.0;
+  c->genextreme_mu = 0.0;
+  c->genextreme_sigma = 5000.0;
+  c->genextreme_xi = 0.001;
+  c->rev_extrusion_speed = c->fwd_extrusion_speed = c->bin_size * 8 / 10;
+  c->rev_extrusion_speed_std = c->fwd_extrusion_speed_std = 0.05;
+  c->number_of_lefs_per_mbp = 20.0;
+  c->avg_lef_processivity = 300000;
+  c->contact_sampling_interval = 50000;
+  c->extrusion_barrier_occupancy = 0.825;
+  c->barrier_occupied_stp = 0.0;
+  c->barrier_not_occupied_stp = 0.70;
+  c->hard_stall_lef_stability_multiplier = 5.0;
+  c->soft_stall_lef_stability_multiplier = 1.0;
+  c->probability_of_extrusion_unit_bypass = 0.1;
+  c->lef_bar_major_collision_pblock = 1.0;
+  c->lef_bar_minor_collision_pblock = 0.0;
+  c->target_simulation_epochs = 2000;
+  c->target_contact_density = 1.0;
+  c->num_cells = 512;
+  c->seed = 0;
+  c->probability_normalization_factor = 8000;
+  c->normalize_probabilities = 1;
+  c->skip_burnin = 0;
+  c->burnin_history_length = 100;
+  c->burnin_smoothing_window_size = 5;
+  c->min_burnin_epochs = 0;
+  c->max_burnin_epochs = std::numeric_limits<uint64_t>::max();
+  c->burnin_speed_coefficient = 1.0;
+  return MGPU_OK;
+}
+
+double mgpu_stp_active_from_occupancy(double stp_inactive, double occupancy) {
+  if (occupancy == 0) return 0.0;
+  const double tp_inactive_to_active = 1.0 - stp_inactive;
+  const double tp_active_to_inactive =
+      (tp_inactive_to_active - (occupancy * tp_inactive_to_active)) / occupancy;
+  return clamp01(1.0 - tp_active_to_inactive);
+}
+
+double mgpu_occupancy_from_stp(double stp_active, double stp_inactive) {
+  if (stp_active + stp_inactive == 0) return 0.0;
+  const double tp_inactive_to_active = 1.0 - stp_inactive;
+  const double tp_active_to_inactive = 1.0 - stp_active;
+  return clamp01(tp_inactive_to_active / (tp_inactive_to_active + tp_active_to_inactive));
+}
+
+// Cli::transform_args (cli.cpp:1192-1215) with its helpers cli_update_extr_speed (:1085-1110),
+// cli_compute_prob_of_lef_release (:1112-1119), cli_update_barrier_stp_and_occupancy (:1122-1135),
+// cli_update_tad_to_loop_contact_ratio (:1169-1182), cli_update_burnin_params (:1184-1190) and
+// cli_normalize_probabilities (:1138-1167).
+int mgpu_config_transform(const mgpu_config* in, uint32_t set_mask, mgpu_params* out) {
+  if (!in || !out) return fail(nullptr, MGPU_EINVAL, "null argument");
+  if (in->bin_size == 0) return fail(nullptr, MGPU_EINVAL, "bin_size must be > 0");
+  if (!in->sample_loop_contacts && !in->sample_tad_contacts)
+    return fail(nullptr, MGPU_EINVAL, "at least one of loop/TAD contact sampling must be enabled");
+  mgpu_config c = *in;
+  if (!(set_mask & MGPU_SET_REV_SPEED)) c.rev_extrusion_speed = c.bin_size * 8 / 10;
+  if (!(set_mask & MGPU_SET_FWD_SPEED)) c.fwd_extrusion_speed = c.bin_size * 8 / 10;
+  if (c.fwd_extrusion_speed_std > 0 && c.fwd_extrusion_speed_std < 1)
+    c.fwd_extrusion_speed_std *= (double)c.fwd_extrusion_speed;
+  if (c.rev_extrusion_speed_std > 0 && c.rev_extrusion_speed_std < 1)
+    c.rev_extrusion_speed_std *= (double)c.rev_extrusion_speed;
+  const uint64_t rev_burnin =
+      (uint64_t)std::round(c.burnin_speed_coefficient * (double)c.rev_extrusion_speed);
+  const uint64_t fwd_burnin =
+      (uint64_t)std::round(c.burnin_speed_coefficient * (double)c.fwd_extrusion_speed);
+  if (c.avg_lef_processivity == 0) return fail(nullptr, MGPU_EINVAL, "processivity must be > 0");
+  if (rev_burnin + fwd_burnin == 0) return fail(nullptr, MGPU_EINVAL, "extrusion speeds are zero");
+
+  mgpu_params p{};
+  p.bin_size = c.bin_size;
+  p.diagonal_width = c.diagonal_width;
+  p.rev_speed = c.rev_extrusion_speed;
+  p.fwd_speed = c.fwd_extrusion_speed;
+  p.rev_speed_burnin = rev_burnin;
+  p.fwd_speed_burnin = fwd_burnin;
+  p.rev_speed_std = c.rev_extrusion_speed_std;
+  p.fwd_speed_std = c.fwd_extrusion_speed_std;
+  p.prob_release =
+      (double)(c.rev_extrusion_speed + c.fwd_extrusion_speed) / (double)c.avg_lef_processivity;
+  p.prob_release_burnin = (double)(rev_burnin + fwd_burnin) / (double)c.avg_lef_processivity;
+
+  double occupancy = c.extrusion_barrier_occupancy;
+  double stp_active = c.barrier_occupied_stp;
+  double stp_inactive = c.barrier_not_occupied_stp;
+  if (set_mask & MGPU_SET_BARRIER_OCCUPANCY) {
+    stp_active = mgpu_stp_active_from_occupancy(stp_inactive, occupancy);
+  } else {
+    occupancy = mgpu_occupancy_from_stp(stp_active, stp_inactive);
+  }
+
+  double ratio_tl = c.tad_to_loop_contact_ratio;
+  if (c.sample_loop_contacts && !c.sample_tad_contacts) ratio_tl = 0.0;
+  if (!c.sample_loop_contacts && c.sample_tad_contacts)
+    ratio_tl = std::numeric_limits<double>::infinity();
+
+  const uint64_t act_bp = 5 * c.avg_lef_processivity;
+  p.burnin_target_epochs_for_lef_activation =
+      std::min(c.max_burnin_epochs, act_bp / (rev_burnin + fwd_burnin));
+
+  double p_bypass = c.probability_of_extrusion_unit_bypass;
+  double pblock_major = c.lef_bar_major_collision_pblock;
+  double pblock_minor = c.lef_bar_minor_collision_pblock;
+  if (c.normalize_probabilities) {
+    const double ratio = (double)(c.rev_extrusion_speed + c.fwd_extrusion_speed) /
+                         (double)c.probability_normalization_factor;
+    if (ratio != 1.0) {
+      auto stable_pow = [](double base, double e) {
+        if (base == 0.0) return 0.0;
+        if (base == 1.0) return 1.0;
+        return std::exp(std::log(base) * e);
+      };
+      stp_inactive = stable_pow(stp_inactive, ratio);
+      stp_active = mgpu_stp_active_from_occupancy(stp_inactive, occupancy);
+      if (p_bypass != 0.0 && p_bypass != 1.0) p_bypass = std::min(p_bypass * ratio, 1.0);
+      pblock_major = stable_pow(pblock_major, ratio);
+      pblock_minor = stable_pow(pblock_minor, ratio);
+    }
+  }
+  p.p_bypass = p_bypass;
+  p.pblock_major = pblock_major;
+  p.pblock_minor = pblock_minor;
+  p.hard_stall_mult = c.hard_stall_lef_stability_multiplier;
+  p.soft_stall_mult = c.soft_stall_lef_stability_multiplier;
+  p.tad_to_loop_ratio = ratio_tl;
+  p.gev_mu = c.genextreme_mu;
+  p.gev_sigma = c.genextreme_sigma;
+  p.gev_xi = c.genextreme_xi;
+  p.noisify = c.noisify_contacts ? 1 : 0;
+  p.skip_burnin = c.skip_burnin ? 1 : 0;
+  p.contact_sampling_interval = c.contact_sampling_interval;
+  p.burnin_history_length = c.burnin_history_length;
+  p.burnin_smoothing_window = c.burnin_smoothing_window_size;
+  p.min_burnin_epochs = c.min_burnin_epochs;
+  p.max_burnin_epochs = c.max_burnin_epochs;
+  p.target_contact_density =
+      c.stopping_criterion == MGPU_STOP_EPOCHS ? -1.0 : c.target_contact_density;
+  p.target_simulation_epochs = c.target_simulation_epochs;
+  p.seed = c.seed;
+  p.lefs_per_mbp = c.number_of_lefs_per_mbp;
+  p.barrier_occupied_stp = stp_active;
+  p.barrier_not_occupied_stp = stp_inactive;
+  p.override_barrier_occupancy = (set_mask & MGPU_SET_BARRIER_OCCUPANCY) ? 1 : 0;
+  *out = p;
+  return MGPU_OK;
+}
+
+int mgpu_prepare_barriers(const mgpu_params* p, const mgpu_bed_record* recs, uint64_t n,
+                          uint64_t chrom_start, uint64_t chrom_end, uint64_t* pos, uint8_t* dir,
+                          double* stp_active, double* stp_inactive, uint64_t* nout) {
+  if (!p || (n && !recs) || !nout) return fail(nullptr, MGPU_EINVAL, "null argument");
+  struct Row {
+    uint64_t pos;
+    uint8_t dir;
+    double sa, si;
+  };
+  std::vector<Row> rows;
+  rows.reserve(n);
+  for (uint64_t i = 0; i < n; ++i) {
+    const mgpu_bed_record& r = recs[i];
+    if (r.strand != '+' && r.strand != '-' && r.strand != '.')
+      return fail(nullptr, MGPU_EINVAL, "record %llu: invalid strand", (unsigned long long)i);
+    if (r.score < 0.0 || r.score > 1.0)
+      return fail(nullptr, MGPU_EINVAL, "record %llu: score must be in [0, 1]",
+                  (unsigned long long)i);
+    const uint64_t bp = (r.chrom_start + r.chrom_end + 1) / 2;
+    if (bp < chrom_start || bp >= chrom_end) continue;
+    double sa = p->barrier_occupied_stp, si = p->barrier_not_occupied_stp;
+    if (!p->override_barrier_occupancy && r.score != 0.0) {
+      sa = mgpu_stp_active_from_occupancy(si, r.score);
+    }
+    // motif '+' blocks extrusion in the rev direction, anything else in the fwd direction
+    rows.push_back({bp, (uint8_t)(r.strand == '+' ? 1 : 2), sa, si});
+  }
+  std::stable_sort(rows.begin(), rows.end(), [](const Row& a, const Row& b) { return a.pos < b.pos; });
+  for (uint64_t i = 0; i < rows.size(); ++i) {
+    pos[i] = rows[i].pos;
+    dir[i] = rows[i].dir;
+    stp_active[i] = rows[i].sa;
+    stp_inactive[i] = rows[i].si;
+  }
+  *nout = rows.size();
+  return MGPU_OK;
+}
+
+int mgpu_plan_chrom(const mgpu_params* p, const mgpu_chrom* chrom, uint64_t ncells,
+                    uint64_t* nlefs, uint64_t* nrows, uint64_t* ncols, uint64_t* target_contacts,
+                    uint64_t* target_epochs) {
+  if (int rc = check_params(p)) return rc;
+  if (int rc = check_chrom(chrom)) return rc;
+  if (ncells == 0) return fail(nullptr, MGPU_EINVAL, "ncells must be > 0");
+  const uint64_t sim_size = chrom->end - chrom->start;
+  const uint64_t nl = std::max<uint64_t>(
+      1, (uint64_t)std::round(p->lefs_per_mbp * ((double)sim_size / 1.0e6)));
+  const uint64_t nc = ceil_div(sim_size, p->bin_size);
+  const uint64_t nr = std::min(ceil_div(p->diagonal_width, p->bin_size), nc);
+  const uint64_t npixels = nr * nc;
+  if (nlefs) *nlefs = nl;
+  if (nrows) *nrows = nr;
+  if (ncols) *ncols = nc;
+  const uint64_t tot = (uint64_t)std::round((double)npixels * p->target_contact_density);
+  if (target_contacts) {
+    const uint64_t per_cell = (tot + ncells - 1) / ncells;
+    uint64_t rolling = 0;
+    for (uint64_t c = 0; c < ncells; ++c) {
+      const uint64_t eff = std::min(per_cell, tot - rolling);
+      rolling += eff;
+      target_contacts[c] = eff;
+    }
+  }
+  if (target_epochs) {
+    if (p->target_contact_density < 0) {
+      *target_epochs = ncells * p->target_simulation_epochs;
+    } else {
+      const double tot_d = std::max(1.0, std::round(p->target_contact_density * (double)npixels));
+      *target_epochs = (uint64_t)std::round(tot_d / (double)contacts_per_epoch(p, nl));
+    }
+  }
+  return MGPU_OK;
+}
+
+uint64_t mgpu_cell_seed(const mgpu_chrom* chrom, uint64_t cell_id, uint64_t seed) {
+  if (!chrom || chrom->name_len > 224) return 0;
+  uint8_t buf[240];
+  memcpy(buf, chrom->name, chrom->name_len);
+  for (int i = 0; i < 8; ++i) buf[chrom->name_len + i] = (uint8_t)(chrom->size >> (8 * i));
+  for (int i = 0; i < 8; ++i) buf[chrom->name_len + 8 + i] = (uint8_t)(cell_id >> (8 * i));
+  uint64_t h = 0;
+  xxh3::hash64(buf, chrom->name_len + 16, seed, &h);
+  return h;
+}
+
+// ---- jobs -----------------------------------------------------------------------------------
+
+}  // extern "C"
+
+struct mgpu_job {
+  mgpu_ctx* ctx = nullptr;
+  int mode = 0;
+  mgpu_params p{};
+  uint64_t nchroms = 0;
+  std::vector<DetChrom> chroms;          // device pointers inside
+  std::vector<uint64_t> band_off, band_len, nrows;
+  std::vector<uint64_t> items_per_chrom;
+  uint64_t nitems = 0;
+  uint64_t band_total = 0;
+  void* d_pool = nullptr;                // one allocation for all inputs and counters
+  uint32_t* d_bands = nullptr;
+  unsigned long long* d_missed = nullptr;
+  unsigned long long* d_next = nullptr;
+  DetChrom* d_chroms = nullptr;
+  WorkItem* d_items = nullptr;
+  uint64_t* d_cell_stats = nullptr;
+  uint8_t* d_scratch = nullptr;
+  uint64_t stride = 0;
+  bool use_lds = false;
+  uint32_t grid = 0;
+  FastJob fast{};
+};
+
+namespace {
+
+struct PoolBuilder {
+  std::vector<std::pair<uint64_t, uint64_t>> spans;  // offset, size
+  uint64_t total = 0;
+  uint64_t add(uint64_t bytes) {
+    const uint64_t off = total;
+    total += (bytes + 255) & ~255ULL;
+    return off;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int mgpu_job_create(mgpu_ctx* ctx, const mgpu_params* p, uint64_t nchroms, const mgpu_chrom* chroms,
+                    const mgpu_barriers* barriers, const uint64_t* nlefs, const uint64_t* ncells,
+                    const uint64_t* const* target_contacts, const uint64_t* target_epochs, int mode,
+                    mgpu_job** out) {
+  if (!ctx || !out) return fail(ctx, MGPU_EINVAL, "null argument");
+  if (int rc = check_params(p)) return fail(ctx, rc, "%s", g_err.c_str());
+  if (nchroms == 0 || !chroms || !barriers || !nlefs || !ncells || !target_contacts ||
+      !target_epochs)
+    return fail(ctx, MGPU_EINVAL, "empty or null chromosome arrays");
+  if (mode != MGPU_MODE_DETERMINISTIC && mode != MGPU_MODE_FAST)
+    return fail(ctx, MGPU_EINVAL, "unknown mode %d", mode);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+
+  auto* job = new mgpu_job();
+  job->ctx = ctx;
+  job->mode = mode;
+  job->p = *p;
+  job->nchroms = nchroms;
+  const DetParams P = det_params(p);
+
+  PoolBuilder pool;
+  std::vector<uint64_t> off_pos(nchroms), off_dir(nchroms), off_sa(nchroms), off_si(nchroms),
+      off_qn(nchroms);
+  uint64_t max_lefs = 0, max_bar = 0;
+  const double p_loop = 1.0 / (p->tad_to_loop_ratio + 1.0);
+  std::vector<std::vector<double>> qn(nchroms);
+  for (uint64_t c = 0; c < nchroms; ++c) {
+    if (int rc = check_chrom(&chroms[c])) {
+      delete job;
+      return fail(ctx, rc, "%s", g_err.c_str());
+    }
+    if (chroms[c].end > 0xFFFFFFF0ULL) {
+      delete job;
+      return fail(ctx, MGPU_ERANGE, "chromosome end %llu beyond the 32-bit position range",
+                  (unsigned long long)chroms[c].end);
+    }
+    if (nlefs[c] == 0 || nlefs[c] > (1u << 24) || ncells[c] == 0) {
+      delete job;
+      return fail(ctx, MGPU_EINVAL, "chromosome %llu: invalid LEF or cell count",
+                  (unsigned long long)c);
+    }
+    const uint64_t nb = barriers[c].n;
+    for (uint64_t i = 1; i < nb; ++i) {
+      if (barriers[c].pos[i] < barriers[c].pos[i - 1]) {
+        delete job;
+        return fail(ctx, MGPU_EINVAL, "chromosome %llu: barriers are not sorted by position",
+                    (unsigned long long)c);
+      }
+    }
+    for (uint64_t i = 0; i < nb; ++i) {
+      if (barriers[c].pos[i] < chroms[c].start || barriers[c].pos[i] >= chroms[c].end ||
+          (barriers[c].dir[i] != 1 && barriers[c].dir[i] != 2)) {
+        delete job;
+        return fail(ctx, MGPU_EINVAL, "chromosome %llu: barrier %llu outside the range or with "
+                    "an invalid direction", (unsigned long long)c, (unsigned long long)i);
+      }
+    }
+    max_lefs = std::max(max_lefs, nlefs[c]);
+    max_bar = std::max(max_bar, nb);
+    off_pos[c] = pool.add(std::max<uint64_t>(nb, 1) * 8);
+    off_dir[c] = pool.add(std::max<uint64_t>(nb, 1));
+    off_sa[c] = pool.add(std::max<uint64_t>(nb, 1) * 8);
+    off_si[c] = pool.add(std::max<uint64_t>(nb, 1) * 8);
+    const uint64_t cpe = contacts_per_epoch(p, nlefs[c]);
+    qn[c].resize(cpe + 1);
+    // binomial inversion's q_n = (1 - p)^t, evaluated on the host libm as the reference does
+    const double pr = p_loop > 0.5 ? 1 - p_loop : p_loop;
+    for (uint64_t t = 0; t <= cpe; ++t) qn[c][t] = std::pow(1 - pr, (double)t);
+    off_qn[c] = pool.add((cpe + 1) * 8);
+    job->nrows.push_back(0);
+    job->items_per_chrom.push_back(ncells[c]);
+    job->nitems += ncells[c];
+  }
+  // bands
+  std::vector<uint64_t> cpe_of(nchroms);
+  for (uint64_t c = 0; c < nchroms; ++c) {
+    const uint64_t sim = chroms[c].end - chroms[c].start;
+    const uint64_t nc = ceil_div(sim, p->bin_size);
+    const uint64_t nr = std::min(ceil_div(p->diagonal_width, p->bin_size), nc);
+    job->nrows[c] = nr;
+    job->band_off.push_back(job->band_total);
+    job->band_len.push_back(nr * nc + 1);
+    job->band_total += ((nr * nc + 1) + 63) & ~63ULL;
+    cpe_of[c] = contacts_per_epoch(p, nlefs[c]);
+  }
+  const uint64_t off_missed = pool.add(nchroms * 8);
+  const uint64_t off_next = pool.add(64);
+  const uint64_t off_chroms = pool.add(nchroms * sizeof(DetChrom));
+  const uint64_t off_items = pool.add(job->nitems * sizeof(WorkItem));
+  const uint64_t off_stats = pool.add(job->nitems * 4 * 8);
+  const uint64_t off_bands = pool.add(job->band_total * 4);
+
+  // deterministic-mode working set per block
+  if (mode == MGPU_MODE_DETERMINISTIC) {
+    job->stride = det_cell_bytes_host(max_lefs, max_bar, p->burnin_history_length);
+    job->use_lds = job->stride <= 64 * 1024;
+    const uint64_t per_cu = job->use_lds ? std::max<uint64_t>(1, (160 * 1024) / job->stride) : 4;
+    job->grid = (uint32_t)std::min<uint64_t>(job->nitems,
+                                             std::min<uint64_t>(per_cu, 8) * ctx->num_cus);
+  }
+  uint64_t off_scratch = 0;
+  if (mode == MGPU_MODE_DETERMINISTIC && !job->use_lds)
+    off_scratch = pool.add(job->stride * job->grid);
+
+  hipError_t e = hipMalloc(&job->d_pool, pool.total);
+  if (e != hipSuccess) {
+    delete job;
+    return fail(ctx, MGPU_ENOMEM, "device allocation of %llu bytes failed: %s",
+                (unsigned long long)pool.total, hipGetErrorString(e));
+  }
+  uint8_t* base = static_cast<uint8_t*>(job->d_pool);
+  job->d_missed = reinterpret_cast<unsigned long long*>(base + off_missed);
+  job->d_next = reinterpret_cast<unsigned long long*>(base + off_next);
+  job->d_chroms = reinterpret_cast<DetChrom*>(base + off_chroms);
+  job->d_items = reinterpret_cast<WorkItem*>(base + off_items);
+  job->d_cell_stats = reinterpret_cast<uint64_t*>(base + off_stats);
+  job->d_bands = reinterpret_cast<uint32_t*>(base + off_bands);
+  if (mode == MGPU_MODE_DETERMINISTIC && !job->use_lds) job->d_scratch = base + off_scratch;
+
+  std::vector<WorkItem> items;
+  items.reserve(job->nitems);
+  for (uint64_t c = 0; c < nchroms; ++c) {
+    const uint64_t nb = barriers[c].n;
+    if (nb) {
+      HIPCHK(ctx, hipMemcpy(base + off_pos[c], barriers[c].pos, nb * 8, hipMemcpyHostToDevice));
+      HIPCHK(ctx, hipMemcpy(base + off_dir[c], barriers[c].dir, nb, hipMemcpyHostToDevice));
+      HIPCHK(ctx, hipMemcpy(base + off_sa[c], barriers[c].stp_active, nb * 8, hipMemcpyHostToDevice));
+      HIPCHK(ctx, hipMemcpy(base + off_si[c], barriers[c].stp_inactive, nb * 8, hipMemcpyHostToDevice));
+    }
+    HIPCHK(ctx, hipMemcpy(base + off_qn[c], qn[c].data(), qn[c].size() * 8, hipMemcpyHostToDevice));
+    DetChrom ch{};
+    ch.start = chroms[c].start;
+    ch.end = chroms[c].end;
+    ch.nlefs = nlefs[c];
+    ch.nbar = nb;
+    ch.bar_pos = reinterpret_cast<const uint64_t*>(base + off_pos[c]);
+    ch.bar_dir = reinterpret_cast<const uint8_t*>(base + off_dir[c]);
+    ch.bar_stp_active = reinterpret_cast<const double*>(base + off_sa[c]);
+    ch.bar_stp_inactive = reinterpret_cast<const double*>(base + off_si[c]);
+    ch.band = job->d_bands + job->band_off[c];
+    ch.missed = job->d_missed + c;
+    ch.nrows = job->nrows[c];
+    ch.cpe = cpe_of[c];
+    ch.target_epochs = target_epochs[c];
+    ch.bind_rate = (double)nlefs[c] / (double)p->burnin_target_epochs_for_lef_activation;
+    ch.qn = reinterpret_cast<const double*>(base + off_qn[c]);
+    job->chroms.push_back(ch);
+    for (uint64_t k = 0; k < ncells[c]; ++k) {
+      WorkItem w{};
+      w.chrom = (uint32_t)c;
+      w.cell_id = k;
+      w.seed = mgpu_cell_seed(&chroms[c], k, p->seed);
+      w.target = target_contacts[c][k];
+      items.push_back(w);
+    }
+  }
+  HIPCHK(ctx, hipMemcpy(job->d_chroms, job->chroms.data(), nchroms * sizeof(DetChrom),
+                        hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(job->d_items, items.data(), items.size() * sizeof(WorkItem),
+                        hipMemcpyHostToDevice));
+  if (mode == MGPU_MODE_FAST) {
+    int rc = fast_job_create(ctx->stream, *p, job->chroms, items, job->d_chroms, &job->fast);
+    if (rc != 0) {
+      hipFree(job->d_pool);
+      delete job;
+      return fail(ctx, MGPU_EDEVICE, "fast-mode setup failed: %s", fast_last_error());
+    }
+  }
+  *out = job;
+  return MGPU_OK;
+}
+
+int mgpu_job_run(mgpu_job* job, mgpu_stats* stats) {
+  if (!job) return fail(nullptr, MGPU_EINVAL, "null job");
+  mgpu_ctx* ctx = job->ctx;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const auto t0 = std::chrono::steady_clock::now();
+  HIPCHK(ctx, hipMemsetAsync(job->d_bands, 0, job->band_total * 4, ctx->stream));
+  HIPCHK(ctx, hipMemsetAsync(job->d_missed, 0, job->nchroms * 8, ctx->stream));
+  HIPCHK(ctx, hipMemsetAsync(job->d_next, 0, 64, ctx->stream));
+  HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  if (job->mode == MGPU_MODE_DETERMINISTIC) {
+    HIPCHK(ctx, launch_det_cells(ctx->stream, det_params(&job->p), job->d_chroms, job->d_items,
+                                 job->nitems, job->d_next, job->d_scratch, job->stride,
+                                 job->use_lds, job->grid, job->d_cell_stats));
+  } else {
+    HIPCHK(ctx, fast_job_run(ctx->stream, &job->fast, job->d_cell_stats));
+  }
+  HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  const auto t1 = std::chrono::steady_clock::now();
+  if (stats) {
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    std::vector<uint64_t> cs(job->nitems * 4);
+    HIPCHK(ctx, hipMemcpy(cs.data(), job->d_cell_stats, cs.size() * 8, hipMemcpyDeviceToHost));
+    std::vector<unsigned long long> missed(job->nchroms);
+    HIPCHK(ctx, hipMemcpy(missed.data(), job->d_missed, job->nchroms * 8, hipMemcpyDeviceToHost));
+    *stats = mgpu_stats{};
+    stats->cells = job->nitems;
+    for (uint64_t i = 0; i < job->nitems; ++i) {
+      stats->epochs += cs[4 * i];
+      stats->burnin_epochs += cs[4 * i + 1];
+      stats->contacts += cs[4 * i + 2];
+    }
+    for (auto m : missed) stats->missed += m;
+    stats->kernel_ms = ms;
+    stats->total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  }
+  return MGPU_OK;
+}
+
+int mgpu_job_read_band(mgpu_job* job, uint64_t c, uint32_t* out_band, uint64_t* out_missed) {
+  if (!job || c >= job->nchroms) return fail(job ? job->ctx : nullptr, MGPU_EINVAL, "bad chromosome index");
+  mgpu_ctx* ctx = job->ctx;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (out_band)
+    HIPCHK(ctx, hipMemcpy(out_band, job->d_bands + job->band_off[c], job->band_len[c] * 4,
+                          hipMemcpyDeviceToHost));
+  if (out_missed) {
+    unsigned long long m = 0;
+    HIPCHK(ctx, hipMemcpy(&m, job->d_missed + c, 8, hipMemcpyDeviceToHost));
+    *out_missed = m;
+  }
+  return MGPU_OK;
+}
+
+int mgpu_job_band_device_ptr(mgpu_job* job, uint64_t c, void** dptr, uint64_t* nelems) {
+  if (!job || c >= job->nchroms) return fail(nullptr, MGPU_EINVAL, "bad chromosome index");
+  if (dptr) *dptr = job->d_bands + job->band_off[c];
+  if (nelems) *nelems = job->band_len[c];
+  return MGPU_OK;
+}
+
+void mgpu_job_destroy(mgpu_job* job) {
+  if (!job) return;
+  hipSetDevice(job->ctx->device);
+  fast_job_destroy(&job->fast);
+  if (job->d_pool) hipFree(job->d_pool);
+  delete job;
+}
+
+int mgpu_simulate_chrom(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom* chrom,
+                        const mgpu_barriers* barriers, uint64_t nlefs, uint64_t cell_begin,
+                        uint64_t ncells, const uint64_t* target_contacts, uint64_t target_epochs,
+                        int mode, uint32_t* out_band, uint64_t* out_missed, mgpu_stats* stats) {
+  if (!ctx || !out_band) return fail(ctx, MGPU_EINVAL, "null argument");
+  mgpu_job* job = nullptr;
+  // cell ids start at cell_begin: shift the ids after creating the items
+  int rc = mgpu_job_create(ctx, p, 1, chrom, barriers, &nlefs, &ncells, &target_contacts,
+                           &target_epochs, mode, &job);
+  if (rc) return rc;
+  if (cell_begin != 0) {
+    std::vector<WorkItem> items(ncells);
+    for (uint64_t k = 0; k < ncells; ++k) {
+      items[k].chrom = 0;
+      items[k].cell_id = cell_begin + k;
+      items[k].seed = mgpu_cell_seed(chrom, cell_begin + k, p->seed);
+      items[k].target = target_contacts[k];
+    }
+    hipError_t e = hipMemcpy(job->d_items, items.data(), ncells * sizeof(WorkItem),
+                             hipMemcpyHostToDevice);
+    if (e == hipSuccess && mode == MGPU_MODE_FAST) e = fast_job_set_items(&job->fast, items);
+    if (e != hipSuccess) {
+      mgpu_job_destroy(job);
+      return fail(ctx, MGPU_EDEVICE, "item upload failed: %s", hipGetErrorString(e));
+    }
+  }
+  rc = mgpu_job_run(job, stats);
+  if (!rc) {
+    std::vector<uint32_t> tmp(job->band_len[0]);
+    rc = mgpu_job_read_band(job, 0, tmp.data(), out_missed);
+    if (!rc) {
+      for (uint64_t i = 0; i < tmp.size(); ++i) out_band[i] += tmp[i];
+    }
+  }
+  mgpu_job_destroy(job);
+  return rc;
+}
+
+// ---- debug / known-answer entry points --------------------------------------------------------
+
+int mgpu_debug_step(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom* chrom, int op,
+                    uint64_t n, uint64_t* rev, uint64_t* fwd, uint64_t* epoch,
+                    uint64_t* rev_ranks, uint64_t* fwd_ranks, uint64_t* rev_moves,
+                    uint64_t* fwd_moves, uint64_t* rev_coll, uint64_t* fwd_coll,
+                    const mgpu_barriers* barriers, const uint8_t* barrier_state,
+                    uint64_t* rng_state) {
+  if (!ctx || !p || !chrom || !barriers || !rng_state) return fail(ctx, MGPU_EINVAL, "null argument");
+  if (n == 0 || n > (1u << 24)) return fail(ctx, MGPU_EINVAL, "n out of range");
+  if (op < 0 || op > 11) return fail(ctx, MGPU_EINVAL, "unknown op %d", op);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const uint64_t nb = barriers->n;
+  const uint64_t bytes = n * 8 * 7 + n * 4 * 3 + std::max<uint64_t>(nb, 1) * (8 * 3 + 2) + 64;
+  uint8_t* d = nullptr;
+  HIPCHK(ctx, hipMalloc(&d, bytes + 4096));
+  uint64_t* d_rev = reinterpret_cast<uint64_t*>(d);
+  uint64_t* d_fwd = d_rev + n;
+  uint64_t* d_ep = d_fwd + n;
+  uint64_t* d_rm = d_ep + n;
+  uint64_t* d_fm = d_rm + n;
+  uint64_t* d_rc = d_fm + n;
+  uint64_t* d_fc = d_rc + n;
+  uint64_t* d_bpos = d_fc + n;
+  double* d_bsa = reinterpret_cast<double*>(d_bpos + std::max<uint64_t>(nb, 1));
+  double* d_bsi = d_bsa + std::max<uint64_t>(nb, 1);
+  uint64_t* d_rng = reinterpret_cast<uint64_t*>(d_bsi + std::max<uint64_t>(nb, 1));
+  uint32_t* d_rr = reinterpret_cast<uint32_t*>(d_rng + 4);
+  uint32_t* d_fr = d_rr + n;
+  uint32_t* d_tmp = d_fr + n;
+  uint8_t* d_bdir = reinterpret_cast<uint8_t*>(d_tmp + n);
+  uint8_t* d_bst = d_bdir + std::max<uint64_t>(nb, 1);
+  std::vector<uint32_t> rr(n), fr(n);
+  for (uint64_t i = 0; i < n; ++i) {
+    rr[i] = (uint32_t)rev_ranks[i];
+    fr[i] = (uint32_t)fwd_ranks[i];
+  }
+  auto up = [&](void* dst, const void* src, size_t sz) {
+    return sz ? hipMemcpy(dst, src, sz, hipMemcpyHostToDevice) : hipSuccess;
+  };
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = up(d_rev, rev, n * 8);
+  if (e == hipSuccess) e = up(d_fwd, fwd, n * 8);
+  if (e == hipSuccess) e = up(d_ep, epoch, n * 8);
+  if (e == hipSuccess) e = up(d_rm, rev_moves, n * 8);
+  if (e == hipSuccess) e = up(d_fm, fwd_moves, n * 8);
+  if (e == hipSuccess) e = up(d_rc, rev_coll, n * 8);
+  if (e == hipSuccess) e = up(d_fc, fwd_coll, n * 8);
+  if (e == hipSuccess) e = up(d_rr, rr.data(), n * 4);
+  if (e == hipSuccess) e = up(d_fr, fr.data(), n * 4);
+  if (e == hipSuccess) e = up(d_rng, rng_state, 32);
+  if (e == hipSuccess) e = up(d_bpos, barriers->pos, nb * 8);
+  if (e == hipSuccess) e = up(d_bsa, barriers->stp_active, nb * 8);
+  if (e == hipSuccess) e = up(d_bsi, barriers->stp_inactive, nb * 8);
+  if (e == hipSuccess) e = up(d_bdir, barriers->dir, nb);
+  if (e == hipSuccess && barrier_state) e = up(d_bst, barrier_state, nb);
+  DetChrom ch{};
+  ch.start = chrom->start;
+  ch.end = chrom->end;
+  ch.nlefs = n;
+  ch.nbar = nb;
+  ch.bar_pos = d_bpos;
+  ch.bar_dir = d_bdir;
+  ch.bar_stp_active = d_bsa;
+  ch.bar_stp_inactive = d_bsi;
+  if (e == hipSuccess)
+    e = launch_det_step(ctx->stream, det_params(p), ch, op, n, d_rev, d_fwd, d_ep, d_rr, d_fr,
+                        d_tmp, d_rm, d_fm, d_rc, d_fc, d_bst, d_rng);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  auto down = [&](void* dst, const void* src, size_t sz) {
+    return hipMemcpy(dst, src, sz, hipMemcpyDeviceToHost);
+  };
+  if (e == hipSuccess) e = down(rev, d_rev, n * 8);
+  if (e == hipSuccess) e = down(fwd, d_fwd, n * 8);
+  if (e == hipSuccess) e = down(epoch, d_ep, n * 8);
+  if (e == hipSuccess) e = down(rev_moves, d_rm, n * 8);
+  if (e == hipSuccess) e = down(fwd_moves, d_fm, n * 8);
+  if (e == hipSuccess) e = down(rev_coll, d_rc, n * 8);
+  if (e == hipSuccess) e = down(fwd_coll, d_fc, n * 8);
+  if (e == hipSuccess) e = down(rr.data(), d_rr, n * 4);
+  if (e == hipSuccess) e = down(fr.data(), d_fr, n * 4);
+  if (e == hipSuccess) e = down(rng_state, d_rng, 32);
+  hipFree(d);
+  if (e != hipSuccess) return fail(ctx, MGPU_EDEVICE, "debug step failed: %s", hipGetErrorString(e));
+  for (uint64_t i = 0; i < n; ++i) {
+    rev_ranks[i] = rr[i];
+    fwd_ranks[i] = fr[i];
+  }
+  return MGPU_OK;
+}
+
+int mgpu_debug_draws(mgpu_ctx* ctx, uint64_t seed, int kind, double a, double b, double c,
+                     uint64_t n, uint64_t* out) {
+  if (!ctx || !out) return fail(ctx, MGPU_EINVAL, "null argument");
+  if (kind < 0 || kind > 8) return fail(ctx, MGPU_EINVAL, "unknown sampler kind %d", kind);
+  if (n == 0) return MGPU_OK;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  uint64_t* d = nullptr;
+  HIPCHK(ctx, hipMalloc(&d, n * 8));
+  hipError_t e = launch_det_draws(ctx->stream, seed, kind, a, b, c, n, d);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = hipMemcpy(out, d, n * 8, hipMemcpyDeviceToHost);
+  hipFree(d);
+  if (e != hipSuccess) return fail(ctx, MGPU_EDEVICE, "debug draws failed: %s", hipGetErrorString(e));
+  return MGPU_OK;
+}
+
+int mgpu_debug_cell_trace(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom* chrom,
+                          const mgpu_barriers* barriers, uint64_t nlefs, uint64_t cell_id,
+                          uint64_t target_contacts, uint64_t target_epochs, uint64_t max_trace,
+                          uint64_t* trace, uint64_t* out_epochs) {
+  if (!ctx || !trace || !out_epochs) return fail(ctx, MGPU_EINVAL, "null argument");
+  mgpu_job* job = nullptr;
+  const uint64_t one = 1;
+  const uint64_t* tc = &target_contacts;
+  int rc = mgpu_job_create(ctx, p, 1, chrom, barriers, &nlefs, &one, &tc, &target_epochs,
+                           MGPU_MODE_DETERMINISTIC, &job);
+  if (rc) return rc;
+  const uint64_t bytes = det_cell_bytes_host(nlefs, barriers->n, p->burnin_history_length);
+  uint8_t* d = nullptr;
+  hipError_t e = hipMalloc(&d, bytes + max_trace * 8 + 64);
+  if (e == hipSuccess) e = hipMemsetAsync(job->d_bands, 0, job->band_total * 4, ctx->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(job->d_missed, 0, 8, ctx->stream);
+  uint64_t* d_trace = reinterpret_cast<uint64_t*>(d + bytes);
+  uint64_t* d_out = d_trace + max_trace;
+  if (e == hipSuccess)
+    e = launch_det_trace(ctx->stream, det_params(p), job->chroms[0],
+                         mgpu_cell_seed(chrom, cell_id, p->seed), target_contacts, d, d_trace,
+                         max_trace, d_out);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess && max_trace) e = hipMemcpy(trace, d_trace, max_trace * 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(out_epochs, d_out, 32, hipMemcpyDeviceToHost);
+  if (d) hipFree(d);
+  mgpu_job_destroy(job);
+  if (e != hipSuccess) return fail(ctx, MGPU_EDEVICE, "trace failed: %s", hipGetErrorString(e));
+  return MGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,214 @@
+"""Host-side mirror of libmodle's simulate interface over the C-ABI.
+
+`Simulation(config).run_simulate(genome)` plays the role of Simulation::run_simulate
+(src/libmodle/cpu/scheduler_simulate.cpp:43-184): for every chromosome with barriers it plans
+the LEF count and per-cell contact targets, simulates all cells on the GPU and returns the
+chromosome's banded contact matrix in ContactMatrixDense<u32> layout
+(data[col * nrows + (col - row)], nrows * ncols + 1 entries) with its missed-update count.
+Option names follow modle::Config (simulation_config.hpp:24-135).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+
+_SET_BITS = {
+    "rev_extrusion_speed": L.SET_REV_SPEED,
+    "fwd_extrusion_speed": L.SET_FWD_SPEED,
+    "extrusion_barrier_occupancy": L.SET_BARRIER_OCCUPANCY,
+}
+
+
+def _p(a: np.ndarray, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def config_defaults() -> L.MgpuConfig:
+    c = L.MgpuConfig()
+    L.check(L.lib().mgpu_config_defaults(C.byref(c)))
+    return c
+
+
+def make_params(options: dict | None = None) -> L.MgpuParams:
+    """Config defaults, the user's options, then Cli::transform_args (cli.cpp:1192-1215)."""
+    c = config_defaults()
+    mask = 0
+    for k, v in (options or {}).items():
+        if k == "num_cells":
+            c.num_cells = int(v)
+            continue
+        if not hasattr(c, k):
+            raise KeyError(f"unknown simulate option {k!r}")
+        ftype = dict(L.MgpuConfig._fields_)[k]
+        setattr(c, k, ftype(v).value if ftype in (C.c_int32,) else v)
+        mask |= _SET_BITS.get(k, 0)
+    p = L.MgpuParams()
+    L.check(L.lib().mgpu_config_transform(C.byref(c), mask, C.byref(p)))
+    return p
+
+
+@dataclass
+class Barriers:
+    pos: np.ndarray
+    dir: np.ndarray
+    stp_active: np.ndarray
+    stp_inactive: np.ndarray
+    view: L.MgpuBarriers = field(init=False)
+
+    def __post_init__(self):
+        self.pos = np.ascontiguousarray(self.pos, dtype=np.uint64)
+        self.dir = np.ascontiguousarray(self.dir, dtype=np.uint8)
+        self.stp_active = np.ascontiguousarray(self.stp_active, dtype=np.float64)
+        self.stp_inactive = np.ascontiguousarray(self.stp_inactive, dtype=np.float64)
+        self.view = L.MgpuBarriers(len(self.pos), _p(self.pos, C.c_uint64), _p(self.dir, C.c_uint8),
+                                   _p(self.stp_active, C.c_double),
+                                   _p(self.stp_inactive, C.c_double))
+
+    def __len__(self):
+        return len(self.pos)
+
+
+def chrom_view(name: str, size: int, start: int = 0, end: int | None = None) -> L.MgpuChrom:
+    b = name.encode()
+    ch = L.MgpuChrom(b, len(b), size, start, size if end is None else end)
+    ch._name_bytes = b
+    return ch
+
+
+def prepare_barriers(params: L.MgpuParams, records, start: int, end: int) -> Barriers:
+    n = len(records)
+    recs = (L.MgpuBedRecord * max(n, 1))()
+    for i, r in enumerate(records):
+        recs[i] = L.MgpuBedRecord(r.start, r.end, r.score, r.strand.encode())
+    pos = np.zeros(max(n, 1), np.uint64)
+    dirs = np.zeros(max(n, 1), np.uint8)
+    sa = np.zeros(max(n, 1), np.float64)
+    si = np.zeros(max(n, 1), np.float64)
+    nout = C.c_uint64(0)
+    L.check(L.lib().mgpu_prepare_barriers(C.byref(params), recs, n, start, end, _p(pos, C.c_uint64),
+                                          _p(dirs, C.c_uint8), _p(sa, C.c_double),
+                                          _p(si, C.c_double), C.byref(nout)))
+    k = nout.value
+    return Barriers(pos[:k], dirs[:k], sa[:k], si[:k])
+
+
+@dataclass
+class ChromPlan:
+    nlefs: int
+    nrows: int
+    ncols: int
+    targets: np.ndarray
+    target_epochs: int
+
+
+def plan_chrom(params: L.MgpuParams, chrom: L.MgpuChrom, ncells: int) -> ChromPlan:
+    nl, nr, nc, te = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint64()
+    targets = np.zeros(ncells, np.uint64)
+    L.check(L.lib().mgpu_plan_chrom(C.byref(params), C.byref(chrom), ncells, C.byref(nl),
+                                    C.byref(nr), C.byref(nc), _p(targets, C.c_uint64), C.byref(te)))
+    return ChromPlan(nl.value, nr.value, nc.value, targets, te.value)
+
+
+class Context:
+    """One device context (a HIP stream and its timing events)."""
+
+    def __init__(self, device: int = 0):
+        self.ptr = C.c_void_p()
+        L.check(L.lib().mgpu_init(device, C.byref(self.ptr)))
+
+    def close(self):
+        if self.ptr:
+            L.lib().mgpu_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def check(self, rc):
+        L.check(rc, self.ptr)
+
+
+class Job:
+    """Device-resident batch of chromosomes (mgpu_job_*): inputs uploaded once, bands in HBM."""
+
+    def __init__(self, ctx: Context, params: L.MgpuParams, chroms, barriers, plans, mode):
+        self.ctx = ctx
+        n = len(chroms)
+        self.plans = plans
+        self._chroms = (L.MgpuChrom * n)(*chroms)
+        self._bars = (L.MgpuBarriers * n)(*[b.view for b in barriers])
+        self._keep = (chroms, barriers)
+        self._nlefs = np.array([p.nlefs for p in plans], np.uint64)
+        self._ncells = np.array([len(p.targets) for p in plans], np.uint64)
+        self._targets = [np.ascontiguousarray(p.targets, np.uint64) for p in plans]
+        self._tptrs = (L.U64P * n)(*[_p(t, C.c_uint64) for t in self._targets])
+        self._tepochs = np.array([p.target_epochs for p in plans], np.uint64)
+        self.ptr = C.c_void_p()
+        ctx.check(L.lib().mgpu_job_create(ctx.ptr, C.byref(params), n, self._chroms, self._bars,
+                                          _p(self._nlefs, C.c_uint64), _p(self._ncells, C.c_uint64),
+                                          self._tptrs, _p(self._tepochs, C.c_uint64), mode,
+                                          C.byref(self.ptr)))
+
+    def run(self) -> L.MgpuStats:
+        st = L.MgpuStats()
+        self.ctx.check(L.lib().mgpu_job_run(self.ptr, C.byref(st)))
+        return st
+
+    def band(self, c: int):
+        p = self.plans[c]
+        out = np.zeros(p.nrows * p.ncols + 1, np.uint32)
+        missed = C.c_uint64()
+        self.ctx.check(L.lib().mgpu_job_read_band(self.ptr, c, _p(out, C.c_uint32), C.byref(missed)))
+        return out, missed.value
+
+    def close(self):
+        if self.ptr:
+            L.lib().mgpu_job_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Simulation:
+    """Simulation::run_simulate over a workload (chromosomes + BED barrier records)."""
+
+    def __init__(self, options: dict, ctx: Context | None = None):
+        self.options = dict(options)
+        self.params = make_params({k: v for k, v in options.items() if k != "num_cells"})
+        self.num_cells = int(options.get("num_cells", 512))
+        self.ctx = ctx or Context(0)
+
+    def prepare(self, workload, simulate_chromosomes_wo_barriers: bool = False):
+        chroms, bars, plans, names = [], [], [], []
+        for name, size in workload.chroms:
+            ch = chrom_view(name, size)
+            b = prepare_barriers(self.params, workload.barriers_of(name), 0, size)
+            if len(b) == 0 and not simulate_chromosomes_wo_barriers:
+                continue  # scheduler_simulate.cpp:115-120
+            chroms.append(ch)
+            bars.append(b)
+            plans.append(plan_chrom(self.params, ch, self.num_cells))
+            names.append(name)
+        return names, chroms, bars, plans
+
+    def run_simulate(self, workload, mode: int = L.MODE_DETERMINISTIC):
+        names, chroms, bars, plans = self.prepare(workload)
+        job = Job(self.ctx, self.params, chroms, bars, plans, mode)
+        stats = job.run()
+        out = {}
+        for i, name in enumerate(names):
+            band, missed = job.band(i)
+            out[name] = (band, missed, plans[i])
+        job.close()
+        return out, stats
