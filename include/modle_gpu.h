@@ -219,6 +219,10 @@ int mgpu_debug_step(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom* chrom
  * 8 exponential(a)); doubles are returned as bit patterns. Runs on the device. */
 int mgpu_debug_draws(mgpu_ctx* ctx, uint64_t seed, int kind, double a, double b, double c,
                      uint64_t n, uint64_t* out);
+/* Fast mode's counter-based generator, Philox4x32-10, on the device: in holds n tuples of
+ * (counter[4], key[2]) words, out receives n x 4 output words. No counterpart in the reference
+ * (fast mode replaces its engine); exposed for known-answer tests. */
+int mgpu_debug_philox(mgpu_ctx* ctx, const uint32_t* in, uint64_t n, uint32_t* out);
 /* Per-epoch state hash of one cell in deterministic mode (for locating a first divergence). */
 int mgpu_debug_cell_trace(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom* chrom,
                           const mgpu_barriers* barriers, uint64_t nlefs, uint64_t cell_id,

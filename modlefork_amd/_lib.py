@@ -182,6 +182,7 @@ SIGNATURES = [
                                   C.POINTER(MgpuBarriers), U8P, U64P]),
     ("mgpu_debug_draws", C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.c_double, C.c_double,
                                    C.c_double, C.c_uint64, U64P]),
+    ("mgpu_debug_philox", C.c_int, [C.c_void_p, U32P, C.c_uint64, U32P]),
     ("mgpu_debug_cell_trace", C.c_int, [C.c_void_p, C.POINTER(MgpuParams), C.POINTER(MgpuChrom),
                                         C.POINTER(MgpuBarriers), C.c_uint64, C.c_uint64, C.c_uint64,
                                         C.c_uint64, C.c_uint64, U64P, U64P]),

@@ -1,4 +1,4 @@
-// Fast-mode job interface (counter-based RNG, wavefront-parallel cells).
+// Fast-mode job interface (counter-based RNG, one workgroup per cell).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -11,17 +11,21 @@
 namespace mgpu {
 
 struct FastJob {
-  void* d_mem = nullptr;
   uint64_t nitems = 0;
   uint64_t nchroms = 0;
   void* impl = nullptr;
 };
 
+// Builds the 32-bit barrier tables and the longest-first work order for a job whose chromosome
+// descriptors (bands, barrier arrays) already live on the device.
 int fast_job_create(hipStream_t stream, const mgpu_params& p, const std::vector<DetChrom>& chroms,
                     const std::vector<WorkItem>& items, const DetChrom* d_chroms, FastJob* job);
-hipError_t fast_job_set_items(FastJob* job, const std::vector<WorkItem>& items);
-hipError_t fast_job_run(hipStream_t stream, FastJob* job, uint64_t* d_cell_stats);
+hipError_t fast_job_run(hipStream_t stream, FastJob* job, const WorkItem* d_items,
+                        uint64_t* d_cell_stats);
 void fast_job_destroy(FastJob* job);
 const char* fast_last_error();
+uint64_t fast_cell_bytes_host(uint32_t n, uint32_t nbar, uint32_t hist);
+// Philox4x32-10 of n (counter[4], key[2]) tuples: in holds 6 words per tuple, out 4.
+hipError_t launch_philox_kat(hipStream_t stream, const uint32_t* d_in, uint32_t* d_out, int n);
 
 }  // namespace mgpu

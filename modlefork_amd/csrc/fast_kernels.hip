@@ -1,20 +1,1056 @@
-// Fast-mode kernels (placeholder until the wavefront-parallel engine lands).
+// Fast-mode kernels: one workgroup per cell behind a device work queue (see fast_sim.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
 #include <string>
 
 #include "fast_kernels.h"
+#include "fast_sim.h"
 
 namespace mgpu {
 
-static thread_local std::string g_fast_err;
+namespace {
 
-int fast_job_create(hipStream_t, const mgpu_params&, const std::vector<DetChrom>&,
-                    const std::vector<WorkItem>&, const DetChrom*, FastJob*) {
-  g_fast_err = "fast mode is not available in this build";
-  return -1;
+// ---------------------------------------------------------------------------------------------
+// block primitives (kFastThreads lanes, wave64)
+// ---------------------------------------------------------------------------------------------
+struct OpMaxI {
+  __device__ int32_t operator()(int32_t a, int32_t b) const { return a > b ? a : b; }
+};
+struct OpMinI {
+  __device__ int32_t operator()(int32_t a, int32_t b) const { return a < b ? a : b; }
+};
+struct OpAddI {
+  __device__ int32_t operator()(int32_t a, int32_t b) const { return a + b; }
+};
+struct OpMaxU {
+  __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a > b ? a : b; }
+};
+
+// Exclusive scan of one value per lane across the block; sh holds kFastWaves entries.
+template <class T, class Op>
+__device__ T block_exclusive(T v, Op op, T ident, T* sh) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  T incl = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const T y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl = op(incl, y);
+  }
+  if (lane == 63) sh[w] = incl;
+  __syncthreads();
+  T pre = ident;
+  for (int k = 0; k < w; ++k) pre = op(pre, sh[k]);
+  T ex = __shfl_up(incl, 1, 64);
+  if (lane == 0) ex = ident;
+  __syncthreads();
+  return op(pre, ex);
 }
-hipError_t fast_job_set_items(FastJob*, const std::vector<WorkItem>&) { return hipErrorNotSupported; }
-hipError_t fast_job_run(hipStream_t, FastJob*, uint64_t*) { return hipErrorNotSupported; }
-void fast_job_destroy(FastJob*) {}
+
+// In-place inclusive scan of a[0, n) (reverse: from index n-1 down to 0). Each lane owns a
+// contiguous chunk of the scan order.
+template <class T, class Op>
+__device__ void block_scan(T* a, int n, Op op, T ident, bool reverse, T* sh) {
+  const int c = (n + kFastThreads - 1) / kFastThreads;
+  const int b = threadIdx.x * c, e = min(n, b + c);
+  T acc = ident;
+  for (int i = b; i < e; ++i) acc = op(acc, a[reverse ? n - 1 - i : i]);
+  acc = block_exclusive(acc, op, ident, sh);
+  for (int i = b; i < e; ++i) {
+    const int k = reverse ? n - 1 - i : i;
+    acc = op(acc, a[k]);
+    a[k] = acc;
+  }
+  __syncthreads();
+}
+
+__device__ double block_sum_f64(double v, double* sh) {
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int k = 0; k < kFastWaves; ++k) t += sh[k];
+  __syncthreads();
+  return t;
+}
+
+__device__ inline uint64_t pos_key(const uint32_t* pos, uint32_t lef) {
+  return ((uint64_t)pos[lef] << 32) | lef;
+}
+
+// ---------------------------------------------------------------------------------------------
+// per-cell working set (LDS, or a global slab for cells that do not fit)
+// ---------------------------------------------------------------------------------------------
+constexpr uint32_t kNewCap = 1024;  // LEFs merged into the rank order per batch
+
+struct Cell {
+  uint32_t *rev, *fwd, *rm, *fm, *rc, *fc;
+  uint16_t *rr, *fr;
+  uint32_t* scratch;  // max(n u32, kNewCap u64): new-LEF keys, sorted positions, chain words
+  uint8_t* bst;
+  double *cfx, *avg;  // burn-in history
+};
+
+__device__ __host__ inline uint64_t align16(uint64_t x) { return (x + 15) & ~15ULL; }
+
+__device__ __host__ inline uint64_t fast_cell_bytes(uint32_t n, uint32_t nbar, uint32_t hist) {
+  uint64_t b = align16(4ull * n) * 6 + align16(2ull * n) * 2;
+  b += align16(4ull * n > 8ull * kNewCap ? 4ull * n : 8ull * kNewCap);
+  b += align16(nbar) + align16(8ull * (hist + 1)) * 2;
+  return b;
+}
+
+__device__ inline void carve(Cell& c, uint8_t* base, uint32_t n, uint32_t nbar, uint32_t hist) {
+  uint8_t* p = base;
+  auto take = [&](uint64_t bytes) {
+    uint8_t* r = p;
+    p += align16(bytes);
+    return r;
+  };
+  c.rev = (uint32_t*)take(4ull * n);
+  c.fwd = (uint32_t*)take(4ull * n);
+  c.rm = (uint32_t*)take(4ull * n);
+  c.fm = (uint32_t*)take(4ull * n);
+  c.rc = (uint32_t*)take(4ull * n);
+  c.fc = (uint32_t*)take(4ull * n);
+  c.rr = (uint16_t*)take(2ull * n);
+  c.fr = (uint16_t*)take(2ull * n);
+  c.scratch = (uint32_t*)take(4ull * n > 8ull * kNewCap ? 4ull * n : 8ull * kNewCap);
+  c.bst = take(nbar);
+  c.cfx = (double*)take(8ull * (hist + 1));
+  c.avg = (double*)take(8ull * (hist + 1));
+}
+
+struct Shared {
+  int32_t shi[kFastWaves];
+  uint32_t shu[kFastWaves];
+  double shd[kFastWaves];
+  uint64_t item;
+  uint64_t contacts;
+  uint64_t burnin_epochs;
+  uint32_t nact, hlen, burnin_done, done, nnew, nkept, n5, n3, stable, navoid_r, navoid_f;
+  uint32_t avoid_r[64], avoid_f[64];
+};
+
+// bitonic sort of keys[0, m) ascending, m a power of two
+__device__ void bitonic(uint64_t* keys, uint32_t m) {
+  for (uint32_t k = 2; k <= m; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = threadIdx.x; i < m; i += kFastThreads) {
+        const uint32_t l = i ^ j;
+        if (l > i) {
+          const uint64_t a = keys[i], b = keys[l];
+          if ((a > b) == ((i & k) == 0)) {
+            keys[i] = b;
+            keys[l] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Re-ranks one unit side. ranks[0, nact_old) hold last epoch's order; LEFs flagged in is_new
+// were (re)bound this epoch and are merged in; tmp_a / tmp_b are n-sized scratch arrays.
+__device__ void rerank(uint16_t* ranks, const uint32_t* pos, uint32_t nact, uint32_t nact_old,
+                       const uint32_t* is_new, uint32_t* tmp_a, uint32_t* tmp_b, uint64_t* keys,
+                       Shared& S) {
+  // kept order: last epoch's entries that were not rebound, compacted stably
+  {
+    const uint32_t c = (nact_old + kFastThreads - 1) / kFastThreads;
+    const uint32_t b = threadIdx.x * c, e = min(nact_old, b + c);
+    int32_t cnt = 0;
+    for (uint32_t i = b; i < e; ++i) cnt += is_new[ranks[i]] == 0;
+    int32_t off = block_exclusive(cnt, OpAddI{}, 0, S.shi);
+    for (uint32_t i = b; i < e; ++i) {
+      const uint32_t l = ranks[i];
+      if (is_new[l] == 0) tmp_a[off++] = l;
+    }
+    if (threadIdx.x == kFastThreads - 1) S.nkept = (uint32_t)off;
+    __syncthreads();
+  }
+  uint32_t nkept = S.nkept;
+  // new LEFs, merged batch by batch (one batch unless more than kNewCap were bound)
+  const uint32_t batches = (nkept + kNewCap > nact || nact - nkept <= kNewCap)
+                               ? 1u
+                               : (nact + kNewCap - 1) / kNewCap;
+  for (uint32_t bt = 0; bt < batches; ++bt) {
+    const uint32_t lo_id = batches == 1 ? 0 : bt * kNewCap;
+    const uint32_t hi_id = batches == 1 ? nact : min(nact, lo_id + kNewCap);
+    if (threadIdx.x == 0) S.nnew = 0;
+    __syncthreads();
+    for (uint32_t l = lo_id + threadIdx.x; l < hi_id; l += kFastThreads) {
+      if (is_new[l]) keys[atomicAdd(&S.nnew, 1u)] = pos_key(pos, l);
+    }
+    __syncthreads();
+    const uint32_t k = S.nnew;
+    if (k == 0) continue;
+    uint32_t m = 1;
+    while (m < k) m <<= 1;
+    for (uint32_t i = k + threadIdx.x; i < m; i += kFastThreads) keys[i] = ~0ULL;
+    __syncthreads();
+    bitonic(keys, m);
+    for (uint32_t a = threadIdx.x; a < nkept; a += kFastThreads) {
+      const uint64_t key = pos_key(pos, tmp_a[a]);
+      uint32_t lo = 0, hi = k;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (keys[mid] < key) lo = mid + 1;
+        else hi = mid;
+      }
+      tmp_b[a + lo] = tmp_a[a];
+    }
+    for (uint32_t b = threadIdx.x; b < k; b += kFastThreads) {
+      const uint64_t key = keys[b];
+      uint32_t lo = 0, hi = nkept;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pos_key(pos, tmp_a[mid]) < key) lo = mid + 1;
+        else hi = mid;
+      }
+      tmp_b[b + lo] = (uint32_t)key;
+    }
+    __syncthreads();
+    nkept += k;
+    for (uint32_t i = threadIdx.x; i < nkept; i += kFastThreads) tmp_a[i] = tmp_b[i];
+    __syncthreads();
+  }
+  for (uint32_t i = threadIdx.x; i < nact; i += kFastThreads) ranks[i] = (uint16_t)tmp_a[i];
+  __syncthreads();
+  // extrusion keeps neighbours ordered almost always; odd-even passes repair what it did not
+  int quiet = 0;
+  for (uint32_t pass = 0; quiet < 2; ++pass) {
+    int swapped = 0;
+    for (uint32_t i = 2 * threadIdx.x + (pass & 1); i + 1 < nact; i += 2 * kFastThreads) {
+      const uint32_t a = ranks[i], b = ranks[i + 1];
+      if (pos_key(pos, b) < pos_key(pos, a)) {
+        ranks[i] = (uint16_t)b;
+        ranks[i + 1] = (uint16_t)a;
+        swapped = 1;
+      }
+    }
+    quiet = __syncthreads_or(swapped) ? 0 : quiet + 1;
+  }
+}
+
+__device__ inline float gev_noise(const FastParams& P, uint32_t x) {
+  // mu + sigma * (1 - (-ln u)^xi) / xi; expm1 keeps the small-xi cancellation exact
+  const float y = logf(-logf(u01_open(x)));
+  if (P.gev_xi == 0.0f) return (P.gev_mu - P.gev_sigma) * y;
+  return P.gev_mu - P.gev_sigma * expm1f(P.gev_xi * y) / P.gev_xi;
+}
+
+__device__ inline void band_add(const FastChrom& ch, uint32_t row, uint32_t col) {
+  const uint32_t i = row > col ? row - col : col - row;
+  const uint32_t j = row > col ? row : col;
+  if (i >= ch.nrows) {
+    atomicAdd(ch.missed, 1ULL);
+    return;
+  }
+  atomicAdd(&ch.band[(uint64_t)j * ch.nrows + i], 1u);
+}
+
+__device__ inline uint32_t poisson(const Rng& R, double mean, uint32_t tag) {
+  const U4 d = R.draw(0, tag, kSPoisson);
+  if (mean > 500.0) {  // normal approximation far from the inversion range
+    const float r = sqrtf(-2.0f * logf(u01_open(d.x)));
+    const double v = mean + sqrt(mean) * (double)(r * cospif(2.0f * u01(d.y)));
+    return v < 0 ? 0u : (uint32_t)(v + 0.5);
+  }
+  // inversion by sequential search
+  const double u = ((double)d.x * 4294967296.0 + (double)d.y) * (1.0 / 18446744073709551616.0);
+  double p = exp(-mean), s = p;
+  uint32_t k = 0;
+  while (u > s && k < 4096) {
+    ++k;
+    p *= mean / (double)k;
+    s += p;
+  }
+  return k;
+}
+
+// One side of the secondary LEF-LEF pass. Chain index j runs along the direction in which
+// stalls propagate (rev: rank order; fwd: reverse rank order). Seeds are units already stalled;
+// unit j behind seed s stalls when every unit in (s, j] reaches its predecessor's target and
+// keeps its collision (does not bypass). X[j] packs (s + 1) << 16 | (last failure + 1), so a
+// single prefix-max yields the last failure inside j's own chain.
+__device__ void secondary_side(const FastParams& P, const Rng& R, uint32_t epoch, Cell& c,
+                               Shared& S, uint32_t n, bool rev_side, uint32_t lo_j) {
+  uint16_t* ranks = rev_side ? c.rr : c.fr;
+  uint32_t* coll = rev_side ? c.rc : c.fc;
+  uint32_t* mv = rev_side ? c.rm : c.fm;
+  const uint32_t* pos = rev_side ? c.rev : c.fwd;
+  uint32_t* X = c.scratch;
+  auto unit_of = [&](uint32_t j) -> uint32_t { return ranks[rev_side ? j : n - 1 - j]; };
+  auto target = [&](uint32_t l) -> int32_t {
+    return rev_side ? (int32_t)pos[l] - (int32_t)mv[l] : (int32_t)pos[l] + (int32_t)mv[l];
+  };
+  for (uint32_t j = threadIdx.x; j < n; j += kFastThreads)
+    X[j] = ck_stalled(coll[unit_of(j)]) ? j + 1 : 0;
+  __syncthreads();
+  block_scan(X, (int)n, OpMaxU{}, 0u, false, S.shu);
+  // failure flags given the chain start
+  for (uint32_t j = threadIdx.x; j < n; j += kFastThreads) {
+    const uint32_t s1 = X[j];  // seed + 1, 0 if none
+    const uint32_t l = unit_of(j);
+    uint32_t fail = 0;
+    if (s1 != 0 && s1 - 1 != j) {
+      bool ok = j >= lo_j;
+      if (ok) {
+        const int32_t ts = target(unit_of(s1 - 1));
+        const int32_t dist = (int32_t)j - (int32_t)(s1 - 1);
+        const bool reach = rev_side ? target(l) <= ts + dist - 1 : target(l) >= ts - dist + 1;
+        const U4 d = R.draw(l, epoch, rev_side ? kSSecondary : kSSecondary + 16);
+        ok = reach && bern(d.x, P.t_bypass_keep);
+      }
+      fail = ok ? 0 : j + 1;
+    }
+    X[j] = (s1 << 16) | fail;
+  }
+  __syncthreads();
+  block_scan(X, (int)n, OpMaxU{}, 0u, false, S.shu);
+  for (uint32_t j = threadIdx.x; j < n; j += kFastThreads) {
+    const uint32_t s1 = X[j] >> 16;
+    if (s1 == 0 || s1 - 1 == j || j < lo_j) continue;
+    const uint32_t l = unit_of(j);
+    if (ck_stalled(coll[l])) continue;
+    const uint32_t s = s1 - 1;
+    const uint32_t lead = unit_of(j - 1);
+    const int32_t dist = (int32_t)j - (int32_t)s;
+    const int32_t ts = target(unit_of(s));
+    const uint32_t last_fail = X[j] & 0xFFFFu;
+    if (last_fail == 0) {
+      // stalled one bp behind the unit ahead of it in the chain
+      const int32_t t = rev_side ? ts + dist : ts - dist;
+      const int32_t m = rev_side ? (int32_t)pos[l] - t : t - (int32_t)pos[l];
+      mv[l] = m > 0 ? (uint32_t)m : 0u;
+      coll[l] = ck(kCkSecondary, lead);
+    } else if (last_fail == j + 1 && (X[j - 1] & 0xFFFFu) == 0) {
+      const bool reach = rev_side ? target(l) <= ts + dist - 1 : target(l) >= ts - dist + 1;
+      if (reach) {
+        coll[l] = ck(kCkAvoided, lead);
+        uint32_t* cnt = rev_side ? &S.navoid_r : &S.navoid_f;
+        const uint32_t slot = atomicAdd(cnt, 1u);
+        if (slot < 64) (rev_side ? S.avoid_r : S.avoid_f)[slot] = rev_side ? j : n - 1 - j;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------------------------
+// the cell loop
+// ---------------------------------------------------------------------------------------------
+__device__ void fast_cell(const FastParams& P, const FastChrom& ch, Cell& c, Shared& S,
+                          uint64_t seed, uint64_t target, uint64_t* stats) {
+  const uint32_t N = ch.nlefs, NB = ch.nbar;
+  const Rng R{(uint32_t)seed, (uint32_t)(seed >> 32)};
+  for (uint32_t i = threadIdx.x; i < N; i += kFastThreads) {
+    c.rev[i] = c.fwd[i] = kFreePos;
+    c.rr[i] = c.fr[i] = (uint16_t)i;
+  }
+  for (uint32_t b = threadIdx.x; b < NB; b += kFastThreads) {
+    const U4 d = R.draw(b, 0xFFFFFFFFu, kSBarInit);
+    c.bst[b] = d.x < ch.bar_t_occ[b] ? 1 : 0;
+  }
+  if (threadIdx.x == 0) {
+    S.nact = P.skip_burnin ? N : 0;
+    S.burnin_done = P.skip_burnin ? 1 : 0;
+    S.hlen = 0;
+    S.contacts = 0;
+    S.burnin_epochs = 0;
+    S.done = 0;
+  }
+  __syncthreads();
+  const uint32_t span = ch.end - ch.start;  // bind positions in [start, end - 1]
+  uint32_t nact_prev = 0;
+  uint32_t epoch = 0;
+  for (;; ++epoch) {
+    // ---- stop test and burn-in (simulation.cpp:1087-1098, 1027-1054) ----
+    if (threadIdx.x == 0) {
+      S.done = (P.target_contact_density >= 0 ? S.contacts < target
+                                              : (uint64_t)epoch - S.burnin_epochs < ch.target_epochs)
+                   ? 0
+                   : 1;
+      S.navoid_r = S.navoid_f = 0;
+    }
+    __syncthreads();
+    if (S.done) break;
+    if (!S.burnin_done) {
+      if (S.nact != N) {
+        if (threadIdx.x == 0) {
+          uint32_t sub = 0;
+          do {
+            ++S.burnin_epochs;
+            S.nact = min(S.nact + poisson(R, ch.bind_rate, epoch * 16 + sub++), N);
+          } while (S.nact == 0);
+        }
+        __syncthreads();
+      } else {
+        double acc = 0.0;
+        for (uint32_t i = threadIdx.x; i < N; i += kFastThreads)
+          acc += c.rev[i] == kFreePos ? 0.0 : (double)(c.fwd[i] - c.rev[i]);
+        const double mean = block_sum_f64(acc, S.shd) / (double)N;
+        double ss = 0.0;
+        for (uint32_t i = threadIdx.x; i < N; i += kFastThreads) {
+          const double v = (c.rev[i] == kFreePos ? 0.0 : (double)(c.fwd[i] - c.rev[i])) - mean;
+          ss += v * v;
+        }
+        const double sd = sqrt(block_sum_f64(ss, S.shd) / (double)N);
+        const uint32_t cap = P.hist_len, w = P.window;
+        if (threadIdx.x == 0) {
+          ++S.burnin_epochs;
+          if (S.hlen == cap) {
+            for (uint32_t i = 1; i < cap; ++i) {
+              c.avg[i - 1] = c.avg[i];
+              c.cfx[i - 1] = c.cfx[i];
+            }
+            --S.hlen;
+          }
+          c.avg[S.hlen] = mean;
+          c.cfx[S.hlen] = sd / mean;
+          ++S.hlen;
+        }
+        __syncthreads();
+        uint32_t stable = 0;
+        if (S.hlen == cap) {
+          // windows whose mean drops, for the CV and for the mean (evaluate_burnin :983-1025)
+          int32_t dc = 0, da = 0;
+          for (uint32_t k = threadIdx.x; k + w + 1 <= cap - 1; k += kFastThreads) {
+            double a1 = 0, a2 = 0, b1 = 0, b2 = 0;
+            for (uint32_t j = 0; j < w; ++j) {
+              a1 += c.cfx[k + j];
+              a2 += c.cfx[k + 1 + j];
+              b1 += c.avg[k + j];
+              b2 += c.avg[k + 1 + j];
+            }
+            dc += a1 > a2;
+            da += b1 > b2;
+          }
+          const int32_t nc = block_exclusive(dc, OpAddI{}, 0, S.shi) + dc;
+          const int32_t na = block_exclusive(da, OpAddI{}, 0, S.shi) + da;
+          if (threadIdx.x == kFastThreads - 1) {
+            const double r1 = (double)nc / (double)((int32_t)(cap - w) - nc);
+            const double r2 = (double)na / (double)((int32_t)(cap - w) - na);
+            S.stable = (r1 >= 0.95 && r1 <= 1.05 && r2 >= 0.95 && r2 <= 1.05) ? 1 : 0;
+          }
+          __syncthreads();
+          stable = S.stable;
+        }
+        if (threadIdx.x == 0) {
+          bool done = stable && epoch > P.min_burnin_epochs;
+          // --max-burnin-epochs, and a hard stop so a cell that never stabilises cannot keep a
+          // workgroup busy indefinitely
+          if (!done && (epoch >= P.max_burnin_epochs || epoch >= kFastBurninCap)) done = true;
+          S.burnin_done = done ? 1 : 0;
+        }
+        __syncthreads();
+      }
+    }
+    const uint32_t n = S.nact;
+    const bool bdone = S.burnin_done != 0;
+
+    // ---- bind (simulation_impl.hpp:28-89) ----
+    for (uint32_t i = threadIdx.x; i < n; i += kFastThreads) {
+      uint32_t flag = 0;
+      if (c.rev[i] == kFreePos) {
+        const U4 d = R.draw(i, epoch, kSBind);
+        c.rev[i] = c.fwd[i] = ch.start + ubounded(d.x, span);
+        flag = 1;
+      }
+      c.rm[i] = flag;  // rm holds the "bound this epoch" flags until moves are drawn
+    }
+    __syncthreads();
+    // ---- ranks (rank_lefs, simulation.cpp:402-487) ----
+    uint64_t* keys = reinterpret_cast<uint64_t*>(c.scratch);
+    rerank(c.rr, c.rev, n, nact_prev, c.rm, c.rc, c.fc, keys, S);
+    rerank(c.fr, c.fwd, n, nact_prev, c.rm, c.rc, c.fc, keys, S);
+    nact_prev = n;
+
+    // ---- contacts (sample_and_register_contacts, register_contacts.cpp:94-198) ----
+    if (bdone) {
+      uint64_t want = ch.cpe;
+      if (P.target_contact_density > 0.0) want = min(want, target - S.contacts);
+      const uint32_t sp = ch.start + 1, ep = ch.end - 1;
+      for (uint32_t s = threadIdx.x; s < want; s += kFastThreads) {
+        for (uint32_t attempt = 0; attempt < 1024; ++attempt) {
+          const U4 d = R.draw(s, epoch, kSContact, attempt);
+          const uint32_t i = ubounded(d.x, n);
+          const uint32_t r = c.rev[i], f = c.fwd[i];
+          if (!(r > sp && r < ep && f > sp && f < ep)) continue;
+          double p1 = (double)r, p2 = (double)f;
+          if (P.noisify) {
+            p1 -= (double)gev_noise(P, d.y);
+            p2 += (double)gev_noise(P, d.z);
+          }
+          if (p2 < p1) {
+            const double t = p1;
+            p1 = p2;
+            p2 = t;
+          }
+          if (!(p1 >= (double)sp && p2 < (double)ep)) continue;
+          uint32_t q1 = (uint32_t)p1, q2 = (uint32_t)p2;
+          if ((uint64_t)d.w >= P.t_loop) {  // TAD contact: two uniform points inside the loop
+            const U4 e = R.draw(s, epoch, kSContact, attempt | 0x80000000u);
+            const uint32_t range = q2 - q1 + 1;
+            const uint32_t a = q1 + ubounded(e.x, range);
+            q2 = q1 + ubounded(e.y, range);
+            q1 = a;
+          }
+          band_add(ch, (q1 - sp) / P.bin_size, (q2 - sp) / P.bin_size);
+          break;
+        }
+      }
+      if (threadIdx.x == 0) S.contacts += want;
+      __syncthreads();
+      if (target != 0 && S.contacts >= target) break;
+    }
+
+    // ---- moves (generate_moves, simulation.cpp:265-323) ----
+    const float vr = bdone ? P.rev_speed : P.rev_speed_burnin;
+    const float vf = bdone ? P.fwd_speed : P.fwd_speed_burnin;
+    for (uint32_t i = threadIdx.x; i < n; i += kFastThreads) {
+      const U4 d = R.draw(i, epoch, kSMove);
+      const float rad = sqrtf(-2.0f * logf(u01_open(d.x)));
+      float sn, cs;
+      sincospif(2.0f * u01(d.y), &sn, &cs);
+      const float a = P.rev_std == 0.0f ? vr : vr + P.rev_std * rad * cs;
+      const float b = P.fwd_std == 0.0f ? vf : vf + P.fwd_std * rad * sn;
+      c.rm[i] = (uint32_t)roundf(a > 0.0f ? a : 0.0f);
+      c.fm[i] = (uint32_t)roundf(b > 0.0f ? b : 0.0f);
+    }
+    __syncthreads();
+    // adjust (simulation.cpp:342-399): rev targets must fall strictly along the rev order,
+    // fwd targets must rise strictly along the fwd order; then clamp to the chromosome.
+    {
+      int32_t* T = reinterpret_cast<int32_t*>(c.scratch);
+      for (uint32_t k = threadIdx.x; k < n; k += kFastThreads) {
+        const uint32_t l = c.rr[k];
+        T[k] = (int32_t)c.rev[l] - (int32_t)c.rm[l] + (int32_t)k;
+      }
+      __syncthreads();
+      block_scan(T, (int)n, OpMinI{}, INT32_MAX, true, S.shi);
+      for (uint32_t k = threadIdx.x; k < n; k += kFastThreads) {
+        const uint32_t l = c.rr[k];
+        int32_t t = T[k] - (int32_t)k;
+        if (t < (int32_t)ch.start) t = (int32_t)ch.start;
+        c.rm[l] = c.rev[l] - (uint32_t)t;
+      }
+      __syncthreads();
+      for (uint32_t k = threadIdx.x; k < n; k += kFastThreads) {
+        const uint32_t l = c.fr[k];
+        T[k] = (int32_t)c.fwd[l] + (int32_t)c.fm[l] - (int32_t)k;
+      }
+      __syncthreads();
+      block_scan(T, (int)n, OpMaxI{}, INT32_MIN, false, S.shi);
+      for (uint32_t k = threadIdx.x; k < n; k += kFastThreads) {
+        const uint32_t l = c.fr[k];
+        int32_t t = T[k] + (int32_t)k;
+        if (t > (int32_t)(ch.end - 1)) t = (int32_t)(ch.end - 1);
+        c.fm[l] = (uint32_t)t - c.fwd[l];
+        c.rc[k] = 0;
+        c.fc[k] = 0;
+      }
+      __syncthreads();
+    }
+
+    // ---- barrier states (extrusion_barriers.cpp:145-161) ----
+    for (uint32_t b = threadIdx.x; b < NB; b += kFastThreads) {
+      const U4 d = R.draw(b, epoch, kSBarrier);
+      const uint8_t st = c.bst[b];
+      if (st == 0 && d.x >= ch.bar_t_inactive[b]) c.bst[b] = 1;
+      else if (st == 1 && d.x >= ch.bar_t_active[b]) c.bst[b] = 0;
+    }
+
+    // ---- units at the chromosome ends (simulation_detect_collisions.cpp:30-125) ----
+    if (threadIdx.x == 0) {
+      const uint32_t first_fwd = c.fwd[c.fr[0]];
+      const uint32_t last_rev = c.rev[c.rr[n - 1]];
+      uint32_t n5 = 0;
+      for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t l = c.rr[k];
+        if (c.rev[l] == ch.start) {
+          ++n5;
+          c.rc[l] = ck(kCkChrom, 5);
+        } else if (c.rev[l] > first_fwd) {
+          break;
+        } else if (c.rev[l] - c.rm[l] == ch.start) {
+          c.rc[l] = ck(kCkChrom, 5);
+          ++n5;
+          break;
+        }
+      }
+      uint32_t n3 = 0;
+      for (uint32_t k = n - 1; k > 0; --k) {
+        const uint32_t l = c.fr[k];
+        if (c.fwd[l] == ch.end - 1) {
+          ++n3;
+          c.fc[l] = ck(kCkChrom, 3);
+        } else if (c.fwd[l] < last_rev) {
+          break;
+        } else if (c.fwd[l] + c.fm[l] == ch.end - 1) {
+          c.fc[l] = ck(kCkChrom, 3);
+          ++n3;
+          break;
+        }
+      }
+      S.n5 = n5;
+      S.n3 = n3;
+    }
+    __syncthreads();
+    const uint32_t n5 = S.n5, n3 = S.n3;
+
+    // ---- LEF-BAR (simulation_detect_collisions.cpp:128-252): per active barrier, the nearest
+    // unit moving towards it; the closest blocking barrier wins ----
+    uint32_t* sorted = c.scratch;
+    for (int side = 0; side < 2; ++side) {
+      const uint16_t* ranks = side == 0 ? c.rr : c.fr;
+      const uint32_t* pos = side == 0 ? c.rev : c.fwd;
+      for (uint32_t k = threadIdx.x; k < n; k += kFastThreads) sorted[k] = pos[ranks[k]];
+      __syncthreads();
+      const uint32_t rev_lo = n5 ? n5 - 1 : 0;
+      const uint32_t fwd_top = n - (n3 ? n3 - 1 : 0);
+      for (uint32_t b = threadIdx.x; b < NB; b += kFastThreads) {
+        if (!c.bst[b]) continue;
+        const uint32_t bp = ch.bar_pos[b];
+        if (side == 0) {
+          uint32_t lo = rev_lo, hi = n;
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (sorted[mid] <= bp) lo = mid + 1;
+            else hi = mid;
+          }
+          if (lo >= n) continue;
+          const uint32_t l = c.rr[lo];
+          if (sorted[lo] - bp > c.rm[l]) continue;
+          const U4 d = R.draw(b, epoch, kSLefBar);
+          if (bern(d.x, ch.bar_dir[b] == 1 ? P.t_pblock_major : P.t_pblock_minor))
+            atomicMax(&c.rc[l], ck(kCkBar, b));
+        } else {
+          uint32_t lo = 0, hi = fwd_top;
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (sorted[mid] < bp) lo = mid + 1;
+            else hi = mid;
+          }
+          if (lo == 0) continue;
+          const uint32_t l = c.fr[lo - 1];
+          if (bp - sorted[lo - 1] > c.fm[l]) continue;
+          const U4 d = R.draw(b, epoch, kSLefBar);
+          if (bern(d.y, ch.bar_dir[b] == 2 ? P.t_pblock_major : P.t_pblock_minor))
+            atomicMax(&c.fc[l], ck(kCkBarFwdTmp, kCkIdxMask - b));
+        }
+      }
+      __syncthreads();
+    }
+    for (uint32_t i = threadIdx.x; i < n; i += kFastThreads) {
+      if (ck_kind(c.fc[i]) == kCkBarFwdTmp) c.fc[i] = ck(kCkBar, kCkIdxMask - ck_idx(c.fc[i]));
+    }
+
+    // ---- primary LEF-LEF (simulation_detect_collisions.cpp:255-401): a fwd unit that is the
+    // last one before a rev unit pairs with that rev unit ----
+    // rev positions in rank order are still in `sorted` only for the fwd side; rebuild them.
+    for (uint32_t k = threadIdx.x; k < n; k += kFastThreads) sorted[k] = c.rev[c.rr[k]];
+    __syncthreads();
+    if (n5 < n && n3 < n) {
+      const uint32_t i2 = n - (n3 ? n3 - 1 : 0);
+      for (uint32_t k = threadIdx.x; k + 1 < i2; k += kFastThreads) {
+        const uint32_t fl = c.fr[k];
+        const uint32_t fp = c.fwd[fl];
+        uint32_t lo = n5, hi = n;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (sorted[mid] <= fp) lo = mid + 1;
+          else hi = mid;
+        }
+        if (lo >= n) continue;
+        const uint32_t rp = sorted[lo];
+        if (c.fwd[c.fr[k + 1]] < rp) continue;  // not the last fwd unit before rp
+        const uint32_t rl = c.rr[lo];
+        const uint32_t delta = rp - fp;
+        if (!(delta < c.rm[rl] + c.fm[fl])) continue;
+        const U4 d = R.draw(k, epoch, kSPrimary);
+        if (!bern(d.x, P.t_bypass_keep)) continue;
+        const uint32_t rv = c.rc[rl], fv = c.fc[fl];
+        const bool rocc = ck_stalled(rv), focc = ck_stalled(fv);
+        // collision site (compute_lef_lef_collision_pos, simulation.cpp:514-542)
+        const uint32_t rel = c.rm[rl] + c.fm[fl];
+        const uint32_t cpos = fp + (uint32_t)rint((double)c.fm[fl] * ((double)delta / (double)rel));
+        const uint32_t cp_rev = cpos == fp ? cpos + 1 : cpos;
+        const uint32_t cp_fwd = cpos == fp ? cpos : cpos - 1;
+        if (!rocc && !focc) {
+          c.rc[rl] = ck(kCkPrimary, fl);
+          c.fc[fl] = ck(kCkPrimary, rl);
+        } else if (rocc && !focc) {
+          if (ck_kind(rv) == kCkBar && cp_fwd > ch.bar_pos[ck_idx(rv)]) c.rc[rl] = ck(kCkPrimary, fl);
+          c.fc[fl] = ck(kCkPrimary, rl);
+        } else if (!rocc && focc) {
+          c.rc[rl] = ck(kCkPrimary, fl);
+          if (ck_kind(fv) == kCkBar && cp_rev < ch.bar_pos[ck_idx(fv)]) c.fc[fl] = ck(kCkPrimary, rl);
+        }
+      }
+    }
+    __syncthreads();
+
+    // ---- move corrections (simulation_correct_moves.cpp:20-122) ----
+    for (uint32_t i = threadIdx.x; i < n; i += kFastThreads) {
+      if (ck_kind(c.rc[i]) == kCkBar) c.rm[i] = (c.rev[i] - ch.bar_pos[ck_idx(c.rc[i])]) - 1;
+      if (ck_kind(c.fc[i]) == kCkBar) c.fm[i] = (ch.bar_pos[ck_idx(c.fc[i])] - c.fwd[i]) - 1;
+    }
+    __syncthreads();
+    for (uint32_t r = threadIdx.x; r < n; r += kFastThreads) {
+      if (ck_kind(c.rc[r]) != kCkPrimary) continue;
+      const uint32_t f = ck_idx(c.rc[r]);
+      if (ck_kind(c.fc[f]) == kCkPrimary) {
+        const uint32_t rp = c.rev[r], fp = c.fwd[f];
+        const uint32_t rel = c.rm[r] + c.fm[f];
+        const uint32_t cpos = fp + (uint32_t)rint((double)c.fm[f] * ((double)(rp - fp) / (double)rel));
+        c.rm[r] = rp - (cpos == fp ? cpos + 1 : cpos);
+        c.fm[f] = (cpos == fp ? cpos : cpos - 1) - fp;
+      } else if (ck_kind(c.fc[f]) == kCkBar) {
+        c.rm[r] = c.rev[r] - (c.fwd[f] + c.fm[f]) - 1;
+      }
+    }
+    __syncthreads();
+    for (uint32_t f = threadIdx.x; f < n; f += kFastThreads) {
+      if (ck_kind(c.fc[f]) != kCkPrimary) continue;
+      const uint32_t r = ck_idx(c.fc[f]);
+      if (ck_kind(c.rc[r]) == kCkBar) c.fm[f] = (c.rev[r] - c.rm[r]) - c.fwd[f] - 1;
+    }
+    __syncthreads();
+
+    // ---- secondary LEF-LEF (simulation_detect_collisions.cpp:404-518) ----
+    secondary_side(P, R, epoch, c, S, n, true, max(1u, n5));
+    secondary_side(P, R, epoch, c, S, n, false, (n3 ? n3 - 1 : 0) + 1);
+
+    // ---- bypass fix-ups (fix_secondary_lef_lef_collisions, :520-647), one lane, rank order ----
+    if (threadIdx.x == 0 && (S.navoid_r || S.navoid_f)) {
+      const uint32_t nr = min(S.navoid_r, 64u), nf = min(S.navoid_f, 64u);
+      for (uint32_t a = 1; a < nr; ++a)  // insertion sort of the few avoided ranks
+        for (uint32_t b = a; b > 0 && S.avoid_r[b] < S.avoid_r[b - 1]; --b) {
+          const uint32_t t = S.avoid_r[b];
+          S.avoid_r[b] = S.avoid_r[b - 1];
+          S.avoid_r[b - 1] = t;
+        }
+      for (uint32_t a = 1; a < nf; ++a)
+        for (uint32_t b = a; b > 0 && S.avoid_f[b] < S.avoid_f[b - 1]; --b) {
+          const uint32_t t = S.avoid_f[b];
+          S.avoid_f[b] = S.avoid_f[b - 1];
+          S.avoid_f[b - 1] = t;
+        }
+      for (uint32_t a = 0; a < nr; ++a) {
+        const uint32_t k = S.avoid_r[a];
+        const uint32_t r2 = c.rr[k];
+        if (k == 0 || ck_kind(c.rc[r2]) != kCkAvoided) continue;
+        const uint32_t r1 = c.rr[k - 1];
+        const uint32_t p1t = c.rev[r1] - c.rm[r1];
+        c.rm[r2] = c.rev[r2] > p1t + 1 ? c.rev[r2] - (p1t + 1) : 0;
+        c.rc[r2] = ck(kCkSecondary, r1);
+        const uint32_t a1 = c.rev[r1], b1 = c.rev[r2];
+        c.rev[r1] = min(c.fwd[r1], b1);
+        c.rev[r2] = min(c.fwd[r2], a1);
+        uint32_t t = c.rc[r1];
+        c.rc[r1] = c.rc[r2];
+        c.rc[r2] = t;
+        t = c.rm[r1];
+        c.rm[r1] = c.rm[r2];
+        c.rm[r2] = t;
+        c.rr[k - 1] = (uint16_t)r2;
+        c.rr[k] = (uint16_t)r1;
+        c.rm[r2] = min(c.rev[r2] - ch.start, c.rm[r2]);
+        c.rm[r1] = min(c.rev[r1] - ch.start, c.rm[r1]);
+      }
+      for (uint32_t a = 0; a < nf; ++a) {
+        const uint32_t k = S.avoid_f[a];
+        const uint32_t f1 = c.fr[k];
+        if (k + 1 >= n || ck_kind(c.fc[f1]) != kCkAvoided) continue;
+        const uint32_t f2 = c.fr[k + 1];
+        const uint32_t p2t = c.fwd[f2] + c.fm[f2];
+        c.fm[f1] = p2t > c.fwd[f1] + 1 ? p2t - (c.fwd[f1] + 1) : 0;
+        c.fc[f1] = ck(kCkSecondary, f2);
+        const uint32_t a1 = c.fwd[f1], b1 = c.fwd[f2];
+        c.fwd[f1] = max(c.rev[f1], b1);
+        c.fwd[f2] = max(c.rev[f2], a1);
+        uint32_t t = c.fc[f1];
+        c.fc[f1] = c.fc[f2];
+        c.fc[f2] = t;
+        t = c.fm[f1];
+        c.fm[f1] = c.fm[f2];
+        c.fm[f2] = t;
+        c.fr[k] = (uint16_t)f2;
+        c.fr[k + 1] = (uint16_t)f1;
+        c.fm[f2] = min(ch.end - 1 - c.fwd[f2], c.fm[f2]);
+        c.fm[f1] = min(ch.end - 1 - c.fwd[f1], c.fm[f1]);
+      }
+    }
+    __syncthreads();
+
+    // ---- extrude and release (simulation.cpp:489-512, 544-592) ----
+    {
+      const double base = bdone ? P.prob_release : P.prob_release_burnin;
+      for (uint32_t i = threadIdx.x; i < n; i += kFastThreads) {
+        c.rev[i] -= c.rm[i];
+        c.fwd[i] += c.fm[i];
+        const int h = (ck_kind(c.rc[i]) == kCkBar && ch.bar_dir[ck_idx(c.rc[i])] == 1) +
+                      (ck_kind(c.fc[i]) == kCkBar && ch.bar_dir[ck_idx(c.fc[i])] == 2);
+        const double p = (h == 0 ? 1.0 : (h == 1 ? P.inv_soft : P.inv_hard)) * base;
+        const U4 d = R.draw(i, epoch, kSRelease);
+        if (bern(d.x, thresh(p))) c.rev[i] = c.fwd[i] = kFreePos;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && stats) {
+    stats[0] = epoch;
+    stats[1] = S.burnin_epochs;
+    stats[2] = S.contacts;
+    stats[3] = 0;
+  }
+}
+
+__global__ void __launch_bounds__(kFastThreads)
+    fast_cells_kernel(FastParams P, const FastChrom* chroms, const WorkItem* items,
+                      const uint32_t* order, uint64_t nitems, unsigned long long* next,
+                      uint8_t* slab, uint64_t slab_stride, int use_lds, uint64_t* cell_stats) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  __shared__ Shared S;
+  for (;;) {
+    if (threadIdx.x == 0) S.item = atomicAdd(next, 1ULL);
+    __syncthreads();
+    const uint64_t q = S.item;
+    __syncthreads();
+    if (q >= nitems) return;
+    const uint64_t it = order[q];
+    const WorkItem w = items[it];
+    const FastChrom ch = chroms[w.chrom];
+    Cell c;
+    uint8_t* base = use_lds ? lds : slab + (uint64_t)blockIdx.x * slab_stride;
+    carve(c, base, ch.nlefs, ch.nbar, P.hist_len);
+    fast_cell(P, ch, c, S, w.seed, w.target, cell_stats ? cell_stats + 4 * it : nullptr);
+    __syncthreads();
+  }
+}
+
+__global__ void philox_kat_kernel(const uint32_t* in, uint32_t* out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const U4 r = philox(U4{in[6 * i], in[6 * i + 1], in[6 * i + 2], in[6 * i + 3]}, in[6 * i + 4],
+                      in[6 * i + 5]);
+  out[4 * i] = r.x;
+  out[4 * i + 1] = r.y;
+  out[4 * i + 2] = r.z;
+  out[4 * i + 3] = r.w;
+}
+
+thread_local std::string g_fast_err;
+
+uint32_t t32(double p) {
+  if (p <= 0.0) return 0;
+  const double v = p * 4294967296.0;
+  return v >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)v;
+}
+
+}  // namespace
+
+struct FastImpl {
+  FastParams P;
+  std::vector<FastChrom> chroms;
+  FastChrom* d_chroms = nullptr;
+  uint32_t* d_order = nullptr;
+  unsigned long long* d_next = nullptr;
+  uint8_t* d_slab = nullptr;
+  uint64_t slab_stride = 0;
+  uint64_t lds_bytes = 0;
+  bool use_lds = true;
+  uint32_t grid = 0;
+  uint64_t nitems = 0;
+  void* d_mem = nullptr;
+};
+
+uint64_t fast_cell_bytes_host(uint32_t n, uint32_t nbar, uint32_t hist) {
+  return fast_cell_bytes(n, nbar, hist);
+}
+
+int fast_job_create(hipStream_t stream, const mgpu_params& p, const std::vector<DetChrom>& dchroms,
+                    const std::vector<WorkItem>& items, const DetChrom* d_det_chroms,
+                    FastJob* job) {
+  (void)d_det_chroms;
+  (void)stream;
+  auto* impl = new FastImpl();
+  FastParams& P = impl->P;
+  P.bin_size = (uint32_t)p.bin_size;
+  P.rev_speed = (float)p.rev_speed;
+  P.fwd_speed = (float)p.fwd_speed;
+  P.rev_speed_burnin = (float)p.rev_speed_burnin;
+  P.fwd_speed_burnin = (float)p.fwd_speed_burnin;
+  P.rev_std = (float)p.rev_speed_std;
+  P.fwd_std = (float)p.fwd_speed_std;
+  P.t_bypass_keep = thresh(1.0 - p.p_bypass);
+  P.t_pblock_major = thresh(p.pblock_major);
+  P.t_pblock_minor = thresh(p.pblock_minor);
+  P.prob_release = p.prob_release;
+  P.prob_release_burnin = p.prob_release_burnin;
+  P.inv_soft = 1.0 / p.soft_stall_mult;
+  P.inv_hard = 1.0 / p.hard_stall_mult;
+  P.t_loop = std::isinf(p.tad_to_loop_ratio) ? 0 : thresh(1.0 / (p.tad_to_loop_ratio + 1.0));
+  P.gev_mu = (float)p.gev_mu;
+  P.gev_sigma = (float)p.gev_sigma;
+  P.gev_xi = (float)p.gev_xi;
+  P.noisify = p.noisify;
+  P.skip_burnin = p.skip_burnin;
+  P.hist_len = (uint32_t)p.burnin_history_length;
+  P.window = (uint32_t)p.burnin_smoothing_window;
+  P.min_burnin_epochs = p.min_burnin_epochs;
+  P.max_burnin_epochs = p.max_burnin_epochs;
+  P.target_contact_density = p.target_contact_density;
+  if (P.hist_len > 4096) {
+    g_fast_err = "fast mode supports burn-in histories of up to 4096 epochs";
+    delete impl;
+    return -1;
+  }
+
+  const uint64_t nch = dchroms.size();
+  uint64_t max_bytes = 0, nbar_total = 0;
+  for (const auto& d : dchroms) {
+    if (d.nlefs > 65535) {
+      g_fast_err = "fast mode supports up to 65535 LEFs per chromosome";
+      delete impl;
+      return -1;
+    }
+    nbar_total += d.nbar;
+    max_bytes = std::max<uint64_t>(max_bytes, fast_cell_bytes((uint32_t)d.nlefs, (uint32_t)d.nbar,
+                                                              P.hist_len));
+  }
+  int dev = 0;
+  hipGetDevice(&dev);
+  hipDeviceProp_t prop;
+  hipGetDeviceProperties(&prop, dev);
+  const uint64_t lds_total = 160 * 1024;
+  const uint64_t lds_limit = lds_total - sizeof(Shared) - 512;
+  impl->use_lds = max_bytes <= lds_limit;
+  impl->lds_bytes = impl->use_lds ? max_bytes : 0;
+  const uint64_t per_cu =
+      impl->use_lds ? std::max<uint64_t>(1, lds_total / (max_bytes + sizeof(Shared) + 512)) : 2;
+  impl->grid = (uint32_t)std::min<uint64_t>(
+      items.size(), std::min<uint64_t>(per_cu, 8) * (uint64_t)prop.multiProcessorCount);
+  impl->slab_stride = impl->use_lds ? 0 : ((max_bytes + 255) & ~255ULL);
+
+  const uint64_t bar_bytes = (4 * nbar_total + 64) * 4;
+  const uint64_t chrom_bytes = (nch * sizeof(FastChrom) + 255) & ~255ULL;
+  const uint64_t order_bytes = (items.size() * 4 + 255) & ~255ULL;
+  const uint64_t slab_bytes = impl->slab_stride * impl->grid;
+  hipError_t e = hipMalloc(&impl->d_mem, bar_bytes + chrom_bytes + order_bytes + 256 + slab_bytes);
+  if (e != hipSuccess) {
+    g_fast_err = std::string("fast-mode allocation failed: ") + hipGetErrorString(e);
+    delete impl;
+    return -1;
+  }
+  uint8_t* base = static_cast<uint8_t*>(impl->d_mem);
+  uint32_t* bars = reinterpret_cast<uint32_t*>(base);
+  impl->d_chroms = reinterpret_cast<FastChrom*>(base + bar_bytes);
+  impl->d_order = reinterpret_cast<uint32_t*>(base + bar_bytes + chrom_bytes);
+  impl->d_next = reinterpret_cast<unsigned long long*>(base + bar_bytes + chrom_bytes + order_bytes);
+  impl->d_slab = impl->use_lds ? nullptr : base + bar_bytes + chrom_bytes + order_bytes + 256;
+
+  // 32-bit barrier tables: positions and the transition / occupancy thresholds
+  std::vector<uint32_t> hb(4 * nbar_total + 64, 0);
+  uint64_t off = 0;
+  for (uint64_t c = 0; c < nch && e == hipSuccess; ++c) {
+    const DetChrom& d = dchroms[c];
+    std::vector<uint64_t> pos(d.nbar);
+    std::vector<double> sa(d.nbar), si(d.nbar);
+    if (d.nbar) {
+      e = hipMemcpy(pos.data(), d.bar_pos, d.nbar * 8, hipMemcpyDeviceToHost);
+      if (e == hipSuccess) e = hipMemcpy(sa.data(), d.bar_stp_active, d.nbar * 8, hipMemcpyDeviceToHost);
+      if (e == hipSuccess) e = hipMemcpy(si.data(), d.bar_stp_inactive, d.nbar * 8, hipMemcpyDeviceToHost);
+    }
+    FastChrom f{};
+    f.start = (uint32_t)d.start;
+    f.end = (uint32_t)d.end;
+    f.nlefs = (uint32_t)d.nlefs;
+    f.nbar = (uint32_t)d.nbar;
+    uint32_t* hp = hb.data() + off;
+    for (uint64_t i = 0; i < d.nbar; ++i) {
+      hp[i] = (uint32_t)pos[i];
+      hp[d.nbar + i] = t32(sa[i]);
+      hp[2 * d.nbar + i] = t32(si[i]);
+      const double occ =
+          (sa[i] + si[i] == 0) ? 0.0 : (1.0 - si[i]) / ((1.0 - si[i]) + (1.0 - sa[i]));
+      hp[3 * d.nbar + i] = t32(occ < 0.0 ? 0.0 : (occ > 1.0 ? 1.0 : occ));
+    }
+    f.bar_pos = bars + off;
+    f.bar_t_active = bars + off + d.nbar;
+    f.bar_t_inactive = bars + off + 2 * d.nbar;
+    f.bar_t_occ = bars + off + 3 * d.nbar;
+    f.bar_dir = d.bar_dir;
+    f.band = d.band;
+    f.missed = d.missed;
+    f.nrows = (uint32_t)d.nrows;
+    f.cpe = (uint32_t)d.cpe;
+    f.target_epochs = d.target_epochs;
+    f.bind_rate = d.bind_rate;
+    f.lds_bytes = fast_cell_bytes(f.nlefs, f.nbar, P.hist_len);
+    impl->chroms.push_back(f);
+    off += 4 * d.nbar;
+  }
+  if (e == hipSuccess) e = hipMemcpy(bars, hb.data(), hb.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(impl->d_chroms, impl->chroms.data(), nch * sizeof(FastChrom),
+                  hipMemcpyHostToDevice);
+  // longest cells first (chromosomes with more LEFs take longer per epoch)
+  std::vector<uint32_t> order(items.size());
+  for (uint32_t i = 0; i < order.size(); ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    return dchroms[items[a].chrom].nlefs > dchroms[items[b].chrom].nlefs;
+  });
+  if (e == hipSuccess)
+    e = hipMemcpy(impl->d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    g_fast_err = std::string("fast-mode upload failed: ") + hipGetErrorString(e);
+    hipFree(impl->d_mem);
+    delete impl;
+    return -1;
+  }
+  impl->nitems = items.size();
+  job->impl = impl;
+  job->nitems = items.size();
+  job->nchroms = nch;
+  return 0;
+}
+
+hipError_t fast_job_run(hipStream_t stream, FastJob* job, const WorkItem* d_items,
+                        uint64_t* d_cell_stats) {
+  auto* impl = static_cast<FastImpl*>(job->impl);
+  if (!impl) return hipErrorInvalidValue;
+  hipError_t e = hipMemsetAsync(impl->d_next, 0, 8, stream);
+  if (e != hipSuccess) return e;
+  const size_t shmem = impl->use_lds ? impl->lds_bytes : 0;
+  if (shmem > 0) {
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(fast_cells_kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(fast_cells_kernel, dim3(impl->grid), dim3(kFastThreads), shmem, stream,
+                     impl->P, impl->d_chroms, d_items, impl->d_order, impl->nitems, impl->d_next,
+                     impl->d_slab, impl->slab_stride, impl->use_lds ? 1 : 0, d_cell_stats);
+  return hipGetLastError();
+}
+
+void fast_job_destroy(FastJob* job) {
+  auto* impl = static_cast<FastImpl*>(job->impl);
+  if (!impl) return;
+  if (impl->d_mem) hipFree(impl->d_mem);
+  delete impl;
+  job->impl = nullptr;
+}
+
 const char* fast_last_error() { return g_fast_err.c_str(); }
+
+hipError_t launch_philox_kat(hipStream_t stream, const uint32_t* d_in, uint32_t* d_out, int n) {
+  hipLaunchKernelGGL(philox_kat_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, d_in, d_out, n);
+  return hipGetLastError();
+}
 
 }  // namespace mgpu

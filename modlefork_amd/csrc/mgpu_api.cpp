@@ -620,7 +620,7 @@ int mgpu_job_run(mgpu_job* job, mgpu_stats* stats) {
                                  job->nitems, job->d_next, job->d_scratch, job->stride,
                                  job->use_lds, job->grid, job->d_cell_stats));
   } else {
-    HIPCHK(ctx, fast_job_run(ctx->stream, &job->fast, job->d_cell_stats));
+    HIPCHK(ctx, fast_job_run(ctx->stream, &job->fast, job->d_items, job->d_cell_stats));
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -696,7 +696,6 @@ int mgpu_simulate_chrom(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom* c
     }
     hipError_t e = hipMemcpy(job->d_items, items.data(), ncells * sizeof(WorkItem),
                              hipMemcpyHostToDevice);
-    if (e == hipSuccess && mode == MGPU_MODE_FAST) e = fast_job_set_items(&job->fast, items);
     if (e != hipSuccess) {
       mgpu_job_destroy(job);
       return fail(ctx, MGPU_EDEVICE, "item upload failed: %s", hipGetErrorString(e));
@@ -849,6 +848,22 @@ int mgpu_debug_cell_trace(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom*
   if (d) hipFree(d);
   mgpu_job_destroy(job);
   if (e != hipSuccess) return fail(ctx, MGPU_EDEVICE, "trace failed: %s", hipGetErrorString(e));
+  return MGPU_OK;
+}
+
+int mgpu_debug_philox(mgpu_ctx* ctx, const uint32_t* in, uint64_t n, uint32_t* out) {
+  if (!ctx || !in || !out) return fail(ctx, MGPU_EINVAL, "null argument");
+  if (n == 0) return MGPU_OK;
+  if (n > (1u << 24)) return fail(ctx, MGPU_EINVAL, "n out of range");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  uint32_t* d = nullptr;
+  HIPCHK(ctx, hipMalloc(&d, n * 40));
+  hipError_t e = hipMemcpy(d, in, n * 24, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_philox_kat(ctx->stream, d, d + 6 * n, (int)n);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = hipMemcpy(out, d + 6 * n, n * 16, hipMemcpyDeviceToHost);
+  hipFree(d);
+  if (e != hipSuccess) return fail(ctx, MGPU_EDEVICE, "philox failed: %s", hipGetErrorString(e));
   return MGPU_OK;
 }
 

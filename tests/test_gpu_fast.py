@@ -1,0 +1,136 @@
+"""Fast mode (counter-based generator, workgroup-parallel epoch loop) on the GPU.
+
+Fast mode draws from a different stream than the reference, so its bands are compared with the
+oracle's statistically: total contacts exactly, the contact-decay curve P(s), barrier-anchored
+stripe enrichment, coarse-grained matrix correlation and the epoch counts, each with the
+tolerance written in the test. Its own determinism and cell additivity are checked exactly."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import contact_stats as CS
+from modlefork_amd import _lib as L
+from modlefork_amd import simulation as S
+from modlefork_amd import workloads as W
+from oracle import config as OC
+from oracle import refcpu as R
+
+pytestmark = pytest.mark.gpu
+
+# Random123 known-answer vectors for philox4x32-10: (counter, key) -> output
+PHILOX_KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF), (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = S.Context(0)
+    yield c
+    c.close()
+
+
+def test_philox_known_answers(ctx):
+    inp = np.array([list(c) + list(k) for c, k, _ in PHILOX_KAT], np.uint32)
+    out = np.zeros((len(PHILOX_KAT), 4), np.uint32)
+    ctx.check(L.lib().mgpu_debug_philox(ctx.ptr, inp.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                        len(PHILOX_KAT), out.ctypes.data_as(C.POINTER(C.c_uint32))))
+    for row, (_, _, want) in zip(out, PHILOX_KAT):
+        assert tuple(int(x) for x in row) == want
+
+
+def run_device(ctx, wl, mode):
+    sim = S.Simulation(wl.options, ctx)
+    out, stats = sim.run_simulate(wl, mode)
+    return out, stats
+
+
+def run_oracle(wl, nthreads=16):
+    res = {}
+    c = OC.transform(wl.options)
+    rc = OC.to_rc(c)
+    cfg = R.make_config(rc)
+    for name, size in wl.chroms:
+        bars = OC.prepare_barriers(wl.barriers_of(name), 0, size, c)
+        nl = R.lib().rc_num_lefs(c["number_of_lefs_per_mbp"], size)
+        ncols = R.lib().rc_band_ncols(c["bin_size"], size)
+        nrows = R.lib().rc_band_nrows(c["diagonal_width"], c["bin_size"], size)
+        ncells = wl.options["num_cells"]
+        t = np.zeros(ncells, np.uint64)
+        te = R.lib().rc_plan_targets(C.byref(cfg), ncells, nl, nrows * ncols,
+                                     t.ctypes.data_as(C.POINTER(C.c_uint64)))
+        ch = R.make_chrom(name, size, 0, size)
+        band, missed, st = R.simulate_cells(rc, ch, bars, nl, 0, ncells, t, te, nrows, ncols,
+                                            nthreads)
+        res[name] = (band, missed, st, nrows, ncols, bars)
+    return res
+
+
+def test_fast_mode_is_deterministic_and_conserves_contacts(ctx):
+    wl = W.small(ncells=16)
+    a, sa = run_device(ctx, wl, L.MODE_FAST)
+    b, sb = run_device(ctx, wl, L.MODE_FAST)
+    band_a, missed_a, plan = a["chrS"]
+    band_b, missed_b, _ = b["chrS"]
+    say(f"fast small/16: {sa.kernel_ms:.1f} ms, epochs {sa.epochs}, burn-in {sa.burnin_epochs}")
+    assert np.array_equal(band_a, band_b) and missed_a == missed_b
+    assert int(band_a.sum()) + missed_a == int(plan.targets.sum())
+    assert sa.contacts == int(plan.targets.sum())
+
+
+def test_fast_mode_statistics_match_oracle(ctx):
+    wl = W.small(ncells=128)
+    say("oracle: 128 cells")
+    o = run_oracle(wl)["chrS"]
+    oband, omissed, ost, nrows, ncols, obars = o
+    say("fast mode: 128 cells")
+    d, st = run_device(ctx, wl, L.MODE_FAST)
+    fband, fmissed, plan = d["chrS"]
+    tot = int(plan.targets.sum())
+    assert int(fband.sum()) + fmissed == tot == int(oband.sum()) + omissed
+
+    # contact decay P(s) in log-spaced diagonal groups: within 10 % in every group
+    po, edges = CS.log_binned_decay(oband, nrows, ncols)
+    pf, _ = CS.log_binned_decay(fband, nrows, ncols)
+    rel = np.abs(pf - po) / po
+    say(f"P(s) relative differences per group: {np.round(rel, 3).tolist()}")
+    assert rel.max() < 0.10
+
+    # coarse-grained matrices (10 x 10 bins): Pearson correlation above 0.9
+    r = CS.pearson(CS.block_sums(fband, nrows, ncols), CS.block_sums(oband, nrows, ncols))
+    say(f"block-sum correlation {r:.4f}")
+    assert r > 0.9
+
+    # barrier-anchored stripes: enrichment within 15 % of the oracle's
+    bins = (obars.pos // 5000).astype(int)
+    eo = CS.barrier_stripe_enrichment(oband, nrows, ncols, bins, obars.dir)
+    ef = CS.barrier_stripe_enrichment(fband, nrows, ncols, bins, obars.dir)
+    say(f"stripe enrichment oracle {eo:.3f} fast {ef:.3f}")
+    assert abs(ef - eo) / eo < 0.15
+
+    # epochs: mean burn-in and mean total epochs per cell within 10 %
+    ob = ost[:, 1].mean()
+    oe = ost[:, 0].mean()
+    fb = st.burnin_epochs / st.cells
+    fe = st.epochs / st.cells
+    say(f"burn-in epochs oracle {ob:.1f} fast {fb:.1f}; total oracle {oe:.1f} fast {fe:.1f}")
+    assert abs(fb - ob) / ob < 0.10
+    assert abs(fe - oe) / oe < 0.10
+
+
+def test_fast_mode_genome_slice_runs(ctx):
+    wl = W.genome(ncells=8, nchroms=3)
+    out, st = run_device(ctx, wl, L.MODE_FAST)
+    say(f"fast genome slice: {st.cells} cells, {st.kernel_ms:.1f} ms, epochs {st.epochs}")
+    for name, (band, missed, plan) in out.items():
+        assert int(band.sum()) + missed == int(plan.targets.sum())

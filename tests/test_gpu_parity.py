@@ -61,15 +61,17 @@ def test_samplers_draw_for_draw(ctx, kind, a, b, c):
         if kind != 7:
             assert np.array_equal(got, want), (seed, np.flatnonzero(got != want)[:5])
         else:
-            # GEV noise goes through log and pow: device and host libm may round the last bit
-            # differently; positions are truncated to integers, which must agree.
+            # GEV noise is mu + sigma (1 - t^xi) / xi with t = -log(u): the device and host libm
+            # may round log/pow in the last bit, and 1 - t^xi cancels ~3 digits, so the noise
+            # may differ by ~1e-9 bp. Positions are truncated to integers, which must agree.
             g = got.view(np.float64)
             w = want.view(np.float64)
-            ulps = np.abs(got.view(np.int64) - want.view(np.int64))
-            assert ulps.max() <= 4, ulps.max()
-            for pos in (1_000_000.0, 12_345_678.0):
+            diff = np.abs(g - w)
+            assert diff.max() < 1e-6, diff.max()
+            for pos in (1_000_000.0, 12_345_678.0, 249_999_999.0):
                 assert np.array_equal(np.floor(pos - g), np.floor(pos - w))
-            say(f"  gev seed={seed}: {np.count_nonzero(ulps)} of {n} differ in the last bits")
+                assert np.array_equal(np.floor(pos + g), np.floor(pos + w))
+            say(f"  gev seed={seed}: {np.count_nonzero(diff)} of {n} differ, max {diff.max():.2e}")
 
 
 def small_setup(size=2_000_000, ncells=8, nbar=40, seed=3):
