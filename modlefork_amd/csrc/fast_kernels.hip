@@ -152,6 +152,24 @@ __device__ void bitonic(uint64_t* keys, uint32_t m) {
   }
 }
 
+// Sorts LEF ids a[0, n) by (position, id) with odd-even transposition passes; returns after two
+// consecutive passes without a swap. Cheap on the nearly sorted orders extrusion leaves.
+__device__ void odd_even_sort(uint32_t* a, uint32_t n, const uint32_t* pos) {
+  int quiet = 0;
+  for (uint32_t pass = 0; quiet < 2; ++pass) {
+    int swapped = 0;
+    for (uint32_t i = 2 * threadIdx.x + (pass & 1); i + 1 < n; i += 2 * kFastThreads) {
+      const uint32_t x = a[i], y = a[i + 1];
+      if (pos_key(pos, y) < pos_key(pos, x)) {
+        a[i] = y;
+        a[i + 1] = x;
+        swapped = 1;
+      }
+    }
+    quiet = __syncthreads_or(swapped) ? 0 : quiet + 1;
+  }
+}
+
 // Re-ranks one unit side. ranks[0, nact_old) hold last epoch's order; LEFs flagged in is_new
 // were (re)bound this epoch and are merged in; tmp_a / tmp_b are n-sized scratch arrays.
 __device__ void rerank(uint16_t* ranks, const uint32_t* pos, uint32_t nact, uint32_t nact_old,
@@ -172,6 +190,9 @@ __device__ void rerank(uint16_t* ranks, const uint32_t* pos, uint32_t nact, uint
     __syncthreads();
   }
   uint32_t nkept = S.nkept;
+  // Extrusion keeps neighbours ordered almost always, but the merge below needs the kept order
+  // sorted exactly: odd-even transposition passes until two consecutive passes swap nothing.
+  odd_even_sort(tmp_a, nkept, pos);
   // new LEFs, merged batch by batch (one batch unless more than kNewCap were bound)
   const uint32_t batches = (nkept + kNewCap > nact || nact - nkept <= kNewCap)
                                ? 1u
@@ -219,20 +240,6 @@ __device__ void rerank(uint16_t* ranks, const uint32_t* pos, uint32_t nact, uint
   }
   for (uint32_t i = threadIdx.x; i < nact; i += kFastThreads) ranks[i] = (uint16_t)tmp_a[i];
   __syncthreads();
-  // extrusion keeps neighbours ordered almost always; odd-even passes repair what it did not
-  int quiet = 0;
-  for (uint32_t pass = 0; quiet < 2; ++pass) {
-    int swapped = 0;
-    for (uint32_t i = 2 * threadIdx.x + (pass & 1); i + 1 < nact; i += 2 * kFastThreads) {
-      const uint32_t a = ranks[i], b = ranks[i + 1];
-      if (pos_key(pos, b) < pos_key(pos, a)) {
-        ranks[i] = (uint16_t)b;
-        ranks[i + 1] = (uint16_t)a;
-        swapped = 1;
-      }
-    }
-    quiet = __syncthreads_or(swapped) ? 0 : quiet + 1;
-  }
 }
 
 __device__ inline float gev_noise(const FastParams& P, uint32_t x) {
@@ -379,7 +386,9 @@ __device__ void fast_cell(const FastParams& P, const FastChrom& ch, Cell& c, Sha
     __syncthreads();
     if (S.done) break;
     if (!S.burnin_done) {
-      if (S.nact != N) {
+      const bool activating = S.nact != N;
+      __syncthreads();  // every lane has read nact before lane 0 changes it
+      if (activating) {
         if (threadIdx.x == 0) {
           uint32_t sub = 0;
           do {
@@ -503,6 +512,7 @@ __device__ void fast_cell(const FastParams& P, const FastChrom& ch, Cell& c, Sha
           break;
         }
       }
+      __syncthreads();  // every lane has read the remaining target before it changes
       if (threadIdx.x == 0) S.contacts += want;
       __syncthreads();
       if (target != 0 && S.contacts >= target) break;
