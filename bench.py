@@ -112,7 +112,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="genome")
-    ap.add_argument("--mode", default=os.environ.get("MGPU_BENCH_MODE", "det"))
+    ap.add_argument("--mode", default=os.environ.get("MGPU_BENCH_MODE", "fast"))
     ap.add_argument("--ncells", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)

@@ -198,6 +198,18 @@ int mgpu_job_read_band(mgpu_job* job, uint64_t c, uint32_t* out_band, uint64_t* 
 int mgpu_job_band_device_ptr(mgpu_job* job, uint64_t c, void** dptr, uint64_t* nelems);
 void mgpu_job_destroy(mgpu_job* job);
 
+/* ---- launch tuning (no counterpart in the reference: it sizes the device launch) ----
+ * threads_per_cell: fast mode's workgroup size per cell, 256, 512 or 1024 (0 = default 512).
+ * Bands, missed counts and statistics do not depend on the tuning. grid (get only) is the number
+ * of workgroups the next run launches. */
+typedef struct mgpu_tuning {
+  uint32_t threads_per_cell;
+  uint32_t grid;
+  uint32_t reserved[6];
+} mgpu_tuning;
+int mgpu_job_set_tuning(mgpu_job* job, const mgpu_tuning* t);
+int mgpu_job_get_tuning(mgpu_job* job, mgpu_tuning* t);
+
 /* ---- step-level entry points (the reference's ENABLE_TESTING hooks, simulation.hpp:499-651),
  * run by the device step code on one LEF state ----
  * rev/fwd/epoch: LEF arrays (UINT64_MAX when unbound); ranks and moves as in the reference;
@@ -223,6 +235,13 @@ int mgpu_debug_draws(mgpu_ctx* ctx, uint64_t seed, int kind, double a, double b,
  * (counter[4], key[2]) words, out receives n x 4 output words. No counterpart in the reference
  * (fast mode replaces its engine); exposed for known-answer tests. */
 int mgpu_debug_philox(mgpu_ctx* ctx, const uint32_t* in, uint64_t n, uint32_t* out);
+/* Fast mode's epoch-phase timers (no counterpart in the reference): enable != 0 makes later
+ * job runs on this context accumulate, per phase, the 100 MHz wall-clock ticks of every cell;
+ * out (n entries, may be null) receives the totals, which are then cleared. Phases: 0 stop test
+ * and burn-in, 1 bind, 2-3 rank rev/fwd, 4 contacts, 5 moves, 6 adjust, 7 barrier states and
+ * chromosome ends, 8 LEF-BAR, 9 primary LEF-LEF, 10 move corrections, 11-12 secondary rev/fwd
+ * with their bypass fix-ups, 14 extrude and release (13 and 15 unused). */
+int mgpu_debug_phase_times(mgpu_ctx* ctx, int enable, uint64_t* out, uint64_t n);
 /* Per-epoch state hash of one cell in deterministic mode (for locating a first divergence). */
 int mgpu_debug_cell_trace(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom* chrom,
                           const mgpu_barriers* barriers, uint64_t nlefs, uint64_t cell_id,

@@ -151,6 +151,19 @@ class MgpuStats(C.Structure):
     ]
 
 
+class MgpuTuning(C.Structure):
+    _fields_ = [
+        ("threads_per_cell", C.c_uint32),
+        ("grid", C.c_uint32),
+        ("reserved", C.c_uint32 * 6),
+    ]
+
+
+PHASE_NAMES = ["stop test and burn-in", "bind", "rank rev", "rank fwd", "contacts", "moves",
+               "adjust", "barrier states and ends", "LEF-BAR", "primary LEF-LEF",
+               "move corrections", "secondary rev", "secondary fwd", "unused",
+               "extrude and release", "unused"]
+
 # (name, restype, argtypes) for every entry point of include/modle_gpu.h
 SIGNATURES = [
     ("mgpu_init", C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
@@ -177,12 +190,15 @@ SIGNATURES = [
     ("mgpu_job_read_band", C.c_int, [C.c_void_p, C.c_uint64, U32P, U64P]),
     ("mgpu_job_band_device_ptr", C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p), U64P]),
     ("mgpu_job_destroy", None, [C.c_void_p]),
+    ("mgpu_job_set_tuning", C.c_int, [C.c_void_p, C.POINTER(MgpuTuning)]),
+    ("mgpu_job_get_tuning", C.c_int, [C.c_void_p, C.POINTER(MgpuTuning)]),
     ("mgpu_debug_step", C.c_int, [C.c_void_p, C.POINTER(MgpuParams), C.POINTER(MgpuChrom), C.c_int,
                                   C.c_uint64, U64P, U64P, U64P, U64P, U64P, U64P, U64P, U64P, U64P,
                                   C.POINTER(MgpuBarriers), U8P, U64P]),
     ("mgpu_debug_draws", C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.c_double, C.c_double,
                                    C.c_double, C.c_uint64, U64P]),
     ("mgpu_debug_philox", C.c_int, [C.c_void_p, U32P, C.c_uint64, U32P]),
+    ("mgpu_debug_phase_times", C.c_int, [C.c_void_p, C.c_int, U64P, C.c_uint64]),
     ("mgpu_debug_cell_trace", C.c_int, [C.c_void_p, C.POINTER(MgpuParams), C.POINTER(MgpuChrom),
                                         C.POINTER(MgpuBarriers), C.c_uint64, C.c_uint64, C.c_uint64,
                                         C.c_uint64, C.c_uint64, U64P, U64P]),

@@ -13,7 +13,7 @@ namespace mgpu {
 namespace {
 
 // ---------------------------------------------------------------------------------------------
-// block primitives (kFastThreads lanes, wave64)
+// block primitives (NT lanes, wave64)
 // ---------------------------------------------------------------------------------------------
 struct OpMaxI {
   __device__ int32_t operator()(int32_t a, int32_t b) const { return a > b ? a : b; }
@@ -28,8 +28,8 @@ struct OpMaxU {
   __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a > b ? a : b; }
 };
 
-// Exclusive scan of one value per lane across the block; sh holds kFastWaves entries.
-template <class T, class Op>
+// Exclusive scan of one value per lane across the block; sh holds NT / 64 entries.
+template <int NT, class T, class Op>
 __device__ T block_exclusive(T v, Op op, T ident, T* sh) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   T incl = v;
@@ -50,13 +50,13 @@ __device__ T block_exclusive(T v, Op op, T ident, T* sh) {
 
 // In-place inclusive scan of a[0, n) (reverse: from index n-1 down to 0). Each lane owns a
 // contiguous chunk of the scan order.
-template <class T, class Op>
+template <int NT, class T, class Op>
 __device__ void block_scan(T* a, int n, Op op, T ident, bool reverse, T* sh) {
-  const int c = (n + kFastThreads - 1) / kFastThreads;
+  const int c = (n + NT - 1) / NT;
   const int b = threadIdx.x * c, e = min(n, b + c);
   T acc = ident;
   for (int i = b; i < e; ++i) acc = op(acc, a[reverse ? n - 1 - i : i]);
-  acc = block_exclusive(acc, op, ident, sh);
+  acc = block_exclusive<NT>(acc, op, ident, sh);
   for (int i = b; i < e; ++i) {
     const int k = reverse ? n - 1 - i : i;
     acc = op(acc, a[k]);
@@ -65,12 +65,13 @@ __device__ void block_scan(T* a, int n, Op op, T ident, bool reverse, T* sh) {
   __syncthreads();
 }
 
-__device__ double block_sum_f64(double v, double* sh) {
+template <int NT>
+__device__ uint64_t block_sum_u64(uint64_t v, uint64_t* sh) {
   for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
-  double t = 0.0;
-  for (int k = 0; k < kFastWaves; ++k) t += sh[k];
+  uint64_t t = 0;
+  for (int k = 0; k < NT / 64; ++k) t += sh[k];
   __syncthreads();
   return t;
 }
@@ -123,21 +124,23 @@ __device__ inline void carve(Cell& c, uint8_t* base, uint32_t n, uint32_t nbar, 
 }
 
 struct Shared {
-  int32_t shi[kFastWaves];
-  uint32_t shu[kFastWaves];
-  double shd[kFastWaves];
+  int32_t shi[kFastMaxWaves];
+  uint32_t shu[kFastMaxWaves];
+  uint64_t phase[kFastPhases];
+  uint64_t tick;
   uint64_t item;
   uint64_t contacts;
   uint64_t burnin_epochs;
-  uint32_t nact, hlen, burnin_done, done, nnew, nkept, n5, n3, stable, navoid_r, navoid_f;
-  uint32_t avoid_r[64], avoid_f[64];
+  uint64_t shx[kFastMaxWaves];
+  uint32_t nact, hlen, burnin_done, done, nnew, nkept, n5, n3, stable, navoid;
 };
 
 // bitonic sort of keys[0, m) ascending, m a power of two
+template <int NT>
 __device__ void bitonic(uint64_t* keys, uint32_t m) {
   for (uint32_t k = 2; k <= m; k <<= 1) {
     for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t i = threadIdx.x; i < m; i += kFastThreads) {
+      for (uint32_t i = threadIdx.x; i < m; i += NT) {
         const uint32_t l = i ^ j;
         if (l > i) {
           const uint64_t a = keys[i], b = keys[l];
@@ -154,11 +157,12 @@ __device__ void bitonic(uint64_t* keys, uint32_t m) {
 
 // Sorts LEF ids a[0, n) by (position, id) with odd-even transposition passes; returns after two
 // consecutive passes without a swap. Cheap on the nearly sorted orders extrusion leaves.
+template <int NT>
 __device__ void odd_even_sort(uint32_t* a, uint32_t n, const uint32_t* pos) {
   int quiet = 0;
   for (uint32_t pass = 0; quiet < 2; ++pass) {
     int swapped = 0;
-    for (uint32_t i = 2 * threadIdx.x + (pass & 1); i + 1 < n; i += 2 * kFastThreads) {
+    for (uint32_t i = 2 * threadIdx.x + (pass & 1); i + 1 < n; i += 2 * NT) {
       const uint32_t x = a[i], y = a[i + 1];
       if (pos_key(pos, y) < pos_key(pos, x)) {
         a[i] = y;
@@ -172,27 +176,28 @@ __device__ void odd_even_sort(uint32_t* a, uint32_t n, const uint32_t* pos) {
 
 // Re-ranks one unit side. ranks[0, nact_old) hold last epoch's order; LEFs flagged in is_new
 // were (re)bound this epoch and are merged in; tmp_a / tmp_b are n-sized scratch arrays.
+template <int NT>
 __device__ void rerank(uint16_t* ranks, const uint32_t* pos, uint32_t nact, uint32_t nact_old,
                        const uint32_t* is_new, uint32_t* tmp_a, uint32_t* tmp_b, uint64_t* keys,
                        Shared& S) {
   // kept order: last epoch's entries that were not rebound, compacted stably
   {
-    const uint32_t c = (nact_old + kFastThreads - 1) / kFastThreads;
+    const uint32_t c = (nact_old + NT - 1) / NT;
     const uint32_t b = threadIdx.x * c, e = min(nact_old, b + c);
     int32_t cnt = 0;
     for (uint32_t i = b; i < e; ++i) cnt += is_new[ranks[i]] == 0;
-    int32_t off = block_exclusive(cnt, OpAddI{}, 0, S.shi);
+    int32_t off = block_exclusive<NT>(cnt, OpAddI{}, 0, S.shi);
     for (uint32_t i = b; i < e; ++i) {
       const uint32_t l = ranks[i];
       if (is_new[l] == 0) tmp_a[off++] = l;
     }
-    if (threadIdx.x == kFastThreads - 1) S.nkept = (uint32_t)off;
+    if (threadIdx.x == NT - 1) S.nkept = (uint32_t)off;
     __syncthreads();
   }
   uint32_t nkept = S.nkept;
   // Extrusion keeps neighbours ordered almost always, but the merge below needs the kept order
   // sorted exactly: odd-even transposition passes until two consecutive passes swap nothing.
-  odd_even_sort(tmp_a, nkept, pos);
+  odd_even_sort<NT>(tmp_a, nkept, pos);
   // new LEFs, merged batch by batch (one batch unless more than kNewCap were bound)
   const uint32_t batches = (nkept + kNewCap > nact || nact - nkept <= kNewCap)
                                ? 1u
@@ -202,7 +207,7 @@ __device__ void rerank(uint16_t* ranks, const uint32_t* pos, uint32_t nact, uint
     const uint32_t hi_id = batches == 1 ? nact : min(nact, lo_id + kNewCap);
     if (threadIdx.x == 0) S.nnew = 0;
     __syncthreads();
-    for (uint32_t l = lo_id + threadIdx.x; l < hi_id; l += kFastThreads) {
+    for (uint32_t l = lo_id + threadIdx.x; l < hi_id; l += NT) {
       if (is_new[l]) keys[atomicAdd(&S.nnew, 1u)] = pos_key(pos, l);
     }
     __syncthreads();
@@ -210,10 +215,10 @@ __device__ void rerank(uint16_t* ranks, const uint32_t* pos, uint32_t nact, uint
     if (k == 0) continue;
     uint32_t m = 1;
     while (m < k) m <<= 1;
-    for (uint32_t i = k + threadIdx.x; i < m; i += kFastThreads) keys[i] = ~0ULL;
+    for (uint32_t i = k + threadIdx.x; i < m; i += NT) keys[i] = ~0ULL;
     __syncthreads();
-    bitonic(keys, m);
-    for (uint32_t a = threadIdx.x; a < nkept; a += kFastThreads) {
+    bitonic<NT>(keys, m);
+    for (uint32_t a = threadIdx.x; a < nkept; a += NT) {
       const uint64_t key = pos_key(pos, tmp_a[a]);
       uint32_t lo = 0, hi = k;
       while (lo < hi) {
@@ -223,7 +228,7 @@ __device__ void rerank(uint16_t* ranks, const uint32_t* pos, uint32_t nact, uint
       }
       tmp_b[a + lo] = tmp_a[a];
     }
-    for (uint32_t b = threadIdx.x; b < k; b += kFastThreads) {
+    for (uint32_t b = threadIdx.x; b < k; b += NT) {
       const uint64_t key = keys[b];
       uint32_t lo = 0, hi = nkept;
       while (lo < hi) {
@@ -235,11 +240,24 @@ __device__ void rerank(uint16_t* ranks, const uint32_t* pos, uint32_t nact, uint
     }
     __syncthreads();
     nkept += k;
-    for (uint32_t i = threadIdx.x; i < nkept; i += kFastThreads) tmp_a[i] = tmp_b[i];
+    for (uint32_t i = threadIdx.x; i < nkept; i += NT) tmp_a[i] = tmp_b[i];
     __syncthreads();
   }
-  for (uint32_t i = threadIdx.x; i < nact; i += kFastThreads) ranks[i] = (uint16_t)tmp_a[i];
+  for (uint32_t i = threadIdx.x; i < nact; i += NT) ranks[i] = (uint16_t)tmp_a[i];
   __syncthreads();
+}
+
+static_assert(kFastPhases == kFastPhaseCount, "phase count of the timers and of the host API");
+__device__ unsigned long long g_fast_phase[kFastPhases];
+
+// Phase timers: called by every lane right after the barrier that ends a phase; lane 0 adds the
+// wall-clock ticks since the previous mark to the phase's total.
+__device__ inline void mark(const FastParams& P, Shared& S, int ph) {
+  if (P.timers && threadIdx.x == 0) {
+    const uint64_t t = wall_clock64();
+    S.phase[ph] += t - S.tick;
+    S.tick = t;
+  }
 }
 
 __device__ inline float gev_noise(const FastParams& P, uint32_t x) {
@@ -283,6 +301,7 @@ __device__ inline uint32_t poisson(const Rng& R, double mean, uint32_t tag) {
 // unit j behind seed s stalls when every unit in (s, j] reaches its predecessor's target and
 // keeps its collision (does not bypass). X[j] packs (s + 1) << 16 | (last failure + 1), so a
 // single prefix-max yields the last failure inside j's own chain.
+template <int NT>
 __device__ void secondary_side(const FastParams& P, const Rng& R, uint32_t epoch, Cell& c,
                                Shared& S, uint32_t n, bool rev_side, uint32_t lo_j) {
   uint16_t* ranks = rev_side ? c.rr : c.fr;
@@ -294,12 +313,12 @@ __device__ void secondary_side(const FastParams& P, const Rng& R, uint32_t epoch
   auto target = [&](uint32_t l) -> int32_t {
     return rev_side ? (int32_t)pos[l] - (int32_t)mv[l] : (int32_t)pos[l] + (int32_t)mv[l];
   };
-  for (uint32_t j = threadIdx.x; j < n; j += kFastThreads)
+  for (uint32_t j = threadIdx.x; j < n; j += NT)
     X[j] = ck_stalled(coll[unit_of(j)]) ? j + 1 : 0;
   __syncthreads();
-  block_scan(X, (int)n, OpMaxU{}, 0u, false, S.shu);
+  block_scan<NT>(X, (int)n, OpMaxU{}, 0u, false, S.shu);
   // failure flags given the chain start
-  for (uint32_t j = threadIdx.x; j < n; j += kFastThreads) {
+  for (uint32_t j = threadIdx.x; j < n; j += NT) {
     const uint32_t s1 = X[j];  // seed + 1, 0 if none
     const uint32_t l = unit_of(j);
     uint32_t fail = 0;
@@ -317,8 +336,8 @@ __device__ void secondary_side(const FastParams& P, const Rng& R, uint32_t epoch
     X[j] = (s1 << 16) | fail;
   }
   __syncthreads();
-  block_scan(X, (int)n, OpMaxU{}, 0u, false, S.shu);
-  for (uint32_t j = threadIdx.x; j < n; j += kFastThreads) {
+  block_scan<NT>(X, (int)n, OpMaxU{}, 0u, false, S.shu);
+  for (uint32_t j = threadIdx.x; j < n; j += NT) {
     const uint32_t s1 = X[j] >> 16;
     if (s1 == 0 || s1 - 1 == j || j < lo_j) continue;
     const uint32_t l = unit_of(j);
@@ -336,29 +355,101 @@ __device__ void secondary_side(const FastParams& P, const Rng& R, uint32_t epoch
       coll[l] = ck(kCkSecondary, lead);
     } else if (last_fail == j + 1 && (X[j - 1] & 0xFFFFu) == 0) {
       const bool reach = rev_side ? target(l) <= ts + dist - 1 : target(l) >= ts - dist + 1;
-      if (reach) {
-        coll[l] = ck(kCkAvoided, lead);
-        uint32_t* cnt = rev_side ? &S.navoid_r : &S.navoid_f;
-        const uint32_t slot = atomicAdd(cnt, 1u);
-        if (slot < 64) (rev_side ? S.avoid_r : S.avoid_f)[slot] = rev_side ? j : n - 1 - j;
-      }
+      if (reach) coll[l] = ck(kCkAvoided, lead);
     }
   }
   __syncthreads();
 }
 
+// Rank positions of the units whose secondary collision was avoided on one side, in chain
+// order (rev: ascending rank, fwd: descending rank), written to list; returns their count.
+// A scan over contiguous chunks keeps the order independent of the workgroup size.
+template <int NT>
+__device__ uint32_t collect_avoided(const uint16_t* ranks, const uint32_t* coll, uint32_t n,
+                                    bool rev_side, uint32_t* list, Shared& S) {
+  const uint32_t c = (n + NT - 1) / NT;
+  const uint32_t b = threadIdx.x * c, e = min(n, b + c);
+  auto avoided = [&](uint32_t j) {
+    return ck_kind(coll[ranks[rev_side ? j : n - 1 - j]]) == kCkAvoided;
+  };
+  int32_t cnt = 0;
+  for (uint32_t j = b; j < e; ++j) cnt += avoided(j);
+  int32_t off = block_exclusive<NT>(cnt, OpAddI{}, 0, S.shi);
+  for (uint32_t j = b; j < e; ++j)
+    if (avoided(j)) list[off++] = rev_side ? j : n - 1 - j;
+  if (threadIdx.x == NT - 1) S.navoid = (uint32_t)off;
+  __syncthreads();
+  return S.navoid;
+}
+
+// fix_secondary_lef_lef_collisions (simulation_detect_collisions.cpp:520-647) for the rev units
+// listed in ascending rank order: the bypassing unit swaps places with the one it passed.
+__device__ void fix_avoided_rev(Cell& c, const FastChrom& ch, const uint32_t* list, uint32_t cnt) {
+  for (uint32_t a = 0; a < cnt; ++a) {
+    const uint32_t k = list[a];
+    const uint32_t r2 = c.rr[k];
+    if (k == 0 || ck_kind(c.rc[r2]) != kCkAvoided) continue;
+    const uint32_t r1 = c.rr[k - 1];
+    const uint32_t p1t = c.rev[r1] - c.rm[r1];
+    c.rm[r2] = c.rev[r2] > p1t + 1 ? c.rev[r2] - (p1t + 1) : 0;
+    c.rc[r2] = ck(kCkSecondary, r1);
+    const uint32_t a1 = c.rev[r1], b1 = c.rev[r2];
+    c.rev[r1] = min(c.fwd[r1], b1);
+    c.rev[r2] = min(c.fwd[r2], a1);
+    uint32_t t = c.rc[r1];
+    c.rc[r1] = c.rc[r2];
+    c.rc[r2] = t;
+    t = c.rm[r1];
+    c.rm[r1] = c.rm[r2];
+    c.rm[r2] = t;
+    c.rr[k - 1] = (uint16_t)r2;
+    c.rr[k] = (uint16_t)r1;
+    c.rm[r2] = min(c.rev[r2] - ch.start, c.rm[r2]);
+    c.rm[r1] = min(c.rev[r1] - ch.start, c.rm[r1]);
+  }
+}
+
+// The same for fwd units; list holds descending ranks, processed from the lowest rank up.
+__device__ void fix_avoided_fwd(Cell& c, const FastChrom& ch, uint32_t n, const uint32_t* list,
+                                uint32_t cnt) {
+  for (uint32_t a = cnt; a-- > 0;) {
+    const uint32_t k = list[a];
+    const uint32_t f1 = c.fr[k];
+    if (k + 1 >= n || ck_kind(c.fc[f1]) != kCkAvoided) continue;
+    const uint32_t f2 = c.fr[k + 1];
+    const uint32_t p2t = c.fwd[f2] + c.fm[f2];
+    c.fm[f1] = p2t > c.fwd[f1] + 1 ? p2t - (c.fwd[f1] + 1) : 0;
+    c.fc[f1] = ck(kCkSecondary, f2);
+    const uint32_t a1 = c.fwd[f1], b1 = c.fwd[f2];
+    c.fwd[f1] = max(c.rev[f1], b1);
+    c.fwd[f2] = max(c.rev[f2], a1);
+    uint32_t t = c.fc[f1];
+    c.fc[f1] = c.fc[f2];
+    c.fc[f2] = t;
+    t = c.fm[f1];
+    c.fm[f1] = c.fm[f2];
+    c.fm[f2] = t;
+    c.fr[k] = (uint16_t)f2;
+    c.fr[k + 1] = (uint16_t)f1;
+    c.fm[f2] = min(ch.end - 1 - c.fwd[f2], c.fm[f2]);
+    c.fm[f1] = min(ch.end - 1 - c.fwd[f1], c.fm[f1]);
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // the cell loop
 // ---------------------------------------------------------------------------------------------
-__device__ void fast_cell(const FastParams& P, const FastChrom& ch, Cell& c, Shared& S,
+template <int NT>
+__device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, const FastChrom& ch,
+                                                         Cell& c, Shared& S,
                           uint64_t seed, uint64_t target, uint64_t* stats) {
   const uint32_t N = ch.nlefs, NB = ch.nbar;
   const Rng R{(uint32_t)seed, (uint32_t)(seed >> 32)};
-  for (uint32_t i = threadIdx.x; i < N; i += kFastThreads) {
+  for (uint32_t i = threadIdx.x; i < N; i += NT) {
     c.rev[i] = c.fwd[i] = kFreePos;
     c.rr[i] = c.fr[i] = (uint16_t)i;
   }
-  for (uint32_t b = threadIdx.x; b < NB; b += kFastThreads) {
+  for (uint32_t b = threadIdx.x; b < NB; b += NT) {
     const U4 d = R.draw(b, 0xFFFFFFFFu, kSBarInit);
     c.bst[b] = d.x < ch.bar_t_occ[b] ? 1 : 0;
   }
@@ -369,6 +460,8 @@ __device__ void fast_cell(const FastParams& P, const FastChrom& ch, Cell& c, Sha
     S.contacts = 0;
     S.burnin_epochs = 0;
     S.done = 0;
+    for (int k = 0; k < kFastPhases; ++k) S.phase[k] = 0;
+    S.tick = P.timers ? wall_clock64() : 0;
   }
   __syncthreads();
   const uint32_t span = ch.end - ch.start;  // bind positions in [start, end - 1]
@@ -381,7 +474,6 @@ __device__ void fast_cell(const FastParams& P, const FastChrom& ch, Cell& c, Sha
                                               : (uint64_t)epoch - S.burnin_epochs < ch.target_epochs)
                    ? 0
                    : 1;
-      S.navoid_r = S.navoid_f = 0;
     }
     __syncthreads();
     if (S.done) break;
@@ -398,16 +490,27 @@ __device__ void fast_cell(const FastParams& P, const FastChrom& ch, Cell& c, Sha
         }
         __syncthreads();
       } else {
-        double acc = 0.0;
-        for (uint32_t i = threadIdx.x; i < N; i += kFastThreads)
-          acc += c.rev[i] == kFreePos ? 0.0 : (double)(c.fwd[i] - c.rev[i]);
-        const double mean = block_sum_f64(acc, S.shd) / (double)N;
-        double ss = 0.0;
-        for (uint32_t i = threadIdx.x; i < N; i += kFastThreads) {
-          const double v = (c.rev[i] == kFreePos ? 0.0 : (double)(c.fwd[i] - c.rev[i])) - mean;
-          ss += v * v;
+        // mean and population standard deviation of the loop sizes, unbound LEFs counting as
+        // 0 (compute_loop_size_stats, simulation.cpp:957-981), from exact integer sums so they
+        // do not depend on the reduction order: x < 2^32, per-lane sums of x^2 are split into
+        // 32-bit halves that cannot overflow across the block
+        uint64_t sx = 0, sq_lo = 0, sq_hi = 0;
+        for (uint32_t i = threadIdx.x; i < N; i += NT) {
+          const uint64_t x = c.rev[i] == kFreePos ? 0 : (uint64_t)(c.fwd[i] - c.rev[i]);
+          const uint64_t x2 = x * x;
+          sx += x;
+          sq_lo += x2 & 0xFFFFFFFFull;
+          sq_hi += x2 >> 32;
         }
-        const double sd = sqrt(block_sum_f64(ss, S.shd) / (double)N);
+        sx = block_sum_u64<NT>(sx, S.shx);
+        sq_lo = block_sum_u64<NT>(sq_lo, S.shx);
+        sq_hi = block_sum_u64<NT>(sq_hi, S.shx);
+        const unsigned __int128 sxx = ((unsigned __int128)sq_hi << 32) + sq_lo;
+        const unsigned __int128 num = (unsigned __int128)N * sxx - (unsigned __int128)sx * sx;
+        const double num_d = (double)(uint64_t)(num >> 64) * 18446744073709551616.0 +
+                             (double)(uint64_t)num;  // N^2 * variance
+        const double mean = (double)sx / (double)N;
+        const double sd = sqrt(num_d / ((double)N * (double)N));
         const uint32_t cap = P.hist_len, w = P.window;
         if (threadIdx.x == 0) {
           ++S.burnin_epochs;
@@ -427,7 +530,7 @@ __device__ void fast_cell(const FastParams& P, const FastChrom& ch, Cell& c, Sha
         if (S.hlen == cap) {
           // windows whose mean drops, for the CV and for the mean (evaluate_burnin :983-1025)
           int32_t dc = 0, da = 0;
-          for (uint32_t k = threadIdx.x; k + w + 1 <= cap - 1; k += kFastThreads) {
+          for (uint32_t k = threadIdx.x; k + w + 1 <= cap - 1; k += NT) {
             double a1 = 0, a2 = 0, b1 = 0, b2 = 0;
             for (uint32_t j = 0; j < w; ++j) {
               a1 += c.cfx[k + j];
@@ -438,9 +541,9 @@ __device__ void fast_cell(const FastParams& P, const FastChrom& ch, Cell& c, Sha
             dc += a1 > a2;
             da += b1 > b2;
           }
-          const int32_t nc = block_exclusive(dc, OpAddI{}, 0, S.shi) + dc;
-          const int32_t na = block_exclusive(da, OpAddI{}, 0, S.shi) + da;
-          if (threadIdx.x == kFastThreads - 1) {
+          const int32_t nc = block_exclusive<NT>(dc, OpAddI{}, 0, S.shi) + dc;
+          const int32_t na = block_exclusive<NT>(da, OpAddI{}, 0, S.shi) + da;
+          if (threadIdx.x == NT - 1) {
             const double r1 = (double)nc / (double)((int32_t)(cap - w) - nc);
             const double r2 = (double)na / (double)((int32_t)(cap - w) - na);
             S.stable = (r1 >= 0.95 && r1 <= 1.05 && r2 >= 0.95 && r2 <= 1.05) ? 1 : 0;
@@ -460,9 +563,10 @@ __device__ void fast_cell(const FastParams& P, const FastChrom& ch, Cell& c, Sha
     }
     const uint32_t n = S.nact;
     const bool bdone = S.burnin_done != 0;
+    mark(P, S, 0);
 
     // ---- bind (simulation_impl.hpp:28-89) ----
-    for (uint32_t i = threadIdx.x; i < n; i += kFastThreads) {
+    for (uint32_t i = threadIdx.x; i < n; i += NT) {
       uint32_t flag = 0;
       if (c.rev[i] == kFreePos) {
         const U4 d = R.draw(i, epoch, kSBind);
@@ -472,10 +576,13 @@ __device__ void fast_cell(const FastParams& P, const FastChrom& ch, Cell& c, Sha
       c.rm[i] = flag;  // rm holds the "bound this epoch" flags until moves are drawn
     }
     __syncthreads();
+    mark(P, S, 1);
     // ---- ranks (rank_lefs, simulation.cpp:402-487) ----
     uint64_t* keys = reinterpret_cast<uint64_t*>(c.scratch);
-    rerank(c.rr, c.rev, n, nact_prev, c.rm, c.rc, c.fc, keys, S);
-    rerank(c.fr, c.fwd, n, nact_prev, c.rm, c.rc, c.fc, keys, S);
+    rerank<NT>(c.rr, c.rev, n, nact_prev, c.rm, c.rc, c.fc, keys, S);
+    mark(P, S, 2);
+    rerank<NT>(c.fr, c.fwd, n, nact_prev, c.rm, c.rc, c.fc, keys, S);
+    mark(P, S, 3);
     nact_prev = n;
 
     // ---- contacts (sample_and_register_contacts, register_contacts.cpp:94-198) ----
@@ -483,7 +590,7 @@ __device__ void fast_cell(const FastParams& P, const FastChrom& ch, Cell& c, Sha
       uint64_t want = ch.cpe;
       if (P.target_contact_density > 0.0) want = min(want, target - S.contacts);
       const uint32_t sp = ch.start + 1, ep = ch.end - 1;
-      for (uint32_t s = threadIdx.x; s < want; s += kFastThreads) {
+      for (uint32_t s = threadIdx.x; s < want; s += NT) {
         for (uint32_t attempt = 0; attempt < 1024; ++attempt) {
           const U4 d = R.draw(s, epoch, kSContact, attempt);
           const uint32_t i = ubounded(d.x, n);
@@ -515,13 +622,14 @@ __device__ void fast_cell(const FastParams& P, const FastChrom& ch, Cell& c, Sha
       __syncthreads();  // every lane has read the remaining target before it changes
       if (threadIdx.x == 0) S.contacts += want;
       __syncthreads();
+      mark(P, S, 4);
       if (target != 0 && S.contacts >= target) break;
     }
 
     // ---- moves (generate_moves, simulation.cpp:265-323) ----
     const float vr = bdone ? P.rev_speed : P.rev_speed_burnin;
     const float vf = bdone ? P.fwd_speed : P.fwd_speed_burnin;
-    for (uint32_t i = threadIdx.x; i < n; i += kFastThreads) {
+    for (uint32_t i = threadIdx.x; i < n; i += NT) {
       const U4 d = R.draw(i, epoch, kSMove);
       const float rad = sqrtf(-2.0f * logf(u01_open(d.x)));
       float sn, cs;
@@ -532,30 +640,31 @@ __device__ void fast_cell(const FastParams& P, const FastChrom& ch, Cell& c, Sha
       c.fm[i] = (uint32_t)roundf(b > 0.0f ? b : 0.0f);
     }
     __syncthreads();
+    mark(P, S, 5);
     // adjust (simulation.cpp:342-399): rev targets must fall strictly along the rev order,
     // fwd targets must rise strictly along the fwd order; then clamp to the chromosome.
     {
       int32_t* T = reinterpret_cast<int32_t*>(c.scratch);
-      for (uint32_t k = threadIdx.x; k < n; k += kFastThreads) {
+      for (uint32_t k = threadIdx.x; k < n; k += NT) {
         const uint32_t l = c.rr[k];
         T[k] = (int32_t)c.rev[l] - (int32_t)c.rm[l] + (int32_t)k;
       }
       __syncthreads();
-      block_scan(T, (int)n, OpMinI{}, INT32_MAX, true, S.shi);
-      for (uint32_t k = threadIdx.x; k < n; k += kFastThreads) {
+      block_scan<NT>(T, (int)n, OpMinI{}, INT32_MAX, true, S.shi);
+      for (uint32_t k = threadIdx.x; k < n; k += NT) {
         const uint32_t l = c.rr[k];
         int32_t t = T[k] - (int32_t)k;
         if (t < (int32_t)ch.start) t = (int32_t)ch.start;
         c.rm[l] = c.rev[l] - (uint32_t)t;
       }
       __syncthreads();
-      for (uint32_t k = threadIdx.x; k < n; k += kFastThreads) {
+      for (uint32_t k = threadIdx.x; k < n; k += NT) {
         const uint32_t l = c.fr[k];
         T[k] = (int32_t)c.fwd[l] + (int32_t)c.fm[l] - (int32_t)k;
       }
       __syncthreads();
-      block_scan(T, (int)n, OpMaxI{}, INT32_MIN, false, S.shi);
-      for (uint32_t k = threadIdx.x; k < n; k += kFastThreads) {
+      block_scan<NT>(T, (int)n, OpMaxI{}, INT32_MIN, false, S.shi);
+      for (uint32_t k = threadIdx.x; k < n; k += NT) {
         const uint32_t l = c.fr[k];
         int32_t t = T[k] + (int32_t)k;
         if (t > (int32_t)(ch.end - 1)) t = (int32_t)(ch.end - 1);
@@ -565,9 +674,10 @@ __device__ void fast_cell(const FastParams& P, const FastChrom& ch, Cell& c, Sha
       }
       __syncthreads();
     }
+    mark(P, S, 6);
 
     // ---- barrier states (extrusion_barriers.cpp:145-161) ----
-    for (uint32_t b = threadIdx.x; b < NB; b += kFastThreads) {
+    for (uint32_t b = threadIdx.x; b < NB; b += NT) {
       const U4 d = R.draw(b, epoch, kSBarrier);
       const uint8_t st = c.bst[b];
       if (st == 0 && d.x >= ch.bar_t_inactive[b]) c.bst[b] = 1;
@@ -610,6 +720,7 @@ __device__ void fast_cell(const FastParams& P, const FastChrom& ch, Cell& c, Sha
       S.n3 = n3;
     }
     __syncthreads();
+    mark(P, S, 7);
     const uint32_t n5 = S.n5, n3 = S.n3;
 
     // ---- LEF-BAR (simulation_detect_collisions.cpp:128-252): per active barrier, the nearest
@@ -618,11 +729,11 @@ __device__ void fast_cell(const FastParams& P, const FastChrom& ch, Cell& c, Sha
     for (int side = 0; side < 2; ++side) {
       const uint16_t* ranks = side == 0 ? c.rr : c.fr;
       const uint32_t* pos = side == 0 ? c.rev : c.fwd;
-      for (uint32_t k = threadIdx.x; k < n; k += kFastThreads) sorted[k] = pos[ranks[k]];
+      for (uint32_t k = threadIdx.x; k < n; k += NT) sorted[k] = pos[ranks[k]];
       __syncthreads();
       const uint32_t rev_lo = n5 ? n5 - 1 : 0;
       const uint32_t fwd_top = n - (n3 ? n3 - 1 : 0);
-      for (uint32_t b = threadIdx.x; b < NB; b += kFastThreads) {
+      for (uint32_t b = threadIdx.x; b < NB; b += NT) {
         if (!c.bst[b]) continue;
         const uint32_t bp = ch.bar_pos[b];
         if (side == 0) {
@@ -655,18 +766,19 @@ __device__ void fast_cell(const FastParams& P, const FastChrom& ch, Cell& c, Sha
       }
       __syncthreads();
     }
-    for (uint32_t i = threadIdx.x; i < n; i += kFastThreads) {
+    for (uint32_t i = threadIdx.x; i < n; i += NT) {
       if (ck_kind(c.fc[i]) == kCkBarFwdTmp) c.fc[i] = ck(kCkBar, kCkIdxMask - ck_idx(c.fc[i]));
     }
 
     // ---- primary LEF-LEF (simulation_detect_collisions.cpp:255-401): a fwd unit that is the
     // last one before a rev unit pairs with that rev unit ----
     // rev positions in rank order are still in `sorted` only for the fwd side; rebuild them.
-    for (uint32_t k = threadIdx.x; k < n; k += kFastThreads) sorted[k] = c.rev[c.rr[k]];
+    for (uint32_t k = threadIdx.x; k < n; k += NT) sorted[k] = c.rev[c.rr[k]];
     __syncthreads();
+    mark(P, S, 8);
     if (n5 < n && n3 < n) {
       const uint32_t i2 = n - (n3 ? n3 - 1 : 0);
-      for (uint32_t k = threadIdx.x; k + 1 < i2; k += kFastThreads) {
+      for (uint32_t k = threadIdx.x; k + 1 < i2; k += NT) {
         const uint32_t fl = c.fr[k];
         const uint32_t fp = c.fwd[fl];
         uint32_t lo = n5, hi = n;
@@ -703,14 +815,15 @@ __device__ void fast_cell(const FastParams& P, const FastChrom& ch, Cell& c, Sha
       }
     }
     __syncthreads();
+    mark(P, S, 9);
 
     // ---- move corrections (simulation_correct_moves.cpp:20-122) ----
-    for (uint32_t i = threadIdx.x; i < n; i += kFastThreads) {
+    for (uint32_t i = threadIdx.x; i < n; i += NT) {
       if (ck_kind(c.rc[i]) == kCkBar) c.rm[i] = (c.rev[i] - ch.bar_pos[ck_idx(c.rc[i])]) - 1;
       if (ck_kind(c.fc[i]) == kCkBar) c.fm[i] = (ch.bar_pos[ck_idx(c.fc[i])] - c.fwd[i]) - 1;
     }
     __syncthreads();
-    for (uint32_t r = threadIdx.x; r < n; r += kFastThreads) {
+    for (uint32_t r = threadIdx.x; r < n; r += NT) {
       if (ck_kind(c.rc[r]) != kCkPrimary) continue;
       const uint32_t f = ck_idx(c.rc[r]);
       if (ck_kind(c.fc[f]) == kCkPrimary) {
@@ -724,83 +837,37 @@ __device__ void fast_cell(const FastParams& P, const FastChrom& ch, Cell& c, Sha
       }
     }
     __syncthreads();
-    for (uint32_t f = threadIdx.x; f < n; f += kFastThreads) {
+    for (uint32_t f = threadIdx.x; f < n; f += NT) {
       if (ck_kind(c.fc[f]) != kCkPrimary) continue;
       const uint32_t r = ck_idx(c.fc[f]);
       if (ck_kind(c.rc[r]) == kCkBar) c.fm[f] = (c.rev[r] - c.rm[r]) - c.fwd[f] - 1;
     }
     __syncthreads();
+    mark(P, S, 10);
 
     // ---- secondary LEF-LEF (simulation_detect_collisions.cpp:404-518) ----
-    secondary_side(P, R, epoch, c, S, n, true, max(1u, n5));
-    secondary_side(P, R, epoch, c, S, n, false, (n3 ? n3 - 1 : 0) + 1);
-
-    // ---- bypass fix-ups (fix_secondary_lef_lef_collisions, :520-647), one lane, rank order ----
-    if (threadIdx.x == 0 && (S.navoid_r || S.navoid_f)) {
-      const uint32_t nr = min(S.navoid_r, 64u), nf = min(S.navoid_f, 64u);
-      for (uint32_t a = 1; a < nr; ++a)  // insertion sort of the few avoided ranks
-        for (uint32_t b = a; b > 0 && S.avoid_r[b] < S.avoid_r[b - 1]; --b) {
-          const uint32_t t = S.avoid_r[b];
-          S.avoid_r[b] = S.avoid_r[b - 1];
-          S.avoid_r[b - 1] = t;
-        }
-      for (uint32_t a = 1; a < nf; ++a)
-        for (uint32_t b = a; b > 0 && S.avoid_f[b] < S.avoid_f[b - 1]; --b) {
-          const uint32_t t = S.avoid_f[b];
-          S.avoid_f[b] = S.avoid_f[b - 1];
-          S.avoid_f[b - 1] = t;
-        }
-      for (uint32_t a = 0; a < nr; ++a) {
-        const uint32_t k = S.avoid_r[a];
-        const uint32_t r2 = c.rr[k];
-        if (k == 0 || ck_kind(c.rc[r2]) != kCkAvoided) continue;
-        const uint32_t r1 = c.rr[k - 1];
-        const uint32_t p1t = c.rev[r1] - c.rm[r1];
-        c.rm[r2] = c.rev[r2] > p1t + 1 ? c.rev[r2] - (p1t + 1) : 0;
-        c.rc[r2] = ck(kCkSecondary, r1);
-        const uint32_t a1 = c.rev[r1], b1 = c.rev[r2];
-        c.rev[r1] = min(c.fwd[r1], b1);
-        c.rev[r2] = min(c.fwd[r2], a1);
-        uint32_t t = c.rc[r1];
-        c.rc[r1] = c.rc[r2];
-        c.rc[r2] = t;
-        t = c.rm[r1];
-        c.rm[r1] = c.rm[r2];
-        c.rm[r2] = t;
-        c.rr[k - 1] = (uint16_t)r2;
-        c.rr[k] = (uint16_t)r1;
-        c.rm[r2] = min(c.rev[r2] - ch.start, c.rm[r2]);
-        c.rm[r1] = min(c.rev[r1] - ch.start, c.rm[r1]);
-      }
-      for (uint32_t a = 0; a < nf; ++a) {
-        const uint32_t k = S.avoid_f[a];
-        const uint32_t f1 = c.fr[k];
-        if (k + 1 >= n || ck_kind(c.fc[f1]) != kCkAvoided) continue;
-        const uint32_t f2 = c.fr[k + 1];
-        const uint32_t p2t = c.fwd[f2] + c.fm[f2];
-        c.fm[f1] = p2t > c.fwd[f1] + 1 ? p2t - (c.fwd[f1] + 1) : 0;
-        c.fc[f1] = ck(kCkSecondary, f2);
-        const uint32_t a1 = c.fwd[f1], b1 = c.fwd[f2];
-        c.fwd[f1] = max(c.rev[f1], b1);
-        c.fwd[f2] = max(c.rev[f2], a1);
-        uint32_t t = c.fc[f1];
-        c.fc[f1] = c.fc[f2];
-        c.fc[f2] = t;
-        t = c.fm[f1];
-        c.fm[f1] = c.fm[f2];
-        c.fm[f2] = t;
-        c.fr[k] = (uint16_t)f2;
-        c.fr[k + 1] = (uint16_t)f1;
-        c.fm[f2] = min(ch.end - 1 - c.fwd[f2], c.fm[f2]);
-        c.fm[f1] = min(ch.end - 1 - c.fwd[f1], c.fm[f1]);
-      }
+    // The rev fix-ups touch only rev-side state (and read fwd positions, which the fwd pass
+    // does not change), so each side is fixed right after its own pass, rev first as in
+    // process_collisions (simulation.cpp:925-955).
+    secondary_side<NT>(P, R, epoch, c, S, n, true, max(1u, n5));
+    {
+      const uint32_t na = collect_avoided<NT>(c.rr, c.rc, n, true, c.scratch, S);
+      if (threadIdx.x == 0 && na) fix_avoided_rev(c, ch, c.scratch, na);
+      __syncthreads();
     }
-    __syncthreads();
+    mark(P, S, 11);
+    secondary_side<NT>(P, R, epoch, c, S, n, false, (n3 ? n3 - 1 : 0) + 1);
+    {
+      const uint32_t na = collect_avoided<NT>(c.fr, c.fc, n, false, c.scratch, S);
+      if (threadIdx.x == 0 && na) fix_avoided_fwd(c, ch, n, c.scratch, na);
+      __syncthreads();
+    }
+    mark(P, S, 12);
 
     // ---- extrude and release (simulation.cpp:489-512, 544-592) ----
     {
       const double base = bdone ? P.prob_release : P.prob_release_burnin;
-      for (uint32_t i = threadIdx.x; i < n; i += kFastThreads) {
+      for (uint32_t i = threadIdx.x; i < n; i += NT) {
         c.rev[i] -= c.rm[i];
         c.fwd[i] += c.fm[i];
         const int h = (ck_kind(c.rc[i]) == kCkBar && ch.bar_dir[ck_idx(c.rc[i])] == 1) +
@@ -811,6 +878,7 @@ __device__ void fast_cell(const FastParams& P, const FastChrom& ch, Cell& c, Sha
       }
     }
     __syncthreads();
+    mark(P, S, 14);
   }
   if (threadIdx.x == 0 && stats) {
     stats[0] = epoch;
@@ -818,12 +886,18 @@ __device__ void fast_cell(const FastParams& P, const FastChrom& ch, Cell& c, Sha
     stats[2] = S.contacts;
     stats[3] = 0;
   }
+  if (P.timers && threadIdx.x == 0)
+    for (int k = 0; k < kFastPhases; ++k) atomicAdd(&g_fast_phase[k], (unsigned long long)S.phase[k]);
 }
 
-__global__ void __launch_bounds__(kFastThreads)
+// Persistent workers: each workgroup takes cells off the queue until it is empty. kLds selects
+// where the working set lives (LDS, or a per-workgroup slab in global memory); it is a template
+// argument so the LDS variant addresses its arrays with LDS instructions.
+template <int NT, bool kLds>
+__global__ void __launch_bounds__(NT)
     fast_cells_kernel(FastParams P, const FastChrom* chroms, const WorkItem* items,
                       const uint32_t* order, uint64_t nitems, unsigned long long* next,
-                      uint8_t* slab, uint64_t slab_stride, int use_lds, uint64_t* cell_stats) {
+                      uint8_t* slab, uint64_t slab_stride, uint64_t* cell_stats) {
   extern __shared__ __align__(16) uint8_t lds[];
   __shared__ Shared S;
   for (;;) {
@@ -836,9 +910,12 @@ __global__ void __launch_bounds__(kFastThreads)
     const WorkItem w = items[it];
     const FastChrom ch = chroms[w.chrom];
     Cell c;
-    uint8_t* base = use_lds ? lds : slab + (uint64_t)blockIdx.x * slab_stride;
-    carve(c, base, ch.nlefs, ch.nbar, P.hist_len);
-    fast_cell(P, ch, c, S, w.seed, w.target, cell_stats ? cell_stats + 4 * it : nullptr);
+    if constexpr (kLds) {
+      carve(c, lds, ch.nlefs, ch.nbar, P.hist_len);
+    } else {
+      carve(c, slab + (uint64_t)blockIdx.x * slab_stride, ch.nlefs, ch.nbar, P.hist_len);
+    }
+    fast_cell<NT>(P, ch, c, S, w.seed, w.target, cell_stats ? cell_stats + 4 * it : nullptr);
     __syncthreads();
   }
 }
@@ -874,10 +951,42 @@ struct FastImpl {
   uint64_t slab_stride = 0;
   uint64_t lds_bytes = 0;
   bool use_lds = true;
-  uint32_t grid = 0;
+  uint32_t per_cu = 1;     // cells resident per CU as the LDS footprint allows
+  uint32_t num_cus = 1;
+  uint32_t threads = 256;  // workgroup size per cell
   uint64_t nitems = 0;
   void* d_mem = nullptr;
 };
+
+// Workgroups of the persistent launch: as many cells per CU as LDS and the 2048-lane CU limit
+// allow (at most 8), never more than there are cells.
+uint32_t fast_grid(const FastImpl* impl) {
+  const uint64_t per_cu = std::min<uint64_t>({impl->per_cu, 2048u / impl->threads, 8u});
+  return (uint32_t)std::min<uint64_t>(impl->nitems, std::max<uint64_t>(1, per_cu) * impl->num_cus);
+}
+
+template <int NT, bool kLds>
+hipError_t launch_fast_as(hipStream_t stream, FastImpl* impl, const WorkItem* d_items,
+                          uint64_t* d_cell_stats, uint32_t grid) {
+  const size_t shmem = kLds ? impl->lds_bytes : 0;
+  if (shmem > 0) {
+    const hipError_t e =
+        hipFuncSetAttribute(reinterpret_cast<const void*>(fast_cells_kernel<NT, kLds>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL((fast_cells_kernel<NT, kLds>), dim3(grid), dim3(NT), shmem, stream, impl->P,
+                     impl->d_chroms, d_items, impl->d_order, impl->nitems, impl->d_next,
+                     impl->d_slab, impl->slab_stride, d_cell_stats);
+  return hipGetLastError();
+}
+
+template <int NT>
+hipError_t launch_fast(hipStream_t stream, FastImpl* impl, const WorkItem* d_items,
+                       uint64_t* d_cell_stats, uint32_t grid) {
+  return impl->use_lds ? launch_fast_as<NT, true>(stream, impl, d_items, d_cell_stats, grid)
+                       : launch_fast_as<NT, false>(stream, impl, d_items, d_cell_stats, grid);
+}
 
 uint64_t fast_cell_bytes_host(uint32_t n, uint32_t nbar, uint32_t hist) {
   return fast_cell_bytes(n, nbar, hist);
@@ -941,16 +1050,20 @@ int fast_job_create(hipStream_t stream, const mgpu_params& p, const std::vector<
   const uint64_t lds_limit = lds_total - sizeof(Shared) - 512;
   impl->use_lds = max_bytes <= lds_limit;
   impl->lds_bytes = impl->use_lds ? max_bytes : 0;
-  const uint64_t per_cu =
-      impl->use_lds ? std::max<uint64_t>(1, lds_total / (max_bytes + sizeof(Shared) + 512)) : 2;
-  impl->grid = (uint32_t)std::min<uint64_t>(
-      items.size(), std::min<uint64_t>(per_cu, 8) * (uint64_t)prop.multiProcessorCount);
+  impl->per_cu = impl->use_lds
+                     ? (uint32_t)std::max<uint64_t>(1, lds_total / (max_bytes + sizeof(Shared) + 512))
+                     : 2u;
+  impl->num_cus = (uint32_t)prop.multiProcessorCount;
+  impl->nitems = items.size();
+  impl->threads = kFastDefaultThreads;
   impl->slab_stride = impl->use_lds ? 0 : ((max_bytes + 255) & ~255ULL);
+  // the slab holds one working set per workgroup of the largest grid any tuning launches
+  const uint64_t max_grid = std::min<uint64_t>(items.size(), 8ull * impl->num_cus);
 
   const uint64_t bar_bytes = (4 * nbar_total + 64) * 4;
   const uint64_t chrom_bytes = (nch * sizeof(FastChrom) + 255) & ~255ULL;
   const uint64_t order_bytes = (items.size() * 4 + 255) & ~255ULL;
-  const uint64_t slab_bytes = impl->slab_stride * impl->grid;
+  const uint64_t slab_bytes = impl->slab_stride * max_grid;
   hipError_t e = hipMalloc(&impl->d_mem, bar_bytes + chrom_bytes + order_bytes + 256 + slab_bytes);
   if (e != hipSuccess) {
     g_fast_err = std::string("fast-mode allocation failed: ") + hipGetErrorString(e);
@@ -1031,21 +1144,50 @@ int fast_job_create(hipStream_t stream, const mgpu_params& p, const std::vector<
 }
 
 hipError_t fast_job_run(hipStream_t stream, FastJob* job, const WorkItem* d_items,
-                        uint64_t* d_cell_stats) {
+                        uint64_t* d_cell_stats, int timers) {
   auto* impl = static_cast<FastImpl*>(job->impl);
   if (!impl) return hipErrorInvalidValue;
   hipError_t e = hipMemsetAsync(impl->d_next, 0, 8, stream);
   if (e != hipSuccess) return e;
-  const size_t shmem = impl->use_lds ? impl->lds_bytes : 0;
-  if (shmem > 0) {
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(fast_cells_kernel),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-    if (e != hipSuccess) return e;
+  impl->P.timers = timers ? 1u : 0u;
+  const uint32_t grid = fast_grid(impl);
+  switch (impl->threads) {
+    case 256: return launch_fast<256>(stream, impl, d_items, d_cell_stats, grid);
+    case 512: return launch_fast<512>(stream, impl, d_items, d_cell_stats, grid);
+    case 1024: return launch_fast<1024>(stream, impl, d_items, d_cell_stats, grid);
+    default: return hipErrorInvalidValue;
   }
-  hipLaunchKernelGGL(fast_cells_kernel, dim3(impl->grid), dim3(kFastThreads), shmem, stream,
-                     impl->P, impl->d_chroms, d_items, impl->d_order, impl->nitems, impl->d_next,
-                     impl->d_slab, impl->slab_stride, impl->use_lds ? 1 : 0, d_cell_stats);
-  return hipGetLastError();
+}
+
+int fast_job_set_threads(FastJob* job, uint32_t threads) {
+  auto* impl = static_cast<FastImpl*>(job->impl);
+  if (!impl) return -1;
+  if (threads == 0) threads = kFastDefaultThreads;
+  if (threads != 256 && threads != 512 && threads != 1024) {
+    g_fast_err = "threads per cell must be 256, 512 or 1024";
+    return -1;
+  }
+  impl->threads = threads;
+  return 0;
+}
+
+uint32_t fast_job_threads(const FastJob* job) {
+  const auto* impl = static_cast<const FastImpl*>(job->impl);
+  return impl ? impl->threads : 0;
+}
+
+uint32_t fast_job_grid(const FastJob* job) {
+  const auto* impl = static_cast<const FastImpl*>(job->impl);
+  return impl ? fast_grid(impl) : 0;
+}
+
+hipError_t fast_phase_times(uint64_t* out, int n) {
+  unsigned long long h[kFastPhases] = {};
+  hipError_t e = hipMemcpyFromSymbol(h, HIP_SYMBOL(g_fast_phase), sizeof(h), 0, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return e;
+  for (int k = 0; k < n && k < kFastPhases; ++k) out[k] = h[k];
+  const unsigned long long z[kFastPhases] = {};
+  return hipMemcpyToSymbol(HIP_SYMBOL(g_fast_phase), z, sizeof(z), 0, hipMemcpyHostToDevice);
 }
 
 void fast_job_destroy(FastJob* job) {

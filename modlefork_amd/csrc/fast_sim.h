@@ -1,4 +1,4 @@
-// Fast-mode epoch loop (device): one workgroup of kFastThreads lanes per cell.
+// Fast-mode epoch loop (device): one workgroup of 256, 512 or 1024 lanes per cell.
 //
 // Same model as the reference's simulate_one_cell (simulation.cpp:1056-1152), restated for a
 // workgroup: every LEF-, unit- and barrier-level step is data-parallel, draws come from a
@@ -28,8 +28,10 @@
 
 namespace mgpu {
 
-constexpr int kFastThreads = 256;
-constexpr int kFastWaves = kFastThreads / 64;
+constexpr int kFastMaxThreads = 1024;
+constexpr int kFastMaxWaves = kFastMaxThreads / 64;
+constexpr int kFastPhases = 16;  // epoch phases timed when FastParams::timers is set
+constexpr uint32_t kFastDefaultThreads = 512;
 constexpr uint32_t kFreePos = 0xFFFFFFFFu;
 constexpr uint32_t kFastBurninCap = 100000;  // burn-in epochs after which fast mode stops it
 
@@ -111,6 +113,7 @@ struct FastParams {
   uint32_t hist_len, window;
   uint64_t min_burnin_epochs, max_burnin_epochs;
   double target_contact_density;
+  uint32_t timers;  // nonzero: lane 0 accumulates wall-clock ticks per epoch phase
 };
 
 struct FastChrom {

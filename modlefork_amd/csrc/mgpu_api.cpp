@@ -25,6 +25,7 @@ struct mgpu_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
   int num_cus = 0;
+  int phase_timers = 0;
 };
 
 namespace {
@@ -620,7 +621,8 @@ int mgpu_job_run(mgpu_job* job, mgpu_stats* stats) {
                                  job->nitems, job->d_next, job->d_scratch, job->stride,
                                  job->use_lds, job->grid, job->d_cell_stats));
   } else {
-    HIPCHK(ctx, fast_job_run(ctx->stream, &job->fast, job->d_items, job->d_cell_stats));
+    HIPCHK(ctx, fast_job_run(ctx->stream, &job->fast, job->d_items, job->d_cell_stats,
+                             ctx->phase_timers));
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -642,6 +644,27 @@ int mgpu_job_run(mgpu_job* job, mgpu_stats* stats) {
     for (auto m : missed) stats->missed += m;
     stats->kernel_ms = ms;
     stats->total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  }
+  return MGPU_OK;
+}
+
+int mgpu_job_set_tuning(mgpu_job* job, const mgpu_tuning* t) {
+  if (!job || !t) return fail(job ? job->ctx : nullptr, MGPU_EINVAL, "null argument");
+  if (job->mode != MGPU_MODE_FAST) return MGPU_OK;  // deterministic mode: one lane per cell
+  if (fast_job_set_threads(&job->fast, t->threads_per_cell) != 0)
+    return fail(job->ctx, MGPU_EINVAL, "%s", fast_last_error());
+  return MGPU_OK;
+}
+
+int mgpu_job_get_tuning(mgpu_job* job, mgpu_tuning* t) {
+  if (!job || !t) return fail(job ? job->ctx : nullptr, MGPU_EINVAL, "null argument");
+  *t = mgpu_tuning{};
+  if (job->mode == MGPU_MODE_FAST) {
+    t->threads_per_cell = fast_job_threads(&job->fast);
+    t->grid = fast_job_grid(&job->fast);
+  } else {
+    t->threads_per_cell = 64;
+    t->grid = job->grid;
   }
   return MGPU_OK;
 }
@@ -848,6 +871,18 @@ int mgpu_debug_cell_trace(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom*
   if (d) hipFree(d);
   mgpu_job_destroy(job);
   if (e != hipSuccess) return fail(ctx, MGPU_EDEVICE, "trace failed: %s", hipGetErrorString(e));
+  return MGPU_OK;
+}
+
+int mgpu_debug_phase_times(mgpu_ctx* ctx, int enable, uint64_t* out, uint64_t n) {
+  if (!ctx) return fail(ctx, MGPU_EINVAL, "null context");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  ctx->phase_timers = enable ? 1 : 0;
+  if (out && n) {
+    std::vector<uint64_t> v(kFastPhaseCount, 0);
+    HIPCHK(ctx, fast_phase_times(v.data(), kFastPhaseCount));
+    for (uint64_t k = 0; k < n; ++k) out[k] = k < v.size() ? v[k] : 0;
+  }
   return MGPU_OK;
 }
 

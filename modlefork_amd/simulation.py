@@ -131,6 +131,13 @@ class Context:
     def __exit__(self, *exc):
         self.close()
 
+    def phase_times(self, enable: bool = True):
+        """Reads and clears fast mode's per-phase tick totals (100 MHz) and sets the timers."""
+        out = np.zeros(16, np.uint64)
+        self.check(L.lib().mgpu_debug_phase_times(self.ptr, 1 if enable else 0,
+                                                  _p(out, C.c_uint64), 16))
+        return out
+
     def check(self, rc):
         L.check(rc, self.ptr)
 
@@ -160,6 +167,16 @@ class Job:
         st = L.MgpuStats()
         self.ctx.check(L.lib().mgpu_job_run(self.ptr, C.byref(st)))
         return st
+
+    def set_tuning(self, threads_per_cell: int = 0):
+        t = L.MgpuTuning()
+        t.threads_per_cell = threads_per_cell
+        self.ctx.check(L.lib().mgpu_job_set_tuning(self.ptr, C.byref(t)))
+
+    def tuning(self) -> L.MgpuTuning:
+        t = L.MgpuTuning()
+        self.ctx.check(L.lib().mgpu_job_get_tuning(self.ptr, C.byref(t)))
+        return t
 
     def band(self, c: int):
         p = self.plans[c]
@@ -202,9 +219,11 @@ class Simulation:
             names.append(name)
         return names, chroms, bars, plans
 
-    def run_simulate(self, workload, mode: int = L.MODE_DETERMINISTIC):
+    def run_simulate(self, workload, mode: int = L.MODE_DETERMINISTIC, threads_per_cell: int = 0):
         names, chroms, bars, plans = self.prepare(workload)
         job = Job(self.ctx, self.params, chroms, bars, plans, mode)
+        if threads_per_cell:
+            job.set_tuning(threads_per_cell)
         stats = job.run()
         out = {}
         for i, name in enumerate(names):

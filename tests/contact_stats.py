@@ -70,3 +70,35 @@ def barrier_stripe_enrichment(band, nrows, ncols, barrier_bins, dirs, reach=20):
     ref = np.nanmean([stripe(b, d) for b, d in
                       zip(rng.integers(reach, ncols - reach, len(barrier_bins)), dirs)])
     return float(s / ref)
+
+
+def decay_groups(band, nrows, ncols, min_contacts_ref=None, nbins=12):
+    """P(s) over log-spaced diagonal groups; groups past the last one holding at least
+    `min_contacts_ref` contacts of the reference band (an array of per-group contact sums) are
+    pooled into one tail group. Returns (mean contacts per pixel per group, contacts per group)."""
+    m = as_diag_major(band, nrows, ncols)
+    edges = np.unique(np.geomspace(1, nrows, nbins + 1).astype(int))
+    groups = list(zip(edges[:-1], edges[1:]))
+    if min_contacts_ref is not None:
+        keep = [g for g, n in zip(groups, min_contacts_ref) if n]
+        groups = keep + [(keep[-1][1], nrows)] if keep and keep[-1][1] < nrows else keep
+    sums = np.array([m[a:b].sum() for a, b in groups])
+    npix = np.array([sum(max(ncols - d, 0) for d in range(a, b)) for a, b in groups], float)
+    return sums / npix, sums, groups
+
+
+def barrier_dot_enrichment(band, nrows, ncols, barrier_bins, dirs, max_dist=40):
+    """Mean observed/expected contacts at convergent barrier pairs: a REV-blocking barrier (dir 1)
+    at bin i and a FWD-blocking one (dir 2) at bin j, 2 <= j - i < max_dist, where the expected
+    value is the mean of diagonal j - i."""
+    m = as_diag_major(band, nrows, ncols)
+    p = decay(band, nrows, ncols)
+    rev = sorted({int(b) for b, d in zip(barrier_bins, dirs) if d == 1})
+    fwd = sorted({int(b) for b, d in zip(barrier_bins, dirs) if d == 2})
+    ratios = []
+    for i in rev:
+        for j in fwd:
+            d = j - i
+            if 2 <= d < min(nrows, max_dist) and j < ncols and p[d] > 0:
+                ratios.append(m[d, j] / p[d])
+    return float(np.mean(ratios)) if ratios else float("nan")

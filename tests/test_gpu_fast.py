@@ -49,9 +49,9 @@ def test_philox_known_answers(ctx):
         assert tuple(int(x) for x in row) == want
 
 
-def run_device(ctx, wl, mode):
+def run_device(ctx, wl, mode, threads=0):
     sim = S.Simulation(wl.options, ctx)
-    out, stats = sim.run_simulate(wl, mode)
+    out, stats = sim.run_simulate(wl, mode, threads_per_cell=threads)
     return out, stats
 
 
@@ -88,7 +88,24 @@ def test_fast_mode_is_deterministic_and_conserves_contacts(ctx):
     assert sa.contacts == int(plan.targets.sum())
 
 
+@pytest.mark.parametrize("threads", [512, 1024])
+def test_fast_mode_band_does_not_depend_on_workgroup_size(ctx, threads):
+    """Every draw is keyed by (element, epoch, stream) and every reduction is exact, so the
+    workgroup size per cell changes nothing in the result."""
+    wl = W.small(ncells=8)
+    a, sa = run_device(ctx, wl, L.MODE_FAST, 256)
+    b, sb = run_device(ctx, wl, L.MODE_FAST, threads)
+    say(f"fast small/8 at 256 threads {sa.kernel_ms:.1f} ms, at {threads} {sb.kernel_ms:.1f} ms")
+    assert np.array_equal(a["chrS"][0], b["chrS"][0]) and a["chrS"][1] == b["chrS"][1]
+    assert (sa.epochs, sa.burnin_epochs, sa.contacts) == (sb.epochs, sb.burnin_epochs, sb.contacts)
+
+
 def test_fast_mode_statistics_match_oracle(ctx):
+    """128 cells of the 20 Mbp workload, fast mode vs the oracle. Tolerances are set from the
+    oracle's own spread between two disjoint sets of 64 cells (same targets, other seeds):
+    P(s) groups holding >= 5000 contacts differ by < 4 %, the pooled far tail (~650 contacts)
+    by ~8 %, block correlation is 0.998, stripe enrichment differs by ~1 %, dot enrichment by
+    ~14 % and mean epochs per cell by ~10 % (per-cell std ~200 epochs)."""
     wl = W.small(ncells=128)
     say("oracle: 128 cells")
     o = run_oracle(wl)["chrS"]
@@ -98,34 +115,51 @@ def test_fast_mode_statistics_match_oracle(ctx):
     fband, fmissed, plan = d["chrS"]
     tot = int(plan.targets.sum())
     assert int(fband.sum()) + fmissed == tot == int(oband.sum()) + omissed
+    failures = []
 
-    # contact decay P(s) in log-spaced diagonal groups: within 10 % in every group
-    po, edges = CS.log_binned_decay(oband, nrows, ncols)
-    pf, _ = CS.log_binned_decay(fband, nrows, ncols)
+    # contact decay P(s) in log-spaced diagonal groups: within 10 % where the oracle holds at
+    # least 5000 contacts, groups beyond pooled into one tail group within 35 %
+    _, osums, _ = CS.decay_groups(oband, nrows, ncols)
+    po, osums, groups = CS.decay_groups(oband, nrows, ncols, osums >= 5000)
+    pf, fsums, _ = CS.decay_groups(fband, nrows, ncols, osums >= 5000)
     rel = np.abs(pf - po) / po
-    say(f"P(s) relative differences per group: {np.round(rel, 3).tolist()}")
-    assert rel.max() < 0.10
+    tol = np.full(len(groups), 0.10)
+    tol[-1] = 0.35 if osums[-1] < 5000 else 0.10
+    say(f"P(s) groups {[(int(a), int(b)) for a, b in groups]}")
+    say(f"oracle contacts per group {osums.astype(int).tolist()}")
+    say(f"P(s) relative differences {np.round(rel, 3).tolist()}")
+    if (rel > tol).any():
+        failures.append(f"P(s) {np.round(rel, 3).tolist()} over {tol.tolist()}")
 
-    # coarse-grained matrices (10 x 10 bins): Pearson correlation above 0.9
+    # coarse-grained matrices (10 x 10 bins): Pearson correlation above 0.98
     r = CS.pearson(CS.block_sums(fband, nrows, ncols), CS.block_sums(oband, nrows, ncols))
     say(f"block-sum correlation {r:.4f}")
-    assert r > 0.9
+    if r <= 0.98:
+        failures.append(f"block correlation {r}")
 
-    # barrier-anchored stripes: enrichment within 15 % of the oracle's
+    # barrier-anchored stripes within 10 %, convergent-pair dots within 30 % of the oracle's
     bins = (obars.pos // 5000).astype(int)
     eo = CS.barrier_stripe_enrichment(oband, nrows, ncols, bins, obars.dir)
     ef = CS.barrier_stripe_enrichment(fband, nrows, ncols, bins, obars.dir)
     say(f"stripe enrichment oracle {eo:.3f} fast {ef:.3f}")
-    assert abs(ef - eo) / eo < 0.15
+    if abs(ef - eo) / eo >= 0.10:
+        failures.append(f"stripe {ef} vs {eo}")
+    do = CS.barrier_dot_enrichment(oband, nrows, ncols, bins, obars.dir)
+    df = CS.barrier_dot_enrichment(fband, nrows, ncols, bins, obars.dir)
+    say(f"dot enrichment oracle {do:.3f} fast {df:.3f}")
+    if abs(df - do) / do >= 0.30:
+        failures.append(f"dot {df} vs {do}")
 
-    # epochs: mean burn-in and mean total epochs per cell within 10 %
-    ob = ost[:, 1].mean()
-    oe = ost[:, 0].mean()
-    fb = st.burnin_epochs / st.cells
-    fe = st.epochs / st.cells
-    say(f"burn-in epochs oracle {ob:.1f} fast {fb:.1f}; total oracle {oe:.1f} fast {fe:.1f}")
-    assert abs(fb - ob) / ob < 0.10
-    assert abs(fe - oe) / oe < 0.10
+    # epochs: mean burn-in and mean total epochs per cell within max(5 %, 4 standard errors)
+    n = len(ost)
+    for label, col, fval in (("burn-in", 1, st.burnin_epochs / st.cells),
+                             ("total", 0, st.epochs / st.cells)):
+        x = ost[:, col].astype(np.float64)
+        allowed = max(0.05 * x.mean(), 4.0 * x.std() * np.sqrt(2.0 / n))
+        say(f"{label} epochs per cell oracle {x.mean():.1f} fast {fval:.1f} (allowed {allowed:.1f})")
+        if abs(fval - x.mean()) > allowed:
+            failures.append(f"{label} epochs {fval} vs {x.mean()}")
+    assert not failures, failures
 
 
 def test_fast_mode_genome_slice_runs(ctx):
