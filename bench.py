@@ -117,6 +117,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--threads", type=int, default=0,
+                    help="fast mode: workgroup size per cell (256, 512 or 1024; 0 = default)")
+    ap.add_argument("--phases", action="store_true",
+                    help="fast mode: after the timed steps, run one more step with the epoch-phase "
+                         "timers on and print each phase's share of the time to stderr")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (reference for HBM counters)")
     args = ap.parse_args()
@@ -151,6 +156,9 @@ def main():
     log(f"workload {wl.name}: {len(chroms)} chromosomes, "
         f"{sum(len(p.targets) for p in plans)} cells, mode {args.mode}")
     job = S.Job(sim.ctx, sim.params, chroms, bars, plans, mode)
+    if args.threads:
+        job.set_tuning(args.threads)
+    tuning = job.tuning()
 
     for i in range(args.warmup):
         st = job.run()
@@ -163,6 +171,14 @@ def main():
         kernel_ms.append(stats.kernel_ms)
         log(f"step {i}: kernel {stats.kernel_ms:.1f} ms, {stats.epochs} epochs")
     wall = time.perf_counter() - t0
+    if args.phases and mode == L.MODE_FAST:
+        sim.ctx.phase_times(True)
+        job.run()
+        ph = sim.ctx.phase_times(False).astype(np.float64)
+        log(f"epoch phases (share of cell time, {int(tuning.threads_per_cell)} threads per cell):")
+        for k, v in enumerate(ph):
+            if v:
+                log(f"  {L.PHASE_NAMES[k]:28s} {100.0 * v / ph.sum():5.1f} %")
     ms_per_step = 1000.0 * wall / args.steps
     cells = int(stats.cells)
     contacts = int(stats.contacts)
@@ -177,7 +193,10 @@ def main():
     tfile = os.path.join(ROOT, "profiles", f"traffic_{args.mode}_{args.workload}.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+            prof = json.load(fh)
+        # counters were taken for one launch shape; other shapes report no traffic figure
+        if prof.get("threads_per_cell", 0) in (0, int(tuning.threads_per_cell)):
+            traffic = prof.get("hbm_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu:
@@ -202,7 +221,8 @@ def main():
                    "mode": "deterministic" if mode == L.MODE_DETERMINISTIC else "fast",
                    "bin_size": int(sim.params.bin_size),
                    "diagonal_width": int(sim.params.diagonal_width),
-                   "lefs_per_mbp": float(sim.params.lefs_per_mbp), "parallelism": "none"},
+                   "lefs_per_mbp": float(sim.params.lefs_per_mbp), "parallelism": "none",
+                   "threads_per_cell": int(tuning.threads_per_cell), "grid": int(tuning.grid)},
         "contacts_per_sec": contacts * args.steps / wall,
         "epochs_per_sec": epochs * args.steps / wall,
         "kernel_ms": float(np.mean(kernel_ms)),
