@@ -190,6 +190,16 @@ int mgpu_job_create(mgpu_ctx* ctx, const mgpu_params* p, uint64_t nchroms,
                     const mgpu_chrom* chroms, const mgpu_barriers* barriers, const uint64_t* nlefs,
                     const uint64_t* ncells, const uint64_t* const* target_contacts,
                     const uint64_t* target_epochs, int mode, mgpu_job** out);
+/* The same with one parameter block per chromosome (params has nchroms entries), for parameter
+ * sweeps and training windows whose chromosomes differ in LEF density and processivity. The
+ * blocks may differ only in lefs_per_mbp, prob_release and prob_release_burnin; in deterministic
+ * mode they must be identical. No counterpart in the reference, which runs one Config per
+ * process (each block is what Cli::transform_args, cli.cpp:1192-1215, makes of one Config). */
+int mgpu_job_create_multi(mgpu_ctx* ctx, const mgpu_params* params, uint64_t nchroms,
+                          const mgpu_chrom* chroms, const mgpu_barriers* barriers,
+                          const uint64_t* nlefs, const uint64_t* ncells,
+                          const uint64_t* const* target_contacts, const uint64_t* target_epochs,
+                          int mode, mgpu_job** out);
 /* Zeroes the bands and simulates every cell of the job. */
 int mgpu_job_run(mgpu_job* job, mgpu_stats* stats);
 /* Copies chromosome c's band to host memory (nrows*ncols + 1 u32) and its missed count. */
@@ -199,7 +209,8 @@ int mgpu_job_band_device_ptr(mgpu_job* job, uint64_t c, void** dptr, uint64_t* n
 void mgpu_job_destroy(mgpu_job* job);
 
 /* ---- launch tuning (no counterpart in the reference: it sizes the device launch) ----
- * threads_per_cell: fast mode's workgroup size per cell, 256, 512 or 1024 (0 = default 512).
+ * threads_per_cell: fast mode's workgroup size per cell, 64, 128, 256, 512 or 1024 (0 = about
+ * four LEFs per lane, at most 512).
  * Bands, missed counts and statistics do not depend on the tuning. grid (get only) is the number
  * of workgroups the next run launches. */
 typedef struct mgpu_tuning {
@@ -209,6 +220,10 @@ typedef struct mgpu_tuning {
 } mgpu_tuning;
 int mgpu_job_set_tuning(mgpu_job* job, const mgpu_tuning* t);
 int mgpu_job_get_tuning(mgpu_job* job, mgpu_tuning* t);
+/* Times each fast-mode workgroup size (64 to 1024 lanes) on sample_cells cells spread over the
+ * job (0: all), keeps the fastest for later runs and reports it in *chosen (may be null); ms (5
+ * entries, may be null) receives the times. Deterministic mode: no-op. */
+int mgpu_job_autotune(mgpu_job* job, uint64_t sample_cells, mgpu_tuning* chosen, float* ms);
 
 /* ---- step-level entry points (the reference's ENABLE_TESTING hooks, simulation.hpp:499-651),
  * run by the device step code on one LEF state ----
@@ -247,6 +262,68 @@ int mgpu_debug_cell_trace(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom*
                           const mgpu_barriers* barriers, uint64_t nlefs, uint64_t cell_id,
                           uint64_t target_contacts, uint64_t target_epochs, uint64_t max_trace,
                           uint64_t* trace, uint64_t* out_epochs);
+
+/* HIP stream (hipStream_t) and device of a context, for code that enqueues its own work in
+ * order with the library's. */
+int mgpu_context_stream(mgpu_ctx* ctx, void** stream, int* device);
+
+/* ---- learned surrogate (no counterpart in the reference) ----
+ * A network that stands in for the simulation in parameter sweeps: it maps the barrier track of
+ * a window (plus LEF density and processivity, broadcast as input channels) to the log1p contact
+ * band of the window's central output bins. BF16 weights and activations on MFMA with FP32
+ * accumulation; fp32 master weights updated by AdamW. Input x: [n][in_len][in_ch] fp32; output
+ * and targets: [n][ndiag][out_len] fp32, entry [d][i] = pixel (i, i + d) of the output window
+ * (i + d >= out_len is outside the window: predicted but not trained). Parameters travel in a
+ * canonical fp32 order: per layer weights [cout][taps][cin] then biases, for the encoder layers,
+ * the two 1 x 1 projections (with a bias on the first), the [ndiag][head_ch] diagonal embedding,
+ * the head layers and the 1-channel output layer. */
+typedef struct mgpu_sg_config {
+  uint32_t in_len;      /* input bins: output window plus context on both sides */
+  uint32_t out_offset;  /* first output bin within the input */
+  uint32_t out_len;     /* output bins */
+  uint32_t ndiag;       /* predicted diagonals (band width in bins) */
+  uint32_t in_ch;       /* input channels per bin, at most 32 */
+  uint32_t enc_ch;      /* encoder channels: 64 or 128 */
+  uint32_t enc_layers;  /* 1D conv layers in the encoder */
+  uint32_t enc_k;       /* encoder kernel width (odd, at most 9) */
+  uint32_t head_ch;     /* pairwise / head channels: 64 */
+  uint32_t head_layers; /* 3 x 3 conv layers of the head, the 1-channel output layer included */
+  uint32_t max_batch;   /* samples per device batch */
+  uint32_t reserved;
+  uint64_t seed;        /* weight initialisation */
+} mgpu_sg_config;
+typedef struct mgpu_sg mgpu_sg;
+
+/* 600 input bins (2 Mbp window of 400 5-kb bins with 500 kb of context each side), 100
+ * diagonals, 5 input channels, 8 x 128-channel encoder (k = 5), 4-layer 64-channel head. */
+int mgpu_sg_config_default(mgpu_sg_config* c);
+int mgpu_sg_create(mgpu_ctx* ctx, const mgpu_sg_config* c, mgpu_sg** out);
+void mgpu_sg_destroy(mgpu_sg* s);
+const char* mgpu_sg_last_error(const mgpu_sg* s);
+uint64_t mgpu_sg_num_params(const mgpu_sg* s);
+int mgpu_sg_get_params(mgpu_sg* s, float* out);
+/* Replaces the weights and resets the optimiser state. */
+int mgpu_sg_set_params(mgpu_sg* s, const float* in);
+/* Predictions for n samples (any n; batched by max_batch). */
+int mgpu_sg_forward(mgpu_sg* s, const float* x, uint64_t n, float* y);
+/* Loss (mean squared error over valid pixels) and its gradient for n <= max_batch samples;
+ * grads (may be null) receives num_params values in the canonical order. */
+int mgpu_sg_gradients(mgpu_sg* s, const float* x, const float* y, uint64_t n, float* loss,
+                      float* grads);
+/* One AdamW step (beta 0.9 / 0.999, eps 1e-8, decoupled weight decay) with the given gradients. */
+int mgpu_sg_adamw_step(mgpu_sg* s, const float* grads, float lr, float weight_decay);
+/* Uploads a training set (n samples) to device memory. */
+int mgpu_sg_set_data(mgpu_sg* s, const float* x, const float* y, uint64_t n);
+/* One training step on samples idx[0, n) of the training set (n <= max_batch): forward,
+ * backward and AdamW on the device. *loss receives the batch loss before the update, *ms (may be
+ * null) the device time of the step. */
+int mgpu_sg_train_step(mgpu_sg* s, const uint32_t* idx, uint64_t n, float lr, float weight_decay,
+                       float* loss, float* ms);
+/* Device time of `repeats` forward passes over the last uploaded batch of n samples. */
+int mgpu_sg_predict_timed(mgpu_sg* s, uint64_t n, uint64_t repeats, float* ms);
+/* One v_mfma_f32_32x32x16_bf16 of a (32 x 16) and b (16 x 32), bf16 row-major, into d (32 x 32
+ * fp32 row-major) through the lane layout the surrogate kernels assume. */
+int mgpu_debug_mfma(mgpu_ctx* ctx, const uint16_t* a, const uint16_t* b, float* d);
 
 #ifdef __cplusplus
 }

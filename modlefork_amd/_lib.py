@@ -159,6 +159,14 @@ class MgpuTuning(C.Structure):
     ]
 
 
+class MgpuSgConfig(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in (
+        "in_len", "out_offset", "out_len", "ndiag", "in_ch", "enc_ch", "enc_layers", "enc_k",
+        "head_ch", "head_layers", "max_batch", "reserved")] + [("seed", C.c_uint64)]
+
+
+FP = C.POINTER(C.c_float)
+
 PHASE_NAMES = ["stop test and burn-in", "bind", "rank rev", "rank fwd", "contacts", "moves",
                "adjust", "barrier states and ends", "LEF-BAR", "primary LEF-LEF",
                "move corrections", "secondary rev", "secondary fwd", "unused",
@@ -186,12 +194,17 @@ SIGNATURES = [
     ("mgpu_job_create", C.c_int, [C.c_void_p, C.POINTER(MgpuParams), C.c_uint64,
                                   C.POINTER(MgpuChrom), C.POINTER(MgpuBarriers), U64P, U64P,
                                   C.POINTER(U64P), U64P, C.c_int, C.POINTER(C.c_void_p)]),
+    ("mgpu_job_create_multi", C.c_int, [C.c_void_p, C.POINTER(MgpuParams), C.c_uint64,
+                                        C.POINTER(MgpuChrom), C.POINTER(MgpuBarriers), U64P, U64P,
+                                        C.POINTER(U64P), U64P, C.c_int, C.POINTER(C.c_void_p)]),
     ("mgpu_job_run", C.c_int, [C.c_void_p, C.POINTER(MgpuStats)]),
     ("mgpu_job_read_band", C.c_int, [C.c_void_p, C.c_uint64, U32P, U64P]),
     ("mgpu_job_band_device_ptr", C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p), U64P]),
     ("mgpu_job_destroy", None, [C.c_void_p]),
     ("mgpu_job_set_tuning", C.c_int, [C.c_void_p, C.POINTER(MgpuTuning)]),
     ("mgpu_job_get_tuning", C.c_int, [C.c_void_p, C.POINTER(MgpuTuning)]),
+    ("mgpu_job_autotune", C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(MgpuTuning),
+                                    C.POINTER(C.c_float)]),
     ("mgpu_debug_step", C.c_int, [C.c_void_p, C.POINTER(MgpuParams), C.POINTER(MgpuChrom), C.c_int,
                                   C.c_uint64, U64P, U64P, U64P, U64P, U64P, U64P, U64P, U64P, U64P,
                                   C.POINTER(MgpuBarriers), U8P, U64P]),
@@ -199,6 +212,21 @@ SIGNATURES = [
                                    C.c_double, C.c_uint64, U64P]),
     ("mgpu_debug_philox", C.c_int, [C.c_void_p, U32P, C.c_uint64, U32P]),
     ("mgpu_debug_phase_times", C.c_int, [C.c_void_p, C.c_int, U64P, C.c_uint64]),
+    ("mgpu_context_stream", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int)]),
+    ("mgpu_sg_config_default", C.c_int, [C.POINTER(MgpuSgConfig)]),
+    ("mgpu_sg_create", C.c_int, [C.c_void_p, C.POINTER(MgpuSgConfig), C.POINTER(C.c_void_p)]),
+    ("mgpu_sg_destroy", None, [C.c_void_p]),
+    ("mgpu_sg_last_error", C.c_char_p, [C.c_void_p]),
+    ("mgpu_sg_num_params", C.c_uint64, [C.c_void_p]),
+    ("mgpu_sg_get_params", C.c_int, [C.c_void_p, FP]),
+    ("mgpu_sg_set_params", C.c_int, [C.c_void_p, FP]),
+    ("mgpu_sg_forward", C.c_int, [C.c_void_p, FP, C.c_uint64, FP]),
+    ("mgpu_sg_gradients", C.c_int, [C.c_void_p, FP, FP, C.c_uint64, FP, FP]),
+    ("mgpu_sg_adamw_step", C.c_int, [C.c_void_p, FP, C.c_float, C.c_float]),
+    ("mgpu_sg_set_data", C.c_int, [C.c_void_p, FP, FP, C.c_uint64]),
+    ("mgpu_sg_train_step", C.c_int, [C.c_void_p, U32P, C.c_uint64, C.c_float, C.c_float, FP, FP]),
+    ("mgpu_sg_predict_timed", C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, FP]),
+    ("mgpu_debug_mfma", C.c_int, [C.c_void_p, C.POINTER(C.c_uint16), C.POINTER(C.c_uint16), FP]),
     ("mgpu_debug_cell_trace", C.c_int, [C.c_void_p, C.POINTER(MgpuParams), C.POINTER(MgpuChrom),
                                         C.POINTER(MgpuBarriers), C.c_uint64, C.c_uint64, C.c_uint64,
                                         C.c_uint64, C.c_uint64, U64P, U64P]),

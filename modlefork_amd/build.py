@@ -27,6 +27,8 @@ UNITS = [
     ("det_kernels.hip", ["-ffp-contract=off"]),
     ("fast_kernels.hip", []),
     ("mgpu_api.cpp", ["-ffp-contract=off"]),
+    ("sg_kernels.hip", []),
+    ("sg_model.cpp", []),
 ]
 
 

@@ -18,14 +18,23 @@ struct FastJob {
   void* impl = nullptr;
 };
 
-// Builds the 32-bit barrier tables and the longest-first work order for a job whose chromosome
-// descriptors (bands, barrier arrays) already live on the device.
-int fast_job_create(hipStream_t stream, const mgpu_params& p, const std::vector<DetChrom>& chroms,
-                    const std::vector<WorkItem>& items, const DetChrom* d_chroms, FastJob* job);
+// Builds the 32-bit barrier tables (from the caller's host arrays, host_bars[c] for chromosome
+// c) and the longest-first work order for a job whose bands already live on the device.
+// release[2c] and release[2c + 1] are chromosome c's release probabilities after and during
+// burn-in.
+int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& chroms,
+                    const mgpu_barriers* host_bars, const std::vector<double>& release,
+                    const std::vector<WorkItem>& items, FastJob* job);
 // timers != 0: accumulate per-phase wall-clock ticks (read with fast_phase_times)
 hipError_t fast_job_run(hipStream_t stream, FastJob* job, const WorkItem* d_items,
                         uint64_t* d_cell_stats, int timers);
-// Workgroup size per cell (256, 512 or 1024; 0 = default); results do not depend on it.
+// Times every workgroup size on `sample` cells spread over the work order (0: all cells), keeps
+// the fastest and returns it in *best; ms (5 entries, may be null) receives the times of 64 to
+// 1024 lanes. The bands are left dirty (runs zero them first).
+int fast_job_autotune(hipStream_t stream, FastJob* job, const WorkItem* d_items,
+                      uint64_t* d_cell_stats, uint64_t sample, uint32_t* best, float* ms);
+// Workgroup size per cell (64 to 1024, a power of two; 0 = about four LEFs per lane, at most
+// 512); results do not depend on it.
 int fast_job_set_threads(FastJob* job, uint32_t threads);
 uint32_t fast_job_threads(const FastJob* job);
 uint32_t fast_job_grid(const FastJob* job);

@@ -88,16 +88,24 @@ constexpr uint32_t kNewCap = 1024;  // LEFs merged into the rank order per batch
 struct Cell {
   uint32_t *rev, *fwd, *rm, *fm, *rc, *fc;
   uint16_t *rr, *fr;
-  uint32_t* scratch;  // max(n u32, kNewCap u64): new-LEF keys, sorted positions, chain words
+  uint32_t* scratch;  // scratch_bytes(n): new-LEF keys, sorted positions, chain words
   uint8_t* bst;
   double *cfx, *avg;  // burn-in history
 };
 
 __device__ __host__ inline uint64_t align16(uint64_t x) { return (x + 15) & ~15ULL; }
 
+// n u32 words, or the u64 keys of one merge batch (at most min(n, kNewCap) keys, padded to a
+// power of two for the bitonic sort), whichever is larger
+__device__ __host__ inline uint64_t scratch_bytes(uint32_t n) {
+  uint64_t m = 1;
+  while (m < (n < kNewCap ? n : kNewCap)) m <<= 1;
+  return 4ull * n > 8ull * m ? 4ull * n : 8ull * m;
+}
+
 __device__ __host__ inline uint64_t fast_cell_bytes(uint32_t n, uint32_t nbar, uint32_t hist) {
   uint64_t b = align16(4ull * n) * 6 + align16(2ull * n) * 2;
-  b += align16(4ull * n > 8ull * kNewCap ? 4ull * n : 8ull * kNewCap);
+  b += align16(scratch_bytes(n));
   b += align16(nbar) + align16(8ull * (hist + 1)) * 2;
   return b;
 }
@@ -117,7 +125,7 @@ __device__ inline void carve(Cell& c, uint8_t* base, uint32_t n, uint32_t nbar, 
   c.fc = (uint32_t*)take(4ull * n);
   c.rr = (uint16_t*)take(2ull * n);
   c.fr = (uint16_t*)take(2ull * n);
-  c.scratch = (uint32_t*)take(4ull * n > 8ull * kNewCap ? 4ull * n : 8ull * kNewCap);
+  c.scratch = (uint32_t*)take(scratch_bytes(n));
   c.bst = take(nbar);
   c.cfx = (double*)take(8ull * (hist + 1));
   c.avg = (double*)take(8ull * (hist + 1));
@@ -866,7 +874,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
 
     // ---- extrude and release (simulation.cpp:489-512, 544-592) ----
     {
-      const double base = bdone ? P.prob_release : P.prob_release_burnin;
+      const double base = bdone ? ch.prob_release : ch.prob_release_burnin;
       for (uint32_t i = threadIdx.x; i < n; i += NT) {
         c.rev[i] -= c.rm[i];
         c.fwd[i] += c.fm[i];
@@ -954,20 +962,22 @@ struct FastImpl {
   uint32_t per_cu = 1;     // cells resident per CU as the LDS footprint allows
   uint32_t num_cus = 1;
   uint32_t threads = 256;  // workgroup size per cell
+  uint64_t max_lefs = 1;
   uint64_t nitems = 0;
   void* d_mem = nullptr;
 };
 
 // Workgroups of the persistent launch: as many cells per CU as LDS and the 2048-lane CU limit
-// allow (at most 8), never more than there are cells.
+// allow (at most 16), never more than there are cells.
 uint32_t fast_grid(const FastImpl* impl) {
-  const uint64_t per_cu = std::min<uint64_t>({impl->per_cu, 2048u / impl->threads, 8u});
+  const uint64_t per_cu = std::min<uint64_t>({impl->per_cu, 2048u / impl->threads, 16u});
   return (uint32_t)std::min<uint64_t>(impl->nitems, std::max<uint64_t>(1, per_cu) * impl->num_cus);
 }
 
 template <int NT, bool kLds>
 hipError_t launch_fast_as(hipStream_t stream, FastImpl* impl, const WorkItem* d_items,
-                          uint64_t* d_cell_stats, uint32_t grid) {
+                          uint64_t* d_cell_stats, uint32_t grid, const uint32_t* order,
+                          uint64_t nitems) {
   const size_t shmem = kLds ? impl->lds_bytes : 0;
   if (shmem > 0) {
     const hipError_t e =
@@ -976,27 +986,46 @@ hipError_t launch_fast_as(hipStream_t stream, FastImpl* impl, const WorkItem* d_
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL((fast_cells_kernel<NT, kLds>), dim3(grid), dim3(NT), shmem, stream, impl->P,
-                     impl->d_chroms, d_items, impl->d_order, impl->nitems, impl->d_next,
-                     impl->d_slab, impl->slab_stride, d_cell_stats);
+                     impl->d_chroms, d_items, order, nitems, impl->d_next, impl->d_slab,
+                     impl->slab_stride, d_cell_stats);
   return hipGetLastError();
 }
 
 template <int NT>
 hipError_t launch_fast(hipStream_t stream, FastImpl* impl, const WorkItem* d_items,
-                       uint64_t* d_cell_stats, uint32_t grid) {
-  return impl->use_lds ? launch_fast_as<NT, true>(stream, impl, d_items, d_cell_stats, grid)
-                       : launch_fast_as<NT, false>(stream, impl, d_items, d_cell_stats, grid);
+                       uint64_t* d_cell_stats, uint32_t grid, const uint32_t* order,
+                       uint64_t nitems) {
+  return impl->use_lds
+             ? launch_fast_as<NT, true>(stream, impl, d_items, d_cell_stats, grid, order, nitems)
+             : launch_fast_as<NT, false>(stream, impl, d_items, d_cell_stats, grid, order, nitems);
+}
+
+// Launches the persistent kernel at the job's current workgroup size over order[0, nitems).
+hipError_t launch_with(hipStream_t stream, FastImpl* impl, const WorkItem* d_items,
+                       uint64_t* d_cell_stats, const uint32_t* order, uint64_t nitems) {
+  hipError_t e = hipMemsetAsync(impl->d_next, 0, 8, stream);
+  if (e != hipSuccess) return e;
+  const uint64_t keep = impl->nitems;
+  impl->nitems = nitems;  // the grid never exceeds the cells launched
+  const uint32_t grid = fast_grid(impl);
+  impl->nitems = keep;
+  switch (impl->threads) {
+    case 64: return launch_fast<64>(stream, impl, d_items, d_cell_stats, grid, order, nitems);
+    case 128: return launch_fast<128>(stream, impl, d_items, d_cell_stats, grid, order, nitems);
+    case 256: return launch_fast<256>(stream, impl, d_items, d_cell_stats, grid, order, nitems);
+    case 512: return launch_fast<512>(stream, impl, d_items, d_cell_stats, grid, order, nitems);
+    case 1024: return launch_fast<1024>(stream, impl, d_items, d_cell_stats, grid, order, nitems);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 uint64_t fast_cell_bytes_host(uint32_t n, uint32_t nbar, uint32_t hist) {
   return fast_cell_bytes(n, nbar, hist);
 }
 
-int fast_job_create(hipStream_t stream, const mgpu_params& p, const std::vector<DetChrom>& dchroms,
-                    const std::vector<WorkItem>& items, const DetChrom* d_det_chroms,
-                    FastJob* job) {
-  (void)d_det_chroms;
-  (void)stream;
+int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& dchroms,
+                    const mgpu_barriers* host_bars, const std::vector<double>& release,
+                    const std::vector<WorkItem>& items, FastJob* job) {
   auto* impl = new FastImpl();
   FastParams& P = impl->P;
   P.bin_size = (uint32_t)p.bin_size;
@@ -1009,8 +1038,6 @@ int fast_job_create(hipStream_t stream, const mgpu_params& p, const std::vector<
   P.t_bypass_keep = thresh(1.0 - p.p_bypass);
   P.t_pblock_major = thresh(p.pblock_major);
   P.t_pblock_minor = thresh(p.pblock_minor);
-  P.prob_release = p.prob_release;
-  P.prob_release_burnin = p.prob_release_burnin;
   P.inv_soft = 1.0 / p.soft_stall_mult;
   P.inv_hard = 1.0 / p.hard_stall_mult;
   P.t_loop = std::isinf(p.tad_to_loop_ratio) ? 0 : thresh(1.0 / (p.tad_to_loop_ratio + 1.0));
@@ -1031,8 +1058,9 @@ int fast_job_create(hipStream_t stream, const mgpu_params& p, const std::vector<
   }
 
   const uint64_t nch = dchroms.size();
-  uint64_t max_bytes = 0, nbar_total = 0;
+  uint64_t max_bytes = 0, nbar_total = 0, max_lefs = 1;
   for (const auto& d : dchroms) {
+    max_lefs = std::max<uint64_t>(max_lefs, d.nlefs);
     if (d.nlefs > 65535) {
       g_fast_err = "fast mode supports up to 65535 LEFs per chromosome";
       delete impl;
@@ -1055,10 +1083,13 @@ int fast_job_create(hipStream_t stream, const mgpu_params& p, const std::vector<
                      : 2u;
   impl->num_cus = (uint32_t)prop.multiProcessorCount;
   impl->nitems = items.size();
-  impl->threads = kFastDefaultThreads;
+  // about four units per lane, between one wave and kFastDefaultThreads lanes
+  impl->max_lefs = max_lefs;
+  impl->threads = 64;
+  while (impl->threads < kFastDefaultThreads && impl->threads * 4 < max_lefs) impl->threads *= 2;
   impl->slab_stride = impl->use_lds ? 0 : ((max_bytes + 255) & ~255ULL);
   // the slab holds one working set per workgroup of the largest grid any tuning launches
-  const uint64_t max_grid = std::min<uint64_t>(items.size(), 8ull * impl->num_cus);
+  const uint64_t max_grid = std::min<uint64_t>(items.size(), 16ull * impl->num_cus);
 
   const uint64_t bar_bytes = (4 * nbar_total + 64) * 4;
   const uint64_t chrom_bytes = (nch * sizeof(FastChrom) + 255) & ~255ULL;
@@ -1080,15 +1111,11 @@ int fast_job_create(hipStream_t stream, const mgpu_params& p, const std::vector<
   // 32-bit barrier tables: positions and the transition / occupancy thresholds
   std::vector<uint32_t> hb(4 * nbar_total + 64, 0);
   uint64_t off = 0;
-  for (uint64_t c = 0; c < nch && e == hipSuccess; ++c) {
+  for (uint64_t c = 0; c < nch; ++c) {
     const DetChrom& d = dchroms[c];
-    std::vector<uint64_t> pos(d.nbar);
-    std::vector<double> sa(d.nbar), si(d.nbar);
-    if (d.nbar) {
-      e = hipMemcpy(pos.data(), d.bar_pos, d.nbar * 8, hipMemcpyDeviceToHost);
-      if (e == hipSuccess) e = hipMemcpy(sa.data(), d.bar_stp_active, d.nbar * 8, hipMemcpyDeviceToHost);
-      if (e == hipSuccess) e = hipMemcpy(si.data(), d.bar_stp_inactive, d.nbar * 8, hipMemcpyDeviceToHost);
-    }
+    const uint64_t* pos = host_bars[c].pos;
+    const double* sa = host_bars[c].stp_active;
+    const double* si = host_bars[c].stp_inactive;
     FastChrom f{};
     f.start = (uint32_t)d.start;
     f.end = (uint32_t)d.end;
@@ -1114,11 +1141,13 @@ int fast_job_create(hipStream_t stream, const mgpu_params& p, const std::vector<
     f.cpe = (uint32_t)d.cpe;
     f.target_epochs = d.target_epochs;
     f.bind_rate = d.bind_rate;
+    f.prob_release = release[2 * c];
+    f.prob_release_burnin = release[2 * c + 1];
     f.lds_bytes = fast_cell_bytes(f.nlefs, f.nbar, P.hist_len);
     impl->chroms.push_back(f);
     off += 4 * d.nbar;
   }
-  if (e == hipSuccess) e = hipMemcpy(bars, hb.data(), hb.size() * 4, hipMemcpyHostToDevice);
+  e = hipMemcpy(bars, hb.data(), hb.size() * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(impl->d_chroms, impl->chroms.data(), nch * sizeof(FastChrom),
                   hipMemcpyHostToDevice);
@@ -1147,24 +1176,66 @@ hipError_t fast_job_run(hipStream_t stream, FastJob* job, const WorkItem* d_item
                         uint64_t* d_cell_stats, int timers) {
   auto* impl = static_cast<FastImpl*>(job->impl);
   if (!impl) return hipErrorInvalidValue;
-  hipError_t e = hipMemsetAsync(impl->d_next, 0, 8, stream);
-  if (e != hipSuccess) return e;
   impl->P.timers = timers ? 1u : 0u;
-  const uint32_t grid = fast_grid(impl);
-  switch (impl->threads) {
-    case 256: return launch_fast<256>(stream, impl, d_items, d_cell_stats, grid);
-    case 512: return launch_fast<512>(stream, impl, d_items, d_cell_stats, grid);
-    case 1024: return launch_fast<1024>(stream, impl, d_items, d_cell_stats, grid);
-    default: return hipErrorInvalidValue;
+  return launch_with(stream, impl, d_items, d_cell_stats, impl->d_order, impl->nitems);
+}
+
+int fast_job_autotune(hipStream_t stream, FastJob* job, const WorkItem* d_items,
+                      uint64_t* d_cell_stats, uint64_t sample, uint32_t* best, float* ms_out) {
+  auto* impl = static_cast<FastImpl*>(job->impl);
+  if (!impl || !best) return -1;
+  if (sample == 0 || sample > impl->nitems) sample = impl->nitems;
+  // every k-th cell of the longest-first order, so the sample spans the chromosome sizes
+  std::vector<uint32_t> all(impl->nitems), pick;
+  hipError_t e = hipMemcpy(all.data(), impl->d_order, all.size() * 4, hipMemcpyDeviceToHost);
+  const uint64_t stride = impl->nitems / sample;
+  for (uint64_t k = 0; k < sample; ++k) pick.push_back(all[k * stride]);
+  uint32_t* d_pick = nullptr;
+  if (e == hipSuccess) e = hipMalloc(&d_pick, pick.size() * 4);
+  if (e == hipSuccess) e = hipMemcpy(d_pick, pick.data(), pick.size() * 4, hipMemcpyHostToDevice);
+  hipEvent_t a = nullptr, b = nullptr;
+  if (e == hipSuccess) e = hipEventCreate(&a);
+  if (e == hipSuccess) e = hipEventCreate(&b);
+  const uint32_t keep = impl->threads;
+  const uint32_t cand[] = {64, 128, 256, 512, 1024};
+  float best_ms = 0.0f;
+  *best = keep;
+  impl->P.timers = 0;
+  for (int k = 0; k < 5 && e == hipSuccess; ++k) {
+    impl->threads = cand[k];
+    e = hipEventRecord(a, stream);
+    if (e == hipSuccess) e = launch_with(stream, impl, d_items, d_cell_stats, d_pick, pick.size());
+    if (e == hipSuccess) e = hipEventRecord(b, stream);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    if (ms_out) ms_out[k] = ms;
+    if (e == hipSuccess && (k == 0 || ms < best_ms)) {
+      best_ms = ms;
+      *best = cand[k];
+    }
   }
+  impl->threads = e == hipSuccess ? *best : keep;
+  if (a) hipEventDestroy(a);
+  if (b) hipEventDestroy(b);
+  if (d_pick) hipFree(d_pick);
+  if (e != hipSuccess) {
+    g_fast_err = std::string("autotune failed: ") + hipGetErrorString(e);
+    return -1;
+  }
+  return 0;
 }
 
 int fast_job_set_threads(FastJob* job, uint32_t threads) {
   auto* impl = static_cast<FastImpl*>(job->impl);
   if (!impl) return -1;
-  if (threads == 0) threads = kFastDefaultThreads;
-  if (threads != 256 && threads != 512 && threads != 1024) {
-    g_fast_err = "threads per cell must be 256, 512 or 1024";
+  if (threads == 0) {
+    uint32_t t = 64;
+    while (t < kFastDefaultThreads && t * 4 < impl->max_lefs) t *= 2;
+    threads = t;
+  }
+  if (threads != 64 && threads != 128 && threads != 256 && threads != 512 && threads != 1024) {
+    g_fast_err = "threads per cell must be 64, 128, 256, 512 or 1024";
     return -1;
   }
   impl->threads = threads;

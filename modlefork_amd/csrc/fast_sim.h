@@ -1,4 +1,4 @@
-// Fast-mode epoch loop (device): one workgroup of 256, 512 or 1024 lanes per cell.
+// Fast-mode epoch loop (device): one workgroup of 64 to 1024 lanes per cell.
 //
 // Same model as the reference's simulate_one_cell (simulation.cpp:1056-1152), restated for a
 // workgroup: every LEF-, unit- and barrier-level step is data-parallel, draws come from a
@@ -105,7 +105,6 @@ struct FastParams {
   float rev_std, fwd_std;
   uint64_t t_bypass_keep;    // threshold of (1 - p_bypass): collision happens
   uint64_t t_pblock_major, t_pblock_minor;
-  double prob_release, prob_release_burnin;
   double inv_soft, inv_hard;
   uint64_t t_loop;           // threshold of 1 / (ratio + 1): a contact is a loop contact
   float gev_mu, gev_sigma, gev_xi;
@@ -130,6 +129,7 @@ struct FastChrom {
   uint32_t cpe;
   uint64_t target_epochs;
   double bind_rate;
+  double prob_release, prob_release_burnin;  // per chromosome: LEF processivity may vary
   uint64_t lds_bytes;  // working-set bytes of one cell
 };
 

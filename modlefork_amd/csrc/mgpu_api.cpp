@@ -433,19 +433,44 @@ struct PoolBuilder {
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int mgpu_job_create(mgpu_ctx* ctx, const mgpu_params* p, uint64_t nchroms, const mgpu_chrom* chroms,
-                    const mgpu_barriers* barriers, const uint64_t* nlefs, const uint64_t* ncells,
-                    const uint64_t* const* target_contacts, const uint64_t* target_epochs, int mode,
-                    mgpu_job** out) {
-  if (!ctx || !out) return fail(ctx, MGPU_EINVAL, "null argument");
-  if (int rc = check_params(p)) return fail(ctx, rc, "%s", g_err.c_str());
+// Parameter blocks of one multi-parameter job may differ only in the fields that describe LEF
+// density and processivity; everything the kernels read job-wide must match the first block.
+bool same_job_wide(const mgpu_params& a, const mgpu_params& b) {
+  mgpu_params x = a, y = b;
+  x.prob_release = y.prob_release = 0;
+  x.prob_release_burnin = y.prob_release_burnin = 0;
+  x.lefs_per_mbp = y.lefs_per_mbp = 0;
+  return std::memcmp(&x, &y, sizeof(mgpu_params)) == 0;
+}
+
+// params has nchroms entries when per_chrom, else one.
+int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_t nchroms,
+               const mgpu_chrom* chroms, const mgpu_barriers* barriers, const uint64_t* nlefs,
+               const uint64_t* ncells, const uint64_t* const* target_contacts,
+               const uint64_t* target_epochs, int mode, mgpu_job** out) {
+  if (!ctx || !out || !params) return fail(ctx, MGPU_EINVAL, "null argument");
   if (nchroms == 0 || !chroms || !barriers || !nlefs || !ncells || !target_contacts ||
       !target_epochs)
     return fail(ctx, MGPU_EINVAL, "empty or null chromosome arrays");
   if (mode != MGPU_MODE_DETERMINISTIC && mode != MGPU_MODE_FAST)
     return fail(ctx, MGPU_EINVAL, "unknown mode %d", mode);
+  const mgpu_params* p = params;  // job-wide fields
+  for (uint64_t c = 0; c < (per_chrom ? nchroms : 1); ++c) {
+    if (int rc = check_params(&params[c]))
+      return fail(ctx, rc, "parameters of chromosome %llu: %s", (unsigned long long)c,
+                  g_err.c_str());
+    if (per_chrom && c > 0) {
+      if (mode == MGPU_MODE_DETERMINISTIC ? std::memcmp(&params[c], p, sizeof(mgpu_params)) != 0
+                                          : !same_job_wide(params[c], *p))
+        return fail(ctx, MGPU_EINVAL,
+                    "chromosome %llu: parameter blocks of one job may differ only in LEF "
+                    "density and release probabilities (and not at all in deterministic mode)",
+                    (unsigned long long)c);
+    }
+  }
+  auto par = [&](uint64_t c) -> const mgpu_params& { return per_chrom ? params[c] : *p; };
   HIPCHK(ctx, hipSetDevice(ctx->device));
 
   auto* job = new mgpu_job();
@@ -453,14 +478,14 @@ int mgpu_job_create(mgpu_ctx* ctx, const mgpu_params* p, uint64_t nchroms, const
   job->mode = mode;
   job->p = *p;
   job->nchroms = nchroms;
-  const DetParams P = det_params(p);
 
+  // Inputs (barrier arrays, q_n tables, chromosome descriptors, work items) are laid out first
+  // in one pool and uploaded with a single copy; counters, statistics and bands follow.
   PoolBuilder pool;
   std::vector<uint64_t> off_pos(nchroms), off_dir(nchroms), off_sa(nchroms), off_si(nchroms),
-      off_qn(nchroms);
+      off_qn(nchroms), cpe_of(nchroms);
   uint64_t max_lefs = 0, max_bar = 0;
   const double p_loop = 1.0 / (p->tad_to_loop_ratio + 1.0);
-  std::vector<std::vector<double>> qn(nchroms);
   for (uint64_t c = 0; c < nchroms; ++c) {
     if (int rc = check_chrom(&chroms[c])) {
       delete job;
@@ -477,6 +502,11 @@ int mgpu_job_create(mgpu_ctx* ctx, const mgpu_params* p, uint64_t nchroms, const
                   (unsigned long long)c);
     }
     const uint64_t nb = barriers[c].n;
+    if (nb && (!barriers[c].pos || !barriers[c].dir || !barriers[c].stp_active ||
+               !barriers[c].stp_inactive)) {
+      delete job;
+      return fail(ctx, MGPU_EINVAL, "chromosome %llu: null barrier arrays", (unsigned long long)c);
+    }
     for (uint64_t i = 1; i < nb; ++i) {
       if (barriers[c].pos[i] < barriers[c].pos[i - 1]) {
         delete job;
@@ -498,18 +528,16 @@ int mgpu_job_create(mgpu_ctx* ctx, const mgpu_params* p, uint64_t nchroms, const
     off_dir[c] = pool.add(std::max<uint64_t>(nb, 1));
     off_sa[c] = pool.add(std::max<uint64_t>(nb, 1) * 8);
     off_si[c] = pool.add(std::max<uint64_t>(nb, 1) * 8);
-    const uint64_t cpe = contacts_per_epoch(p, nlefs[c]);
-    qn[c].resize(cpe + 1);
-    // binomial inversion's q_n = (1 - p)^t, evaluated on the host libm as the reference does
-    const double pr = p_loop > 0.5 ? 1 - p_loop : p_loop;
-    for (uint64_t t = 0; t <= cpe; ++t) qn[c][t] = std::pow(1 - pr, (double)t);
-    off_qn[c] = pool.add((cpe + 1) * 8);
+    cpe_of[c] = contacts_per_epoch(&par(c), nlefs[c]);
+    off_qn[c] = pool.add((cpe_of[c] + 1) * 8);
     job->nrows.push_back(0);
     job->items_per_chrom.push_back(ncells[c]);
     job->nitems += ncells[c];
   }
+  const uint64_t off_chroms = pool.add(nchroms * sizeof(DetChrom));
+  const uint64_t off_items = pool.add(job->nitems * sizeof(WorkItem));
+  const uint64_t inputs_end = pool.total;
   // bands
-  std::vector<uint64_t> cpe_of(nchroms);
   for (uint64_t c = 0; c < nchroms; ++c) {
     const uint64_t sim = chroms[c].end - chroms[c].start;
     const uint64_t nc = ceil_div(sim, p->bin_size);
@@ -518,12 +546,9 @@ int mgpu_job_create(mgpu_ctx* ctx, const mgpu_params* p, uint64_t nchroms, const
     job->band_off.push_back(job->band_total);
     job->band_len.push_back(nr * nc + 1);
     job->band_total += ((nr * nc + 1) + 63) & ~63ULL;
-    cpe_of[c] = contacts_per_epoch(p, nlefs[c]);
   }
   const uint64_t off_missed = pool.add(nchroms * 8);
   const uint64_t off_next = pool.add(64);
-  const uint64_t off_chroms = pool.add(nchroms * sizeof(DetChrom));
-  const uint64_t off_items = pool.add(job->nitems * sizeof(WorkItem));
   const uint64_t off_stats = pool.add(job->nitems * 4 * 8);
   const uint64_t off_bands = pool.add(job->band_total * 4);
 
@@ -554,17 +579,23 @@ int mgpu_job_create(mgpu_ctx* ctx, const mgpu_params* p, uint64_t nchroms, const
   job->d_bands = reinterpret_cast<uint32_t*>(base + off_bands);
   if (mode == MGPU_MODE_DETERMINISTIC && !job->use_lds) job->d_scratch = base + off_scratch;
 
+  std::vector<uint8_t> stage(inputs_end, 0);
   std::vector<WorkItem> items;
+  std::vector<double> release(2 * nchroms);
   items.reserve(job->nitems);
   for (uint64_t c = 0; c < nchroms; ++c) {
+    const mgpu_params& pc = par(c);
     const uint64_t nb = barriers[c].n;
     if (nb) {
-      HIPCHK(ctx, hipMemcpy(base + off_pos[c], barriers[c].pos, nb * 8, hipMemcpyHostToDevice));
-      HIPCHK(ctx, hipMemcpy(base + off_dir[c], barriers[c].dir, nb, hipMemcpyHostToDevice));
-      HIPCHK(ctx, hipMemcpy(base + off_sa[c], barriers[c].stp_active, nb * 8, hipMemcpyHostToDevice));
-      HIPCHK(ctx, hipMemcpy(base + off_si[c], barriers[c].stp_inactive, nb * 8, hipMemcpyHostToDevice));
+      std::memcpy(stage.data() + off_pos[c], barriers[c].pos, nb * 8);
+      std::memcpy(stage.data() + off_dir[c], barriers[c].dir, nb);
+      std::memcpy(stage.data() + off_sa[c], barriers[c].stp_active, nb * 8);
+      std::memcpy(stage.data() + off_si[c], barriers[c].stp_inactive, nb * 8);
     }
-    HIPCHK(ctx, hipMemcpy(base + off_qn[c], qn[c].data(), qn[c].size() * 8, hipMemcpyHostToDevice));
+    // binomial inversion's q_n = (1 - p)^t, evaluated on the host libm as the reference does
+    const double pr = p_loop > 0.5 ? 1 - p_loop : p_loop;
+    double* qn = reinterpret_cast<double*>(stage.data() + off_qn[c]);
+    for (uint64_t t = 0; t <= cpe_of[c]; ++t) qn[t] = std::pow(1 - pr, (double)t);
     DetChrom ch{};
     ch.start = chroms[c].start;
     ch.end = chroms[c].end;
@@ -582,6 +613,8 @@ int mgpu_job_create(mgpu_ctx* ctx, const mgpu_params* p, uint64_t nchroms, const
     ch.bind_rate = (double)nlefs[c] / (double)p->burnin_target_epochs_for_lef_activation;
     ch.qn = reinterpret_cast<const double*>(base + off_qn[c]);
     job->chroms.push_back(ch);
+    release[2 * c] = pc.prob_release;
+    release[2 * c + 1] = pc.prob_release_burnin;
     for (uint64_t k = 0; k < ncells[c]; ++k) {
       WorkItem w{};
       w.chrom = (uint32_t)c;
@@ -591,12 +624,16 @@ int mgpu_job_create(mgpu_ctx* ctx, const mgpu_params* p, uint64_t nchroms, const
       items.push_back(w);
     }
   }
-  HIPCHK(ctx, hipMemcpy(job->d_chroms, job->chroms.data(), nchroms * sizeof(DetChrom),
-                        hipMemcpyHostToDevice));
-  HIPCHK(ctx, hipMemcpy(job->d_items, items.data(), items.size() * sizeof(WorkItem),
-                        hipMemcpyHostToDevice));
+  std::memcpy(stage.data() + off_chroms, job->chroms.data(), nchroms * sizeof(DetChrom));
+  std::memcpy(stage.data() + off_items, items.data(), items.size() * sizeof(WorkItem));
+  e = hipMemcpy(base, stage.data(), stage.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hipFree(job->d_pool);
+    delete job;
+    return fail(ctx, MGPU_EDEVICE, "upload failed: %s", hipGetErrorString(e));
+  }
   if (mode == MGPU_MODE_FAST) {
-    int rc = fast_job_create(ctx->stream, *p, job->chroms, items, job->d_chroms, &job->fast);
+    int rc = fast_job_create(*p, job->chroms, barriers, release, items, &job->fast);
     if (rc != 0) {
       hipFree(job->d_pool);
       delete job;
@@ -605,6 +642,27 @@ int mgpu_job_create(mgpu_ctx* ctx, const mgpu_params* p, uint64_t nchroms, const
   }
   *out = job;
   return MGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mgpu_job_create(mgpu_ctx* ctx, const mgpu_params* p, uint64_t nchroms, const mgpu_chrom* chroms,
+                    const mgpu_barriers* barriers, const uint64_t* nlefs, const uint64_t* ncells,
+                    const uint64_t* const* target_contacts, const uint64_t* target_epochs, int mode,
+                    mgpu_job** out) {
+  return job_create(ctx, p, false, nchroms, chroms, barriers, nlefs, ncells, target_contacts,
+                    target_epochs, mode, out);
+}
+
+int mgpu_job_create_multi(mgpu_ctx* ctx, const mgpu_params* params, uint64_t nchroms,
+                          const mgpu_chrom* chroms, const mgpu_barriers* barriers,
+                          const uint64_t* nlefs, const uint64_t* ncells,
+                          const uint64_t* const* target_contacts, const uint64_t* target_epochs,
+                          int mode, mgpu_job** out) {
+  return job_create(ctx, params, true, nchroms, chroms, barriers, nlefs, ncells, target_contacts,
+                    target_epochs, mode, out);
 }
 
 int mgpu_job_run(mgpu_job* job, mgpu_stats* stats) {
@@ -654,6 +712,19 @@ int mgpu_job_set_tuning(mgpu_job* job, const mgpu_tuning* t) {
   if (fast_job_set_threads(&job->fast, t->threads_per_cell) != 0)
     return fail(job->ctx, MGPU_EINVAL, "%s", fast_last_error());
   return MGPU_OK;
+}
+
+int mgpu_job_autotune(mgpu_job* job, uint64_t sample_cells, mgpu_tuning* chosen, float* ms) {
+  if (!job) return fail(nullptr, MGPU_EINVAL, "null job");
+  mgpu_ctx* ctx = job->ctx;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (job->mode == MGPU_MODE_FAST) {
+    uint32_t best = 0;
+    if (fast_job_autotune(ctx->stream, &job->fast, job->d_items, job->d_cell_stats, sample_cells,
+                          &best, ms) != 0)
+      return fail(ctx, MGPU_EDEVICE, "%s", fast_last_error());
+  }
+  return chosen ? mgpu_job_get_tuning(job, chosen) : MGPU_OK;
 }
 
 int mgpu_job_get_tuning(mgpu_job* job, mgpu_tuning* t) {
@@ -871,6 +942,13 @@ int mgpu_debug_cell_trace(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom*
   if (d) hipFree(d);
   mgpu_job_destroy(job);
   if (e != hipSuccess) return fail(ctx, MGPU_EDEVICE, "trace failed: %s", hipGetErrorString(e));
+  return MGPU_OK;
+}
+
+int mgpu_context_stream(mgpu_ctx* ctx, void** stream, int* device) {
+  if (!ctx) return fail(ctx, MGPU_EINVAL, "null context");
+  if (stream) *stream = ctx->stream;
+  if (device) *device = ctx->device;
   return MGPU_OK;
 }
 

@@ -1,0 +1,580 @@
+// Surrogate network kernels: BF16 operands on MFMA with FP32 accumulation (gfx950).
+//
+// Layouts (channels innermost, "positions" p = (b, d, i) on a D x I grid; 1D layers use D = 1):
+//   activations  bf16 [B][D][I][C]
+//   conv weights bf16 / fp32 [Cout][T][Cin], T = KD x KI taps in row-major (dd, di) order,
+//                centred: tap (td, ti) reads position (d + td - KD/2, i + ti - KI/2), zero outside
+// Convolutions are implicit GEMMs: out[p][co] = sum_{t,ci} W[co][t][ci] * X[p + off_t][ci].
+// v_mfma_f32_32x32x16_bf16: lane l supplies row (l % 32) of A and column (l % 32) of B with the
+// 8 consecutive k values 8 * (l / 32) .. + 7; the accumulator holds rows
+// 8 * (r / 4) + 4 * (l / 32) + (r % 4) of column l % 32 in register r (checked on the device by
+// the layout probe, mgpu_debug_mfma).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sg_kernels.h"
+
+namespace mgpu {
+namespace sg {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+namespace {
+
+__device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// round to nearest even (values here are finite)
+__device__ inline uint16_t f2bf(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+union Pack8 {
+  uint4 u;
+  bf16x8 v;
+  uint16_t h[8];
+};
+
+__device__ inline int acc_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+
+// ---------------------------------------------------------------------------------------------
+// layout probe: D (32 x 32) = A (32 x 16) * B (16 x 32), bf16 inputs row-major
+// ---------------------------------------------------------------------------------------------
+__global__ void mfma_probe_kernel(const uint16_t* A, const uint16_t* B, float* D) {
+  const int l = threadIdx.x;
+  Pack8 a, b;
+  for (int s = 0; s < 8; ++s) {
+    a.h[s] = A[(l & 31) * 16 + 8 * (l >> 5) + s];
+    b.h[s] = B[(8 * (l >> 5) + s) * 32 + (l & 31)];
+  }
+  f32x16 acc = {};
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.v, b.v, acc, 0, 0, 0);
+  for (int r = 0; r < 16; ++r) D[acc_row(r, l) * 32 + (l & 31)] = acc[r];
+}
+
+// ---------------------------------------------------------------------------------------------
+// implicit-GEMM convolution, forward form
+// ---------------------------------------------------------------------------------------------
+// One workgroup: 4 waves x 64 rows = 256 positions, 64 output channels (blockIdx.y picks the
+// 64-channel slice). Each wave owns a 64 x 64 tile = 2 x 2 MFMA tiles.
+constexpr int kConvRowsPerWave = 64;
+constexpr int kConvWaves = 4;
+constexpr int kConvRows = kConvRowsPerWave * kConvWaves;
+
+__global__ void __launch_bounds__(256)
+    conv_fwd_kernel(ConvArgs a) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t m0 = (int64_t)blockIdx.x * kConvRows + wave * kConvRowsPerWave;
+  const int n0 = blockIdx.y * 64;
+  const int64_t npos = (int64_t)a.B * a.D * a.I;
+  const int kh = a.KD / 2, kw = a.KI / 2;
+  const int T = a.KD * a.KI;
+  // positions of this lane's two A rows
+  int64_t prow[2];
+  int pd[2], pi[2];
+  bool pvalid[2];
+  for (int r = 0; r < 2; ++r) {
+    const int64_t m = m0 + 32 * r + (lane & 31);
+    pvalid[r] = m < npos;
+    const int64_t mm = pvalid[r] ? m : 0;
+    pi[r] = (int)(mm % a.I);
+    pd[r] = (int)((mm / a.I) % a.D);
+    prow[r] = mm - (int64_t)pd[r] * a.I - pi[r];  // first position of this lane's (b) image
+  }
+  f32x16 acc[2][2];
+  for (int r = 0; r < 2; ++r)
+    for (int c = 0; c < 2; ++c) acc[r][c] = f32x16{};
+  const int kofs = 8 * (lane >> 5);
+  for (int t = 0; t < T; ++t) {
+    const int dd = t / a.KI - kh, di = t % a.KI - kw;
+    const uint16_t* xrow[2];
+    for (int r = 0; r < 2; ++r) {
+      const int d = pd[r] + dd, i = pi[r] + di;
+      const bool ok = pvalid[r] && d >= 0 && d < a.D && i >= 0 && i < a.I;
+      xrow[r] = ok ? a.X + (prow[r] + (int64_t)d * a.I + i) * a.Cin : nullptr;
+    }
+    const uint16_t* wrow[2];
+    for (int c = 0; c < 2; ++c)
+      wrow[c] = a.W + ((int64_t)(n0 + 32 * c + (lane & 31)) * T + t) * a.Cin;
+    for (int ci = 0; ci < a.Cin; ci += 16) {
+      Pack8 A[2], Bv[2];
+      for (int r = 0; r < 2; ++r)
+        A[r].u = xrow[r] ? *reinterpret_cast<const uint4*>(xrow[r] + ci + kofs) : make_uint4(0, 0, 0, 0);
+      for (int c = 0; c < 2; ++c) Bv[c].u = *reinterpret_cast<const uint4*>(wrow[c] + ci + kofs);
+      for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 2; ++c)
+          acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[r].v, Bv[c].v, acc[r][c], 0, 0, 0);
+    }
+  }
+  // epilogue: v = acc + bias; relu; out1 = v; out2 = res + v (all bf16)
+  for (int c = 0; c < 2; ++c) {
+    const int co = n0 + 32 * c + (lane & 31);
+    const float bias = a.bias ? a.bias[co] : 0.0f;
+    for (int r = 0; r < 2; ++r) {
+      for (int k = 0; k < 16; ++k) {
+        const int64_t m = m0 + 32 * r + acc_row(k, lane);
+        if (m >= npos) continue;
+        float v = acc[r][c][k] + bias;
+        if (a.relu) v = v > 0.0f ? v : 0.0f;
+        const int64_t o = m * a.Cout + co;
+        if (a.out1) a.out1[o] = f2bf(v);
+        if (a.out2) a.out2[o] = f2bf((a.res ? bf2f(a.res[o]) : 0.0f) + v);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// weight gradient: dW[co][t][ci] += sum_p G[p][co] * X[p + off_t][ci]
+// ---------------------------------------------------------------------------------------------
+// Workgroup: one tap (blockIdx.y), a range of positions (blockIdx.x), all (co, ci) tiles spread
+// over 4 waves. Chunks of 32 positions of G and of the shifted X are staged in LDS transposed
+// ([channel][position]), so each lane reads its 8 k-values (positions) as one 16-byte word.
+constexpr int kWgChunk = 32;
+constexpr int kWgMaxC = 128;
+
+__global__ void __launch_bounds__(256)
+    conv_wgrad_kernel(WgradArgs a) {
+  __shared__ __align__(16) uint16_t Gs[kWgMaxC][kWgChunk + 8];
+  __shared__ __align__(16) uint16_t Xs[kWgMaxC][kWgChunk + 8];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int t = blockIdx.y;
+  const int dd = t / a.KI - a.KD / 2, di = t % a.KI - a.KI / 2;
+  const int T = a.KD * a.KI;
+  const int64_t npos = (int64_t)a.B * a.D * a.I;
+  const int64_t p_begin = (int64_t)blockIdx.x * a.pos_per_block;
+  const int64_t p_end = p_begin + a.pos_per_block < npos ? p_begin + a.pos_per_block : npos;
+  const int tiles_co = a.Cout / 32, tiles_ci = a.Cin / 32;
+  const int ntiles = tiles_co * tiles_ci;
+  // each wave owns tiles wave, wave + 4, ... (at most 4 per wave for 128 x 128)
+  f32x16 acc[4];
+  for (int k = 0; k < 4; ++k) acc[k] = f32x16{};
+  for (int64_t p0 = p_begin; p0 < p_end; p0 += kWgChunk) {
+    __syncthreads();
+    // stage G[p0 .. p0+31][0 .. Cout) and X[p + off][0 .. Cin) transposed
+    for (int e = threadIdx.x; e < kWgChunk * a.Cout; e += 256) {
+      const int pp = e / a.Cout, ch = e % a.Cout;
+      const int64_t p = p0 + pp;
+      Gs[ch][pp] = p < p_end ? a.G[p * a.Cout + ch] : (uint16_t)0;
+    }
+    for (int e = threadIdx.x; e < kWgChunk * a.Cin; e += 256) {
+      const int pp = e / a.Cin, ch = e % a.Cin;
+      const int64_t p = p0 + pp;
+      uint16_t v = 0;
+      if (p < p_end) {
+        const int i = (int)(p % a.I), d = (int)((p / a.I) % a.D);
+        const int d2 = d + dd, i2 = i + di;
+        if (d2 >= 0 && d2 < a.D && i2 >= 0 && i2 < a.I)
+          v = a.X[(p + (int64_t)dd * a.I + di) * a.Cin + ch];
+      }
+      Xs[ch][pp] = v;
+    }
+    __syncthreads();
+    for (int k = 0; k < 4; ++k) {
+      const int tile = wave + 4 * k;
+      if (tile >= ntiles) break;
+      const int tco = tile / tiles_ci, tci = tile % tiles_ci;
+      for (int kk = 0; kk < kWgChunk; kk += 16) {
+        Pack8 A, Bv;
+        A.u = *reinterpret_cast<const uint4*>(&Gs[32 * tco + (lane & 31)][kk + 8 * (lane >> 5)]);
+        Bv.u = *reinterpret_cast<const uint4*>(&Xs[32 * tci + (lane & 31)][kk + 8 * (lane >> 5)]);
+        acc[k] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.v, Bv.v, acc[k], 0, 0, 0);
+      }
+    }
+  }
+  for (int k = 0; k < 4; ++k) {
+    const int tile = wave + 4 * k;
+    if (tile >= ntiles) break;
+    const int tco = tile / tiles_ci, tci = tile % tiles_ci;
+    const int ci = 32 * tci + (lane & 31);
+    for (int r = 0; r < 16; ++r) {
+      const int co = 32 * tco + acc_row(r, lane);
+      atomicAdd(&a.dW[((int64_t)co * T + t) * a.Cin + ci], acc[k][r]);
+    }
+  }
+}
+
+// column sums of a bf16 [npos][C] matrix, added to out[C] (bias gradients)
+__global__ void colsum_kernel(const uint16_t* G, int64_t npos, int C, int64_t rows_per_block,
+                              float* out) {
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+  const int64_t r1 = r0 + rows_per_block < npos ? r0 + rows_per_block : npos;
+  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    float s = 0.0f;
+    for (int64_t r = r0; r < r1; ++r) s += bf2f(G[r * C + c]);
+    atomicAdd(&out[c], s);
+  }
+}
+
+// y = x * [mask > 0] (bf16), the gradient through a ReLU whose output is mask
+__global__ void relu_mask_kernel(const uint16_t* g, const uint16_t* mask, uint16_t* out, int64_t n) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    out[e] = (mask[e] & 0x7FFFu) != 0 && (mask[e] & 0x8000u) == 0 ? g[e] : (uint16_t)0;
+}
+
+// rows [off, off + n_out) of every image of a [B][n_in][C] tensor, or the reverse with zeros
+__global__ void crop_kernel(const uint16_t* in, uint16_t* out, int B, int n_in, int n_out, int off,
+                            int C, int uncrop) {
+  const int64_t total = (int64_t)B * (uncrop ? n_in : n_out) * C;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % C);
+    const int64_t rest = e / C;
+    if (!uncrop) {
+      const int l = (int)(rest % n_out);
+      const int64_t b = rest / n_out;
+      out[e] = in[(b * n_in + l + off) * C + c];
+    } else {
+      const int l = (int)(rest % n_in);
+      const int64_t b = rest / n_in;
+      const int lo = l - off;
+      out[e] = (lo >= 0 && lo < n_out) ? in[(b * n_out + lo) * C + c] : (uint16_t)0;
+    }
+  }
+}
+
+// z[b][d][i][c] = relu(P[b][i][c] + Q[b][i + d][c] + E[d][c]) for i + d < L, else 0
+__global__ void pair_fwd_kernel(const uint16_t* P, const uint16_t* Q, const float* E, uint16_t* Z,
+                                int B, int ND, int L, int C) {
+  const int64_t total = (int64_t)B * ND * L * C;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % C);
+    int64_t rest = e / C;
+    const int i = (int)(rest % L);
+    rest /= L;
+    const int d = (int)(rest % ND);
+    const int64_t b = rest / ND;
+    float v = 0.0f;
+    if (i + d < L) {
+      v = bf2f(P[(b * L + i) * C + c]) + bf2f(Q[(b * L + i + d) * C + c]) + E[d * C + c];
+      v = v > 0.0f ? v : 0.0f;
+    }
+    Z[e] = f2bf(v);
+  }
+}
+
+// gs = gz * [z > 0]; dP[b][i][c] = sum_d gs[b][d][i][c]; dQ[b][j][c] = sum_d gs[b][d][j - d][c]
+__global__ void pair_bwd_pq_kernel(const uint16_t* GZ, const uint16_t* Z, uint16_t* dP, uint16_t* dQ,
+                                   int B, int ND, int L, int C) {
+  const int64_t total = (int64_t)B * L * C;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % C);
+    const int64_t rest = e / C;
+    const int i = (int)(rest % L);
+    const int64_t b = rest / L;
+    float sp = 0.0f, sq = 0.0f;
+    for (int d = 0; d < ND; ++d) {
+      if (i + d < L) {
+        const int64_t o = ((b * ND + d) * L + i) * C + c;
+        if (bf2f(Z[o]) > 0.0f) sp += bf2f(GZ[o]);
+      }
+      if (i - d >= 0) {
+        const int64_t o = ((b * ND + d) * L + (i - d)) * C + c;
+        if (bf2f(Z[o]) > 0.0f) sq += bf2f(GZ[o]);
+      }
+    }
+    dP[e] = f2bf(sp);
+    dQ[e] = f2bf(sq);
+  }
+}
+
+// dE[d][c] += sum_{b,i} gz * [z > 0]; one block per diagonal d
+__global__ void pair_bwd_e_kernel(const uint16_t* GZ, const uint16_t* Z, float* dE, int B, int ND,
+                                  int L, int C) {
+  const int d = blockIdx.x;
+  for (int c = threadIdx.x % C; c < C; c += C) {
+    float s = 0.0f;
+    for (int64_t r = threadIdx.x / C; r < (int64_t)B * L; r += blockDim.x / C) {
+      const int64_t b = r / L;
+      const int i = (int)(r % L);
+      const int64_t o = ((b * ND + d) * L + i) * C + c;
+      if (bf2f(Z[o]) > 0.0f) s += bf2f(GZ[o]);
+    }
+    atomicAdd(&dE[d * C + c], s);
+  }
+}
+
+// out[p] = c0 + sum_{t,ci} V[t][ci] * Z[p + off_t][ci] (3 x 3 taps, one output channel, fp32 out)
+__global__ void out_fwd_kernel(const uint16_t* Z, const float* V, const float* c0, float* out,
+                               int B, int D, int I, int C) {
+  const int64_t npos = (int64_t)B * D * I;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npos;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(p % I), d = (int)((p / I) % D);
+    float s = c0[0];
+    for (int t = 0; t < 9; ++t) {
+      const int d2 = d + t / 3 - 1, i2 = i + t % 3 - 1;
+      if (d2 < 0 || d2 >= D || i2 < 0 || i2 >= I) continue;
+      const uint16_t* z = Z + (p + (int64_t)(t / 3 - 1) * I + (t % 3 - 1)) * C;
+      for (int ci = 0; ci < C; ci += 8) {
+        Pack8 v;
+        v.u = *reinterpret_cast<const uint4*>(z + ci);
+        for (int s8 = 0; s8 < 8; ++s8) s += V[t * C + ci + s8] * bf2f(v.h[s8]);
+      }
+    }
+    out[p] = s;
+  }
+}
+
+// loss: masked squared error; g[p] = scale * (out - y) on valid pixels (i + d < I), 0 elsewhere;
+// sums of (out - y)^2 over valid pixels added to loss[0]
+__global__ void loss_kernel(const float* out, const float* y, float* g, float scale, float* loss,
+                            int B, int D, int I) {
+  const int64_t npos = (int64_t)B * D * I;
+  float acc = 0.0f;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npos;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(p % I), d = (int)((p / I) % D);
+    if (i + d < I) {
+      const float r = out[p] - y[p];
+      acc += r * r;
+      g[p] = scale * r;
+    } else {
+      g[p] = 0.0f;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(loss, acc);
+}
+
+// gZ[p][ci] = sum_t V[t][ci] * g[p - off_t]  (bf16 out)
+__global__ void out_bwd_data_kernel(const float* g, const float* V, uint16_t* GZ, int B, int D,
+                                    int I, int C) {
+  const int64_t total = (int64_t)B * D * I * C;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int ci = (int)(e % C);
+    const int64_t p = e / C;
+    const int i = (int)(p % I), d = (int)((p / I) % D);
+    float s = 0.0f;
+    for (int t = 0; t < 9; ++t) {
+      const int d2 = d - (t / 3 - 1), i2 = i - (t % 3 - 1);
+      if (d2 < 0 || d2 >= D || i2 < 0 || i2 >= I) continue;
+      s += V[t * C + ci] * g[p - (int64_t)(t / 3 - 1) * I - (t % 3 - 1)];
+    }
+    GZ[e] = f2bf(s);
+  }
+}
+
+// dV[t][ci] += sum_p Z[p + off_t][ci] * g[p]; dc += sum_p g[p]. Block: a range of positions,
+// threads over (t, ci) pairs.
+__global__ void out_bwd_weight_kernel(const uint16_t* Z, const float* g, float* dV, float* dc,
+                                      int B, int D, int I, int C, int64_t pos_per_block) {
+  const int64_t npos = (int64_t)B * D * I;
+  const int64_t p0 = (int64_t)blockIdx.x * pos_per_block;
+  const int64_t p1 = p0 + pos_per_block < npos ? p0 + pos_per_block : npos;
+  for (int e = threadIdx.x; e < 9 * C + 1; e += blockDim.x) {
+    float s = 0.0f;
+    if (e == 9 * C) {
+      for (int64_t p = p0; p < p1; ++p) s += g[p];
+      atomicAdd(dc, s);
+      continue;
+    }
+    const int t = e / C, ci = e % C;
+    const int dd = t / 3 - 1, di = t % 3 - 1;
+    for (int64_t p = p0; p < p1; ++p) {
+      const int i = (int)(p % I), d = (int)((p / I) % D);
+      const int d2 = d + dd, i2 = i + di;
+      if (d2 < 0 || d2 >= D || i2 < 0 || i2 >= I) continue;
+      s += bf2f(Z[(p + (int64_t)dd * I + di) * C + ci]) * g[p];
+    }
+    atomicAdd(&dV[e], s);
+  }
+}
+
+// Wt[ci][T-1-t][co] = W[co][t][ci] (bf16), the weights of the backward-data convolution
+__global__ void flip_weights_kernel(const uint16_t* W, uint16_t* Wt, int Cout, int T, int Cin) {
+  const int64_t total = (int64_t)Cout * T * Cin;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int ci = (int)(e % Cin);
+    const int t = (int)((e / Cin) % T);
+    const int co = (int)(e / ((int64_t)Cin * T));
+    Wt[((int64_t)ci * T + (T - 1 - t)) * Cout + co] = W[e];
+  }
+}
+
+// AdamW (decoupled weight decay) on fp32 masters, then the bf16 copy
+__global__ void adamw_kernel(float* p, float* m, float* v, const float* g, uint16_t* pb, int64_t n,
+                             float lr, float b1, float b2, float eps, float wd, float c1, float c2) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const float gi = g[e];
+    const float mi = b1 * m[e] + (1.0f - b1) * gi;
+    const float vi = b2 * v[e] + (1.0f - b2) * gi * gi;
+    m[e] = mi;
+    v[e] = vi;
+    const float upd = (mi / c1) / (sqrtf(vi / c2) + eps);
+    const float pi = p[e] - lr * (upd + wd * p[e]);
+    p[e] = pi;
+    if (pb) pb[e] = f2bf(pi);
+  }
+}
+
+__global__ void to_bf16_kernel(const float* in, uint16_t* out, int64_t n) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    out[e] = f2bf(in[e]);
+}
+
+__global__ void fill_kernel(float* p, float v, int64_t n) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    p[e] = v;
+}
+
+// gathers batch rows idx[0, nb) of a [N][row] array into a dense [nb][row] array
+__global__ void gather_rows_u16_kernel(const uint16_t* src, const uint32_t* idx, uint16_t* dst,
+                                       int nb, int64_t row) {
+  const int64_t total = (int64_t)nb * row;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x)
+    dst[e] = src[(int64_t)idx[e / row] * row + e % row];
+}
+
+__global__ void gather_rows_f32_kernel(const float* src, const uint32_t* idx, float* dst, int nb,
+                                       int64_t row) {
+  const int64_t total = (int64_t)nb * row;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x)
+    dst[e] = src[(int64_t)idx[e / row] * row + e % row];
+}
+
+inline unsigned grid_for(int64_t n, int threads = 256) {
+  const int64_t g = (n + threads - 1) / threads;
+  return (unsigned)(g < 1 ? 1 : (g > 65536 ? 65536 : g));
+}
+
+}  // namespace
+
+hipError_t launch_mfma_probe(hipStream_t s, const uint16_t* A, const uint16_t* B, float* D) {
+  hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, s, A, B, D);
+  return hipGetLastError();
+}
+
+hipError_t launch_conv_fwd(hipStream_t s, const ConvArgs& a) {
+  if (a.Cin % 16 || a.Cout % 64 || a.KD < 1 || a.KI < 1 || !a.X || !a.W) return hipErrorInvalidValue;
+  const int64_t npos = (int64_t)a.B * a.D * a.I;
+  const dim3 grid((unsigned)((npos + kConvRows - 1) / kConvRows), (unsigned)(a.Cout / 64));
+  hipLaunchKernelGGL(conv_fwd_kernel, grid, dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_conv_wgrad(hipStream_t s, const WgradArgs& a) {
+  if (a.Cin % 32 || a.Cout % 32 || a.Cin > kWgMaxC || a.Cout > kWgMaxC ||
+      (a.Cout / 32) * (a.Cin / 32) > 16 || a.pos_per_block <= 0)
+    return hipErrorInvalidValue;
+  const int64_t npos = (int64_t)a.B * a.D * a.I;
+  const dim3 grid((unsigned)((npos + a.pos_per_block - 1) / a.pos_per_block),
+                  (unsigned)(a.KD * a.KI));
+  hipLaunchKernelGGL(conv_wgrad_kernel, grid, dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_colsum(hipStream_t s, const uint16_t* G, int64_t npos, int C, float* out) {
+  const int64_t rows = 512;
+  hipLaunchKernelGGL(colsum_kernel, dim3((unsigned)((npos + rows - 1) / rows)),
+                     dim3(C < 256 ? C : 256), 0, s, G, npos, C, rows, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_relu_mask(hipStream_t s, const uint16_t* g, const uint16_t* mask, uint16_t* out,
+                            int64_t n) {
+  hipLaunchKernelGGL(relu_mask_kernel, dim3(grid_for(n)), dim3(256), 0, s, g, mask, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_crop(hipStream_t s, const uint16_t* in, uint16_t* out, int B, int n_in, int n_out,
+                       int off, int C, bool uncrop) {
+  const int64_t n = (int64_t)B * (uncrop ? n_in : n_out) * C;
+  hipLaunchKernelGGL(crop_kernel, dim3(grid_for(n)), dim3(256), 0, s, in, out, B, n_in, n_out, off,
+                     C, uncrop ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_fwd(hipStream_t s, const uint16_t* P, const uint16_t* Q, const float* E,
+                           uint16_t* Z, int B, int ND, int L, int C) {
+  const int64_t n = (int64_t)B * ND * L * C;
+  hipLaunchKernelGGL(pair_fwd_kernel, dim3(grid_for(n)), dim3(256), 0, s, P, Q, E, Z, B, ND, L, C);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_bwd(hipStream_t s, const uint16_t* GZ, const uint16_t* Z, uint16_t* dP,
+                           uint16_t* dQ, float* dE, int B, int ND, int L, int C) {
+  if (C > 256 || 256 % C) return hipErrorInvalidValue;
+  const int64_t n = (int64_t)B * L * C;
+  hipLaunchKernelGGL(pair_bwd_pq_kernel, dim3(grid_for(n)), dim3(256), 0, s, GZ, Z, dP, dQ, B, ND,
+                     L, C);
+  hipLaunchKernelGGL(pair_bwd_e_kernel, dim3(ND), dim3(256), 0, s, GZ, Z, dE, B, ND, L, C);
+  return hipGetLastError();
+}
+
+hipError_t launch_out_fwd(hipStream_t s, const uint16_t* Z, const float* V, const float* c0,
+                          float* out, int B, int D, int I, int C) {
+  if (C % 8) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(out_fwd_kernel, dim3(grid_for((int64_t)B * D * I)), dim3(256), 0, s, Z, V, c0,
+                     out, B, D, I, C);
+  return hipGetLastError();
+}
+
+hipError_t launch_loss(hipStream_t s, const float* out, const float* y, float* g, float scale,
+                       float* loss, int B, int D, int I) {
+  hipLaunchKernelGGL(loss_kernel, dim3(grid_for((int64_t)B * D * I)), dim3(256), 0, s, out, y, g,
+                     scale, loss, B, D, I);
+  return hipGetLastError();
+}
+
+hipError_t launch_out_bwd(hipStream_t s, const uint16_t* Z, const float* g, const float* V,
+                          uint16_t* GZ, float* dV, float* dc, int B, int D, int I, int C) {
+  const int64_t n = (int64_t)B * D * I * C;
+  hipLaunchKernelGGL(out_bwd_data_kernel, dim3(grid_for(n)), dim3(256), 0, s, g, V, GZ, B, D, I, C);
+  const int64_t npos = (int64_t)B * D * I, ppb = 2048;
+  hipLaunchKernelGGL(out_bwd_weight_kernel, dim3((unsigned)((npos + ppb - 1) / ppb)), dim3(256), 0,
+                     s, Z, g, dV, dc, B, D, I, C, ppb);
+  return hipGetLastError();
+}
+
+hipError_t launch_flip_weights(hipStream_t s, const uint16_t* W, uint16_t* Wt, int Cout, int T,
+                               int Cin) {
+  const int64_t n = (int64_t)Cout * T * Cin;
+  hipLaunchKernelGGL(flip_weights_kernel, dim3(grid_for(n)), dim3(256), 0, s, W, Wt, Cout, T, Cin);
+  return hipGetLastError();
+}
+
+hipError_t launch_adamw(hipStream_t s, float* p, float* m, float* v, const float* g, uint16_t* pb,
+                        int64_t n, float lr, float b1, float b2, float eps, float wd, int step) {
+  const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+  hipLaunchKernelGGL(adamw_kernel, dim3(grid_for(n)), dim3(256), 0, s, p, m, v, g, pb, n, lr, b1, b2,
+                     eps, wd, c1, c2);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_bf16(hipStream_t s, const float* in, uint16_t* out, int64_t n) {
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(grid_for(n)), dim3(256), 0, s, in, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(hipStream_t s, float* p, float v, int64_t n) {
+  hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(256), 0, s, p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_rows_u16(hipStream_t s, const uint16_t* src, const uint32_t* idx,
+                                  uint16_t* dst, int nb, int64_t row) {
+  hipLaunchKernelGGL(gather_rows_u16_kernel, dim3(grid_for((int64_t)nb * row)), dim3(256), 0, s, src,
+                     idx, dst, nb, row);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_rows_f32(hipStream_t s, const float* src, const uint32_t* idx, float* dst,
+                                  int nb, int64_t row) {
+  hipLaunchKernelGGL(gather_rows_f32_kernel, dim3(grid_for((int64_t)nb * row)), dim3(256), 0, s, src,
+                     idx, dst, nb, row);
+  return hipGetLastError();
+}
+
+}  // namespace sg
+}  // namespace mgpu

@@ -145,7 +145,9 @@ class Context:
 class Job:
     """Device-resident batch of chromosomes (mgpu_job_*): inputs uploaded once, bands in HBM."""
 
-    def __init__(self, ctx: Context, params: L.MgpuParams, chroms, barriers, plans, mode):
+    def __init__(self, ctx: Context, params, chroms, barriers, plans, mode):
+        """params: one MgpuParams for every chromosome, or a list with one per chromosome
+        (mgpu_job_create_multi: they may differ in LEF density and processivity only)."""
         self.ctx = ctx
         n = len(chroms)
         self.plans = plans
@@ -158,10 +160,16 @@ class Job:
         self._tptrs = (L.U64P * n)(*[_p(t, C.c_uint64) for t in self._targets])
         self._tepochs = np.array([p.target_epochs for p in plans], np.uint64)
         self.ptr = C.c_void_p()
-        ctx.check(L.lib().mgpu_job_create(ctx.ptr, C.byref(params), n, self._chroms, self._bars,
-                                          _p(self._nlefs, C.c_uint64), _p(self._ncells, C.c_uint64),
-                                          self._tptrs, _p(self._tepochs, C.c_uint64), mode,
-                                          C.byref(self.ptr)))
+        if isinstance(params, (list, tuple)):
+            if len(params) != n:
+                raise ValueError("one parameter block per chromosome expected")
+            self._params = (L.MgpuParams * n)(*params)
+            create, pref = L.lib().mgpu_job_create_multi, self._params
+        else:
+            create, pref = L.lib().mgpu_job_create, C.byref(params)
+        ctx.check(create(ctx.ptr, pref, n, self._chroms, self._bars,
+                         _p(self._nlefs, C.c_uint64), _p(self._ncells, C.c_uint64),
+                         self._tptrs, _p(self._tepochs, C.c_uint64), mode, C.byref(self.ptr)))
 
     def run(self) -> L.MgpuStats:
         st = L.MgpuStats()
@@ -172,6 +180,14 @@ class Job:
         t = L.MgpuTuning()
         t.threads_per_cell = threads_per_cell
         self.ctx.check(L.lib().mgpu_job_set_tuning(self.ptr, C.byref(t)))
+
+    def autotune(self, sample_cells: int = 0):
+        """Times each workgroup size on a sample of cells and keeps the fastest; returns
+        (threads per cell, {threads: ms})."""
+        t = L.MgpuTuning()
+        ms = (C.c_float * 5)()
+        self.ctx.check(L.lib().mgpu_job_autotune(self.ptr, sample_cells, C.byref(t), ms))
+        return int(t.threads_per_cell), dict(zip((64, 128, 256, 512, 1024), list(ms)))
 
     def tuning(self) -> L.MgpuTuning:
         t = L.MgpuTuning()

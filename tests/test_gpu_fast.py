@@ -88,7 +88,7 @@ def test_fast_mode_is_deterministic_and_conserves_contacts(ctx):
     assert sa.contacts == int(plan.targets.sum())
 
 
-@pytest.mark.parametrize("threads", [512, 1024])
+@pytest.mark.parametrize("threads", [64, 128, 512, 1024])
 def test_fast_mode_band_does_not_depend_on_workgroup_size(ctx, threads):
     """Every draw is keyed by (element, epoch, stream) and every reduction is exact, so the
     workgroup size per cell changes nothing in the result."""
@@ -98,6 +98,24 @@ def test_fast_mode_band_does_not_depend_on_workgroup_size(ctx, threads):
     say(f"fast small/8 at 256 threads {sa.kernel_ms:.1f} ms, at {threads} {sb.kernel_ms:.1f} ms")
     assert np.array_equal(a["chrS"][0], b["chrS"][0]) and a["chrS"][1] == b["chrS"][1]
     assert (sa.epochs, sa.burnin_epochs, sa.contacts) == (sb.epochs, sb.burnin_epochs, sb.contacts)
+
+
+def test_autotune_keeps_results(ctx):
+    """The autotuner times each workgroup size on a sample of cells; the run that follows gives
+    the band of the default launch."""
+    wl = W.small(ncells=8)
+    a, sa = run_device(ctx, wl, L.MODE_FAST)
+    sim = S.Simulation(wl.options, ctx)
+    names, chroms, bars, plans = sim.prepare(wl)
+    job = S.Job(ctx, sim.params, chroms, bars, plans, L.MODE_FAST)
+    best, times = job.autotune(4)
+    say(f"autotune: {best} threads per cell; times {times}")
+    assert best in times
+    st = job.run()
+    band, missed = job.band(0)
+    job.close()
+    assert np.array_equal(band, a["chrS"][0]) and missed == a["chrS"][1]
+    assert st.epochs == sa.epochs
 
 
 def test_fast_mode_statistics_match_oracle(ctx):
