@@ -192,8 +192,8 @@ int mgpu_job_create(mgpu_ctx* ctx, const mgpu_params* p, uint64_t nchroms,
                     const uint64_t* target_epochs, int mode, mgpu_job** out);
 /* The same with one parameter block per chromosome (params has nchroms entries), for parameter
  * sweeps and training windows whose chromosomes differ in LEF density and processivity. The
- * blocks may differ only in lefs_per_mbp, prob_release and prob_release_burnin; in deterministic
- * mode they must be identical. No counterpart in the reference, which runs one Config per
+ * blocks may differ only in lefs_per_mbp, prob_release, prob_release_burnin and
+ * burnin_target_epochs_for_lef_activation; in deterministic mode they must be identical. No counterpart in the reference, which runs one Config per
  * process (each block is what Cli::transform_args, cli.cpp:1192-1215, makes of one Config). */
 int mgpu_job_create_multi(mgpu_ctx* ctx, const mgpu_params* params, uint64_t nchroms,
                           const mgpu_chrom* chroms, const mgpu_barriers* barriers,
@@ -291,11 +291,13 @@ typedef struct mgpu_sg_config {
   uint32_t max_batch;   /* samples per device batch */
   uint32_t reserved;
   uint64_t seed;        /* weight initialisation */
+  uint32_t enc_dilation[16]; /* dilation of encoder layer l (0 means 1) */
 } mgpu_sg_config;
 typedef struct mgpu_sg mgpu_sg;
 
 /* 600 input bins (2 Mbp window of 400 5-kb bins with 500 kb of context each side), 100
- * diagonals, 5 input channels, 8 x 128-channel encoder (k = 5), 4-layer 64-channel head. */
+ * diagonals, 6 input channels, 8 x 128-channel encoder (k = 5, dilations 1 1 2 4 8 16 32 1: each
+ * output bin sees 130 bins to either side), 4-layer 64-channel head. */
 int mgpu_sg_config_default(mgpu_sg_config* c);
 int mgpu_sg_create(mgpu_ctx* ctx, const mgpu_sg_config* c, mgpu_sg** out);
 void mgpu_sg_destroy(mgpu_sg* s);

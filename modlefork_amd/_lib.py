@@ -162,7 +162,8 @@ class MgpuTuning(C.Structure):
 class MgpuSgConfig(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in (
         "in_len", "out_offset", "out_len", "ndiag", "in_ch", "enc_ch", "enc_layers", "enc_k",
-        "head_ch", "head_layers", "max_batch", "reserved")] + [("seed", C.c_uint64)]
+        "head_ch", "head_layers", "max_batch", "reserved")] + [("seed", C.c_uint64),
+                                                                ("enc_dilation", C.c_uint32 * 16)]
 
 
 FP = C.POINTER(C.c_float)

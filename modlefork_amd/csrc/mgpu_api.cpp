@@ -436,12 +436,15 @@ struct PoolBuilder {
 namespace {
 
 // Parameter blocks of one multi-parameter job may differ only in the fields that describe LEF
-// density and processivity; everything the kernels read job-wide must match the first block.
+// density and processivity (the release probabilities and the burn-in activation horizon that
+// Cli::transform_args derives from it); everything the kernels read job-wide must match the
+// first block.
 bool same_job_wide(const mgpu_params& a, const mgpu_params& b) {
   mgpu_params x = a, y = b;
   x.prob_release = y.prob_release = 0;
   x.prob_release_burnin = y.prob_release_burnin = 0;
   x.lefs_per_mbp = y.lefs_per_mbp = 0;
+  x.burnin_target_epochs_for_lef_activation = y.burnin_target_epochs_for_lef_activation = 0;
   return std::memcmp(&x, &y, sizeof(mgpu_params)) == 0;
 }
 
@@ -610,7 +613,7 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
     ch.nrows = job->nrows[c];
     ch.cpe = cpe_of[c];
     ch.target_epochs = target_epochs[c];
-    ch.bind_rate = (double)nlefs[c] / (double)p->burnin_target_epochs_for_lef_activation;
+    ch.bind_rate = (double)nlefs[c] / (double)pc.burnin_target_epochs_for_lef_activation;
     ch.qn = reinterpret_cast<const double*>(base + off_qn[c]);
     job->chroms.push_back(ch);
     release[2 * c] = pc.prob_release;

@@ -18,6 +18,7 @@ struct ConvArgs {
   int B, D, I;
   int Cin, Cout;
   int KD, KI;
+  int dil;  // dilation along I (tap ti reads i + (ti - KI/2) * dil); 0 means 1
   int relu;
 };
 
@@ -30,6 +31,7 @@ struct WgradArgs {
   int B, D, I;
   int Cin, Cout;
   int KD, KI;
+  int dil;  // as in ConvArgs
   int64_t pos_per_block;
 };
 

@@ -3,7 +3,8 @@
 // Layouts (channels innermost, "positions" p = (b, d, i) on a D x I grid; 1D layers use D = 1):
 //   activations  bf16 [B][D][I][C]
 //   conv weights bf16 / fp32 [Cout][T][Cin], T = KD x KI taps in row-major (dd, di) order,
-//                centred: tap (td, ti) reads position (d + td - KD/2, i + ti - KI/2), zero outside
+//                centred: tap (td, ti) reads position (d + td - KD/2, i + (ti - KI/2) * dil),
+//                zero outside
 // Convolutions are implicit GEMMs: out[p][co] = sum_{t,ci} W[co][t][ci] * X[p + off_t][ci].
 // v_mfma_f32_32x32x16_bf16: lane l supplies row (l % 32) of A and column (l % 32) of B with the
 // 8 consecutive k values 8 * (l / 32) .. + 7; the accumulator holds rows
@@ -69,6 +70,7 @@ __global__ void __launch_bounds__(256)
   const int64_t npos = (int64_t)a.B * a.D * a.I;
   const int kh = a.KD / 2, kw = a.KI / 2;
   const int T = a.KD * a.KI;
+  const int dil = a.dil > 0 ? a.dil : 1;
   // positions of this lane's two A rows
   int64_t prow[2];
   int pd[2], pi[2];
@@ -86,7 +88,7 @@ __global__ void __launch_bounds__(256)
     for (int c = 0; c < 2; ++c) acc[r][c] = f32x16{};
   const int kofs = 8 * (lane >> 5);
   for (int t = 0; t < T; ++t) {
-    const int dd = t / a.KI - kh, di = t % a.KI - kw;
+    const int dd = t / a.KI - kh, di = (t % a.KI - kw) * dil;
     const uint16_t* xrow[2];
     for (int r = 0; r < 2; ++r) {
       const int d = pd[r] + dd, i = pi[r] + di;
@@ -139,7 +141,7 @@ __global__ void __launch_bounds__(256)
   __shared__ __align__(16) uint16_t Xs[kWgMaxC][kWgChunk + 8];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int t = blockIdx.y;
-  const int dd = t / a.KI - a.KD / 2, di = t % a.KI - a.KI / 2;
+  const int dd = t / a.KI - a.KD / 2, di = (t % a.KI - a.KI / 2) * (a.dil > 0 ? a.dil : 1);
   const int T = a.KD * a.KI;
   const int64_t npos = (int64_t)a.B * a.D * a.I;
   const int64_t p_begin = (int64_t)blockIdx.x * a.pos_per_block;

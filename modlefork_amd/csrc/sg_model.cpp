@@ -3,7 +3,8 @@
 // Architecture (per sample; bins along a window, channels innermost):
 //   x      [Lin][in_ch]           barrier tracks + broadcast scalars (padded to 32 channels)
 //   h1     = relu(conv1d_k(x))                                  enc_ch channels
-//   h(l+1) = h(l) + relu(conv1d_k(h(l)))     l = 1 .. enc_layers-1
+//   h(l+1) = h(l) + relu(conv1d_k(h(l)))     l = 1 .. enc_layers-1 (layer l dilated by
+//                                            enc_dilation[l])
 //   hc     = h[out_offset, out_offset + Lout)                   crop to the output bins
 //   P = hc Wp + bp, Q = hc Wq                                   head_ch channels (1 x 1)
 //   z0[d][i] = relu(P[i] + Q[i + d] + E[d])  for i + d < Lout, else 0    (pairwise outer sum)
@@ -160,6 +161,10 @@ std::vector<Tensor*> conv_tensors(mgpu_sg* s) {
   return v;
 }
 
+int dil_of(const mgpu_sg* s, int l) {
+  return l < 16 && s->cfg.enc_dilation[l] > 0 ? (int)s->cfg.enc_dilation[l] : 1;
+}
+
 int64_t valid_pixels(const mgpu_sg* s) {
   int64_t n = 0;
   for (int d = 0; d < s->ND; ++d) n += std::max(0, s->Lout - d);
@@ -178,6 +183,7 @@ int forward(mgpu_sg* s, int B) {
   a.KI = s->K;
   a.relu = 1;
   for (int l = 0; l < s->Le; ++l) {
+    a.dil = dil_of(s, l);
     a.X = l == 0 ? s->x : s->h[l];
     a.Cin = l == 0 ? kInPad : s->C;
     a.Cout = s->C;
@@ -325,6 +331,7 @@ int backward(mgpu_sg* s, int B) {
     we.Cout = s->C;
     we.KD = 1;
     we.KI = s->K;
+    we.dil = dil_of(s, l);
     we.pos_per_block = 1024;
     SGCHK(s, launch_conv_wgrad(st, we));
     SGCHK(s, launch_colsum(st, s->gm1, pos0, s->C, s->g32 + s->enc_b[l].off));
@@ -341,6 +348,7 @@ int backward(mgpu_sg* s, int B) {
     ae.Cin = ae.Cout = s->C;
     ae.KD = 1;
     ae.KI = s->K;
+    ae.dil = dil_of(s, l);
     SGCHK(s, launch_conv_fwd(st, ae));
     std::swap(s->gh, s->gh2);
   }
@@ -374,6 +382,9 @@ int check_cfg(const mgpu_sg_config* c) {
     return sg_fail(nullptr, MGPU_EINVAL, "output window outside the input or invalid band");
   if (c->max_batch == 0 || c->max_batch > 4096)
     return sg_fail(nullptr, MGPU_EINVAL, "max_batch must be in [1, 4096]");
+  if (c->enc_layers > 16) return sg_fail(nullptr, MGPU_EINVAL, "at most 16 encoder layers");
+  for (int l = 0; l < 16; ++l)
+    if (c->enc_dilation[l] > 256) return sg_fail(nullptr, MGPU_EINVAL, "dilation above 256");
   return MGPU_OK;
 }
 
@@ -388,7 +399,7 @@ int mgpu_sg_config_default(mgpu_sg_config* c) {
   c->out_offset = 100;
   c->out_len = 400;
   c->ndiag = 100;
-  c->in_ch = 5;
+  c->in_ch = 6;
   c->enc_ch = 128;
   c->enc_layers = 8;
   c->enc_k = 5;
@@ -396,6 +407,8 @@ int mgpu_sg_config_default(mgpu_sg_config* c) {
   c->head_layers = 4;
   c->max_batch = 32;
   c->seed = 1;
+  const uint32_t dil[8] = {1, 1, 2, 4, 8, 16, 32, 1};
+  for (int l = 0; l < 8; ++l) c->enc_dilation[l] = dil[l];
   return MGPU_OK;
 }
 

@@ -24,7 +24,11 @@ def default_config(**over) -> L.MgpuSgConfig:
     c = L.MgpuSgConfig()
     L.check(L.lib().mgpu_sg_config_default(C.byref(c)))
     for k, v in over.items():
-        setattr(c, k, v)
+        if k == "enc_dilation":
+            for i in range(16):
+                c.enc_dilation[i] = v[i] if i < len(v) else 0
+        else:
+            setattr(c, k, v)
     return c
 
 
@@ -130,3 +134,136 @@ def from_bf16_bits(h: np.ndarray) -> np.ndarray:
 
 def bf16_round(x: np.ndarray) -> np.ndarray:
     return from_bf16_bits(to_bf16_bits(x))
+
+
+# ---------------------------------------------------------------------------------------------
+# data, training and evaluation
+# ---------------------------------------------------------------------------------------------
+def make_dataset(ctx, nchrom: int, seed: int, ncells: int = 512, contact_density: float = 30.0,
+                 cell_seed: int | None = None):
+    """Windows tiling nchrom simulated 20 Mbp chromosomes: (x, y = normalised log targets, raw
+    counts, device seconds). Layouts come from `seed`, the cells' random streams from
+    `cell_seed` (default: seed)."""
+    from . import windows as Wn
+
+    specs = Wn.sample_chromosomes(nchrom, seed)
+    x, counts, dev_s = Wn.simulate_chromosomes(ctx, specs, ncells, contact_density,
+                                               seed if cell_seed is None else cell_seed)
+    return x, Wn.normalise(counts), counts, dev_s
+
+
+def pair_correlations(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Per-window mean per-diagonal correlation of two sets of count bands."""
+    from . import sweep as Sw
+    from . import windows as Wn
+
+    m = Wn.valid_mask()
+    return np.array([Sw.diag_correlation(np.where(m, u, 0), np.where(m, v, 0))
+                     for u, v in zip(a, b)])
+
+
+def train(net: Surrogate, x: np.ndarray, y: np.ndarray, epochs: float, batch: int, lr: float,
+          weight_decay: float = 1e-4, warmup: int = 100, seed: int = 0, log=print):
+    """AdamW with linear warm-up and cosine decay over epochs * n / batch steps of random
+    batches; returns the per-step losses and the device milliseconds of all steps."""
+    net.set_data(x, y)
+    n = len(x)
+    steps = max(1, int(epochs * n / batch))
+    rng = np.random.default_rng(seed)
+    losses, dev_ms = [], 0.0
+    for t in range(steps):
+        lr_t = lr * min(1.0, (t + 1) / warmup) * 0.5 * (1 + np.cos(np.pi * t / steps))
+        idx = rng.choice(n, size=min(batch, n), replace=False).astype(np.uint32)
+        loss, ms = net.train_step(idx, float(lr_t), weight_decay)
+        losses.append(loss)
+        dev_ms += ms
+        if log and (t % max(1, steps // 20) == 0 or t == steps - 1):
+            log(f"  step {t + 1}/{steps}: loss {np.mean(losses[-50:]):.4f}, lr {lr_t:.2e}")
+    return np.array(losses), dev_ms
+
+
+def window_correlations(net: Surrogate, x: np.ndarray, counts: np.ndarray) -> np.ndarray:
+    """Mean per-diagonal correlation (25 kb, diagonals up to 500 kb) of each window's prediction
+    with its simulated counts."""
+    from . import sweep as Sw
+    from . import windows as Wn
+
+    return pair_correlations(Wn.denormalise(net.forward(x)), counts)
+
+
+def save_params(net: Surrogate, path: str):
+    np.save(path, net.params())
+
+
+def load_params(net: Surrogate, path: str):
+    p = np.load(path, allow_pickle=False)
+    net.set_params(p)
+
+
+def main(argv=None):
+    import argparse
+    import json
+    import time
+
+    from . import simulation as S
+
+    ap = argparse.ArgumentParser(description="generate simulated windows and train the surrogate")
+    ap.add_argument("--chromosomes", type=int, default=1540,
+                    help="simulated 20 Mbp chromosomes (13 windows each)")
+    ap.add_argument("--val", type=int, default=40, help="validation chromosomes")
+    ap.add_argument("--cells", type=int, default=512)
+    ap.add_argument("--density", type=float, default=30.0, help="target contact density")
+    ap.add_argument("--epochs", type=float, default=10.0)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", required=True, help="weights (.npy)")
+    ap.add_argument("--summary", default=None, help="JSON summary")
+    a = ap.parse_args(argv)
+
+    def log(msg):
+        print(msg, flush=True)
+
+    ctx = S.Context(0)
+    t0 = time.perf_counter()
+    log(f"simulating {a.chromosomes} training + {a.val} validation chromosomes, {a.cells} cells")
+    x, y, _, dev_tr = make_dataset(ctx, a.chromosomes, a.seed, a.cells, a.density)
+    xv, yv, cv, dev_va = make_dataset(ctx, a.val, a.seed + 1000, a.cells, a.density)
+    t_data = time.perf_counter() - t0
+    log(f"data: {len(x)} + {len(xv)} windows, {t_data:.1f} s wall, {dev_tr + dev_va:.1f} s in "
+        f"the simulation kernel, {(a.chromosomes + a.val) / (dev_tr + dev_va):.1f} chromosomes/s")
+    # noise ceiling: the validation layouts simulated again with other cell streams
+    _, _, cv2, _ = make_dataset(ctx, a.val, a.seed + 1000, a.cells, a.density,
+                                cell_seed=a.seed + 2000)
+    ceil = pair_correlations(cv2, cv)
+    nbar = xv[:, 100:500, 0].sum(axis=1)
+    a.windows = len(x)
+    log(f"noise ceiling (simulation vs simulation): mean {ceil.mean():.4f}; "
+        f"windows with >= 20 barriers {ceil[nbar >= 20].mean():.4f}")
+    net = Surrogate(ctx, default_config(max_batch=a.batch, seed=a.seed + 1))
+    t0 = time.perf_counter()
+    losses, dev_ms = train(net, x, y, a.epochs, a.batch, a.lr, seed=a.seed, log=log)
+    t_train = time.perf_counter() - t0
+    corr = window_correlations(net, xv, cv)
+    log(f"validation: mean per-diagonal correlation {corr.mean():.4f} "
+        f"(median {np.median(corr):.4f}, 5th percentile {np.percentile(corr, 5):.4f}); "
+        f"windows with >= 20 barriers {corr[nbar >= 20].mean():.4f}")
+    save_params(net, a.out)
+    if a.summary:
+        with open(a.summary, "w") as fh:
+            json.dump({"windows": a.windows, "cells": a.cells, "contact_density": a.density,
+                       "epochs": a.epochs, "batch": a.batch, "lr": a.lr, "params": net.nparams,
+                       "data_wall_s": t_data, "data_kernel_s": dev_tr + dev_va,
+                       "train_wall_s": t_train, "train_device_s": dev_ms / 1000.0,
+                       "final_loss": float(np.mean(losses[-50:])),
+                       "val_corr_mean": float(corr.mean()), "val_corr_median": float(np.median(corr)),
+                       "val_corr_p5": float(np.percentile(corr, 5)),
+                       "val_corr_20bar": float(corr[nbar >= 20].mean()),
+                       "ceiling_mean": float(ceil.mean()),
+                       "ceiling_20bar": float(ceil[nbar >= 20].mean())}, fh, indent=1)
+    net.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

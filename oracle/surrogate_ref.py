@@ -8,7 +8,8 @@ layer, as listed by param_shapes(). Pinned by tests/test_surrogate_oracle.py aga
 autograd on the CPU and against central finite differences in float64.
 
     x   [B][Lin][in_ch]
-    h1 = relu(conv1d(x)),  h(l+1) = h(l) + relu(conv1d(h(l)))          (kernel K, zero padding)
+    h1 = relu(conv1d(x)),  h(l+1) = h(l) + relu(conv1d(h(l)))   (kernel K, layer l dilated by
+                                                                 enc_dilation[l], zero padding)
     hc = h[:, out_offset : out_offset + Lout]
     P = hc Wp^T + bp,  Q = hc Wq^T
     z0[b, d, i] = relu(P[b, i] + Q[b, i + d] + E[d])  if i + d < Lout else 0
@@ -29,12 +30,16 @@ class SgConfig:
     out_offset: int = 100
     out_len: int = 400
     ndiag: int = 100
-    in_ch: int = 5
+    in_ch: int = 6
     enc_ch: int = 128
     enc_layers: int = 8
     enc_k: int = 5
     head_ch: int = 64
     head_layers: int = 4
+    enc_dilation: tuple = (1, 1, 2, 4, 8, 16, 32, 1)
+
+    def dil(self, l: int) -> int:
+        return self.enc_dilation[l] if l < len(self.enc_dilation) and self.enc_dilation[l] else 1
 
 
 def param_shapes(c: SgConfig):
@@ -76,28 +81,30 @@ def valid_mask(c: SgConfig) -> np.ndarray:
     return (i + d) < c.out_len
 
 
-# ---- convolutions (zero padding, centred taps) ----
-def conv1d(x, w, b):
-    """x [B, L, Cin], w [Cout, K, Cin] -> [B, L, Cout]"""
+# ---- convolutions (zero padding, centred taps; 1D taps dilated by dil) ----
+def conv1d(x, w, b, dil=1):
+    """x [B, L, Cin], w [Cout, K, Cin] -> [B, L, Cout]; tap t reads l + (t - K // 2) * dil"""
     K = w.shape[1]
     L = x.shape[1]
-    xp = np.pad(x, ((0, 0), (K // 2, K // 2), (0, 0)))
+    pad = (K // 2) * dil
+    xp = np.pad(x, ((0, 0), (pad, pad), (0, 0)))
     y = np.zeros(x.shape[:2] + (w.shape[0],), dtype=x.dtype)
     for t in range(K):
-        y += xp[:, t:t + L, :] @ w[:, t, :].T
+        y += xp[:, t * dil:t * dil + L, :] @ w[:, t, :].T
     return y + b
 
 
-def conv1d_backward(x, w, g):
+def conv1d_backward(x, w, g, dil=1):
     K = w.shape[1]
     L = x.shape[1]
-    xp = np.pad(x, ((0, 0), (K // 2, K // 2), (0, 0)))
+    pad = (K // 2) * dil
+    xp = np.pad(x, ((0, 0), (pad, pad), (0, 0)))
     dw = np.zeros_like(w)
     dxp = np.zeros_like(xp)
     for t in range(K):
-        dw[:, t, :] = np.einsum("blo,bli->oi", g, xp[:, t:t + L, :])
-        dxp[:, t:t + L, :] += g @ w[:, t, :]
-    return dxp[:, K // 2:K // 2 + L, :], dw, g.sum(axis=(0, 1))
+        dw[:, t, :] = np.einsum("blo,bli->oi", g, xp[:, t * dil:t * dil + L, :])
+        dxp[:, t * dil:t * dil + L, :] += g @ w[:, t, :]
+    return dxp[:, pad:pad + L, :], dw, g.sum(axis=(0, 1))
 
 
 def conv2d3(x, w, b):
@@ -130,11 +137,11 @@ def relu(v):
 # ---- network ----
 def forward(c: SgConfig, p: dict, x: np.ndarray):
     cache = {"x": x}
-    h = relu(conv1d(x, p["enc0.w"], p["enc0.b"]))
+    h = relu(conv1d(x, p["enc0.w"], p["enc0.b"], c.dil(0)))
     cache["h1"] = h
     hs, rs = [None, h], [None]
     for l in range(1, c.enc_layers):
-        r = relu(conv1d(h, p[f"enc{l}.w"], p[f"enc{l}.b"]))
+        r = relu(conv1d(h, p[f"enc{l}.w"], p[f"enc{l}.b"], c.dil(l)))
         h = h + r
         hs.append(h)
         rs.append(r)
@@ -193,10 +200,10 @@ def loss_and_grads(c: SgConfig, p: dict, x: np.ndarray, y: np.ndarray):
     hs, rs = cache["hs"], cache["rs"]
     for l in range(c.enc_layers - 1, 0, -1):
         gm = gh * (rs[l] > 0)
-        dx, g[f"enc{l}.w"], g[f"enc{l}.b"] = conv1d_backward(hs[l], p[f"enc{l}.w"], gm)
+        dx, g[f"enc{l}.w"], g[f"enc{l}.b"] = conv1d_backward(hs[l], p[f"enc{l}.w"], gm, c.dil(l))
         gh = gh + dx
     gm = gh * (cache["h1"] > 0)
-    _, g["enc0.w"], g["enc0.b"] = conv1d_backward(x, p["enc0.w"], gm)
+    _, g["enc0.w"], g["enc0.b"] = conv1d_backward(x, p["enc0.w"], gm, c.dil(0))
     return loss, flatten(c, g), out
 
 

@@ -16,7 +16,7 @@ from oracle import surrogate_ref as SR
 pytestmark = pytest.mark.gpu
 
 TINY = dict(in_len=24, out_offset=4, out_len=16, ndiag=6, in_ch=5, enc_ch=64, enc_layers=3,
-            enc_k=5, head_ch=64, head_layers=3)
+            enc_k=5, head_ch=64, head_layers=3, enc_dilation=(1, 2, 3))
 
 
 def say(msg):

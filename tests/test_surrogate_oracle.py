@@ -10,7 +10,7 @@ import pytest
 from oracle import surrogate_ref as SR
 
 TINY = SR.SgConfig(in_len=24, out_offset=4, out_len=16, ndiag=6, in_ch=5, enc_ch=8, enc_layers=3,
-                   enc_k=5, head_ch=6, head_layers=3)
+                   enc_k=5, head_ch=6, head_layers=3, enc_dilation=(1, 2, 3))
 
 
 def random_problem(c, seed, dtype=np.float64, batch=2):
@@ -27,16 +27,17 @@ def torch_loss(c, flat, x, y):
     p = {k: torch.tensor(v, requires_grad=True) for k, v in SR.unflatten(c, flat.copy()).items()}
     xt = torch.tensor(x)
 
-    def c1(h, w, b):  # [B, L, Cin], w [Cout, K, Cin]
-        return F.conv1d(h.transpose(1, 2), w.permute(0, 2, 1), b, padding=w.shape[1] // 2).transpose(1, 2)
+    def c1(h, w, b, dil):  # [B, L, Cin], w [Cout, K, Cin]
+        return F.conv1d(h.transpose(1, 2), w.permute(0, 2, 1), b, padding=(w.shape[1] // 2) * dil,
+                        dilation=dil).transpose(1, 2)
 
     def c2(z, w, b):  # [B, D, I, Cin], w [Cout, 9, Cin]
         wk = w.reshape(w.shape[0], 3, 3, w.shape[2]).permute(0, 3, 1, 2)
         return F.conv2d(z.permute(0, 3, 1, 2), wk, b, padding=1).permute(0, 2, 3, 1)
 
-    h = torch.relu(c1(xt, p["enc0.w"], p["enc0.b"]))
+    h = torch.relu(c1(xt, p["enc0.w"], p["enc0.b"], c.dil(0)))
     for l in range(1, c.enc_layers):
-        h = h + torch.relu(c1(h, p[f"enc{l}.w"], p[f"enc{l}.b"]))
+        h = h + torch.relu(c1(h, p[f"enc{l}.w"], p[f"enc{l}.b"], c.dil(l)))
     hc = h[:, c.out_offset:c.out_offset + c.out_len]
     P = hc @ p["proj_p.w"][:, 0, :].T + p["proj_p.b"]
     Q = hc @ p["proj_q.w"][:, 0, :].T
@@ -61,7 +62,7 @@ def test_param_count_of_default_network():
     c = SR.SgConfig()
     # 8 x 128-ch k=5 encoder, two 1x1 projections to 64, 100 x 64 embedding, 3 x (64-ch 3x3),
     # 1-channel 3x3 output
-    expect = (128 * 5 * 5 + 128) + 7 * (128 * 5 * 128 + 128) + (64 * 128 + 64) + 64 * 128 + \
+    expect = (128 * 5 * 6 + 128) + 7 * (128 * 5 * 128 + 128) + (64 * 128 + 64) + 64 * 128 + \
         100 * 64 + 3 * (64 * 9 * 64 + 64) + (9 * 64 + 1)
     assert SR.num_params(c) == expect
 
