@@ -62,8 +62,19 @@ class Surrogate:
         assert p.size == self.nparams
         self._check(L.lib().mgpu_sg_set_params(self.ptr, _f(p)))
 
+    def _check_x(self, x):
+        want = (int(self.cfg.in_len), int(self.cfg.in_ch))
+        if x.ndim != 3 or tuple(x.shape[1:]) != want:
+            raise ValueError(f"inputs must be [n][{want[0]}][{want[1]}], got {list(x.shape)}")
+
+    def _check_y(self, y, n):
+        if tuple(y.shape) != (n,) + self.out_shape:
+            raise ValueError(f"targets must be [{n}][{self.out_shape[0]}][{self.out_shape[1]}], "
+                             f"got {list(y.shape)}")
+
     def forward(self, x: np.ndarray) -> np.ndarray:
         x = np.ascontiguousarray(x, np.float32)
+        self._check_x(x)
         n = x.shape[0]
         y = np.zeros((n,) + self.out_shape, np.float32)
         self._check(L.lib().mgpu_sg_forward(self.ptr, _f(x), n, _f(y)))
@@ -72,6 +83,8 @@ class Surrogate:
     def gradients(self, x: np.ndarray, y: np.ndarray):
         x = np.ascontiguousarray(x, np.float32)
         y = np.ascontiguousarray(y, np.float32)
+        self._check_x(x)
+        self._check_y(y, x.shape[0])
         g = np.zeros(self.nparams, np.float32)
         loss = C.c_float()
         self._check(L.lib().mgpu_sg_gradients(self.ptr, _f(x), _f(y), x.shape[0], C.byref(loss),
@@ -85,6 +98,8 @@ class Surrogate:
     def set_data(self, x: np.ndarray, y: np.ndarray):
         self._x = np.ascontiguousarray(x, np.float32)
         self._y = np.ascontiguousarray(y, np.float32)
+        self._check_x(self._x)
+        self._check_y(self._y, self._x.shape[0])
         self._check(L.lib().mgpu_sg_set_data(self.ptr, _f(self._x), _f(self._y), x.shape[0]))
         self._x = self._y = None
 
@@ -147,9 +162,9 @@ def make_dataset(ctx, nchrom: int, seed: int, ncells: int = 512, contact_density
     from . import windows as Wn
 
     specs = Wn.sample_chromosomes(nchrom, seed)
-    x, counts, dev_s = Wn.simulate_chromosomes(ctx, specs, ncells, contact_density,
-                                               seed if cell_seed is None else cell_seed)
-    return x, Wn.normalise(counts), counts, dev_s
+    x, counts, means, dev_s = Wn.simulate_chromosomes(ctx, specs, ncells, contact_density,
+                                                      seed if cell_seed is None else cell_seed)
+    return x, Wn.normalise(counts, means), counts, dev_s
 
 
 def pair_correlations(a: np.ndarray, b: np.ndarray) -> np.ndarray:
@@ -208,11 +223,11 @@ def main(argv=None):
     from . import simulation as S
 
     ap = argparse.ArgumentParser(description="generate simulated windows and train the surrogate")
-    ap.add_argument("--chromosomes", type=int, default=1540,
+    ap.add_argument("--chromosomes", type=int, default=600,
                     help="simulated 20 Mbp chromosomes (13 windows each)")
-    ap.add_argument("--val", type=int, default=40, help="validation chromosomes")
+    ap.add_argument("--val", type=int, default=30, help="validation chromosomes")
     ap.add_argument("--cells", type=int, default=512)
-    ap.add_argument("--density", type=float, default=30.0, help="target contact density")
+    ap.add_argument("--density", type=float, default=300.0, help="target contact density")
     ap.add_argument("--epochs", type=float, default=10.0)
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--lr", type=float, default=1e-3)

@@ -4,7 +4,8 @@ A window is a 2 Mbp stretch (400 bins of 5 kb) whose contact band out to 100 dia
 the surrogate predicts. Its input covers 500 kb of context on each side (600 bins). Training
 windows tile long simulated chromosomes (20 Mbp, one LEF density and processivity each, barrier
 density varying from one 2 Mbp segment to the next) exactly as a sweep tiles its chromosome, so
-the network learns windows inside a chromosome and at its ends.
+the network learns windows inside a chromosome and at its ends. Training targets are scaled by
+the mean of the whole chromosome's band, so a window keeps its level relative to the rest.
 
 Input channels per bin: barrier count, net blocking direction (+1 per REV-blocking, -1 per
 FWD-blocking barrier), summed occupancy, 1 inside the chromosome (0 in the padding beyond its
@@ -105,11 +106,22 @@ def valid_mask(nbins: int = OUT_BINS):
     return (i + d) < nbins
 
 
-def normalise(counts: np.ndarray) -> np.ndarray:
-    """log1p(10 * count / mean count over valid pixels), per window ([..., NDIAG, OUT_BINS])."""
-    m = valid_mask(counts.shape[-1])
-    mean = np.where(m, counts, 0).sum(axis=(-2, -1), keepdims=True) / m.sum()
+def normalise(counts: np.ndarray, mean=None) -> np.ndarray:
+    """log1p(10 * count / mean) per window ([..., NDIAG, OUT_BINS]); mean defaults to the
+    window's own mean count over its valid pixels (a scalar or one value per window)."""
+    if mean is None:
+        m = valid_mask(counts.shape[-1])
+        mean = np.where(m, counts, 0).sum(axis=(-2, -1), keepdims=True) / m.sum()
+    else:
+        mean = np.asarray(mean, np.float64).reshape((-1,) + (1,) * 2 if np.ndim(mean) else ())
     return np.log1p(TARGET_SCALE * counts / np.maximum(mean, 1e-12)).astype(np.float32)
+
+
+def band_mean(counts: np.ndarray) -> float:
+    """Mean count over the pixels (i, i + d < N) of a [NDIAG][N] band."""
+    nd, n = counts.shape
+    m = (np.arange(n)[None, :] + np.arange(nd)[:, None]) < n
+    return float(np.where(m, counts, 0).sum() / m.sum())
 
 
 def denormalise(y: np.ndarray) -> np.ndarray:
@@ -201,8 +213,9 @@ def chrom_options(spec: ChromSpec, contact_density: float, seed: int) -> dict:
 def simulate_chromosomes(ctx: S.Context, specs, ncells: int = 512, contact_density: float = 30.0,
                          seed: int = 0, chunk: int = 64):
     """Simulates each chromosome (fast mode, one parameter block per chromosome) and cuts it into
-    windows: (x [n * nwin][IN_BINS][6], counts [n * nwin][NDIAG][OUT_BINS], kernel seconds)."""
-    xs, ys = [], []
+    windows: (x [n * nwin][IN_BINS][6], counts [n * nwin][NDIAG][OUT_BINS], the band mean of each
+    window's chromosome [n * nwin], kernel seconds)."""
+    xs, ys, means = [], [], []
     kernel_s = 0.0
     starts = window_starts(CHROM_BINS)
     for c0 in range(0, len(specs), chunk):
@@ -223,5 +236,6 @@ def simulate_chromosomes(ctx: S.Context, specs, ncells: int = 512, contact_densi
             band, _ = job.band(k)
             full = band_to_window(band, plans[k].nrows, 0, CHROM_BINS)
             ys.extend(full[:, s:s + OUT_BINS] for s in starts)
+            means.extend([band_mean(full)] * len(starts))
         job.close()
-    return np.concatenate(xs), np.stack(ys), kernel_s
+    return np.concatenate(xs), np.stack(ys), np.array(means), kernel_s
