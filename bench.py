@@ -4,9 +4,11 @@ A step is one pass of the hot path over one batch: every (chromosome, cell) task
 simulated to its contact target, with the inputs and the bands resident in HBM. Prints one JSON
 line (cells/s as `value`, contacts/s and epochs/s beside it), with the roofline of the simulation
 kernel and the CPU oracle timed on this host's cores on a bounded sample of the same workload.
+A "surrogate" object beside them reports the surrogate-accelerated parameter sweep: matrices/s of
+the network and of the plain simulation, the judged correlation and the network's MFMA rate.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload genome|small|stress]
-                    [--mode fast|det] [--cpu-seconds S]
+                    [--mode fast|det] [--cpu-seconds S] [--sweep-points P | --no-sweep]
 """
 from __future__ import annotations
 
@@ -106,6 +108,58 @@ def cpu_baseline(wl, options, budget_s, nthreads):
     }
 
 
+def surrogate_flops(cfg) -> float:
+    """Algorithmic FLOPs of one surrogate forward pass over one window (multiply-adds of the real
+    channels counted as 2 FLOPs; padding and the pairwise sum not counted)."""
+    L, C, K = cfg.in_len, cfg.enc_ch, cfg.enc_k
+    f = 2.0 * L * C * cfg.in_ch * K + (cfg.enc_layers - 1) * 2.0 * L * C * C * K
+    f += 2 * 2.0 * cfg.out_len * C * cfg.head_ch
+    pix = cfg.ndiag * cfg.out_len
+    f += (cfg.head_layers - 1) * 2.0 * pix * cfg.head_ch * cfg.head_ch * 9 + 2.0 * pix * cfg.head_ch * 9
+    return f
+
+
+def surrogate_leg(ctx, npoints: int):
+    """The surrogate-accelerated sweep (4096 points over LEF density, processivity and an
+    occupancy scale on a fixed 20 Mbp layout, 5 % judged by 512-cell simulations), with the
+    surrogate's MFMA throughput."""
+    from modlefork_amd import surrogate as SG
+    from modlefork_amd import sweep as Sw
+
+    weights = os.path.join(ROOT, "modlefork_amd", "data", "surrogate_weights.npy")
+    if not os.path.exists(weights):
+        log("no trained surrogate weights: sweep leg skipped")
+        return None
+    net = SG.Surrogate(ctx, SG.default_config(max_batch=32))
+    SG.load_params(net, weights)
+    layout = Sw.fixed_layout()
+    res, _ = Sw.run_sweep(ctx, net, layout, Sw.sample_points(npoints), log=log)
+    nwin = len(Sw.window_starts(layout.size // 5000))
+    ms = net.predict_timed(32, 10) / 10
+    flops = surrogate_flops(net.cfg)
+    tflops = flops * 32 / (ms / 1000.0) / 1e12
+    dev_rate = 32 / (ms / 1000.0) / nwin
+    net.close()
+    corr = np.array(res.correlations, dtype=np.float64)
+    return {
+        "metric": "surrogate vs simulation matrices/s at stated correlation",
+        "points": res.npoints, "judged": len(res.judged), "cells_per_judged_point": 512,
+        "corr_threshold": 0.95,
+        "corr_mean": float(np.nanmean(corr)), "corr_min": float(np.nanmin(corr)),
+        "raw_corr_mean": float(np.nanmean(res.raw_correlations)),
+        "judge_reliability": float(np.nanmean(res.reliabilities)),
+        "passed": int(np.sum(corr >= 0.95)), "replaced": len(res.replaced),
+        "surrogate_rejected": res.fell_back,
+        "sim_matrices_per_s": res.sim_rate, "surrogate_matrices_per_s": res.surrogate_rate,
+        "surrogate_device_matrices_per_s": dev_rate,
+        "sweep_matrices_per_s": res.sweep_rate,
+        "speedup_inference": res.surrogate_rate / res.sim_rate,
+        "speedup_sweep": res.sweep_rate / res.sim_rate,
+        "mfma": {"bound": "mfma", "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s",
+                 "frac": tflops / 2500.0, "flops_per_window": flops, "dtype": "bf16"},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +171,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sweep-points", type=int, default=4096)
+    ap.add_argument("--no-sweep", action="store_true", help="skip the surrogate sweep leg")
     ap.add_argument("--threads", type=int, default=0,
                     help="fast mode: workgroup size per cell (256, 512 or 1024; 0 = default)")
     ap.add_argument("--phases", action="store_true",
@@ -233,6 +289,9 @@ def main():
     if cpu:
         line["speedup_vs_cpu"] = value / cpu["value"]
     job.close()
+    if not args.no_sweep:
+        log(f"surrogate sweep: {args.sweep_points} points")
+        line["surrogate"] = surrogate_leg(sim.ctx, args.sweep_points)
     print(json.dumps(line), flush=True)
 
 

@@ -202,6 +202,11 @@ int mgpu_job_create_multi(mgpu_ctx* ctx, const mgpu_params* params, uint64_t nch
                           int mode, mgpu_job** out);
 /* Zeroes the bands and simulates every cell of the job. */
 int mgpu_job_run(mgpu_job* job, mgpu_stats* stats);
+/* Records one run (the clears of bands and counters, then the persistent simulation launch, in
+ * which every epoch of every cell runs) into a HIP graph; later mgpu_job_run calls replay it.
+ * A tuning change or mgpu_job_release_graph drops it. No counterpart in the reference. */
+int mgpu_job_capture(mgpu_job* job);
+int mgpu_job_release_graph(mgpu_job* job);
 /* Copies chromosome c's band to host memory (nrows*ncols + 1 u32) and its missed count. */
 int mgpu_job_read_band(mgpu_job* job, uint64_t c, uint32_t* out_band, uint64_t* out_missed);
 /* Device pointer of chromosome c's band (valid until mgpu_job_destroy). */
@@ -210,13 +215,15 @@ void mgpu_job_destroy(mgpu_job* job);
 
 /* ---- launch tuning (no counterpart in the reference: it sizes the device launch) ----
  * threads_per_cell: fast mode's workgroup size per cell, 64, 128, 256, 512 or 1024 (0 = about
- * four LEFs per lane, at most 512).
+ * four LEFs per lane, at most 512). global_working_set: 1 keeps each cell's working set in
+ * device memory even where it fits in LDS (chromosomes too large for LDS always use memory).
  * Bands, missed counts and statistics do not depend on the tuning. grid (get only) is the number
  * of workgroups the next run launches. */
 typedef struct mgpu_tuning {
   uint32_t threads_per_cell;
   uint32_t grid;
-  uint32_t reserved[6];
+  uint32_t global_working_set;
+  uint32_t reserved[5];
 } mgpu_tuning;
 int mgpu_job_set_tuning(mgpu_job* job, const mgpu_tuning* t);
 int mgpu_job_get_tuning(mgpu_job* job, mgpu_tuning* t);

@@ -155,7 +155,8 @@ class MgpuTuning(C.Structure):
     _fields_ = [
         ("threads_per_cell", C.c_uint32),
         ("grid", C.c_uint32),
-        ("reserved", C.c_uint32 * 6),
+        ("global_working_set", C.c_uint32),
+        ("reserved", C.c_uint32 * 5),
     ]
 
 
@@ -199,6 +200,8 @@ SIGNATURES = [
                                         C.POINTER(MgpuChrom), C.POINTER(MgpuBarriers), U64P, U64P,
                                         C.POINTER(U64P), U64P, C.c_int, C.POINTER(C.c_void_p)]),
     ("mgpu_job_run", C.c_int, [C.c_void_p, C.POINTER(MgpuStats)]),
+    ("mgpu_job_capture", C.c_int, [C.c_void_p]),
+    ("mgpu_job_release_graph", C.c_int, [C.c_void_p]),
     ("mgpu_job_read_band", C.c_int, [C.c_void_p, C.c_uint64, U32P, U64P]),
     ("mgpu_job_band_device_ptr", C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p), U64P]),
     ("mgpu_job_destroy", None, [C.c_void_p]),

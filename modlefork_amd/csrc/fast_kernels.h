@@ -37,12 +37,22 @@ int fast_job_autotune(hipStream_t stream, FastJob* job, const WorkItem* d_items,
 // 512); results do not depend on it.
 int fast_job_set_threads(FastJob* job, uint32_t threads);
 uint32_t fast_job_threads(const FastJob* job);
+// global != 0: keep every cell's working set in a per-workgroup slab of device memory even when
+// it fits in LDS (the layout large chromosomes need); 0: back to LDS where it fits.
+int fast_job_set_storage(FastJob* job, int global);
+int fast_job_global_storage(const FastJob* job);
 uint32_t fast_job_grid(const FastJob* job);
 // Copies the first n per-phase tick totals (100 MHz wall clock, summed over cells) and clears them.
 hipError_t fast_phase_times(uint64_t* out, int n);
 void fast_job_destroy(FastJob* job);
 const char* fast_last_error();
 uint64_t fast_cell_bytes_host(uint32_t n, uint32_t nbar, uint32_t hist);
+// Clears `bytes` (a multiple of 4) of device memory with a kernel (capture-safe).
+hipError_t launch_zero_words(hipStream_t stream, void* p, uint64_t bytes);
+// Sets the dynamic-LDS limit of the kernel the next run launches (host-side, outside captures).
+hipError_t fast_job_prepare(FastJob* job);
+// While on, launches leave kernel attributes alone (they are being recorded into a graph).
+void fast_job_set_capturing(FastJob* job, bool on);
 // Philox4x32-10 of n (counter[4], key[2]) tuples: in holds 6 words per tuple, out 4.
 hipError_t launch_philox_kat(hipStream_t stream, const uint32_t* d_in, uint32_t* d_out, int n);
 

@@ -928,6 +928,14 @@ __global__ void __launch_bounds__(NT)
   }
 }
 
+// Clears n 32-bit words. Runs and captured graphs clear bands and counters with this kernel
+// rather than with memset nodes.
+__global__ void zero_words_kernel(uint32_t* p, uint64_t n) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (uint64_t)gridDim.x * blockDim.x)
+    p[e] = 0u;
+}
+
 __global__ void philox_kat_kernel(const uint32_t* in, uint32_t* out, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -963,6 +971,11 @@ struct FastImpl {
   uint32_t num_cus = 1;
   uint32_t threads = 256;  // workgroup size per cell
   uint64_t max_lefs = 1;
+  uint64_t max_bytes = 0;     // largest working set of one cell
+  uint64_t max_grid = 0;      // workgroups the slab has room for
+  bool fits_lds = true;
+  void* d_slab_own = nullptr;  // slab allocated when a job that fits LDS is moved to memory
+  bool capturing = false;      // launches are being recorded into a graph
   uint64_t nitems = 0;
   void* d_mem = nullptr;
 };
@@ -979,7 +992,7 @@ hipError_t launch_fast_as(hipStream_t stream, FastImpl* impl, const WorkItem* d_
                           uint64_t* d_cell_stats, uint32_t grid, const uint32_t* order,
                           uint64_t nitems) {
   const size_t shmem = kLds ? impl->lds_bytes : 0;
-  if (shmem > 0) {
+  if (shmem > 0 && !impl->capturing) {  // set before a capture by fast_job_prepare
     const hipError_t e =
         hipFuncSetAttribute(reinterpret_cast<const void*>(fast_cells_kernel<NT, kLds>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
@@ -1003,7 +1016,7 @@ hipError_t launch_fast(hipStream_t stream, FastImpl* impl, const WorkItem* d_ite
 // Launches the persistent kernel at the job's current workgroup size over order[0, nitems).
 hipError_t launch_with(hipStream_t stream, FastImpl* impl, const WorkItem* d_items,
                        uint64_t* d_cell_stats, const uint32_t* order, uint64_t nitems) {
-  hipError_t e = hipMemsetAsync(impl->d_next, 0, 8, stream);
+  hipError_t e = launch_zero_words(stream, impl->d_next, 8);
   if (e != hipSuccess) return e;
   const uint64_t keep = impl->nitems;
   impl->nitems = nitems;  // the grid never exceeds the cells launched
@@ -1077,6 +1090,8 @@ int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& dchroms,
   const uint64_t lds_total = 160 * 1024;
   const uint64_t lds_limit = lds_total - sizeof(Shared) - 512;
   impl->use_lds = max_bytes <= lds_limit;
+  impl->fits_lds = impl->use_lds;
+  impl->max_bytes = max_bytes;
   impl->lds_bytes = impl->use_lds ? max_bytes : 0;
   impl->per_cu = impl->use_lds
                      ? (uint32_t)std::max<uint64_t>(1, lds_total / (max_bytes + sizeof(Shared) + 512))
@@ -1090,6 +1105,7 @@ int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& dchroms,
   impl->slab_stride = impl->use_lds ? 0 : ((max_bytes + 255) & ~255ULL);
   // the slab holds one working set per workgroup of the largest grid any tuning launches
   const uint64_t max_grid = std::min<uint64_t>(items.size(), 16ull * impl->num_cus);
+  impl->max_grid = max_grid;
 
   const uint64_t bar_bytes = (4 * nbar_total + 64) * 4;
   const uint64_t chrom_bytes = (nch * sizeof(FastChrom) + 255) & ~255ULL;
@@ -1226,6 +1242,62 @@ int fast_job_autotune(hipStream_t stream, FastJob* job, const WorkItem* d_items,
   return 0;
 }
 
+template <int NT>
+hipError_t set_lds_limit(FastImpl* impl) {
+  if (!impl->use_lds || impl->lds_bytes == 0) return hipSuccess;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(fast_cells_kernel<NT, true>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)impl->lds_bytes);
+}
+
+hipError_t fast_job_prepare(FastJob* job) {
+  auto* impl = static_cast<FastImpl*>(job->impl);
+  if (!impl) return hipErrorInvalidValue;
+  switch (impl->threads) {
+    case 64: return set_lds_limit<64>(impl);
+    case 128: return set_lds_limit<128>(impl);
+    case 256: return set_lds_limit<256>(impl);
+    case 512: return set_lds_limit<512>(impl);
+    case 1024: return set_lds_limit<1024>(impl);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+void fast_job_set_capturing(FastJob* job, bool on) {
+  auto* impl = static_cast<FastImpl*>(job->impl);
+  if (impl) impl->capturing = on;
+}
+
+int fast_job_set_storage(FastJob* job, int global) {
+  auto* impl = static_cast<FastImpl*>(job->impl);
+  if (!impl) return -1;
+  if (!global) {
+    if (!impl->fits_lds) {
+      g_fast_err = "the working set of this job does not fit in LDS";
+      return -1;
+    }
+    impl->use_lds = true;
+    return 0;
+  }
+  if (impl->use_lds && !impl->d_slab) {
+    const uint64_t stride = (impl->max_bytes + 255) & ~255ULL;
+    const hipError_t e = hipMalloc(&impl->d_slab_own, stride * impl->max_grid);
+    if (e != hipSuccess) {
+      g_fast_err = std::string("slab allocation failed: ") + hipGetErrorString(e);
+      return -1;
+    }
+    impl->d_slab = static_cast<uint8_t*>(impl->d_slab_own);
+    impl->slab_stride = stride;
+  }
+  impl->use_lds = false;
+  impl->per_cu = 2;
+  return 0;
+}
+
+int fast_job_global_storage(const FastJob* job) {
+  const auto* impl = static_cast<const FastImpl*>(job->impl);
+  return impl && !impl->use_lds ? 1 : 0;
+}
+
 int fast_job_set_threads(FastJob* job, uint32_t threads) {
   auto* impl = static_cast<FastImpl*>(job->impl);
   if (!impl) return -1;
@@ -1265,11 +1337,21 @@ void fast_job_destroy(FastJob* job) {
   auto* impl = static_cast<FastImpl*>(job->impl);
   if (!impl) return;
   if (impl->d_mem) hipFree(impl->d_mem);
+  if (impl->d_slab_own) hipFree(impl->d_slab_own);
   delete impl;
   job->impl = nullptr;
 }
 
 const char* fast_last_error() { return g_fast_err.c_str(); }
+
+hipError_t launch_zero_words(hipStream_t stream, void* p, uint64_t bytes) {
+  const uint64_t n = bytes / 4;
+  if (n == 0) return hipSuccess;
+  const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 16384);
+  hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned)blocks), dim3(256), 0, stream,
+                     static_cast<uint32_t*>(p), n);
+  return hipGetLastError();
+}
 
 hipError_t launch_philox_kat(hipStream_t stream, const uint32_t* d_in, uint32_t* d_out, int n) {
   hipLaunchKernelGGL(philox_kat_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, d_in, d_out, n);

@@ -417,6 +417,8 @@ struct mgpu_job {
   bool use_lds = false;
   uint32_t grid = 0;
   FastJob fast{};
+  hipGraph_t graph = nullptr;       // captured run (mgpu_job_capture)
+  hipGraphExec_t graph_exec = nullptr;
 };
 
 namespace {
@@ -668,22 +670,81 @@ int mgpu_job_create_multi(mgpu_ctx* ctx, const mgpu_params* params, uint64_t nch
                     target_epochs, mode, out);
 }
 
+}  // extern "C"
+
+namespace {
+
+// The device work of one run: clear the bands, missed counts and queue counter (with a kernel,
+// so the sequence can be captured into a graph), then the simulation launch.
+hipError_t enqueue_clears(mgpu_job* job, hipStream_t s) {
+  hipError_t e = launch_zero_words(s, job->d_bands, job->band_total * 4);
+  if (e == hipSuccess) e = launch_zero_words(s, job->d_missed, job->nchroms * 8);
+  if (e == hipSuccess) e = launch_zero_words(s, job->d_next, 64);
+  return e;
+}
+
+hipError_t enqueue_simulation(mgpu_job* job, hipStream_t s) {
+  if (job->mode == MGPU_MODE_DETERMINISTIC)
+    return launch_det_cells(s, det_params(&job->p), job->d_chroms, job->d_items, job->nitems,
+                            job->d_next, job->d_scratch, job->stride, job->use_lds, job->grid,
+                            job->d_cell_stats);
+  return fast_job_run(s, &job->fast, job->d_items, job->d_cell_stats, job->ctx->phase_timers);
+}
+
+void drop_graph(mgpu_job* job) {
+  if (job->graph_exec) hipGraphExecDestroy(job->graph_exec);
+  if (job->graph) hipGraphDestroy(job->graph);
+  job->graph_exec = nullptr;
+  job->graph = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mgpu_job_capture(mgpu_job* job) {
+  if (!job) return fail(nullptr, MGPU_EINVAL, "null job");
+  mgpu_ctx* ctx = job->ctx;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  drop_graph(job);
+  if (job->mode == MGPU_MODE_FAST) HIPCHK(ctx, fast_job_prepare(&job->fast));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  if (job->mode == MGPU_MODE_FAST) fast_job_set_capturing(&job->fast, true);
+  hipError_t e = enqueue_clears(job, ctx->stream);
+  if (e == hipSuccess) e = enqueue_simulation(job, ctx->stream);
+  if (job->mode == MGPU_MODE_FAST) fast_job_set_capturing(&job->fast, false);
+  hipGraph_t g = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+  if (e == hipSuccess) e = e2;
+  if (e == hipSuccess) e = hipGraphInstantiate(&job->graph_exec, g, nullptr, nullptr, 0);
+  job->graph = g;
+  if (e != hipSuccess) {
+    drop_graph(job);
+    return fail(ctx, MGPU_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
+  }
+  return MGPU_OK;
+}
+
+int mgpu_job_release_graph(mgpu_job* job) {
+  if (!job) return fail(nullptr, MGPU_EINVAL, "null job");
+  drop_graph(job);
+  return MGPU_OK;
+}
+
 int mgpu_job_run(mgpu_job* job, mgpu_stats* stats) {
   if (!job) return fail(nullptr, MGPU_EINVAL, "null job");
   mgpu_ctx* ctx = job->ctx;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const auto t0 = std::chrono::steady_clock::now();
-  HIPCHK(ctx, hipMemsetAsync(job->d_bands, 0, job->band_total * 4, ctx->stream));
-  HIPCHK(ctx, hipMemsetAsync(job->d_missed, 0, job->nchroms * 8, ctx->stream));
-  HIPCHK(ctx, hipMemsetAsync(job->d_next, 0, 64, ctx->stream));
-  HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-  if (job->mode == MGPU_MODE_DETERMINISTIC) {
-    HIPCHK(ctx, launch_det_cells(ctx->stream, det_params(&job->p), job->d_chroms, job->d_items,
-                                 job->nitems, job->d_next, job->d_scratch, job->stride,
-                                 job->use_lds, job->grid, job->d_cell_stats));
+  if (job->graph_exec) {
+    // the captured clears and launch; the events bracket the whole graph
+    HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    HIPCHK(ctx, hipGraphLaunch(job->graph_exec, ctx->stream));
   } else {
-    HIPCHK(ctx, fast_job_run(ctx->stream, &job->fast, job->d_items, job->d_cell_stats,
-                             ctx->phase_timers));
+    HIPCHK(ctx, enqueue_clears(job, ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    HIPCHK(ctx, enqueue_simulation(job, ctx->stream));
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -711,8 +772,12 @@ int mgpu_job_run(mgpu_job* job, mgpu_stats* stats) {
 
 int mgpu_job_set_tuning(mgpu_job* job, const mgpu_tuning* t) {
   if (!job || !t) return fail(job ? job->ctx : nullptr, MGPU_EINVAL, "null argument");
+  drop_graph(job);  // a captured run keeps the launch shape it was recorded with
   if (job->mode != MGPU_MODE_FAST) return MGPU_OK;  // deterministic mode: one lane per cell
   if (fast_job_set_threads(&job->fast, t->threads_per_cell) != 0)
+    return fail(job->ctx, MGPU_EINVAL, "%s", fast_last_error());
+  if (t->global_working_set != (uint32_t)fast_job_global_storage(&job->fast) &&
+      fast_job_set_storage(&job->fast, t->global_working_set ? 1 : 0) != 0)
     return fail(job->ctx, MGPU_EINVAL, "%s", fast_last_error());
   return MGPU_OK;
 }
@@ -721,6 +786,7 @@ int mgpu_job_autotune(mgpu_job* job, uint64_t sample_cells, mgpu_tuning* chosen,
   if (!job) return fail(nullptr, MGPU_EINVAL, "null job");
   mgpu_ctx* ctx = job->ctx;
   HIPCHK(ctx, hipSetDevice(ctx->device));
+  drop_graph(job);
   if (job->mode == MGPU_MODE_FAST) {
     uint32_t best = 0;
     if (fast_job_autotune(ctx->stream, &job->fast, job->d_items, job->d_cell_stats, sample_cells,
@@ -736,6 +802,7 @@ int mgpu_job_get_tuning(mgpu_job* job, mgpu_tuning* t) {
   if (job->mode == MGPU_MODE_FAST) {
     t->threads_per_cell = fast_job_threads(&job->fast);
     t->grid = fast_job_grid(&job->fast);
+    t->global_working_set = (uint32_t)fast_job_global_storage(&job->fast);
   } else {
     t->threads_per_cell = 64;
     t->grid = job->grid;
@@ -768,6 +835,7 @@ int mgpu_job_band_device_ptr(mgpu_job* job, uint64_t c, void** dptr, uint64_t* n
 void mgpu_job_destroy(mgpu_job* job) {
   if (!job) return;
   hipSetDevice(job->ctx->device);
+  drop_graph(job);
   fast_job_destroy(&job->fast);
   if (job->d_pool) hipFree(job->d_pool);
   delete job;

@@ -176,10 +176,19 @@ class Job:
         self.ctx.check(L.lib().mgpu_job_run(self.ptr, C.byref(st)))
         return st
 
-    def set_tuning(self, threads_per_cell: int = 0):
-        t = L.MgpuTuning()
+    def set_tuning(self, threads_per_cell: int = 0, global_working_set: bool | None = None):
+        t = self.tuning()
         t.threads_per_cell = threads_per_cell
+        if global_working_set is not None:
+            t.global_working_set = 1 if global_working_set else 0
         self.ctx.check(L.lib().mgpu_job_set_tuning(self.ptr, C.byref(t)))
+
+    def capture(self):
+        """Records a run into a HIP graph that later runs replay."""
+        self.ctx.check(L.lib().mgpu_job_capture(self.ptr))
+
+    def release_graph(self):
+        self.ctx.check(L.lib().mgpu_job_release_graph(self.ptr))
 
     def autotune(self, sample_cells: int = 0):
         """Times each workgroup size on a sample of cells and keeps the fastest; returns

@@ -155,7 +155,7 @@ def bf16_round(x: np.ndarray) -> np.ndarray:
 # data, training and evaluation
 # ---------------------------------------------------------------------------------------------
 def make_dataset(ctx, nchrom: int, seed: int, ncells: int = 512, contact_density: float = 30.0,
-                 cell_seed: int | None = None):
+                 cell_seed: int | None = None, log=None):
     """Windows tiling nchrom simulated 20 Mbp chromosomes: (x, y = normalised log targets, raw
     counts, device seconds). Layouts come from `seed`, the cells' random streams from
     `cell_seed` (default: seed)."""
@@ -163,7 +163,8 @@ def make_dataset(ctx, nchrom: int, seed: int, ncells: int = 512, contact_density
 
     specs = Wn.sample_chromosomes(nchrom, seed)
     x, counts, means, dev_s = Wn.simulate_chromosomes(ctx, specs, ncells, contact_density,
-                                                      seed if cell_seed is None else cell_seed)
+                                                      seed if cell_seed is None else cell_seed,
+                                                      log=log)
     return x, Wn.normalise(counts, means), counts, dev_s
 
 
@@ -242,8 +243,8 @@ def main(argv=None):
     ctx = S.Context(0)
     t0 = time.perf_counter()
     log(f"simulating {a.chromosomes} training + {a.val} validation chromosomes, {a.cells} cells")
-    x, y, _, dev_tr = make_dataset(ctx, a.chromosomes, a.seed, a.cells, a.density)
-    xv, yv, cv, dev_va = make_dataset(ctx, a.val, a.seed + 1000, a.cells, a.density)
+    x, y, _, dev_tr = make_dataset(ctx, a.chromosomes, a.seed, a.cells, a.density, log=log)
+    xv, yv, cv, dev_va = make_dataset(ctx, a.val, a.seed + 1000, a.cells, a.density, log=log)
     t_data = time.perf_counter() - t0
     log(f"data: {len(x)} + {len(xv)} windows, {t_data:.1f} s wall, {dev_tr + dev_va:.1f} s in "
         f"the simulation kernel, {(a.chromosomes + a.val) / (dev_tr + dev_va):.1f} chromosomes/s")

@@ -211,7 +211,7 @@ def chrom_options(spec: ChromSpec, contact_density: float, seed: int) -> dict:
 
 
 def simulate_chromosomes(ctx: S.Context, specs, ncells: int = 512, contact_density: float = 30.0,
-                         seed: int = 0, chunk: int = 64):
+                         seed: int = 0, chunk: int = 64, log=None):
     """Simulates each chromosome (fast mode, one parameter block per chromosome) and cuts it into
     windows: (x [n * nwin][IN_BINS][6], counts [n * nwin][NDIAG][OUT_BINS], the band mean of each
     window's chromosome [n * nwin], kernel seconds)."""
@@ -232,6 +232,8 @@ def simulate_chromosomes(ctx: S.Context, specs, ncells: int = 512, contact_densi
         job = S.Job(ctx, params, chroms, bars, plans, L.MODE_FAST)
         st = job.run()
         kernel_s += st.kernel_ms / 1000.0
+        if log:
+            log(f"  simulated chromosomes {c0 + len(part)}/{len(specs)} ({kernel_s:.1f} s in kernel)")
         for k in range(len(part)):
             band, _ = job.band(k)
             full = band_to_window(band, plans[k].nrows, 0, CHROM_BINS)

@@ -100,6 +100,50 @@ def test_fast_mode_band_does_not_depend_on_workgroup_size(ctx, threads):
     assert (sa.epochs, sa.burnin_epochs, sa.contacts) == (sb.epochs, sb.burnin_epochs, sb.contacts)
 
 
+def test_working_set_in_memory_gives_the_lds_band(ctx):
+    """The same cells with their working sets in a device-memory slab (the layout chromosomes too
+    large for LDS use) instead of LDS: identical band and statistics."""
+    wl = W.small(ncells=8)
+    a, sa = run_device(ctx, wl, L.MODE_FAST)
+    sim = S.Simulation(wl.options, ctx)
+    names, chroms, bars, plans = sim.prepare(wl)
+    job = S.Job(ctx, sim.params, chroms, bars, plans, L.MODE_FAST)
+    job.set_tuning(0, global_working_set=True)
+    assert job.tuning().global_working_set == 1
+    st = job.run()
+    band, missed = job.band(0)
+    job.close()
+    say(f"memory working set: {st.kernel_ms:.1f} ms, LDS: {sa.kernel_ms:.1f} ms")
+    assert np.array_equal(band, a["chrS"][0]) and missed == a["chrS"][1]
+    assert (st.epochs, st.burnin_epochs, st.contacts) == (sa.epochs, sa.burnin_epochs, sa.contacts)
+
+
+@pytest.mark.parametrize("mode", [L.MODE_FAST, L.MODE_DETERMINISTIC])
+def test_captured_graph_replays_match_eager_runs(ctx, mode):
+    """A run captured into a HIP graph and replayed three times, with eager work between the
+    replays (an eager run, band reads, synchronisations): every replay gives the eager band."""
+    wl = W.small(ncells=4)
+    sim = S.Simulation(wl.options, ctx)
+    names, chroms, bars, plans = sim.prepare(wl)
+    job = S.Job(ctx, sim.params, chroms, bars, plans, mode)
+    se = job.run()
+    eager, eager_missed = job.band(0)
+    job.capture()
+    for r in range(3):
+        sg = job.run()
+        band, missed = job.band(0)
+        say(f"mode {mode} replay {r}: {sg.kernel_ms:.1f} ms (eager {se.kernel_ms:.1f} ms)")
+        assert np.array_equal(band, eager) and missed == eager_missed
+        assert (sg.epochs, sg.contacts) == (se.epochs, se.contacts)
+        if r == 1:  # eager work between replays
+            job.release_graph()
+            job.run()
+            band2, _ = job.band(0)
+            assert np.array_equal(band2, eager)
+            job.capture()
+    job.close()
+
+
 def test_autotune_keeps_results(ctx):
     """The autotuner times each workgroup size on a sample of cells; the run that follows gives
     the band of the default launch."""
