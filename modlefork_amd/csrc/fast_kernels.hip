@@ -210,6 +210,7 @@ __device__ void rerank(uint16_t* ranks, const uint32_t* pos, uint32_t nact, uint
   const uint32_t batches = (nkept + kNewCap > nact || nact - nkept <= kNewCap)
                                ? 1u
                                : (nact + kNewCap - 1) / kNewCap;
+  bool written = false;
   for (uint32_t bt = 0; bt < batches; ++bt) {
     const uint32_t lo_id = batches == 1 ? 0 : bt * kNewCap;
     const uint32_t hi_id = batches == 1 ? nact : min(nact, lo_id + kNewCap);
@@ -226,6 +227,8 @@ __device__ void rerank(uint16_t* ranks, const uint32_t* pos, uint32_t nact, uint
     for (uint32_t i = k + threadIdx.x; i < m; i += NT) keys[i] = ~0ULL;
     __syncthreads();
     bitonic<NT>(keys, m);
+    // the last batch merges straight into the rank array (nothing reads it any more)
+    const bool last = bt + 1 == batches;
     for (uint32_t a = threadIdx.x; a < nkept; a += NT) {
       const uint64_t key = pos_key(pos, tmp_a[a]);
       uint32_t lo = 0, hi = k;
@@ -234,7 +237,8 @@ __device__ void rerank(uint16_t* ranks, const uint32_t* pos, uint32_t nact, uint
         if (keys[mid] < key) lo = mid + 1;
         else hi = mid;
       }
-      tmp_b[a + lo] = tmp_a[a];
+      if (last) ranks[a + lo] = (uint16_t)tmp_a[a];
+      else tmp_b[a + lo] = tmp_a[a];
     }
     for (uint32_t b = threadIdx.x; b < k; b += NT) {
       const uint64_t key = keys[b];
@@ -244,15 +248,22 @@ __device__ void rerank(uint16_t* ranks, const uint32_t* pos, uint32_t nact, uint
         if (pos_key(pos, tmp_a[mid]) < key) lo = mid + 1;
         else hi = mid;
       }
-      tmp_b[b + lo] = (uint32_t)key;
+      if (last) ranks[b + lo] = (uint16_t)key;
+      else tmp_b[b + lo] = (uint32_t)key;
     }
     __syncthreads();
     nkept += k;
-    for (uint32_t i = threadIdx.x; i < nkept; i += NT) tmp_a[i] = tmp_b[i];
+    if (last) {
+      written = true;
+    } else {
+      for (uint32_t i = threadIdx.x; i < nkept; i += NT) tmp_a[i] = tmp_b[i];
+      __syncthreads();
+    }
+  }
+  if (!written) {
+    for (uint32_t i = threadIdx.x; i < nact; i += NT) ranks[i] = (uint16_t)tmp_a[i];
     __syncthreads();
   }
-  for (uint32_t i = threadIdx.x; i < nact; i += NT) ranks[i] = (uint16_t)tmp_a[i];
-  __syncthreads();
 }
 
 static_assert(kFastPhases == kFastPhaseCount, "phase count of the timers and of the host API");
@@ -336,8 +347,9 @@ __device__ void secondary_side(const FastParams& P, const Rng& R, uint32_t epoch
         const int32_t ts = target(unit_of(s1 - 1));
         const int32_t dist = (int32_t)j - (int32_t)(s1 - 1);
         const bool reach = rev_side ? target(l) <= ts + dist - 1 : target(l) >= ts - dist + 1;
-        const U4 d = R.draw(l, epoch, rev_side ? kSSecondary : kSSecondary + 16);
-        ok = reach && bern(d.x, P.t_bypass_keep);
+        // the bypass trial only for units that reach the one ahead of them
+        ok = reach && bern(R.draw(l, epoch, rev_side ? kSSecondary : kSSecondary + 16).x,
+                           P.t_bypass_keep);
       }
       fail = ok ? 0 : j + 1;
     }
@@ -734,7 +746,9 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     // ---- LEF-BAR (simulation_detect_collisions.cpp:128-252): per active barrier, the nearest
     // unit moving towards it; the closest blocking barrier wins ----
     uint32_t* sorted = c.scratch;
-    for (int side = 0; side < 2; ++side) {
+    // fwd side first, so the rev positions in rank order stay in `sorted` for the primary step
+    for (int it = 0; it < 2; ++it) {
+      const int side = 1 - it;
       const uint16_t* ranks = side == 0 ? c.rr : c.fr;
       const uint32_t* pos = side == 0 ? c.rev : c.fwd;
       for (uint32_t k = threadIdx.x; k < n; k += NT) sorted[k] = pos[ranks[k]];
@@ -780,8 +794,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
 
     // ---- primary LEF-LEF (simulation_detect_collisions.cpp:255-401): a fwd unit that is the
     // last one before a rev unit pairs with that rev unit ----
-    // rev positions in rank order are still in `sorted` only for the fwd side; rebuild them.
-    for (uint32_t k = threadIdx.x; k < n; k += NT) sorted[k] = c.rev[c.rr[k]];
+    // `sorted` holds the rev positions in rank order (the LEF-BAR pass above ended with them)
     __syncthreads();
     mark(P, S, 8);
     if (n5 < n && n3 < n) {
