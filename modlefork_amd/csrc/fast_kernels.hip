@@ -320,9 +320,13 @@ __device__ inline uint32_t poisson(const Rng& R, double mean, uint32_t tag) {
 // unit j behind seed s stalls when every unit in (s, j] reaches its predecessor's target and
 // keeps its collision (does not bypass). X[j] packs (s + 1) << 16 | (last failure + 1), so a
 // single prefix-max yields the last failure inside j's own chain.
+// Each lane owns a contiguous run of j; the two prefix-max scans are fused with the passes
+// around them: seed flags are formed in the first scan's reduction, failure flags while its
+// prefix is applied, and the moves are corrected while the second scan's prefix is applied.
+// Returns the number of units that bypassed, listed (chain order) at the start of c.scratch.
 template <int NT>
-__device__ void secondary_side(const FastParams& P, const Rng& R, uint32_t epoch, Cell& c,
-                               Shared& S, uint32_t n, bool rev_side, uint32_t lo_j) {
+__device__ uint32_t secondary_side(const FastParams& P, const Rng& R, uint32_t epoch, Cell& c,
+                                   Shared& S, uint32_t n, bool rev_side, uint32_t lo_j) {
   uint16_t* ranks = rev_side ? c.rr : c.fr;
   uint32_t* coll = rev_side ? c.rc : c.fc;
   uint32_t* mv = rev_side ? c.rm : c.fm;
@@ -332,18 +336,26 @@ __device__ void secondary_side(const FastParams& P, const Rng& R, uint32_t epoch
   auto target = [&](uint32_t l) -> int32_t {
     return rev_side ? (int32_t)pos[l] - (int32_t)mv[l] : (int32_t)pos[l] + (int32_t)mv[l];
   };
-  for (uint32_t j = threadIdx.x; j < n; j += NT)
-    X[j] = ck_stalled(coll[unit_of(j)]) ? j + 1 : 0;
-  __syncthreads();
-  block_scan<NT>(X, (int)n, OpMaxU{}, 0u, false, S.shu);
-  // failure flags given the chain start
-  for (uint32_t j = threadIdx.x; j < n; j += NT) {
-    const uint32_t s1 = X[j];  // seed + 1, 0 if none
-    const uint32_t l = unit_of(j);
+  const uint32_t chunk = (n + NT - 1) / NT;
+  const uint32_t jb = threadIdx.x * chunk, je = min(n, jb + chunk);
+  // seeds: X[j] = j + 1 where unit j is already stalled
+  uint32_t m = 0;
+  for (uint32_t j = jb; j < je; ++j) {
+    const uint32_t v = ck_stalled(coll[unit_of(j)]) ? j + 1 : 0;
+    X[j] = v;
+    m = v > m ? v : m;
+  }
+  uint32_t run = block_exclusive<NT>(m, OpMaxU{}, 0u, S.shu);
+  // chain start s1 (prefix max of the seeds) and the failure flag of each j, packed
+  uint32_t mf = 0;
+  for (uint32_t j = jb; j < je; ++j) {
+    run = X[j] > run ? X[j] : run;
+    const uint32_t s1 = run;  // seed + 1, 0 if none
     uint32_t fail = 0;
     if (s1 != 0 && s1 - 1 != j) {
       bool ok = j >= lo_j;
       if (ok) {
+        const uint32_t l = unit_of(j);
         const int32_t ts = target(unit_of(s1 - 1));
         const int32_t dist = (int32_t)j - (int32_t)(s1 - 1);
         const bool reach = rev_side ? target(l) <= ts + dist - 1 : target(l) >= ts - dist + 1;
@@ -353,12 +365,17 @@ __device__ void secondary_side(const FastParams& P, const Rng& R, uint32_t epoch
       }
       fail = ok ? 0 : j + 1;
     }
-    X[j] = (s1 << 16) | fail;
+    const uint32_t w = (s1 << 16) | fail;
+    X[j] = w;
+    mf = w > mf ? w : mf;
   }
-  __syncthreads();
-  block_scan<NT>(X, (int)n, OpMaxU{}, 0u, false, S.shu);
-  for (uint32_t j = threadIdx.x; j < n; j += NT) {
-    const uint32_t s1 = X[j] >> 16;
+  uint32_t run2 = block_exclusive<NT>(mf, OpMaxU{}, 0u, S.shu);
+  // corrections: X[j] becomes the prefix max; prev is the prefix max at j - 1
+  int32_t navoid = 0;
+  for (uint32_t j = jb; j < je; ++j) {
+    const uint32_t prev = run2;
+    run2 = X[j] > run2 ? X[j] : run2;
+    const uint32_t s1 = run2 >> 16;
     if (s1 == 0 || s1 - 1 == j || j < lo_j) continue;
     const uint32_t l = unit_of(j);
     if (ck_stalled(coll[l])) continue;
@@ -366,37 +383,28 @@ __device__ void secondary_side(const FastParams& P, const Rng& R, uint32_t epoch
     const uint32_t lead = unit_of(j - 1);
     const int32_t dist = (int32_t)j - (int32_t)s;
     const int32_t ts = target(unit_of(s));
-    const uint32_t last_fail = X[j] & 0xFFFFu;
+    const uint32_t last_fail = run2 & 0xFFFFu;
     if (last_fail == 0) {
       // stalled one bp behind the unit ahead of it in the chain
       const int32_t t = rev_side ? ts + dist : ts - dist;
-      const int32_t m = rev_side ? (int32_t)pos[l] - t : t - (int32_t)pos[l];
-      mv[l] = m > 0 ? (uint32_t)m : 0u;
+      const int32_t mm = rev_side ? (int32_t)pos[l] - t : t - (int32_t)pos[l];
+      mv[l] = mm > 0 ? (uint32_t)mm : 0u;
       coll[l] = ck(kCkSecondary, lead);
-    } else if (last_fail == j + 1 && (X[j - 1] & 0xFFFFu) == 0) {
+    } else if (last_fail == j + 1 && (prev & 0xFFFFu) == 0) {
       const bool reach = rev_side ? target(l) <= ts + dist - 1 : target(l) >= ts - dist + 1;
-      if (reach) coll[l] = ck(kCkAvoided, lead);
+      if (reach) {
+        coll[l] = ck(kCkAvoided, lead);
+        ++navoid;
+      }
     }
   }
-  __syncthreads();
-}
-
-// Rank positions of the units whose secondary collision was avoided on one side, in chain
-// order (rev: ascending rank, fwd: descending rank), written to list; returns their count.
-// A scan over contiguous chunks keeps the order independent of the workgroup size.
-template <int NT>
-__device__ uint32_t collect_avoided(const uint16_t* ranks, const uint32_t* coll, uint32_t n,
-                                    bool rev_side, uint32_t* list, Shared& S) {
-  const uint32_t c = (n + NT - 1) / NT;
-  const uint32_t b = threadIdx.x * c, e = min(n, b + c);
-  auto avoided = [&](uint32_t j) {
-    return ck_kind(coll[ranks[rev_side ? j : n - 1 - j]]) == kCkAvoided;
-  };
-  int32_t cnt = 0;
-  for (uint32_t j = b; j < e; ++j) cnt += avoided(j);
-  int32_t off = block_exclusive<NT>(cnt, OpAddI{}, 0, S.shi);
-  for (uint32_t j = b; j < e; ++j)
-    if (avoided(j)) list[off++] = rev_side ? j : n - 1 - j;
+  // bypassing units, compacted in chain order (the exclusive scan's barriers also end every
+  // lane's reads of X, which the list overwrites)
+  int32_t off = block_exclusive<NT>(navoid, OpAddI{}, 0, S.shi);
+  if (navoid) {
+    for (uint32_t j = jb; j < je; ++j)
+      if (ck_kind(coll[unit_of(j)]) == kCkAvoided) X[off++] = rev_side ? j : n - 1 - j;
+  }
   if (threadIdx.x == NT - 1) S.navoid = (uint32_t)off;
   __syncthreads();
   return S.navoid;
@@ -799,14 +807,24 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     mark(P, S, 8);
     if (n5 < n && n3 < n) {
       const uint32_t i2 = n - (n3 ? n3 - 1 : 0);
-      for (uint32_t k = threadIdx.x; k + 1 < i2; k += NT) {
+      // contiguous runs of fwd ranks per lane: the first rev unit beyond each fwd unit only moves
+      // forward along the run (both orders are sorted by position)
+      const uint32_t kmax = i2 > 0 ? i2 - 1 : 0;
+      const uint32_t ckr = (kmax + NT - 1) / NT;
+      const uint32_t k_begin = threadIdx.x * ckr, k_end = min(kmax, k_begin + ckr);
+      uint32_t lo = n5;
+      for (uint32_t k = k_begin; k < k_end; ++k) {
         const uint32_t fl = c.fr[k];
         const uint32_t fp = c.fwd[fl];
-        uint32_t lo = n5, hi = n;
-        while (lo < hi) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (sorted[mid] <= fp) lo = mid + 1;
-          else hi = mid;
+        if (k == k_begin) {
+          uint32_t hi = n;
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (sorted[mid] <= fp) lo = mid + 1;
+            else hi = mid;
+          }
+        } else {
+          while (lo < n && sorted[lo] <= fp) ++lo;
         }
         if (lo >= n) continue;
         const uint32_t rp = sorted[lo];
@@ -870,16 +888,14 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     // The rev fix-ups touch only rev-side state (and read fwd positions, which the fwd pass
     // does not change), so each side is fixed right after its own pass, rev first as in
     // process_collisions (simulation.cpp:925-955).
-    secondary_side<NT>(P, R, epoch, c, S, n, true, max(1u, n5));
     {
-      const uint32_t na = collect_avoided<NT>(c.rr, c.rc, n, true, c.scratch, S);
+      const uint32_t na = secondary_side<NT>(P, R, epoch, c, S, n, true, max(1u, n5));
       if (threadIdx.x == 0 && na) fix_avoided_rev(c, ch, c.scratch, na);
       __syncthreads();
     }
     mark(P, S, 11);
-    secondary_side<NT>(P, R, epoch, c, S, n, false, (n3 ? n3 - 1 : 0) + 1);
     {
-      const uint32_t na = collect_avoided<NT>(c.fr, c.fc, n, false, c.scratch, S);
+      const uint32_t na = secondary_side<NT>(P, R, epoch, c, S, n, false, (n3 ? n3 - 1 : 0) + 1);
       if (threadIdx.x == 0 && na) fix_avoided_fwd(c, ch, n, c.scratch, na);
       __syncthreads();
     }
