@@ -215,6 +215,10 @@ def main():
     if args.threads:
         job.set_tuning(args.threads)
     tuning = job.tuning()
+    launches = [{"threads": int(tuning.tier_threads[k]), "grid": int(tuning.tier_grid[k])}
+                for k in range(int(tuning.launches))]
+    shape = "+".join(str(x["threads"]) for x in launches)
+    log(f"launches (threads per cell, grid): {[(x['threads'], x['grid']) for x in launches]}")
 
     for i in range(args.warmup):
         st = job.run()
@@ -231,7 +235,7 @@ def main():
         sim.ctx.phase_times(True)
         job.run()
         ph = sim.ctx.phase_times(False).astype(np.float64)
-        log(f"epoch phases (share of cell time, {int(tuning.threads_per_cell)} threads per cell):")
+        log(f"epoch phases (share of cell time, {shape} threads per cell):")
         for k, v in enumerate(ph):
             if v:
                 log(f"  {L.PHASE_NAMES[k]:28s} {100.0 * v / ph.sum():5.1f} %")
@@ -251,8 +255,8 @@ def main():
         with open(tfile) as fh:
             prof = json.load(fh)
         # counters were taken for one launch shape; other shapes report no traffic figure
-        if prof.get("threads_per_cell", 0) in (0, int(tuning.threads_per_cell)):
-            traffic = prof.get("hbm_bytes_per_launch")
+        if prof.get("launch_shape") == shape:
+            traffic = prof.get("hbm_bytes_per_run")
 
     cpu = None
     if not args.no_cpu:
@@ -278,7 +282,7 @@ def main():
                    "bin_size": int(sim.params.bin_size),
                    "diagonal_width": int(sim.params.diagonal_width),
                    "lefs_per_mbp": float(sim.params.lefs_per_mbp), "parallelism": "none",
-                   "threads_per_cell": int(tuning.threads_per_cell), "grid": int(tuning.grid)},
+                   "launches": launches},
         "contacts_per_sec": contacts * args.steps / wall,
         "epochs_per_sec": epochs * args.steps / wall,
         "kernel_ms": float(np.mean(kernel_ms)),

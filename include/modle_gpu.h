@@ -214,22 +214,29 @@ int mgpu_job_band_device_ptr(mgpu_job* job, uint64_t c, void** dptr, uint64_t* n
 void mgpu_job_destroy(mgpu_job* job);
 
 /* ---- launch tuning (no counterpart in the reference: it sizes the device launch) ----
- * threads_per_cell: fast mode's workgroup size per cell, 64, 128, 256, 512 or 1024 (0 = about
- * four LEFs per lane, at most 512). global_working_set: 1 keeps each cell's working set in
- * device memory even where it fits in LDS (chromosomes too large for LDS always use memory).
- * Bands, missed counts and statistics do not depend on the tuning. grid (get only) is the number
- * of workgroups the next run launches. */
+ * Fast mode runs its cells in persistent launches. By default (threads_per_cell 0) it makes one
+ * launch per size tier: cells whose working set lets 1, 2 or 4 of them share one CU's LDS run at
+ * 1024, 512 or 256 lanes per cell (16 waves per CU in every tier), cells too large for LDS run
+ * from device memory at 512 lanes; tier_threads[k] (set, when nonzero) overrides tier k's size.
+ * threads_per_cell 64, 128, 256, 512 or 1024 instead runs every cell in one launch of that size.
+ * global_working_set: 1 keeps every cell's working set in device memory, in one launch (0 or the
+ * chosen threads_per_cell: 512 lanes by default). Bands, missed counts and statistics do not
+ * depend on the tuning. Get only: launches (the launches the next run makes, at most 4), their
+ * sizes in tier_threads and grids in tier_grid, and grid (the largest of them). */
 typedef struct mgpu_tuning {
   uint32_t threads_per_cell;
   uint32_t grid;
   uint32_t global_working_set;
-  uint32_t reserved[5];
+  uint32_t launches;
+  uint32_t tier_threads[4];
+  uint32_t tier_grid[4];
 } mgpu_tuning;
 int mgpu_job_set_tuning(mgpu_job* job, const mgpu_tuning* t);
 int mgpu_job_get_tuning(mgpu_job* job, mgpu_tuning* t);
 /* Times each fast-mode workgroup size (64 to 1024 lanes) on sample_cells cells spread over the
- * job (0: all), keeps the fastest for later runs and reports it in *chosen (may be null); ms (5
- * entries, may be null) receives the times. Deterministic mode: no-op. */
+ * job (0: all), per size tier (or for the single launch a global_working_set job makes), keeps
+ * the fastest for later runs and reports the tuning in *chosen (may be null); ms (5 entries,
+ * may be null) receives the times summed over the tiers. Deterministic mode: no-op. */
 int mgpu_job_autotune(mgpu_job* job, uint64_t sample_cells, mgpu_tuning* chosen, float* ms);
 
 /* ---- step-level entry points (the reference's ENABLE_TESTING hooks, simulation.hpp:499-651),

@@ -156,7 +156,9 @@ class MgpuTuning(C.Structure):
         ("threads_per_cell", C.c_uint32),
         ("grid", C.c_uint32),
         ("global_working_set", C.c_uint32),
-        ("reserved", C.c_uint32 * 5),
+        ("launches", C.c_uint32),
+        ("tier_threads", C.c_uint32 * 4),
+        ("tier_grid", C.c_uint32 * 4),
     ]
 
 

@@ -28,19 +28,27 @@ int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& chroms,
 // timers != 0: accumulate per-phase wall-clock ticks (read with fast_phase_times)
 hipError_t fast_job_run(hipStream_t stream, FastJob* job, const WorkItem* d_items,
                         uint64_t* d_cell_stats, int timers);
-// Times every workgroup size on `sample` cells spread over the work order (0: all cells), keeps
-// the fastest and returns it in *best; ms (5 entries, may be null) receives the times of 64 to
-// 1024 lanes. The bands are left dirty (runs zero them first).
+// Times every workgroup size on `sample` cells spread over the work order (0: all cells) and
+// keeps the fastest: per size tier when the job launches by tiers, else for its single launch.
+// ms (5 entries, may be null) receives the summed times of 64 to 1024 lanes. The bands are left
+// dirty (runs zero them first).
 int fast_job_autotune(hipStream_t stream, FastJob* job, const WorkItem* d_items,
-                      uint64_t* d_cell_stats, uint64_t sample, uint32_t* best, float* ms);
-// Workgroup size per cell (64 to 1024, a power of two; 0 = about four LEFs per lane, at most
-// 512); results do not depend on it.
+                      uint64_t* d_cell_stats, uint64_t sample, float* ms);
+// Workgroup size per cell (64 to 1024, a power of two) for one launch over every cell; 0 (the
+// default) launches once per size tier instead. Results do not depend on it.
 int fast_job_set_threads(FastJob* job, uint32_t threads);
 uint32_t fast_job_threads(const FastJob* job);
+// The launches the next run makes (one per size tier, or the single launch): workgroup size,
+// grid and cells of each, at most max of them; returns their number.
+uint32_t fast_job_tiers(const FastJob* job, uint32_t* threads, uint32_t* grid, uint64_t* cells,
+                        uint32_t max);
+// Workgroup size of one size tier (index as listed by fast_job_tiers in tiered mode).
+int fast_job_set_tier_threads(FastJob* job, uint32_t tier, uint32_t threads);
 // global != 0: keep every cell's working set in a per-workgroup slab of device memory even when
-// it fits in LDS (the layout large chromosomes need); 0: back to LDS where it fits.
+// it fits in LDS (the layout large chromosomes need), in one launch; 0: back to LDS where it fits.
 int fast_job_set_storage(FastJob* job, int global);
 int fast_job_global_storage(const FastJob* job);
+// Largest grid of the launches the next run makes.
 uint32_t fast_job_grid(const FastJob* job);
 // Copies the first n per-phase tick totals (100 MHz wall clock, summed over cells) and clears them.
 hipError_t fast_phase_times(uint64_t* out, int n);
@@ -49,10 +57,9 @@ const char* fast_last_error();
 uint64_t fast_cell_bytes_host(uint32_t n, uint32_t nbar, uint32_t hist);
 // Clears `bytes` (a multiple of 4) of device memory with a kernel (capture-safe).
 hipError_t launch_zero_words(hipStream_t stream, void* p, uint64_t bytes);
-// Sets the dynamic-LDS limit of the kernel the next run launches (host-side, outside captures).
+// Sets the dynamic-LDS limit of every fast-mode kernel to the most one workgroup may take
+// (host-side, outside captures; job creation does it too).
 hipError_t fast_job_prepare(FastJob* job);
-// While on, launches leave kernel attributes alone (they are being recorded into a graph).
-void fast_job_set_capturing(FastJob* job, bool on);
 // Philox4x32-10 of n (counter[4], key[2]) tuples: in holds 6 words per tuple, out 4.
 hipError_t launch_philox_kat(hipStream_t stream, const uint32_t* d_in, uint32_t* d_out, int n);
 

@@ -929,9 +929,12 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
 
 // Persistent workers: each workgroup takes cells off the queue until it is empty. kLds selects
 // where the working set lives (LDS, or a per-workgroup slab in global memory); it is a template
-// argument so the LDS variant addresses its arrays with LDS instructions.
+// argument so the LDS variant addresses its arrays with LDS instructions. Every size is compiled
+// for four waves per SIMD (at most 128 VGPRs), so 16 waves share a CU at every workgroup size
+// whose cells fit 1024 / NT to a CU: the epoch loop waits on LDS and atomics, and the waves
+// of other workgroups hide that latency.
 template <int NT, bool kLds>
-__global__ void __launch_bounds__(NT)
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
     fast_cells_kernel(FastParams P, const FastChrom* chroms, const WorkItem* items,
                       const uint32_t* order, uint64_t nitems, unsigned long long* next,
                       uint8_t* slab, uint64_t slab_stride, uint64_t* cell_stats) {
@@ -986,6 +989,14 @@ uint32_t t32(double p) {
 
 }  // namespace
 
+// One launch of the persistent kernel over a contiguous run of the work order.
+struct Tier {
+  uint32_t threads = 0;    // workgroup size
+  uint32_t per_cu = 1;     // workgroups one CU holds by LDS (2 for device-memory working sets)
+  uint64_t shmem = 0;      // dynamic LDS per workgroup; 0: working sets in the slab
+  uint64_t begin = 0, count = 0;  // cells order[begin, begin + count)
+};
+
 struct FastImpl {
   FastParams P;
   std::vector<FastChrom> chroms;
@@ -994,71 +1005,115 @@ struct FastImpl {
   unsigned long long* d_next = nullptr;
   uint8_t* d_slab = nullptr;
   uint64_t slab_stride = 0;
-  uint64_t lds_bytes = 0;
-  bool use_lds = true;
-  uint32_t per_cu = 1;     // cells resident per CU as the LDS footprint allows
   uint32_t num_cus = 1;
-  uint32_t threads = 256;  // workgroup size per cell
+  uint32_t threads = 0;          // 0: one launch per size tier; else one launch at this size
+  bool global = false;           // every working set in the slab, one launch
+  bool fits_lds = true;          // every cell's working set fits in LDS
   uint64_t max_lefs = 1;
-  uint64_t max_bytes = 0;     // largest working set of one cell
-  uint64_t max_grid = 0;      // workgroups the slab has room for
-  bool fits_lds = true;
-  void* d_slab_own = nullptr;  // slab allocated when a job that fits LDS is moved to memory
-  bool capturing = false;      // launches are being recorded into a graph
+  uint64_t max_bytes = 0;        // largest working set of one cell
+  uint64_t max_grid = 0;         // workgroups the slab has room for
+  std::vector<Tier> tiers;       // size tiers, in launch order (their runs tile the order)
+  void* d_slab_own = nullptr;    // slab allocated when a job that fits LDS is moved to memory
   uint64_t nitems = 0;
   void* d_mem = nullptr;
 };
 
-// Workgroups of the persistent launch: as many cells per CU as LDS and the 2048-lane CU limit
-// allow (at most 16), never more than there are cells.
-uint32_t fast_grid(const FastImpl* impl) {
-  const uint64_t per_cu = std::min<uint64_t>({impl->per_cu, 2048u / impl->threads, 16u});
-  return (uint32_t)std::min<uint64_t>(impl->nitems, std::max<uint64_t>(1, per_cu) * impl->num_cus);
+constexpr uint64_t kLdsTotal = 160 * 1024;                          // LDS of one CU
+constexpr uint64_t kLdsReserve = sizeof(Shared) + 512;               // static LDS and slack
+constexpr uint64_t kLdsLimit = kLdsTotal - kLdsReserve;              // largest working set in LDS
+
+// Workgroups of one persistent launch: as many as the CU holds (LDS, 2048 lanes, at most 16 per
+// CU), never more than there are cells.
+uint32_t launch_grid(const FastImpl* impl, uint32_t threads, uint32_t per_cu, uint64_t cells) {
+  const uint64_t k = std::min<uint64_t>({per_cu, 2048u / threads, 16u});
+  return (uint32_t)std::min<uint64_t>(cells, std::max<uint64_t>(1, k) * impl->num_cus);
 }
 
-template <int NT, bool kLds>
-hipError_t launch_fast_as(hipStream_t stream, FastImpl* impl, const WorkItem* d_items,
-                          uint64_t* d_cell_stats, uint32_t grid, const uint32_t* order,
-                          uint64_t nitems) {
-  const size_t shmem = kLds ? impl->lds_bytes : 0;
-  if (shmem > 0 && !impl->capturing) {  // set before a capture by fast_job_prepare
-    const hipError_t e =
-        hipFuncSetAttribute(reinterpret_cast<const void*>(fast_cells_kernel<NT, kLds>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-    if (e != hipSuccess) return e;
-  }
-  hipLaunchKernelGGL((fast_cells_kernel<NT, kLds>), dim3(grid), dim3(NT), shmem, stream, impl->P,
-                     impl->d_chroms, d_items, order, nitems, impl->d_next, impl->d_slab,
-                     impl->slab_stride, d_cell_stats);
-  return hipGetLastError();
+// The launch a job makes when it is not split into tiers: the whole order at one workgroup
+// size, from LDS when every cell fits there (sized for the largest), else from the slab.
+Tier single_launch(const FastImpl* impl) {
+  Tier t;
+  const bool lds = impl->fits_lds && !impl->global;
+  t.threads = impl->threads ? impl->threads : kFastDefaultThreads;
+  t.shmem = lds ? impl->max_bytes : 0;
+  t.per_cu = lds ? (uint32_t)std::max<uint64_t>(1, kLdsTotal / (impl->max_bytes + kLdsReserve)) : 2u;
+  t.begin = 0;
+  t.count = impl->nitems;
+  return t;
 }
+
+bool tiered(const FastImpl* impl) { return impl->threads == 0 && !impl->global; }
 
 template <int NT>
 hipError_t launch_fast(hipStream_t stream, FastImpl* impl, const WorkItem* d_items,
-                       uint64_t* d_cell_stats, uint32_t grid, const uint32_t* order,
-                       uint64_t nitems) {
-  return impl->use_lds
-             ? launch_fast_as<NT, true>(stream, impl, d_items, d_cell_stats, grid, order, nitems)
-             : launch_fast_as<NT, false>(stream, impl, d_items, d_cell_stats, grid, order, nitems);
+                       uint64_t* d_cell_stats, uint32_t grid, uint64_t shmem,
+                       const uint32_t* order, uint64_t nitems) {
+  if (shmem)
+    hipLaunchKernelGGL((fast_cells_kernel<NT, true>), dim3(grid), dim3(NT), shmem, stream,
+                       impl->P, impl->d_chroms, d_items, order, nitems, impl->d_next, nullptr,
+                       (uint64_t)0, d_cell_stats);
+  else
+    hipLaunchKernelGGL((fast_cells_kernel<NT, false>), dim3(grid), dim3(NT), 0, stream, impl->P,
+                       impl->d_chroms, d_items, order, nitems, impl->d_next, impl->d_slab,
+                       impl->slab_stride, d_cell_stats);
+  return hipGetLastError();
 }
 
-// Launches the persistent kernel at the job's current workgroup size over order[0, nitems).
-hipError_t launch_with(hipStream_t stream, FastImpl* impl, const WorkItem* d_items,
+// Launches the persistent kernel with tier t's shape over order[0, nitems) (the work counter is
+// cleared first, on the same stream, so launches in a row reuse it).
+hipError_t launch_tier(hipStream_t stream, FastImpl* impl, const Tier& t, const WorkItem* d_items,
                        uint64_t* d_cell_stats, const uint32_t* order, uint64_t nitems) {
+  if (nitems == 0) return hipSuccess;
   hipError_t e = launch_zero_words(stream, impl->d_next, 8);
   if (e != hipSuccess) return e;
-  const uint64_t keep = impl->nitems;
-  impl->nitems = nitems;  // the grid never exceeds the cells launched
-  const uint32_t grid = fast_grid(impl);
-  impl->nitems = keep;
-  switch (impl->threads) {
-    case 64: return launch_fast<64>(stream, impl, d_items, d_cell_stats, grid, order, nitems);
-    case 128: return launch_fast<128>(stream, impl, d_items, d_cell_stats, grid, order, nitems);
-    case 256: return launch_fast<256>(stream, impl, d_items, d_cell_stats, grid, order, nitems);
-    case 512: return launch_fast<512>(stream, impl, d_items, d_cell_stats, grid, order, nitems);
-    case 1024: return launch_fast<1024>(stream, impl, d_items, d_cell_stats, grid, order, nitems);
+  const uint32_t grid = launch_grid(impl, t.threads, t.per_cu, nitems);
+  switch (t.threads) {
+    case 64: return launch_fast<64>(stream, impl, d_items, d_cell_stats, grid, t.shmem, order, nitems);
+    case 128: return launch_fast<128>(stream, impl, d_items, d_cell_stats, grid, t.shmem, order, nitems);
+    case 256: return launch_fast<256>(stream, impl, d_items, d_cell_stats, grid, t.shmem, order, nitems);
+    case 512: return launch_fast<512>(stream, impl, d_items, d_cell_stats, grid, t.shmem, order, nitems);
+    case 1024: return launch_fast<1024>(stream, impl, d_items, d_cell_stats, grid, t.shmem, order, nitems);
     default: return hipErrorInvalidValue;
   }
+}
+
+// The dynamic-LDS limit of every LDS instantiation, set to the most a workgroup may take, so no
+// launch changes kernel attributes (launches may be recorded into a graph).
+template <int NT>
+hipError_t set_lds_limit() {
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(fast_cells_kernel<NT, true>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsLimit);
+}
+
+hipError_t set_lds_limits() {
+  hipError_t e = set_lds_limit<64>();
+  if (e == hipSuccess) e = set_lds_limit<128>();
+  if (e == hipSuccess) e = set_lds_limit<256>();
+  if (e == hipSuccess) e = set_lds_limit<512>();
+  if (e == hipSuccess) e = set_lds_limit<1024>();
+  return e;
+}
+
+// Size tiers: a cell whose working set leaves room for k = 1, 2 or 4 of them in one CU's LDS
+// runs in the tier of that k, at 1024 / k lanes, so every tier keeps 16 waves on each CU; cells
+// too large for LDS form a tier of their own that works from the slab (two workgroups of
+// kFastDefaultThreads per CU). Tier of chromosome c: 0 slab, 1 k = 1, 2 k = 2, 3 k = 4.
+uint32_t tier_of(uint64_t bytes) {
+  if (bytes > kLdsLimit) return 0;
+  const uint64_t k = kLdsTotal / (bytes + kLdsReserve);
+  return k >= 4 ? 3 : (k >= 2 ? 2 : 1);
+}
+
+Tier tier_shape(uint32_t tier) {
+  Tier t;
+  if (tier == 0) {
+    t.threads = kFastDefaultThreads;
+    t.per_cu = 2;
+  } else {
+    t.per_cu = 1u << (tier - 1);
+    t.threads = 1024 / t.per_cu;
+  }
+  return t;
 }
 
 uint64_t fast_cell_bytes_host(uint32_t n, uint32_t nbar, uint32_t hist) {
@@ -1113,34 +1168,32 @@ int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& dchroms,
                                                               P.hist_len));
   }
   int dev = 0;
-  hipGetDevice(&dev);
   hipDeviceProp_t prop;
-  hipGetDeviceProperties(&prop, dev);
-  const uint64_t lds_total = 160 * 1024;
-  const uint64_t lds_limit = lds_total - sizeof(Shared) - 512;
-  impl->use_lds = max_bytes <= lds_limit;
-  impl->fits_lds = impl->use_lds;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
+  if (e == hipSuccess) e = set_lds_limits();
+  if (e != hipSuccess) {
+    g_fast_err = std::string("fast-mode set-up failed: ") + hipGetErrorString(e);
+    delete impl;
+    return -1;
+  }
+  impl->fits_lds = max_bytes <= kLdsLimit;
   impl->max_bytes = max_bytes;
-  impl->lds_bytes = impl->use_lds ? max_bytes : 0;
-  impl->per_cu = impl->use_lds
-                     ? (uint32_t)std::max<uint64_t>(1, lds_total / (max_bytes + sizeof(Shared) + 512))
-                     : 2u;
   impl->num_cus = (uint32_t)prop.multiProcessorCount;
   impl->nitems = items.size();
-  // about four units per lane, between one wave and kFastDefaultThreads lanes
   impl->max_lefs = max_lefs;
-  impl->threads = 64;
-  while (impl->threads < kFastDefaultThreads && impl->threads * 4 < max_lefs) impl->threads *= 2;
-  impl->slab_stride = impl->use_lds ? 0 : ((max_bytes + 255) & ~255ULL);
-  // the slab holds one working set per workgroup of the largest grid any tuning launches
+  impl->threads = 0;
+  // the slab holds one working set (sized for the largest cell) per workgroup of the largest
+  // grid any launch makes; it exists from the start when some cell does not fit in LDS
+  impl->slab_stride = (max_bytes + 255) & ~255ULL;
   const uint64_t max_grid = std::min<uint64_t>(items.size(), 16ull * impl->num_cus);
   impl->max_grid = max_grid;
 
   const uint64_t bar_bytes = (4 * nbar_total + 64) * 4;
   const uint64_t chrom_bytes = (nch * sizeof(FastChrom) + 255) & ~255ULL;
   const uint64_t order_bytes = (items.size() * 4 + 255) & ~255ULL;
-  const uint64_t slab_bytes = impl->slab_stride * max_grid;
-  hipError_t e = hipMalloc(&impl->d_mem, bar_bytes + chrom_bytes + order_bytes + 256 + slab_bytes);
+  const uint64_t slab_bytes = impl->fits_lds ? 0 : impl->slab_stride * max_grid;
+  e = hipMalloc(&impl->d_mem, bar_bytes + chrom_bytes + order_bytes + 256 + slab_bytes);
   if (e != hipSuccess) {
     g_fast_err = std::string("fast-mode allocation failed: ") + hipGetErrorString(e);
     delete impl;
@@ -1151,7 +1204,7 @@ int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& dchroms,
   impl->d_chroms = reinterpret_cast<FastChrom*>(base + bar_bytes);
   impl->d_order = reinterpret_cast<uint32_t*>(base + bar_bytes + chrom_bytes);
   impl->d_next = reinterpret_cast<unsigned long long*>(base + bar_bytes + chrom_bytes + order_bytes);
-  impl->d_slab = impl->use_lds ? nullptr : base + bar_bytes + chrom_bytes + order_bytes + 256;
+  impl->d_slab = impl->fits_lds ? nullptr : base + bar_bytes + chrom_bytes + order_bytes + 256;
 
   // 32-bit barrier tables: positions and the transition / occupancy thresholds
   std::vector<uint32_t> hb(4 * nbar_total + 64, 0);
@@ -1196,12 +1249,29 @@ int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& dchroms,
   if (e == hipSuccess)
     e = hipMemcpy(impl->d_chroms, impl->chroms.data(), nch * sizeof(FastChrom),
                   hipMemcpyHostToDevice);
-  // longest cells first (chromosomes with more LEFs take longer per epoch)
+  // work order: by size tier, longest cells first within a tier (chromosomes with more LEFs
+  // take longer per epoch)
+  std::vector<uint32_t> tier(nch);
+  for (uint64_t c = 0; c < nch; ++c) tier[c] = tier_of(impl->chroms[c].lds_bytes);
   std::vector<uint32_t> order(items.size());
   for (uint32_t i = 0; i < order.size(); ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-    return dchroms[items[a].chrom].nlefs > dchroms[items[b].chrom].nlefs;
+    const uint32_t ca = items[a].chrom, cb = items[b].chrom;
+    if (tier[ca] != tier[cb]) return tier[ca] < tier[cb];
+    return dchroms[ca].nlefs > dchroms[cb].nlefs;
   });
+  for (uint32_t k = 0; k < 4; ++k) {
+    Tier t = tier_shape(k);
+    uint64_t bytes = 0;
+    for (uint64_t i = 0; i < order.size(); ++i) {
+      const uint32_t c = items[order[i]].chrom;
+      if (tier[c] != k) continue;
+      if (t.count++ == 0) t.begin = i;
+      bytes = std::max<uint64_t>(bytes, impl->chroms[c].lds_bytes);
+    }
+    t.shmem = k == 0 ? 0 : bytes;
+    if (t.count) impl->tiers.push_back(t);
+  }
   if (e == hipSuccess)
     e = hipMemcpy(impl->d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -1222,48 +1292,84 @@ hipError_t fast_job_run(hipStream_t stream, FastJob* job, const WorkItem* d_item
   auto* impl = static_cast<FastImpl*>(job->impl);
   if (!impl) return hipErrorInvalidValue;
   impl->P.timers = timers ? 1u : 0u;
-  return launch_with(stream, impl, d_items, d_cell_stats, impl->d_order, impl->nitems);
+  if (!tiered(impl))
+    return launch_tier(stream, impl, single_launch(impl), d_items, d_cell_stats, impl->d_order,
+                       impl->nitems);
+  for (const Tier& t : impl->tiers) {
+    const hipError_t e =
+        launch_tier(stream, impl, t, d_items, d_cell_stats, impl->d_order + t.begin, t.count);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
+namespace {
+// Milliseconds of one launch of shape t over order[0, n).
+hipError_t time_launch(hipStream_t stream, FastImpl* impl, const Tier& t, const WorkItem* d_items,
+                       uint64_t* d_cell_stats, const uint32_t* order, uint64_t n, hipEvent_t a,
+                       hipEvent_t b, float* ms) {
+  hipError_t e = hipEventRecord(a, stream);
+  if (e == hipSuccess) e = launch_tier(stream, impl, t, d_items, d_cell_stats, order, n);
+  if (e == hipSuccess) e = hipEventRecord(b, stream);
+  if (e == hipSuccess) e = hipEventSynchronize(b);
+  if (e == hipSuccess) e = hipEventElapsedTime(ms, a, b);
+  return e;
+}
+}  // namespace
+
 int fast_job_autotune(hipStream_t stream, FastJob* job, const WorkItem* d_items,
-                      uint64_t* d_cell_stats, uint64_t sample, uint32_t* best, float* ms_out) {
+                      uint64_t* d_cell_stats, uint64_t sample, float* ms_out) {
   auto* impl = static_cast<FastImpl*>(job->impl);
-  if (!impl || !best) return -1;
+  if (!impl) return -1;
   if (sample == 0 || sample > impl->nitems) sample = impl->nitems;
-  // every k-th cell of the longest-first order, so the sample spans the chromosome sizes
+  // the runs timed: the tiers (or the single launch of a job kept in device memory), each on
+  // every k-th cell of its part of the order, so a sample spans the chromosome sizes
+  std::vector<Tier> runs = tiered(impl) ? impl->tiers : std::vector<Tier>{single_launch(impl)};
   std::vector<uint32_t> all(impl->nitems), pick;
+  std::vector<uint64_t> at, cnt;
   hipError_t e = hipMemcpy(all.data(), impl->d_order, all.size() * 4, hipMemcpyDeviceToHost);
-  const uint64_t stride = impl->nitems / sample;
-  for (uint64_t k = 0; k < sample; ++k) pick.push_back(all[k * stride]);
+  for (const Tier& t : runs) {
+    const uint64_t s = std::min<uint64_t>(t.count, std::max<uint64_t>(1, sample * t.count / impl->nitems));
+    const uint64_t stride = t.count / s;
+    at.push_back(pick.size());
+    cnt.push_back(s);
+    for (uint64_t k = 0; k < s; ++k) pick.push_back(all[t.begin + k * stride]);
+  }
   uint32_t* d_pick = nullptr;
   if (e == hipSuccess) e = hipMalloc(&d_pick, pick.size() * 4);
   if (e == hipSuccess) e = hipMemcpy(d_pick, pick.data(), pick.size() * 4, hipMemcpyHostToDevice);
   hipEvent_t a = nullptr, b = nullptr;
   if (e == hipSuccess) e = hipEventCreate(&a);
   if (e == hipSuccess) e = hipEventCreate(&b);
-  const uint32_t keep = impl->threads;
   const uint32_t cand[] = {64, 128, 256, 512, 1024};
-  float best_ms = 0.0f;
-  *best = keep;
+  std::vector<float> best_ms(runs.size(), 0.0f);
+  std::vector<uint32_t> best(runs.size(), 0);
   impl->P.timers = 0;
   for (int k = 0; k < 5 && e == hipSuccess; ++k) {
-    impl->threads = cand[k];
-    e = hipEventRecord(a, stream);
-    if (e == hipSuccess) e = launch_with(stream, impl, d_items, d_cell_stats, d_pick, pick.size());
-    if (e == hipSuccess) e = hipEventRecord(b, stream);
-    if (e == hipSuccess) e = hipEventSynchronize(b);
-    float ms = 0.0f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
-    if (ms_out) ms_out[k] = ms;
-    if (e == hipSuccess && (k == 0 || ms < best_ms)) {
-      best_ms = ms;
-      *best = cand[k];
+    float total = 0.0f;
+    for (size_t r = 0; r < runs.size() && e == hipSuccess; ++r) {
+      Tier t = runs[r];
+      t.threads = cand[k];
+      float ms = 0.0f;
+      e = time_launch(stream, impl, t, d_items, d_cell_stats, d_pick + at[r], cnt[r], a, b, &ms);
+      total += ms;
+      if (e == hipSuccess && (best[r] == 0 || ms < best_ms[r])) {
+        best_ms[r] = ms;
+        best[r] = cand[k];
+      }
+    }
+    if (ms_out) ms_out[k] = total;
+  }
+  if (e == hipSuccess) {
+    if (tiered(impl)) {
+      for (size_t r = 0; r < runs.size(); ++r) impl->tiers[r].threads = best[r];
+    } else {
+      impl->threads = best[0];
     }
   }
-  impl->threads = e == hipSuccess ? *best : keep;
-  if (a) hipEventDestroy(a);
-  if (b) hipEventDestroy(b);
-  if (d_pick) hipFree(d_pick);
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+  if (d_pick) (void)hipFree(d_pick);
   if (e != hipSuccess) {
     g_fast_err = std::string("autotune failed: ") + hipGetErrorString(e);
     return -1;
@@ -1271,72 +1377,44 @@ int fast_job_autotune(hipStream_t stream, FastJob* job, const WorkItem* d_items,
   return 0;
 }
 
-template <int NT>
-hipError_t set_lds_limit(FastImpl* impl) {
-  if (!impl->use_lds || impl->lds_bytes == 0) return hipSuccess;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(fast_cells_kernel<NT, true>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)impl->lds_bytes);
-}
-
 hipError_t fast_job_prepare(FastJob* job) {
-  auto* impl = static_cast<FastImpl*>(job->impl);
-  if (!impl) return hipErrorInvalidValue;
-  switch (impl->threads) {
-    case 64: return set_lds_limit<64>(impl);
-    case 128: return set_lds_limit<128>(impl);
-    case 256: return set_lds_limit<256>(impl);
-    case 512: return set_lds_limit<512>(impl);
-    case 1024: return set_lds_limit<1024>(impl);
-    default: return hipErrorInvalidValue;
-  }
-}
-
-void fast_job_set_capturing(FastJob* job, bool on) {
-  auto* impl = static_cast<FastImpl*>(job->impl);
-  if (impl) impl->capturing = on;
+  if (!job->impl) return hipErrorInvalidValue;
+  return set_lds_limits();
 }
 
 int fast_job_set_storage(FastJob* job, int global) {
   auto* impl = static_cast<FastImpl*>(job->impl);
   if (!impl) return -1;
   if (!global) {
-    if (!impl->fits_lds) {
-      g_fast_err = "the working set of this job does not fit in LDS";
-      return -1;
-    }
-    impl->use_lds = true;
+    impl->global = false;
     return 0;
   }
-  if (impl->use_lds && !impl->d_slab) {
-    const uint64_t stride = (impl->max_bytes + 255) & ~255ULL;
-    const hipError_t e = hipMalloc(&impl->d_slab_own, stride * impl->max_grid);
+  if (!impl->d_slab) {
+    const hipError_t e = hipMalloc(&impl->d_slab_own, impl->slab_stride * impl->max_grid);
     if (e != hipSuccess) {
       g_fast_err = std::string("slab allocation failed: ") + hipGetErrorString(e);
       return -1;
     }
     impl->d_slab = static_cast<uint8_t*>(impl->d_slab_own);
-    impl->slab_stride = stride;
   }
-  impl->use_lds = false;
-  impl->per_cu = 2;
+  impl->global = true;
   return 0;
 }
 
 int fast_job_global_storage(const FastJob* job) {
   const auto* impl = static_cast<const FastImpl*>(job->impl);
-  return impl && !impl->use_lds ? 1 : 0;
+  return impl && impl->global ? 1 : 0;
+}
+
+static bool valid_threads(uint32_t t) {
+  return t == 64 || t == 128 || t == 256 || t == 512 || t == 1024;
 }
 
 int fast_job_set_threads(FastJob* job, uint32_t threads) {
   auto* impl = static_cast<FastImpl*>(job->impl);
   if (!impl) return -1;
-  if (threads == 0) {
-    uint32_t t = 64;
-    while (t < kFastDefaultThreads && t * 4 < impl->max_lefs) t *= 2;
-    threads = t;
-  }
-  if (threads != 64 && threads != 128 && threads != 256 && threads != 512 && threads != 1024) {
-    g_fast_err = "threads per cell must be 64, 128, 256, 512 or 1024";
+  if (threads != 0 && !valid_threads(threads)) {
+    g_fast_err = "threads per cell must be 0, 64, 128, 256, 512 or 1024";
     return -1;
   }
   impl->threads = threads;
@@ -1348,9 +1426,38 @@ uint32_t fast_job_threads(const FastJob* job) {
   return impl ? impl->threads : 0;
 }
 
-uint32_t fast_job_grid(const FastJob* job) {
+uint32_t fast_job_tiers(const FastJob* job, uint32_t* threads, uint32_t* grid, uint64_t* cells,
+                        uint32_t max) {
   const auto* impl = static_cast<const FastImpl*>(job->impl);
-  return impl ? fast_grid(impl) : 0;
+  if (!impl) return 0;
+  std::vector<Tier> runs = tiered(impl) ? impl->tiers : std::vector<Tier>{single_launch(impl)};
+  uint32_t n = 0;
+  for (const Tier& t : runs) {
+    if (n == max) break;
+    if (threads) threads[n] = t.threads;
+    if (grid) grid[n] = launch_grid(impl, t.threads, t.per_cu, t.count);
+    if (cells) cells[n] = t.count;
+    ++n;
+  }
+  return n;
+}
+
+int fast_job_set_tier_threads(FastJob* job, uint32_t tier, uint32_t threads) {
+  auto* impl = static_cast<FastImpl*>(job->impl);
+  if (!impl || tier >= impl->tiers.size() || !valid_threads(threads)) {
+    g_fast_err = "no such tier, or threads per cell not 64, 128, 256, 512 or 1024";
+    return -1;
+  }
+  impl->tiers[tier].threads = threads;
+  return 0;
+}
+
+uint32_t fast_job_grid(const FastJob* job) {
+  uint32_t g[4] = {};
+  const uint32_t n = fast_job_tiers(job, nullptr, g, nullptr, 4);
+  uint32_t m = 0;
+  for (uint32_t k = 0; k < n; ++k) m = std::max(m, g[k]);
+  return m;
 }
 
 hipError_t fast_phase_times(uint64_t* out, int n) {

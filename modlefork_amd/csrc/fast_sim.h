@@ -31,7 +31,7 @@ namespace mgpu {
 constexpr int kFastMaxThreads = 1024;
 constexpr int kFastMaxWaves = kFastMaxThreads / 64;
 constexpr int kFastPhases = 16;  // epoch phases timed when FastParams::timers is set
-constexpr uint32_t kFastDefaultThreads = 512;
+constexpr uint32_t kFastDefaultThreads = 512;  // lanes per cell of launches from device memory
 constexpr uint32_t kFreePos = 0xFFFFFFFFu;
 constexpr uint32_t kFastBurninCap = 100000;  // burn-in epochs after which fast mode stops it
 

@@ -710,10 +710,8 @@ int mgpu_job_capture(mgpu_job* job) {
   if (job->mode == MGPU_MODE_FAST) HIPCHK(ctx, fast_job_prepare(&job->fast));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-  if (job->mode == MGPU_MODE_FAST) fast_job_set_capturing(&job->fast, true);
   hipError_t e = enqueue_clears(job, ctx->stream);
   if (e == hipSuccess) e = enqueue_simulation(job, ctx->stream);
-  if (job->mode == MGPU_MODE_FAST) fast_job_set_capturing(&job->fast, false);
   hipGraph_t g = nullptr;
   const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
   if (e == hipSuccess) e = e2;
@@ -776,6 +774,12 @@ int mgpu_job_set_tuning(mgpu_job* job, const mgpu_tuning* t) {
   if (job->mode != MGPU_MODE_FAST) return MGPU_OK;  // deterministic mode: one lane per cell
   if (fast_job_set_threads(&job->fast, t->threads_per_cell) != 0)
     return fail(job->ctx, MGPU_EINVAL, "%s", fast_last_error());
+  if (t->threads_per_cell == 0) {
+    const uint32_t n = fast_job_tiers(&job->fast, nullptr, nullptr, nullptr, 4);
+    for (uint32_t k = 0; k < n && k < 4; ++k)
+      if (t->tier_threads[k] && fast_job_set_tier_threads(&job->fast, k, t->tier_threads[k]) != 0)
+        return fail(job->ctx, MGPU_EINVAL, "%s", fast_last_error());
+  }
   if (t->global_working_set != (uint32_t)fast_job_global_storage(&job->fast) &&
       fast_job_set_storage(&job->fast, t->global_working_set ? 1 : 0) != 0)
     return fail(job->ctx, MGPU_EINVAL, "%s", fast_last_error());
@@ -788,9 +792,8 @@ int mgpu_job_autotune(mgpu_job* job, uint64_t sample_cells, mgpu_tuning* chosen,
   HIPCHK(ctx, hipSetDevice(ctx->device));
   drop_graph(job);
   if (job->mode == MGPU_MODE_FAST) {
-    uint32_t best = 0;
     if (fast_job_autotune(ctx->stream, &job->fast, job->d_items, job->d_cell_stats, sample_cells,
-                          &best, ms) != 0)
+                          ms) != 0)
       return fail(ctx, MGPU_EDEVICE, "%s", fast_last_error());
   }
   return chosen ? mgpu_job_get_tuning(job, chosen) : MGPU_OK;
@@ -803,6 +806,7 @@ int mgpu_job_get_tuning(mgpu_job* job, mgpu_tuning* t) {
     t->threads_per_cell = fast_job_threads(&job->fast);
     t->grid = fast_job_grid(&job->fast);
     t->global_working_set = (uint32_t)fast_job_global_storage(&job->fast);
+    t->launches = fast_job_tiers(&job->fast, t->tier_threads, t->tier_grid, nullptr, 4);
   } else {
     t->threads_per_cell = 64;
     t->grid = job->grid;

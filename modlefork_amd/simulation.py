@@ -176,12 +176,22 @@ class Job:
         self.ctx.check(L.lib().mgpu_job_run(self.ptr, C.byref(st)))
         return st
 
-    def set_tuning(self, threads_per_cell: int = 0, global_working_set: bool | None = None):
+    def set_tuning(self, threads_per_cell: int = 0, global_working_set: bool | None = None,
+                   tier_threads=None):
+        """threads_per_cell 0: one launch per size tier (tier_threads overrides their sizes);
+        otherwise one launch of that size over every cell."""
         t = self.tuning()
         t.threads_per_cell = threads_per_cell
+        for k in range(4):
+            t.tier_threads[k] = int(tier_threads[k]) if tier_threads and k < len(tier_threads) else 0
         if global_working_set is not None:
             t.global_working_set = 1 if global_working_set else 0
         self.ctx.check(L.lib().mgpu_job_set_tuning(self.ptr, C.byref(t)))
+
+    def launches(self):
+        """[(threads per cell, grid)] of the launches the next run makes."""
+        t = self.tuning()
+        return [(int(t.tier_threads[k]), int(t.tier_grid[k])) for k in range(int(t.launches))]
 
     def capture(self):
         """Records a run into a HIP graph that later runs replay."""
@@ -191,12 +201,13 @@ class Job:
         self.ctx.check(L.lib().mgpu_job_release_graph(self.ptr))
 
     def autotune(self, sample_cells: int = 0):
-        """Times each workgroup size on a sample of cells and keeps the fastest; returns
-        (threads per cell, {threads: ms})."""
+        """Times each workgroup size on a sample of cells, per size tier, and keeps the fastest;
+        returns (the workgroup size of each launch, {threads: ms summed over the launches})."""
         t = L.MgpuTuning()
         ms = (C.c_float * 5)()
         self.ctx.check(L.lib().mgpu_job_autotune(self.ptr, sample_cells, C.byref(t), ms))
-        return int(t.threads_per_cell), dict(zip((64, 128, 256, 512, 1024), list(ms)))
+        sizes = [int(t.tier_threads[k]) for k in range(int(t.launches))]
+        return sizes, dict(zip((64, 128, 256, 512, 1024), list(ms)))
 
     def tuning(self) -> L.MgpuTuning:
         t = L.MgpuTuning()

@@ -144,6 +144,52 @@ def test_captured_graph_replays_match_eager_runs(ctx, mode):
     job.close()
 
 
+def tier_workload(ncells=4):
+    """Four chromosomes at 40 LEFs/Mbp whose working sets need device memory (5200 LEFs) or let
+    1 (4000), 2 (2000) and 4 (800) of them share one CU's LDS: one of each launch tier."""
+    rng = np.random.default_rng(5)
+    chroms = [("chrA", 130_000_000), ("chrB", 100_000_000), ("chrC", 50_000_000),
+              ("chrD", 20_000_000)]
+    bars = []
+    for name, size in chroms:
+        bars += W._uniform_barriers(rng, name, size, 100, lambda n: rng.uniform(0.5, 0.95, n))
+    return W.Workload("tiers", chroms, bars,
+                      dict(bin_size=5_000, diagonal_width=1_000_000, number_of_lefs_per_mbp=40.0,
+                           num_cells=ncells, target_contact_density=0.05, seed=3))
+
+
+def test_size_tiers_give_the_single_launch_band(ctx):
+    """The default run makes one launch per size tier (device memory, then 1, 2 and 4 cells per
+    CU); one launch of every cell from device memory gives the same bands and statistics, and so
+    do replays of the tiered run captured into a graph."""
+    wl = tier_workload()
+    sim = S.Simulation(wl.options, ctx)
+    names, chroms, bars, plans = sim.prepare(wl)
+    job = S.Job(ctx, sim.params, chroms, bars, plans, L.MODE_FAST)
+    launches = job.launches()
+    say(f"tier launches (threads, grid): {launches}")
+    assert [t for t, _ in launches] == [512, 1024, 512, 256]
+    st = job.run()
+    tiered = [job.band(i) for i in range(len(names))]
+    job.set_tuning(512)
+    assert [t for t, _ in job.launches()] == [512]
+    ss = job.run()
+    say(f"tiered {st.kernel_ms:.1f} ms, single launch {ss.kernel_ms:.1f} ms")
+    for i in range(len(names)):
+        band, missed = job.band(i)
+        assert np.array_equal(band, tiered[i][0]) and missed == tiered[i][1], names[i]
+    assert (st.epochs, st.burnin_epochs, st.contacts) == (ss.epochs, ss.burnin_epochs, ss.contacts)
+    job.set_tuning(0)
+    job.capture()
+    for r in range(2):
+        sg = job.run()
+        assert sg.contacts == st.contacts
+        for i in range(len(names)):
+            band, missed = job.band(i)
+            assert np.array_equal(band, tiered[i][0]) and missed == tiered[i][1], (r, names[i])
+    job.close()
+
+
 def test_autotune_keeps_results(ctx):
     """The autotuner times each workgroup size on a sample of cells; the run that follows gives
     the band of the default launch."""
@@ -152,9 +198,10 @@ def test_autotune_keeps_results(ctx):
     sim = S.Simulation(wl.options, ctx)
     names, chroms, bars, plans = sim.prepare(wl)
     job = S.Job(ctx, sim.params, chroms, bars, plans, L.MODE_FAST)
-    best, times = job.autotune(4)
-    say(f"autotune: {best} threads per cell; times {times}")
-    assert best in times
+    sizes, times = job.autotune(4)
+    say(f"autotune: {sizes} threads per cell; times {times}")
+    assert sizes and all(x in times for x in sizes)
+    assert job.tuning().threads_per_cell == 0  # still one launch per size tier
     st = job.run()
     band, missed = job.band(0)
     job.close()
