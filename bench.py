@@ -43,13 +43,12 @@ def build_workload(name, ncells):
     raise SystemExit(f"unknown workload {name}")
 
 
-def algorithmic_bytes(plans, barriers, contacts, spill_bytes):
-    """Bytes one simulation launch must move (per cell-epoch 8 B per registered contact, a
-    read and a write of an int32 pixel, plus LEF state that spills out of LDS; per cell the
-    barrier table (pos 8 B, direction 1 B, two probabilities 16 B... counted as 21 B of
-    position/direction/probability data the kernel reads) and 32 B of engine state)."""
-    per_cell = sum(len(p.targets) * (21 * len(b) + 32) for p, b in zip(plans, barriers))
-    return 8 * contacts + per_cell + spill_bytes
+def launch_bytes(ls) -> int:
+    """Algorithmic bytes of one simulation launch: 8 B per registered contact (a read and a
+    write of one int32 pixel), 21 B per barrier-table entry each cell reads (position, direction
+    and transition probabilities) and 32 B of engine state per cell. Working sets stay on chip
+    (LDS) in the measured configurations, so no LEF state spills."""
+    return 8 * int(ls.contacts) + 21 * int(ls.barriers) + 32 * int(ls.cells)
 
 
 def cpu_baseline(wl, options, budget_s, nthreads):
@@ -231,6 +230,7 @@ def main():
         kernel_ms.append(stats.kernel_ms)
         log(f"step {i}: kernel {stats.kernel_ms:.1f} ms, {stats.epochs} epochs")
     wall = time.perf_counter() - t0
+    ls = job.launch_stats()  # per-launch times of the last timed run
     if args.phases and mode == L.MODE_FAST:
         sim.ctx.phase_times(True)
         job.run()
@@ -245,10 +245,13 @@ def main():
     epochs = int(stats.epochs)
     value = cells * args.steps / wall
 
-    avg_kernel_s = float(np.mean(kernel_ms)) / 1000.0
-    spill = 0
-    abytes = algorithmic_bytes(plans, bars, contacts, spill)
-    achieved = abytes / avg_kernel_s / 1e9
+    # roofline of the dominant launch (the size tier that takes longest), from the last timed
+    # run's per-launch HIP events and the algorithmic bytes of that launch's cells
+    dom = max(range(len(ls)), key=lambda k: ls[k].ms)
+    d = ls[dom]
+    dom_bytes = launch_bytes(d)
+    achieved = dom_bytes / (d.ms / 1000.0) / 1e9
+    all_bytes = sum(launch_bytes(x) for x in ls)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{args.mode}_{args.workload}.json")
     if os.path.exists(tfile):
@@ -256,7 +259,7 @@ def main():
             prof = json.load(fh)
         # counters were taken for one launch shape; other shapes report no traffic figure
         if prof.get("launch_shape") == shape:
-            traffic = prof.get("hbm_bytes_per_run")
+            traffic = prof.get("hbm_bytes_per_launch", {}).get(str(d.threads))
 
     cpu = None
     if not args.no_cpu:
@@ -287,7 +290,15 @@ def main():
         "epochs_per_sec": epochs * args.steps / wall,
         "kernel_ms": float(np.mean(kernel_ms)),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": f"fast_cells_kernel<{d.threads}>" if mode == L.MODE_FAST
+                               else "det_cells_kernel",
+                     "launch": {"threads": int(d.threads), "grid": int(d.grid),
+                                "cells": int(d.cells), "contacts": int(d.contacts),
+                                "ms": float(d.ms), "algorithmic_bytes": dom_bytes},
+                     "all_launches": {"algorithmic_bytes": all_bytes,
+                                      "ms": float(sum(x.ms for x in ls)),
+                                      "achieved": all_bytes / (sum(x.ms for x in ls) / 1000.0) / 1e9}},
         "cpu_baseline": cpu,
     }
     if cpu:

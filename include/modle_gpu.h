@@ -68,6 +68,8 @@ typedef struct mgpu_config {
   uint64_t burnin_history_length, burnin_smoothing_window_size;
   uint64_t min_burnin_epochs, max_burnin_epochs;
   double burnin_speed_coefficient;
+  int32_t track_1d_lef_position;    /* --track-1d-lef-position */
+  int32_t log_model_internal_state; /* --log-model-internal-state */
 } mgpu_config;
 
 /* Options given explicitly on the command line (the CLI's "option parsed" tests). */
@@ -103,6 +105,8 @@ typedef struct mgpu_params {
   double lefs_per_mbp;
   double barrier_occupied_stp, barrier_not_occupied_stp; /* defaults for barriers w/o score */
   int32_t override_barrier_occupancy; /* --extrusion-barrier-occupancy given */
+  int32_t track_1d_lef_position;      /* a job keeps each chromosome's 1D LEF occupancy */
+  int32_t log_model_internal_state;   /* a job logs every cell's per-epoch state */
   int32_t reserved;
 } mgpu_params;
 
@@ -212,6 +216,52 @@ int mgpu_job_read_band(mgpu_job* job, uint64_t c, uint32_t* out_band, uint64_t* 
 /* Device pointer of chromosome c's band (valid until mgpu_job_destroy). */
 int mgpu_job_band_device_ptr(mgpu_job* job, uint64_t c, void** dptr, uint64_t* nelems);
 void mgpu_job_destroy(mgpu_job* job);
+
+/* Per-launch figures of the last run that was not replayed from a graph (no counterpart in the
+ * reference: they time the device launches). Fast mode launches once per working-set size tier;
+ * deterministic mode once. *n receives the number of launches; up to max are written. */
+typedef struct mgpu_launch_stats {
+  uint32_t threads, grid; /* workgroup size and workgroups */
+  uint64_t cells;
+  uint64_t epochs;   /* summed over the launch's cells */
+  uint64_t contacts; /* registered by the launch's cells (incl. missed) */
+  uint64_t barriers; /* barrier-table entries the launch's cells read (per cell, summed) */
+  double ms;         /* device time between HIP events around the launch */
+} mgpu_launch_stats;
+int mgpu_job_launch_stats(mgpu_job* job, mgpu_launch_stats* out, uint32_t max, uint32_t* n);
+
+/* ---- optional outputs ----
+ * 1D LEF occupancy (Chromosome::lef_1d_occupancy, genome.cpp:236-245, filled by
+ * Simulation::register_1d_lef_occupancy, register_contacts.cpp:200-232): jobs created with
+ * track_1d_lef_position keep one u64 histogram of ceil(size / bin_size) bins per chromosome,
+ * cleared by every run. Each contact sampling event adds the bins of both units of one more
+ * sampled LEF. Copies chromosome c's histogram to out (nbins entries, nbins may be NULL). */
+int mgpu_job_read_occupancy(mgpu_job* job, uint64_t c, uint64_t* out, uint64_t* nbins);
+
+/* One line of the model internal-state log (Simulation::dump_stats, simulation.cpp:1166-1224),
+ * taken after extrusion and before LEF release each epoch. */
+typedef struct mgpu_state_record {
+  uint64_t chrom; /* index of the chromosome in the job */
+  uint64_t cell_id;
+  uint64_t epoch;
+  int32_t burnin; /* 1 while burn-in runs */
+  uint32_t num_active_lefs;
+  uint64_t num_barriers, active_barriers;
+  uint64_t stalls_rev, stalls_fwd, stalls_both;
+  uint64_t lef_bar, primary, secondary; /* collisions that occurred, both sides summed */
+  double avg_loop_size;
+} mgpu_state_record;
+
+/* Replaces the per-thread compressed_io::Writer of simulate_worker
+ * (scheduler_simulate.cpp:205-226): device memory for `capacity` records, filled by every later
+ * run (a job created with log_model_internal_state starts with room for about twice the epochs
+ * its cells are expected to need); 0 turns logging off. */
+int mgpu_job_set_state_log(mgpu_job* job, uint64_t capacity);
+/* Copies the last run's records, sorted by (chromosome, cell, epoch), into out (room for cap).
+ * *n_records receives the number the run produced; MGPU_ERANGE when that exceeds the job's
+ * capacity or cap (nothing is copied then: enlarge and run again). */
+int mgpu_job_read_state_log(mgpu_job* job, mgpu_state_record* out, uint64_t cap,
+                            uint64_t* n_records);
 
 /* ---- launch tuning (no counterpart in the reference: it sizes the device launch) ----
  * Fast mode runs its cells in persistent launches. By default (threads_per_cell 0) it makes one

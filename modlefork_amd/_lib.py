@@ -16,6 +16,11 @@ U32P = C.POINTER(C.c_uint32)
 U8P = C.POINTER(C.c_uint8)
 DP = C.POINTER(C.c_double)
 
+MGPU_OK = 0
+MGPU_EINVAL = -1
+MGPU_EDEVICE = -2
+MGPU_ENOMEM = -3
+MGPU_ERANGE = -4
 MODE_DETERMINISTIC = 0
 MODE_FAST = 1
 STOP_CONTACT_DENSITY = 0
@@ -64,6 +69,8 @@ class MgpuConfig(C.Structure):
         ("min_burnin_epochs", C.c_uint64),
         ("max_burnin_epochs", C.c_uint64),
         ("burnin_speed_coefficient", C.c_double),
+        ("track_1d_lef_position", C.c_int32),
+        ("log_model_internal_state", C.c_int32),
     ]
 
 
@@ -103,6 +110,8 @@ class MgpuParams(C.Structure):
         ("barrier_occupied_stp", C.c_double),
         ("barrier_not_occupied_stp", C.c_double),
         ("override_barrier_occupancy", C.c_int32),
+        ("track_1d_lef_position", C.c_int32),
+        ("log_model_internal_state", C.c_int32),
         ("reserved", C.c_int32),
     ]
 
@@ -162,6 +171,37 @@ class MgpuTuning(C.Structure):
     ]
 
 
+class MgpuLaunchStats(C.Structure):
+    _fields_ = [
+        ("threads", C.c_uint32),
+        ("grid", C.c_uint32),
+        ("cells", C.c_uint64),
+        ("epochs", C.c_uint64),
+        ("contacts", C.c_uint64),
+        ("barriers", C.c_uint64),
+        ("ms", C.c_double),
+    ]
+
+
+class MgpuStateRecord(C.Structure):
+    _fields_ = [
+        ("chrom", C.c_uint64),
+        ("cell_id", C.c_uint64),
+        ("epoch", C.c_uint64),
+        ("burnin", C.c_int32),
+        ("num_active_lefs", C.c_uint32),
+        ("num_barriers", C.c_uint64),
+        ("active_barriers", C.c_uint64),
+        ("stalls_rev", C.c_uint64),
+        ("stalls_fwd", C.c_uint64),
+        ("stalls_both", C.c_uint64),
+        ("lef_bar", C.c_uint64),
+        ("primary", C.c_uint64),
+        ("secondary", C.c_uint64),
+        ("avg_loop_size", C.c_double),
+    ]
+
+
 class MgpuSgConfig(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in (
         "in_len", "out_offset", "out_len", "ndiag", "in_ch", "enc_ch", "enc_layers", "enc_k",
@@ -207,6 +247,12 @@ SIGNATURES = [
     ("mgpu_job_read_band", C.c_int, [C.c_void_p, C.c_uint64, U32P, U64P]),
     ("mgpu_job_band_device_ptr", C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p), U64P]),
     ("mgpu_job_destroy", None, [C.c_void_p]),
+    ("mgpu_job_launch_stats", C.c_int, [C.c_void_p, C.POINTER(MgpuLaunchStats), C.c_uint32,
+                                        C.POINTER(C.c_uint32)]),
+    ("mgpu_job_read_occupancy", C.c_int, [C.c_void_p, C.c_uint64, U64P, U64P]),
+    ("mgpu_job_set_state_log", C.c_int, [C.c_void_p, C.c_uint64]),
+    ("mgpu_job_read_state_log", C.c_int, [C.c_void_p, C.POINTER(MgpuStateRecord), C.c_uint64,
+                                          U64P]),
     ("mgpu_job_set_tuning", C.c_int, [C.c_void_p, C.POINTER(MgpuTuning)]),
     ("mgpu_job_get_tuning", C.c_int, [C.c_void_p, C.POINTER(MgpuTuning)]),
     ("mgpu_job_autotune", C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(MgpuTuning),

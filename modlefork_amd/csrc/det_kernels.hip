@@ -26,7 +26,7 @@ __global__ void __launch_bounds__(64) det_cells_kernel(DetParams P, const DetChr
     const DetChrom ch = chroms[w.chrom];
     DetCell s;
     det_carve(s, base, ch.nlefs, ch.nbar, P.hist_len);
-    det_simulate_cell(P, ch, s, w.seed, w.target, nullptr, 0);
+    det_simulate_cell(P, ch, s, w.seed, w.target, nullptr, 0, (uint32_t)it);
     if (cell_stats) {
       cell_stats[4 * it + 0] = s.epoch;
       cell_stats[4 * it + 1] = s.burnin_epochs;

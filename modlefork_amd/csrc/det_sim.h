@@ -536,6 +536,30 @@ __device__ inline void det_register(const DetParams& P, const DetChrom& ch, DetC
   }
 }
 
+// register_1d_lef_occupancy (register_contacts.cpp:200-232): nevents LEFs sampled with
+// replacement; each accepted one adds the bins of its two (noisified) units.
+__device__ inline void det_register_occupancy(const DetParams& P, const DetChrom& ch, DetCell& s,
+                                              uint64_t nevents) {
+  const uint64_t sp = ch.start + 1, epp = ch.end - 1;
+  const double spd = (double)sp, epd = (double)epp;
+  const uint64_t n = s.nact;
+  const uint64_t bucket = bucket_of(n - 1);
+  while (nevents != 0) {
+    const uint64_t i = det_uniform_b(s.g, 0, n - 1, bucket);
+    if (s.ep[i] == kUnbound) continue;
+    const uint64_t r = s.rev[i], f = s.fwd[i];
+    if (!(r > sp && r < epp && f > sp && f < epp)) continue;
+    const double a = (double)r - (P.noisify ? det_gev(s.g, P.gev_mu, P.gev_sigma, P.gev_xi) : 0.0);
+    const double b = (double)f + (P.noisify ? det_gev(s.g, P.gev_mu, P.gev_sigma, P.gev_xi) : 0.0);
+    const double p1 = b < a ? b : a;
+    const double p2 = a < b ? b : a;
+    if (!(p1 >= spd && p2 >= spd && p1 < epd && p2 < epd)) continue;
+    atomicAdd(&ch.occupancy[((uint64_t)p1 - sp) / P.bin_size], 1ULL);
+    atomicAdd(&ch.occupancy[((uint64_t)p2 - sp) / P.bin_size], 1ULL);
+    --nevents;
+  }
+}
+
 __device__ inline void det_sample_contacts(const DetParams& P, const DetChrom& ch, DetCell& s,
                                            uint64_t target) {
   uint64_t n = ch.cpe;
@@ -552,6 +576,7 @@ __device__ inline void det_sample_contacts(const DetParams& P, const DetChrom& c
   }
   det_register(P, ch, s, nloop, false);
   det_register(P, ch, s, n - nloop, true);
+  if (ch.occupancy) det_register_occupancy(P, ch, s, n);
   s.contacts += n;
 }
 
@@ -577,10 +602,38 @@ __device__ inline uint64_t det_epoch_hash(const DetCell& s) {
   return h ^ (s.contacts << 1);
 }
 
+// dump_stats (simulation.cpp:1166-1224): stalls are collisions that occurred (COLLISION bit),
+// the typed counts sum both sides, the average loop size is stats::mean over the active LEFs
+// (descriptive_impl.hpp:21-30, a sequential double sum).
+__device__ inline void det_log_state(const DetParams& P, const DetChrom& ch, const DetCell& s,
+                                     uint32_t item) {
+  const unsigned long long slot = atomicAdd(P.log.count, 1ULL);
+  if (slot >= P.log.cap) return;
+  StateRecord r{};
+  r.item = item;
+  r.epoch = (uint32_t)s.epoch;
+  r.burnin = s.burnin_done ? 0u : 1u;
+  r.nact = (uint32_t)s.nact;
+  for (uint64_t i = 0; i < ch.nbar; ++i) r.active_barriers += s.bst[i] == 1;
+  double acc = 0.0;
+  for (uint64_t i = 0; i < s.nact; ++i) {
+    const bool sr = c_any(s.rc[i]), sf = c_any(s.fc[i]);
+    r.stalls_rev += sr;
+    r.stalls_fwd += sf;
+    r.stalls_both += sr && sf;
+    r.lef_bar += (uint32_t)c_is(s.rc[i], kEvLefBar) + (uint32_t)c_is(s.fc[i], kEvLefBar);
+    r.primary += (uint32_t)c_is(s.rc[i], kEvPrimary) + (uint32_t)c_is(s.fc[i], kEvPrimary);
+    r.secondary += (uint32_t)c_is(s.rc[i], kEvSecondary) + (uint32_t)c_is(s.fc[i], kEvSecondary);
+    acc = acc + (double)(s.fwd[i] - s.rev[i]);
+  }
+  r.avg_loop_size = s.nact == 0 ? 0.0 : acc / (double)s.nact;
+  P.log.rec[slot] = r;
+}
+
 // simulate_one_cell (simulation.cpp:1056-1152) on lane 0.
 __device__ inline void det_simulate_cell(const DetParams& P, const DetChrom& ch, DetCell& s,
                                          uint64_t seed, uint64_t target, uint64_t* trace,
-                                         uint64_t max_trace) {
+                                         uint64_t max_trace, uint32_t item = 0) {
   const uint64_t N = ch.nlefs;
   s.n = N;
   s.nact = 0;
@@ -641,6 +694,7 @@ __device__ inline void det_simulate_cell(const DetParams& P, const DetChrom& ch,
       s.rev[i] -= s.rm[i];
       s.fwd[i] += s.fm[i];
     }
+    if (P.log.rec) det_log_state(P, ch, s, item);
     det_release(P, ch, s);
     if (trace && s.epoch < max_trace) trace[s.epoch] = det_epoch_hash(s);
   }

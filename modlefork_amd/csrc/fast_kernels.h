@@ -25,9 +25,12 @@ struct FastJob {
 int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& chroms,
                     const mgpu_barriers* host_bars, const std::vector<double>& release,
                     const std::vector<WorkItem>& items, FastJob* job);
-// timers != 0: accumulate per-phase wall-clock ticks (read with fast_phase_times)
+// timers != 0: accumulate per-phase wall-clock ticks (read with fast_phase_times). With nev
+// greater than the number of launches, ev[k] is recorded before launch k and ev[launches] after
+// the last one.
 hipError_t fast_job_run(hipStream_t stream, FastJob* job, const WorkItem* d_items,
-                        uint64_t* d_cell_stats, int timers);
+                        uint64_t* d_cell_stats, int timers, hipEvent_t* ev = nullptr,
+                        uint32_t nev = 0);
 // Times every workgroup size on `sample` cells spread over the work order (0: all cells) and
 // keeps the fastest: per size tier when the job launches by tiers, else for its single launch.
 // ms (5 entries, may be null) receives the summed times of 64 to 1024 lanes. The bands are left
@@ -39,9 +42,12 @@ int fast_job_autotune(hipStream_t stream, FastJob* job, const WorkItem* d_items,
 int fast_job_set_threads(FastJob* job, uint32_t threads);
 uint32_t fast_job_threads(const FastJob* job);
 // The launches the next run makes (one per size tier, or the single launch): workgroup size,
-// grid and cells of each, at most max of them; returns their number.
+// grid, cells and first position in the work order of each, at most max of them; returns their
+// number.
 uint32_t fast_job_tiers(const FastJob* job, uint32_t* threads, uint32_t* grid, uint64_t* cells,
-                        uint32_t max);
+                        uint32_t max, uint64_t* begin = nullptr);
+// The work order (work-item indices, launch by launch).
+const uint32_t* fast_job_order(const FastJob* job);
 // Workgroup size of one size tier (index as listed by fast_job_tiers in tiered mode).
 int fast_job_set_tier_threads(FastJob* job, uint32_t tier, uint32_t threads);
 // global != 0: keep every cell's working set in a per-workgroup slab of device memory even when
@@ -60,6 +66,8 @@ hipError_t launch_zero_words(hipStream_t stream, void* p, uint64_t bytes);
 // Sets the dynamic-LDS limit of every fast-mode kernel to the most one workgroup may take
 // (host-side, outside captures; job creation does it too).
 hipError_t fast_job_prepare(FastJob* job);
+// Per-epoch state records of later runs go to log (log.rec == nullptr: none).
+void fast_job_set_log(FastJob* job, const StateLog& log);
 // Philox4x32-10 of n (counter[4], key[2]) tuples: in holds 6 words per tuple, out 4.
 hipError_t launch_philox_kat(hipStream_t stream, const uint32_t* d_in, uint32_t* d_out, int n);
 

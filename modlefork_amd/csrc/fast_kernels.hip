@@ -65,13 +65,17 @@ __device__ void block_scan(T* a, int n, Op op, T ident, bool reverse, T* sh) {
   __syncthreads();
 }
 
+// Sum over the block: a butterfly within each wave, then every wave reduces the NT / 64 wave
+// sums the same way (a sequential loop over them, unrolled, held 16 u64 loads in flight and
+// made this the register-pressure peak of the epoch loop).
 template <int NT>
 __device__ uint64_t block_sum_u64(uint64_t v, uint64_t* sh) {
+  constexpr int NW = NT / 64;
   for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
-  uint64_t t = 0;
-  for (int k = 0; k < NT / 64; ++k) t += sh[k];
+  uint64_t t = (threadIdx.x & 63) < NW ? sh[threadIdx.x & 63] : 0;
+  for (int d = NW / 2; d >= 1; d >>= 1) t += __shfl_xor(t, d, 64);
   __syncthreads();
   return t;
 }
@@ -464,13 +468,63 @@ __device__ void fix_avoided_fwd(Cell& c, const FastChrom& ch, uint32_t n, const 
   }
 }
 
+// dump_stats (simulation.cpp:1166-1224) for one epoch, on the state after the move corrections
+// and before extrusion applies them: stalls are collisions that occurred, the typed counts sum
+// both sides, loop sizes are taken at the extruded positions. Lane 0 writes the record.
+template <int NT>
+__device__ __attribute__((always_inline)) void log_state(const FastParams& P, const FastChrom& ch,
+                                                    Cell& c, Shared& S, uint32_t n,
+                                                    uint32_t epoch, bool bdone, uint32_t item) {
+  uint64_t st_rev = 0, st_fwd = 0, st_both = 0, n_bar = 0, n_pri = 0, n_sec = 0, loops = 0;
+  for (uint32_t i = threadIdx.x; i < n; i += NT) {
+    const uint32_t kr = ck_kind(c.rc[i]), kf = ck_kind(c.fc[i]);
+    const bool sr = ck_stalled(c.rc[i]), sf = ck_stalled(c.fc[i]);
+    st_rev += sr;
+    st_fwd += sf;
+    st_both += sr && sf;
+    n_bar += (kr == kCkBar) + (kf == kCkBar);
+    n_pri += (kr == kCkPrimary) + (kf == kCkPrimary);
+    n_sec += (kr == kCkSecondary) + (kf == kCkSecondary);
+    loops += (uint64_t)((c.fwd[i] + c.fm[i]) - (c.rev[i] - c.rm[i]));
+  }
+  uint64_t n_act = 0;
+  for (uint32_t b = threadIdx.x; b < ch.nbar; b += NT) n_act += c.bst[b];
+  // counts < 2^17 and loop sums < 2^48: two counts share one 64-bit reduction
+  const uint64_t a0 = block_sum_u64<NT>(st_rev | (st_fwd << 32), S.shx);
+  const uint64_t a1 = block_sum_u64<NT>(st_both | (n_act << 32), S.shx);
+  const uint64_t a2 = block_sum_u64<NT>(n_bar | (n_pri << 32), S.shx);
+  const uint64_t a3 = block_sum_u64<NT>(n_sec, S.shx);
+  const uint64_t a4 = block_sum_u64<NT>(loops, S.shx);
+  if (threadIdx.x == 0) {
+    const unsigned long long slot = atomicAdd(P.log.count, 1ULL);
+    if (slot < P.log.cap) {
+      StateRecord r{};
+      r.item = item;
+      r.epoch = epoch;
+      r.burnin = bdone ? 0u : 1u;
+      r.nact = n;
+      r.stalls_rev = (uint32_t)a0;
+      r.stalls_fwd = (uint32_t)(a0 >> 32);
+      r.stalls_both = (uint32_t)a1;
+      r.active_barriers = (uint32_t)(a1 >> 32);
+      r.lef_bar = (uint32_t)a2;
+      r.primary = (uint32_t)(a2 >> 32);
+      r.secondary = (uint32_t)a3;
+      r.avg_loop_size = n ? (double)a4 / (double)n : 0.0;
+      P.log.rec[slot] = r;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // the cell loop
 // ---------------------------------------------------------------------------------------------
-template <int NT>
+// kOut: the optional outputs (1D occupancy, state log) are compiled in; the plain variant keeps
+// their code, and the registers it would hold, out of the epoch loop.
+template <int NT, bool kOut>
 __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, const FastChrom& ch,
                                                          Cell& c, Shared& S,
-                          uint64_t seed, uint64_t target, uint64_t* stats) {
+                          uint64_t seed, uint64_t target, uint64_t* stats, uint32_t item) {
   const uint32_t N = ch.nlefs, NB = ch.nbar;
   const Rng R{(uint32_t)seed, (uint32_t)(seed >> 32)};
   for (uint32_t i = threadIdx.x; i < N; i += NT) {
@@ -645,6 +699,30 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
           }
           band_add(ch, (q1 - sp) / P.bin_size, (q2 - sp) / P.bin_size);
           break;
+        }
+        // 1D LEF occupancy (register_1d_lef_occupancy, register_contacts.cpp:200-232): one more
+        // LEF per sampling event, both of its (noisified) units
+        if (kOut && ch.occupancy) {
+          for (uint32_t attempt = 0; attempt < 1024; ++attempt) {
+            const U4 d = R.draw(s, epoch, kSOccupancy, attempt);
+            const uint32_t i = ubounded(d.x, n);
+            const uint32_t r = c.rev[i], f = c.fwd[i];
+            if (!(r > sp && r < ep && f > sp && f < ep)) continue;
+            double p1 = (double)r, p2 = (double)f;
+            if (P.noisify) {
+              p1 -= (double)gev_noise(P, d.y);
+              p2 += (double)gev_noise(P, d.z);
+            }
+            if (p2 < p1) {
+              const double t = p1;
+              p1 = p2;
+              p2 = t;
+            }
+            if (!(p1 >= (double)sp && p2 < (double)ep)) continue;
+            atomicAdd(&ch.occupancy[((uint32_t)p1 - sp) / P.bin_size], 1ULL);
+            atomicAdd(&ch.occupancy[((uint32_t)p2 - sp) / P.bin_size], 1ULL);
+            break;
+          }
         }
       }
       __syncthreads();  // every lane has read the remaining target before it changes
@@ -901,6 +979,9 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     }
     mark(P, S, 12);
 
+    if constexpr (kOut) {
+      if (P.log.rec) log_state<NT>(P, ch, c, S, n, epoch, bdone, item);
+    }
     // ---- extrude and release (simulation.cpp:489-512, 544-592) ----
     {
       const double base = bdone ? ch.prob_release : ch.prob_release_burnin;
@@ -933,7 +1014,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
 // for four waves per SIMD (at most 128 VGPRs), so 16 waves share a CU at every workgroup size
 // whose cells fit 1024 / NT to a CU: the epoch loop waits on LDS and atomics, and the waves
 // of other workgroups hide that latency.
-template <int NT, bool kLds>
+template <int NT, bool kLds, bool kOut>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
     fast_cells_kernel(FastParams P, const FastChrom* chroms, const WorkItem* items,
                       const uint32_t* order, uint64_t nitems, unsigned long long* next,
@@ -955,7 +1036,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
     } else {
       carve(c, slab + (uint64_t)blockIdx.x * slab_stride, ch.nlefs, ch.nbar, P.hist_len);
     }
-    fast_cell<NT>(P, ch, c, S, w.seed, w.target, cell_stats ? cell_stats + 4 * it : nullptr);
+    fast_cell<NT, kOut>(P, ch, c, S, w.seed, w.target,
+                        cell_stats ? cell_stats + 4 * it : nullptr, (uint32_t)it);
     __syncthreads();
   }
 }
@@ -1013,6 +1095,8 @@ struct FastImpl {
   uint64_t max_bytes = 0;        // largest working set of one cell
   uint64_t max_grid = 0;         // workgroups the slab has room for
   std::vector<Tier> tiers;       // size tiers, in launch order (their runs tile the order)
+  std::vector<uint32_t> order_host;  // the work order (a copy of d_order)
+  bool occupancy = false;        // chromosomes keep a 1D LEF occupancy histogram
   void* d_slab_own = nullptr;    // slab allocated when a job that fits LDS is moved to memory
   uint64_t nitems = 0;
   void* d_mem = nullptr;
@@ -1044,19 +1128,30 @@ Tier single_launch(const FastImpl* impl) {
 
 bool tiered(const FastImpl* impl) { return impl->threads == 0 && !impl->global; }
 
+template <int NT, bool kOut>
+hipError_t launch_fast_out(hipStream_t stream, FastImpl* impl, const WorkItem* d_items,
+                           uint64_t* d_cell_stats, uint32_t grid, uint64_t shmem,
+                           const uint32_t* order, uint64_t nitems) {
+  if (shmem)
+    hipLaunchKernelGGL((fast_cells_kernel<NT, true, kOut>), dim3(grid), dim3(NT), shmem, stream,
+                       impl->P, impl->d_chroms, d_items, order, nitems, impl->d_next, nullptr,
+                       (uint64_t)0, d_cell_stats);
+  else
+    hipLaunchKernelGGL((fast_cells_kernel<NT, false, kOut>), dim3(grid), dim3(NT), 0, stream,
+                       impl->P, impl->d_chroms, d_items, order, nitems, impl->d_next,
+                       impl->d_slab, impl->slab_stride, d_cell_stats);
+  return hipGetLastError();
+}
+
 template <int NT>
 hipError_t launch_fast(hipStream_t stream, FastImpl* impl, const WorkItem* d_items,
                        uint64_t* d_cell_stats, uint32_t grid, uint64_t shmem,
                        const uint32_t* order, uint64_t nitems) {
-  if (shmem)
-    hipLaunchKernelGGL((fast_cells_kernel<NT, true>), dim3(grid), dim3(NT), shmem, stream,
-                       impl->P, impl->d_chroms, d_items, order, nitems, impl->d_next, nullptr,
-                       (uint64_t)0, d_cell_stats);
-  else
-    hipLaunchKernelGGL((fast_cells_kernel<NT, false>), dim3(grid), dim3(NT), 0, stream, impl->P,
-                       impl->d_chroms, d_items, order, nitems, impl->d_next, impl->d_slab,
-                       impl->slab_stride, d_cell_stats);
-  return hipGetLastError();
+  if (impl->occupancy || impl->P.log.rec)
+    return launch_fast_out<NT, true>(stream, impl, d_items, d_cell_stats, grid, shmem, order,
+                                     nitems);
+  return launch_fast_out<NT, false>(stream, impl, d_items, d_cell_stats, grid, shmem, order,
+                                    nitems);
 }
 
 // Launches the persistent kernel with tier t's shape over order[0, nitems) (the work counter is
@@ -1081,7 +1176,11 @@ hipError_t launch_tier(hipStream_t stream, FastImpl* impl, const Tier& t, const 
 // launch changes kernel attributes (launches may be recorded into a graph).
 template <int NT>
 hipError_t set_lds_limit() {
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(fast_cells_kernel<NT, true>),
+  hipError_t e =
+      hipFuncSetAttribute(reinterpret_cast<const void*>(fast_cells_kernel<NT, true, false>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsLimit);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(fast_cells_kernel<NT, true, true>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsLimit);
 }
 
@@ -1242,6 +1341,8 @@ int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& dchroms,
     f.prob_release = release[2 * c];
     f.prob_release_burnin = release[2 * c + 1];
     f.lds_bytes = fast_cell_bytes(f.nlefs, f.nbar, P.hist_len);
+    f.occupancy = d.occupancy;
+    impl->occupancy |= d.occupancy != nullptr;
     impl->chroms.push_back(f);
     off += 4 * d.nbar;
   }
@@ -1274,6 +1375,7 @@ int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& dchroms,
   }
   if (e == hipSuccess)
     e = hipMemcpy(impl->d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice);
+  impl->order_host = order;
   if (e != hipSuccess) {
     g_fast_err = std::string("fast-mode upload failed: ") + hipGetErrorString(e);
     hipFree(impl->d_mem);
@@ -1288,19 +1390,21 @@ int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& dchroms,
 }
 
 hipError_t fast_job_run(hipStream_t stream, FastJob* job, const WorkItem* d_items,
-                        uint64_t* d_cell_stats, int timers) {
+                        uint64_t* d_cell_stats, int timers, hipEvent_t* ev, uint32_t nev) {
   auto* impl = static_cast<FastImpl*>(job->impl);
   if (!impl) return hipErrorInvalidValue;
   impl->P.timers = timers ? 1u : 0u;
-  if (!tiered(impl))
-    return launch_tier(stream, impl, single_launch(impl), d_items, d_cell_stats, impl->d_order,
-                       impl->nitems);
-  for (const Tier& t : impl->tiers) {
-    const hipError_t e =
-        launch_tier(stream, impl, t, d_items, d_cell_stats, impl->d_order + t.begin, t.count);
+  const std::vector<Tier> runs =
+      tiered(impl) ? impl->tiers : std::vector<Tier>{single_launch(impl)};
+  const bool mark = ev != nullptr && nev > runs.size();
+  for (size_t k = 0; k < runs.size(); ++k) {
+    const Tier& t = runs[k];
+    hipError_t e = mark ? hipEventRecord(ev[k], stream) : hipSuccess;
+    if (e == hipSuccess)
+      e = launch_tier(stream, impl, t, d_items, d_cell_stats, impl->d_order + t.begin, t.count);
     if (e != hipSuccess) return e;
   }
-  return hipSuccess;
+  return mark ? hipEventRecord(ev[runs.size()], stream) : hipSuccess;
 }
 
 namespace {
@@ -1345,6 +1449,8 @@ int fast_job_autotune(hipStream_t stream, FastJob* job, const WorkItem* d_items,
   std::vector<float> best_ms(runs.size(), 0.0f);
   std::vector<uint32_t> best(runs.size(), 0);
   impl->P.timers = 0;
+  const StateLog log = impl->P.log;
+  impl->P.log = StateLog{};  // timing runs leave the state log alone
   for (int k = 0; k < 5 && e == hipSuccess; ++k) {
     float total = 0.0f;
     for (size_t r = 0; r < runs.size() && e == hipSuccess; ++r) {
@@ -1360,6 +1466,7 @@ int fast_job_autotune(hipStream_t stream, FastJob* job, const WorkItem* d_items,
     }
     if (ms_out) ms_out[k] = total;
   }
+  impl->P.log = log;
   if (e == hipSuccess) {
     if (tiered(impl)) {
       for (size_t r = 0; r < runs.size(); ++r) impl->tiers[r].threads = best[r];
@@ -1375,6 +1482,10 @@ int fast_job_autotune(hipStream_t stream, FastJob* job, const WorkItem* d_items,
     return -1;
   }
   return 0;
+}
+
+void fast_job_set_log(FastJob* job, const StateLog& log) {
+  if (auto* impl = static_cast<FastImpl*>(job->impl)) impl->P.log = log;
 }
 
 hipError_t fast_job_prepare(FastJob* job) {
@@ -1427,7 +1538,7 @@ uint32_t fast_job_threads(const FastJob* job) {
 }
 
 uint32_t fast_job_tiers(const FastJob* job, uint32_t* threads, uint32_t* grid, uint64_t* cells,
-                        uint32_t max) {
+                        uint32_t max, uint64_t* begin) {
   const auto* impl = static_cast<const FastImpl*>(job->impl);
   if (!impl) return 0;
   std::vector<Tier> runs = tiered(impl) ? impl->tiers : std::vector<Tier>{single_launch(impl)};
@@ -1437,9 +1548,15 @@ uint32_t fast_job_tiers(const FastJob* job, uint32_t* threads, uint32_t* grid, u
     if (threads) threads[n] = t.threads;
     if (grid) grid[n] = launch_grid(impl, t.threads, t.per_cu, t.count);
     if (cells) cells[n] = t.count;
+    if (begin) begin[n] = t.begin;
     ++n;
   }
   return n;
+}
+
+const uint32_t* fast_job_order(const FastJob* job) {
+  const auto* impl = static_cast<const FastImpl*>(job->impl);
+  return impl ? impl->order_host.data() : nullptr;
 }
 
 int fast_job_set_tier_threads(FastJob* job, uint32_t tier, uint32_t threads) {

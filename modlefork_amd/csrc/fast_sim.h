@@ -74,6 +74,7 @@ enum Stream : uint32_t {
   kSSecondary = 8,
   kSRelease = 9,
   kSPoisson = 10,
+  kSOccupancy = 11,
 };
 
 struct Rng {
@@ -113,6 +114,7 @@ struct FastParams {
   uint64_t min_burnin_epochs, max_burnin_epochs;
   double target_contact_density;
   uint32_t timers;  // nonzero: lane 0 accumulates wall-clock ticks per epoch phase
+  StateLog log;     // per-epoch state records (rec == nullptr: off)
 };
 
 struct FastChrom {
@@ -131,6 +133,7 @@ struct FastChrom {
   double bind_rate;
   double prob_release, prob_release_burnin;  // per chromosome: LEF processivity may vary
   uint64_t lds_bytes;  // working-set bytes of one cell
+  unsigned long long* occupancy;  // 1D LEF occupancy, ceil(size / bin) bins, or null
 };
 
 }  // namespace mgpu

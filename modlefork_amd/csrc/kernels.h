@@ -14,6 +14,26 @@ struct WorkItem {
   uint64_t target;  // contacts this cell must register
 };
 
+// One line of the model internal-state log (Simulation::dump_stats, simulation.cpp:1166-1224),
+// as a cell writes it: item is the work-item index, counts are 32-bit (LEFs per cell < 2^31).
+struct StateRecord {
+  uint32_t item, epoch;
+  uint32_t burnin, nact;
+  uint32_t active_barriers, stalls_rev;
+  uint32_t stalls_fwd, stalls_both;
+  uint32_t lef_bar, primary;
+  uint32_t secondary, reserved;
+  double avg_loop_size;
+};
+
+// Device log of StateRecords: cells take slots from `count`; slots at or past `cap` are counted
+// but not written (the host reports the shortfall). rec == nullptr: logging is off.
+struct StateLog {
+  StateRecord* rec;
+  unsigned long long* count;
+  uint64_t cap;
+};
+
 // Post-transform parameters in the form the deterministic kernels read them.
 struct DetParams {
   uint64_t bin_size;
@@ -28,6 +48,7 @@ struct DetParams {
   uint64_t hist_len, window;
   uint64_t min_burnin_epochs, max_burnin_epochs;
   double target_contact_density;
+  StateLog log;
 };
 
 // Device view of one chromosome: barrier table (sorted), band and its counters.
@@ -45,6 +66,7 @@ struct DetChrom {
   uint64_t target_epochs;  // epochs criterion
   double bind_rate;        // nlefs / burnin_target_epochs_for_lef_activation
   const double* qn;        // pow(1 - p_loop, t) for t in [0, cpe], or null
+  unsigned long long* occupancy;  // 1D LEF occupancy, ceil(size / bin) bins, or null
 };
 
 hipError_t launch_det_cells(hipStream_t stream, const DetParams& P, const DetChrom* d_chroms,

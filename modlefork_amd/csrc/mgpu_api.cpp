@@ -23,6 +23,7 @@ struct mgpu_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t lev[5] = {};  // around each launch of a run (fast mode launches by size tier)
   std::string err;
   int num_cus = 0;
   int phase_timers = 0;
@@ -131,6 +132,7 @@ int mgpu_init(int device, mgpu_ctx** out) {
   HIPCHK(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   HIPCHK(ctx, hipEventCreate(&ctx->ev0));
   HIPCHK(ctx, hipEventCreate(&ctx->ev1));
+  for (auto& e : ctx->lev) HIPCHK(ctx, hipEventCreate(&e));
   hipDeviceProp_t prop;
   HIPCHK(ctx, hipGetDeviceProperties(&prop, device));
   ctx->num_cus = prop.multiProcessorCount;
@@ -143,6 +145,8 @@ void mgpu_destroy(mgpu_ctx* ctx) {
   hipSetDevice(ctx->device);
   if (ctx->ev0) hipEventDestroy(ctx->ev0);
   if (ctx->ev1) hipEventDestroy(ctx->ev1);
+  for (auto e : ctx->lev)
+    if (e) hipEventDestroy(e);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -302,6 +306,8 @@ int mgpu_config_transform(const mgpu_config* in, uint32_t set_mask, mgpu_params*
   p.barrier_occupied_stp = stp_active;
   p.barrier_not_occupied_stp = stp_inactive;
   p.override_barrier_occupancy = (set_mask & MGPU_SET_BARRIER_OCCUPANCY) ? 1 : 0;
+  p.track_1d_lef_position = c.track_1d_lef_position ? 1 : 0;
+  p.log_model_internal_state = c.log_model_internal_state ? 1 : 0;
   *out = p;
   return MGPU_OK;
 }
@@ -419,6 +425,15 @@ struct mgpu_job {
   FastJob fast{};
   hipGraph_t graph = nullptr;       // captured run (mgpu_job_capture)
   hipGraphExec_t graph_exec = nullptr;
+  // optional outputs: 1D LEF occupancy (in the pool) and the state log (own allocation)
+  unsigned long long* d_occ = nullptr;
+  std::vector<uint64_t> occ_off, occ_len;
+  uint64_t occ_total = 0;
+  StateLog log{};
+  void* d_log = nullptr;
+  uint64_t log_default = 0;  // records a job created with log_model_internal_state starts with
+  std::vector<float> launch_ms;  // device time of each launch of the last uncaptured run
+  std::vector<uint32_t> item_chrom;  // chromosome of each work item
 };
 
 namespace {
@@ -556,6 +571,17 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
   const uint64_t off_next = pool.add(64);
   const uint64_t off_stats = pool.add(job->nitems * 4 * 8);
   const uint64_t off_bands = pool.add(job->band_total * 4);
+  // 1D LEF occupancy: ceil(size / bin) u64 bins per chromosome (Chromosome::
+  // allocate_lef_occupancy_buffer, genome.cpp:236-245)
+  uint64_t off_occ = 0;
+  if (p->track_1d_lef_position) {
+    for (uint64_t c = 0; c < nchroms; ++c) {
+      job->occ_off.push_back(job->occ_total);
+      job->occ_len.push_back(ceil_div(chroms[c].size, p->bin_size));
+      job->occ_total += (job->occ_len.back() + 31) & ~31ULL;
+    }
+    off_occ = pool.add(job->occ_total * 8);
+  }
 
   // deterministic-mode working set per block
   if (mode == MGPU_MODE_DETERMINISTIC) {
@@ -582,6 +608,7 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
   job->d_items = reinterpret_cast<WorkItem*>(base + off_items);
   job->d_cell_stats = reinterpret_cast<uint64_t*>(base + off_stats);
   job->d_bands = reinterpret_cast<uint32_t*>(base + off_bands);
+  if (p->track_1d_lef_position) job->d_occ = reinterpret_cast<unsigned long long*>(base + off_occ);
   if (mode == MGPU_MODE_DETERMINISTIC && !job->use_lds) job->d_scratch = base + off_scratch;
 
   std::vector<uint8_t> stage(inputs_end, 0);
@@ -617,6 +644,7 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
     ch.target_epochs = target_epochs[c];
     ch.bind_rate = (double)nlefs[c] / (double)pc.burnin_target_epochs_for_lef_activation;
     ch.qn = reinterpret_cast<const double*>(base + off_qn[c]);
+    ch.occupancy = job->d_occ ? job->d_occ + job->occ_off[c] : nullptr;
     job->chroms.push_back(ch);
     release[2 * c] = pc.prob_release;
     release[2 * c + 1] = pc.prob_release_burnin;
@@ -627,6 +655,7 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
       w.seed = mgpu_cell_seed(&chroms[c], k, p->seed);
       w.target = target_contacts[c][k];
       items.push_back(w);
+      job->item_chrom.push_back((uint32_t)c);
     }
   }
   std::memcpy(stage.data() + off_chroms, job->chroms.data(), nchroms * sizeof(DetChrom));
@@ -643,6 +672,29 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
       hipFree(job->d_pool);
       delete job;
       return fail(ctx, MGPU_EDEVICE, "fast-mode setup failed: %s", fast_last_error());
+    }
+  }
+  if (p->log_model_internal_state) {
+    // about twice the epochs a cell is expected to need: its sampling epochs plus a burn-in of
+    // LEF activation, the history window and the minimum burn-in
+    uint64_t want = 0;
+    for (uint64_t c = 0; c < nchroms; ++c) {
+      const mgpu_params& pc = par(c);
+      const uint64_t burn = pc.burnin_target_epochs_for_lef_activation + pc.burnin_history_length +
+                            std::min<uint64_t>(pc.min_burnin_epochs, 1u << 20);
+      for (uint64_t k = 0; k < ncells[c]; ++k) {
+        const uint64_t sampling = pc.target_contact_density >= 0
+                                      ? ceil_div(target_contacts[c][k], std::max<uint64_t>(1, cpe_of[c]))
+                                      : target_epochs[c];
+        want += 2 * (sampling + burn) + 16;
+      }
+    }
+    job->log_default = want;
+    if (int rc = mgpu_job_set_state_log(job, want)) {
+      fast_job_destroy(&job->fast);
+      hipFree(job->d_pool);
+      delete job;
+      return rc;
     }
   }
   *out = job;
@@ -680,15 +732,26 @@ hipError_t enqueue_clears(mgpu_job* job, hipStream_t s) {
   hipError_t e = launch_zero_words(s, job->d_bands, job->band_total * 4);
   if (e == hipSuccess) e = launch_zero_words(s, job->d_missed, job->nchroms * 8);
   if (e == hipSuccess) e = launch_zero_words(s, job->d_next, 64);
+  if (e == hipSuccess && job->d_occ) e = launch_zero_words(s, job->d_occ, job->occ_total * 8);
+  if (e == hipSuccess && job->log.rec) e = launch_zero_words(s, job->log.count, 8);
   return e;
 }
 
-hipError_t enqueue_simulation(mgpu_job* job, hipStream_t s) {
-  if (job->mode == MGPU_MODE_DETERMINISTIC)
-    return launch_det_cells(s, det_params(&job->p), job->d_chroms, job->d_items, job->nitems,
-                            job->d_next, job->d_scratch, job->stride, job->use_lds, job->grid,
-                            job->d_cell_stats);
-  return fast_job_run(s, &job->fast, job->d_items, job->d_cell_stats, job->ctx->phase_timers);
+hipError_t enqueue_simulation(mgpu_job* job, hipStream_t s, hipEvent_t* ev = nullptr,
+                              uint32_t nev = 0) {
+  if (job->mode == MGPU_MODE_DETERMINISTIC) {
+    DetParams P = det_params(&job->p);
+    P.log = job->log;
+    hipError_t e = ev && nev >= 2 ? hipEventRecord(ev[0], s) : hipSuccess;
+    if (e == hipSuccess)
+      e = launch_det_cells(s, P, job->d_chroms, job->d_items, job->nitems, job->d_next,
+                           job->d_scratch, job->stride, job->use_lds, job->grid,
+                           job->d_cell_stats);
+    if (e == hipSuccess && ev && nev >= 2) e = hipEventRecord(ev[1], s);
+    return e;
+  }
+  return fast_job_run(s, &job->fast, job->d_items, job->d_cell_stats, job->ctx->phase_timers, ev,
+                      nev);
 }
 
 void drop_graph(mgpu_job* job) {
@@ -735,18 +798,28 @@ int mgpu_job_run(mgpu_job* job, mgpu_stats* stats) {
   mgpu_ctx* ctx = job->ctx;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const auto t0 = std::chrono::steady_clock::now();
+  uint32_t nlaunch = 0;
+  job->launch_ms.clear();
   if (job->graph_exec) {
     // the captured clears and launch; the events bracket the whole graph
     HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     HIPCHK(ctx, hipGraphLaunch(job->graph_exec, ctx->stream));
   } else {
+    nlaunch = job->mode == MGPU_MODE_FAST
+                  ? fast_job_tiers(&job->fast, nullptr, nullptr, nullptr, 4)
+                  : 1;
     HIPCHK(ctx, enqueue_clears(job, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    HIPCHK(ctx, enqueue_simulation(job, ctx->stream));
+    HIPCHK(ctx, enqueue_simulation(job, ctx->stream, ctx->lev, 5));
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   const auto t1 = std::chrono::steady_clock::now();
+  for (uint32_t k = 0; k < nlaunch; ++k) {
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->lev[k], ctx->lev[k + 1]));
+    job->launch_ms.push_back(ms);
+  }
   if (stats) {
     float ms = 0.f;
     HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -829,6 +902,125 @@ int mgpu_job_read_band(mgpu_job* job, uint64_t c, uint32_t* out_band, uint64_t* 
   return MGPU_OK;
 }
 
+int mgpu_job_launch_stats(mgpu_job* job, mgpu_launch_stats* out, uint32_t max, uint32_t* n) {
+  if (!job || !n) return fail(job ? job->ctx : nullptr, MGPU_EINVAL, "null argument");
+  mgpu_ctx* ctx = job->ctx;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  uint32_t threads[4] = {64, 0, 0, 0}, grid[4] = {job->grid, 0, 0, 0};
+  uint64_t cells[4] = {job->nitems, 0, 0, 0}, begin[4] = {0, 0, 0, 0};
+  uint32_t k = 1;
+  const uint32_t* order = nullptr;
+  if (job->mode == MGPU_MODE_FAST) {
+    k = fast_job_tiers(&job->fast, threads, grid, cells, 4, begin);
+    order = fast_job_order(&job->fast);
+  }
+  *n = k;
+  if (job->launch_ms.size() != k)
+    return fail(ctx, MGPU_EINVAL, "no uncaptured run with the current launch shape yet");
+  std::vector<uint64_t> cs(job->nitems * 4);
+  HIPCHK(ctx, hipMemcpy(cs.data(), job->d_cell_stats, cs.size() * 8, hipMemcpyDeviceToHost));
+  for (uint32_t l = 0; l < k && l < max && out; ++l) {
+    mgpu_launch_stats o{};
+    o.threads = threads[l];
+    o.grid = grid[l];
+    o.cells = cells[l];
+    for (uint64_t i = begin[l]; i < begin[l] + cells[l]; ++i) {
+      const uint64_t it = order ? order[i] : i;
+      o.epochs += cs[4 * it];
+      o.contacts += cs[4 * it + 2];
+      o.barriers += job->chroms[job->item_chrom[it]].nbar;
+    }
+    o.ms = job->launch_ms[l];
+    out[l] = o;
+  }
+  return MGPU_OK;
+}
+
+int mgpu_job_read_occupancy(mgpu_job* job, uint64_t c, uint64_t* out, uint64_t* nbins) {
+  if (!job || c >= job->nchroms) return fail(job ? job->ctx : nullptr, MGPU_EINVAL, "bad chromosome index");
+  if (!job->d_occ)
+    return fail(job->ctx, MGPU_EINVAL, "the job was created without track_1d_lef_position");
+  mgpu_ctx* ctx = job->ctx;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (nbins) *nbins = job->occ_len[c];
+  if (out)
+    HIPCHK(ctx, hipMemcpy(out, job->d_occ + job->occ_off[c], job->occ_len[c] * 8,
+                          hipMemcpyDeviceToHost));
+  return MGPU_OK;
+}
+
+int mgpu_job_set_state_log(mgpu_job* job, uint64_t capacity) {
+  if (!job) return fail(nullptr, MGPU_EINVAL, "null job");
+  mgpu_ctx* ctx = job->ctx;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (capacity > (1ull << 32)) return fail(ctx, MGPU_ERANGE, "state log capacity above 2^32 records");
+  drop_graph(job);  // a captured run keeps the log pointers it was recorded with
+  if (job->d_log) {
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipFree(job->d_log));
+  }
+  job->d_log = nullptr;
+  job->log = StateLog{};
+  if (capacity) {
+    const uint64_t bytes = 256 + capacity * sizeof(StateRecord);
+    hipError_t e = hipMalloc(&job->d_log, bytes);
+    if (e != hipSuccess)
+      return fail(ctx, MGPU_ENOMEM, "state log of %llu records: %s", (unsigned long long)capacity,
+                  hipGetErrorString(e));
+    job->log.count = static_cast<unsigned long long*>(job->d_log);
+    job->log.rec = reinterpret_cast<StateRecord*>(static_cast<uint8_t*>(job->d_log) + 256);
+    job->log.cap = capacity;
+    HIPCHK(ctx, hipMemset(job->log.count, 0, 8));
+  }
+  if (job->mode == MGPU_MODE_FAST) fast_job_set_log(&job->fast, job->log);
+  return MGPU_OK;
+}
+
+int mgpu_job_read_state_log(mgpu_job* job, mgpu_state_record* out, uint64_t cap,
+                            uint64_t* n_records) {
+  if (!job) return fail(nullptr, MGPU_EINVAL, "null job");
+  mgpu_ctx* ctx = job->ctx;
+  if (!job->log.rec) return fail(ctx, MGPU_EINVAL, "the job has no state log");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  unsigned long long n = 0;
+  HIPCHK(ctx, hipMemcpy(&n, job->log.count, 8, hipMemcpyDeviceToHost));
+  if (n_records) *n_records = n;
+  if (n > job->log.cap || n > cap)
+    return fail(ctx, MGPU_ERANGE, "the state log holds %llu records: capacity %llu, output room %llu",
+                (unsigned long long)n, (unsigned long long)job->log.cap, (unsigned long long)cap);
+  if (!out || n == 0) return MGPU_OK;
+  std::vector<StateRecord> rec(n);
+  HIPCHK(ctx, hipMemcpy(rec.data(), job->log.rec, n * sizeof(StateRecord), hipMemcpyDeviceToHost));
+  std::vector<WorkItem> items(job->nitems);
+  HIPCHK(ctx, hipMemcpy(items.data(), job->d_items, items.size() * sizeof(WorkItem),
+                        hipMemcpyDeviceToHost));
+  std::sort(rec.begin(), rec.end(), [](const StateRecord& a, const StateRecord& b) {
+    return a.item != b.item ? a.item < b.item : a.epoch < b.epoch;
+  });
+  for (uint64_t i = 0; i < n; ++i) {
+    const StateRecord& r = rec[i];
+    if (r.item >= job->nitems) return fail(ctx, MGPU_EDEVICE, "state log record with a bad item");
+    const WorkItem& w = items[r.item];
+    mgpu_state_record o{};
+    o.chrom = w.chrom;
+    o.cell_id = w.cell_id;
+    o.epoch = r.epoch;
+    o.burnin = (int32_t)r.burnin;
+    o.num_active_lefs = r.nact;
+    o.num_barriers = job->chroms[w.chrom].nbar;
+    o.active_barriers = r.active_barriers;
+    o.stalls_rev = r.stalls_rev;
+    o.stalls_fwd = r.stalls_fwd;
+    o.stalls_both = r.stalls_both;
+    o.lef_bar = r.lef_bar;
+    o.primary = r.primary;
+    o.secondary = r.secondary;
+    o.avg_loop_size = r.avg_loop_size;
+    out[i] = o;
+  }
+  return MGPU_OK;
+}
+
 int mgpu_job_band_device_ptr(mgpu_job* job, uint64_t c, void** dptr, uint64_t* nelems) {
   if (!job || c >= job->nchroms) return fail(nullptr, MGPU_EINVAL, "bad chromosome index");
   if (dptr) *dptr = job->d_bands + job->band_off[c];
@@ -842,6 +1034,7 @@ void mgpu_job_destroy(mgpu_job* job) {
   drop_graph(job);
   fast_job_destroy(&job->fast);
   if (job->d_pool) hipFree(job->d_pool);
+  if (job->d_log) hipFree(job->d_log);
   delete job;
 }
 

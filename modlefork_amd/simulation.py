@@ -10,6 +10,7 @@ Option names follow modle::Config (simulation_config.hpp:24-135).
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -214,6 +215,41 @@ class Job:
         self.ctx.check(L.lib().mgpu_job_get_tuning(self.ptr, C.byref(t)))
         return t
 
+    def launch_stats(self):
+        """[MgpuLaunchStats] of the last uncaptured run, one per launch."""
+        out = (L.MgpuLaunchStats * 4)()
+        n = C.c_uint32()
+        self.ctx.check(L.lib().mgpu_job_launch_stats(self.ptr, out, 4, C.byref(n)))
+        return [out[k] for k in range(min(n.value, 4))]
+
+    def occupancy(self, c: int) -> np.ndarray:
+        """Chromosome c's 1D LEF occupancy of the last run (u64 per bin); the job must have been
+        created with track_1d_lef_position."""
+        nb = C.c_uint64()
+        self.ctx.check(L.lib().mgpu_job_read_occupancy(self.ptr, c, None, C.byref(nb)))
+        out = np.zeros(nb.value, np.uint64)
+        self.ctx.check(L.lib().mgpu_job_read_occupancy(self.ptr, c, _p(out, C.c_uint64), None))
+        return out
+
+    def set_state_log(self, capacity: int):
+        """Room for `capacity` per-epoch state records in later runs (0: logging off)."""
+        self.ctx.check(L.lib().mgpu_job_set_state_log(self.ptr, int(capacity)))
+
+    def state_log(self):
+        """The last run's state records, sorted by (chromosome, cell, epoch), as a structured
+        array; None when the log overflowed (the second item is then the records needed)."""
+        n = C.c_uint64()
+        rc = L.lib().mgpu_job_read_state_log(self.ptr, None, 0, C.byref(n))
+        if rc == L.MGPU_ERANGE and n.value:
+            buf = (L.MgpuStateRecord * n.value)()
+            rc = L.lib().mgpu_job_read_state_log(self.ptr, buf, n.value, C.byref(n))
+            if rc == L.MGPU_ERANGE:
+                return None, n.value
+            self.ctx.check(rc)
+            return np.frombuffer(buf, dtype=STATE_DTYPE, count=n.value).copy(), n.value
+        self.ctx.check(rc)
+        return np.zeros(0, STATE_DTYPE), 0
+
     def band(self, c: int):
         p = self.plans[c]
         out = np.zeros(p.nrows * p.ncols + 1, np.uint32)
@@ -238,9 +274,14 @@ class Simulation:
 
     def __init__(self, options: dict, ctx: Context | None = None):
         self.options = dict(options)
-        self.params = make_params({k: v for k, v in options.items() if k != "num_cells"})
+        self.params = make_params({k: v for k, v in options.items()
+                                   if k not in ("num_cells", "nthreads")})
         self.num_cells = int(options.get("num_cells", 512))
+        self.nthreads = int(options.get("nthreads", 0)) or (os.cpu_count() or 1)
         self.ctx = ctx or Context(0)
+        self.lef_1d_occupancy = {}
+        self.state_log = None
+        self.state_log_tsv = ""
 
     def prepare(self, workload, simulate_chromosomes_wo_barriers: bool = False):
         chroms, bars, plans, names = [], [], [], []
@@ -256,14 +297,86 @@ class Simulation:
         return names, chroms, bars, plans
 
     def run_simulate(self, workload, mode: int = L.MODE_DETERMINISTIC, threads_per_cell: int = 0):
+        """Returns ({chromosome: (band, missed, plan)}, stats). With track_1d_lef_position the
+        chromosomes' 1D LEF occupancy lands in self.lef_1d_occupancy ({name: u64 bins}); with
+        log_model_internal_state the per-epoch state records land in self.state_log and its
+        text (the reference's TSV) in self.state_log_tsv."""
         names, chroms, bars, plans = self.prepare(workload)
         job = Job(self.ctx, self.params, chroms, bars, plans, mode)
         if threads_per_cell:
             job.set_tuning(threads_per_cell)
         stats = job.run()
+        if self.params.log_model_internal_state:
+            recs, need = job.state_log()
+            if recs is None:  # the log overflowed: the run is deterministic, so size it and rerun
+                job.set_state_log(need)
+                stats = job.run()
+                recs, need = job.state_log()
+            self.state_log = recs
+            self.state_log_tsv = format_state_log(recs, names, task_id_base(
+                [len(p.targets) for p in plans], self.num_cells, self.nthreads))
         out = {}
+        self.lef_1d_occupancy = {}
         for i, name in enumerate(names):
             band, missed = job.band(i)
             out[name] = (band, missed, plans[i])
+            if self.params.track_1d_lef_position:
+                self.lef_1d_occupancy[name] = job.occupancy(i)
         job.close()
         return out, stats
+
+
+STATE_DTYPE = np.dtype([(n, {C.c_uint64: np.uint64, C.c_int32: np.int32, C.c_uint32: np.uint32,
+                             C.c_double: np.float64}[t]) for n, t in L.MgpuStateRecord._fields_])
+
+# Config::model_internal_state_log_header (simulation_config.hpp:126-131)
+STATE_LOG_HEADER = ("task_id\tepoch\tcell_id\tchrom\tburnin\tbarrier_occupancy\tnum_active_lefs\t"
+                    "num_stalls_rev\tnum_stalls_fwd\tnum_stalls_both\tnum_lef_bar_collisions\t"
+                    "num_primary_lef_lef_collisions\tnum_secondary_lef_lef_collisions\t"
+                    "avg_loop_size\n")
+
+
+def task_id_base(cells_per_chrom, num_cells: int, nthreads: int):
+    """First task id of each simulated chromosome as run_simulate numbers them
+    (scheduler_simulate.cpp:70-78, 141-153): tasks are generated in batches of
+    2 * min(16, num_cells / nthreads) (1 when num_cells <= nthreads) and ids advance over whole
+    batches, including the unused tail of a chromosome's last batch."""
+    deq = 1 if num_cells <= nthreads else min(16, num_cells // nthreads)
+    enq = 2 * deq
+    base, t = [], 0
+    for n in cells_per_chrom:
+        base.append(t)
+        t += ((n + enq - 1) // enq) * enq
+    return base
+
+
+def format_state_log(recs, names, task_base) -> str:
+    """The model internal-state log as Simulation::dump_stats writes it (simulation.cpp:1206-1220),
+    one line per record, after the header."""
+    lines = [STATE_LOG_HEADER]
+    for r in recs:
+        c = int(r["chrom"])
+        nb = int(r["num_barriers"])
+        occ = float(r["active_barriers"]) / nb if nb else float("nan")
+        lines.append(f"{task_base[c] + int(r['cell_id'])}\t{int(r['epoch'])}\t{int(r['cell_id'])}\t"
+                     f"{names[c]}\t{'True' if r['burnin'] else 'False'}\t{_fmt(occ)}\t"
+                     f"{int(r['num_active_lefs'])}\t{int(r['stalls_rev'])}\t{int(r['stalls_fwd'])}\t"
+                     f"{int(r['stalls_both'])}\t{int(r['lef_bar'])}\t{int(r['primary'])}\t"
+                     f"{int(r['secondary'])}\t{_fmt(float(r['avg_loop_size']))}\n")
+    return "".join(lines)
+
+
+def _fmt(x: float) -> str:
+    """fmt's default for a double: the shortest representation that round-trips."""
+    if x != x:
+        return "nan"
+    if x == int(x) and abs(x) < 1e16:
+        return str(int(x))
+    return repr(x)
+
+
+def occupancy_track(counts: np.ndarray) -> np.ndarray:
+    """The values write_1d_lef_occupancy_to_disk hands to the BigWig writer
+    (simulation.cpp:244-252): each bin's count over the chromosome's largest count, as float."""
+    m = float(counts.max()) if len(counts) else 0.0
+    return (counts.astype(np.float64) / m).astype(np.float32) if m else counts.astype(np.float32)

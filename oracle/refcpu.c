@@ -1410,9 +1410,31 @@ static void register_contacts(const rc_config* c, cell_state* s, uint64_t sp, ui
   }
 }
 
+/* register_1d_lef_occupancy, register_contacts.cpp:200-232: nevents LEFs sampled with
+ * replacement; each accepted one adds its two (noisified) unit positions to the 1D histogram. */
+static void register_occupancy(const rc_config* c, cell_state* s, uint64_t sp, uint64_t ep,
+                               uint64_t nevents, uint64_t* occ) {
+  const size_t n = s->nact;
+  while (nevents != 0) {
+    const size_t i = (size_t)rc_uniform_u64(&s->g, 0, n - 1);
+    if (!RC_BOUND(s->ep[i])) continue;
+    const uint64_t r = s->rev[i], f = s->fwd[i];
+    if (!(r > sp && r < ep && f > sp && f < ep)) continue;
+    const double a = (double)r - (c->noisify ? rc_gev(&s->g, c->gev_mu, c->gev_sigma, c->gev_xi) : 0.0);
+    const double b = (double)f + (c->noisify ? rc_gev(&s->g, c->gev_mu, c->gev_sigma, c->gev_xi) : 0.0);
+    const double p1 = b < a ? b : a;
+    const double p2 = a < b ? b : a;
+    const double spd = (double)sp, epd = (double)ep;
+    if (!(p1 >= spd && p2 >= spd && p1 < epd && p2 < epd)) continue;
+    __atomic_fetch_add(&occ[((uint64_t)p1 - sp) / c->bin_size], 1, __ATOMIC_RELAXED);
+    __atomic_fetch_add(&occ[((uint64_t)p2 - sp) / c->bin_size], 1, __ATOMIC_RELAXED);
+    --nevents;
+  }
+}
+
 /* sample_and_register_contacts + compute_num_contacts_loop, register_contacts.cpp:73-127 */
 static void sample_contacts(const rc_config* c, const rc_chrom* ch, cell_state* s, uint64_t cpe,
-                            uint64_t target, rc_band* band) {
+                            uint64_t target, rc_band* band, uint64_t* occ) {
   uint64_t n = cpe;
   if (c->target_contact_density > 0.0) n = umin(n, target - s->contacts);
   if (n == 0) return;
@@ -1423,6 +1445,7 @@ static void sample_contacts(const rc_config* c, const rc_chrom* ch, cell_state* 
   const uint64_t ntad = n - nloop;
   register_contacts(c, s, ch->start + 1, ch->end - 1, nloop, 0, band);
   register_contacts(c, s, ch->start + 1, ch->end - 1, ntad, 1, band);
+  if (occ) register_occupancy(c, s, ch->start + 1, ch->end - 1, n, occ);
   s->contacts += n;
 }
 
@@ -1452,10 +1475,57 @@ static uint64_t epoch_hash(const cell_state* s) {
 
 static void* xalloc(size_t n) { return calloc(n ? n : 1, 1); }
 
-/* simulate_one_cell, simulation.cpp:1056-1152 */
+/* The counts of Simulation::dump_stats (simulation.cpp:1166-1224): stalls are collisions that
+ * occurred (event has the COLLISION bit), per side and on both sides of one LEF; LEF-BAR,
+ * primary and secondary count occurred collisions of that type over both sides; the average
+ * loop size is stats::mean over the active LEFs (descriptive_impl.hpp:21-30, sequential double
+ * sum); the effective occupancy is active barriers over all barriers. */
+void rc_count_state(const uint64_t* rev, const uint64_t* fwd, size_t n, const uint64_t* rcoll,
+                    const uint64_t* fcoll, const uint8_t* bst, size_t nbar, rc_state_record* r) {
+  r->num_active_lefs = (uint32_t)n;
+  r->num_barriers = nbar;
+  r->active_barriers = 0;
+  for (size_t i = 0; i < nbar; ++i) r->active_barriers += bst[i] == BAR_ACTIVE;
+  r->stalls_rev = r->stalls_fwd = r->stalls_both = 0;
+  r->lef_bar = r->primary = r->secondary = 0;
+  double acc = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    const int sr = coll_occurred_any(rcoll[i]), sf = coll_occurred_any(fcoll[i]);
+    r->stalls_rev += (uint64_t)sr;
+    r->stalls_fwd += (uint64_t)sf;
+    r->stalls_both += (uint64_t)(sr && sf);
+    r->lef_bar += (uint64_t)coll_occurred(rcoll[i], EV_LEF_BAR) + (uint64_t)coll_occurred(fcoll[i], EV_LEF_BAR);
+    r->primary += (uint64_t)coll_occurred(rcoll[i], EV_PRIMARY) + (uint64_t)coll_occurred(fcoll[i], EV_PRIMARY);
+    r->secondary +=
+        (uint64_t)coll_occurred(rcoll[i], EV_SECONDARY) + (uint64_t)coll_occurred(fcoll[i], EV_SECONDARY);
+    acc = acc + (double)(fwd[i] - rev[i]);
+  }
+  r->avg_loop_size = n == 0 ? 0.0 : acc / (double)n;
+}
+
+static void log_state(const cell_state* s, const rc_barriers* b, uint64_t cell_id, rc_outputs* out) {
+  const uint64_t slot = __atomic_fetch_add(&out->log_n, 1, __ATOMIC_RELAXED);
+  if (slot >= out->log_cap) return;
+  rc_state_record* r = &out->log[slot];
+  rc_count_state(s->rev, s->fwd, s->nact, s->rc, s->fc, s->bst, b->n, r);
+  r->cell_id = cell_id;
+  r->epoch = s->epoch;
+  r->burnin = !s->burnin_done;
+}
+
 int rc_simulate_cell(const rc_config* c, const rc_chrom* ch, const rc_barriers* b, uint64_t nlefs,
                      uint64_t cell_id, uint64_t target_contacts, uint64_t target_epochs,
                      rc_band* band, rc_cell_stats* stats, uint64_t* trace, uint64_t max_trace) {
+  return rc_simulate_cell_out(c, ch, b, nlefs, cell_id, target_contacts, target_epochs, band,
+                              stats, trace, max_trace, NULL);
+}
+
+/* simulate_one_cell, simulation.cpp:1056-1152 */
+int rc_simulate_cell_out(const rc_config* c, const rc_chrom* ch, const rc_barriers* b,
+                         uint64_t nlefs, uint64_t cell_id, uint64_t target_contacts,
+                         uint64_t target_epochs, rc_band* band, rc_cell_stats* stats,
+                         uint64_t* trace, uint64_t max_trace, rc_outputs* out) {
+  uint64_t* occ = out ? out->occupancy : NULL;
   if (nlefs == 0 || ch->end <= ch->start) return -1;
   cell_state s;
   memset(&s, 0, sizeof(s));
@@ -1503,7 +1573,7 @@ int rc_simulate_cell(const rc_config* c, const rc_chrom* ch, const rc_barriers* 
     rc_bind_lefs(ch->start, ch->end, s.rev, s.fwd, s.ep, n, s.rr, s.fr, s.mask, &s.g, s.epoch);
 
     if (s.burnin_done) {
-      sample_contacts(c, ch, &s, cpe, target_contacts, band);
+      sample_contacts(c, ch, &s, cpe, target_contacts, band, occ);
       if (target_contacts != 0 && s.contacts >= target_contacts) break;
     }
 
@@ -1528,6 +1598,7 @@ int rc_simulate_cell(const rc_config* c, const rc_chrom* ch, const rc_barriers* 
       s.fwd[i] += s.fm[i];
     }
 
+    if (out && out->log) log_state(&s, b, cell_id, out);
     release_lefs(c, &s, b);
     if (trace && s.epoch < max_trace) trace[s.epoch] = epoch_hash(&s);
   }
@@ -1555,6 +1626,7 @@ typedef struct worker_args {
   uint64_t* next;
   rc_band* band;
   rc_cell_stats* stats;
+  rc_outputs* out;
   int err;
 } worker_args;
 
@@ -1564,8 +1636,8 @@ static void* worker_main(void* p) {
     const uint64_t cell = __atomic_fetch_add(a->next, 1, __ATOMIC_RELAXED);
     if (cell >= a->cell_end) break;
     rc_cell_stats st;
-    if (rc_simulate_cell(a->c, a->ch, a->b, a->nlefs, cell, a->targets[cell - a->cell_begin],
-                         a->target_epochs, a->band, &st, NULL, 0) != 0) {
+    if (rc_simulate_cell_out(a->c, a->ch, a->b, a->nlefs, cell, a->targets[cell - a->cell_begin],
+                             a->target_epochs, a->band, &st, NULL, 0, a->out) != 0) {
       a->err = 1;
     }
     if (a->stats) a->stats[cell - a->cell_begin] = st;
@@ -1577,13 +1649,21 @@ int rc_simulate_cells(const rc_config* c, const rc_chrom* ch, const rc_barriers*
                       uint64_t nlefs, uint64_t cell_begin, uint64_t cell_end,
                       const uint64_t* targets, uint64_t target_epochs, rc_band* band,
                       int nthreads, rc_cell_stats* stats) {
+  return rc_simulate_cells_out(c, ch, b, nlefs, cell_begin, cell_end, targets, target_epochs,
+                               band, nthreads, stats, NULL);
+}
+
+int rc_simulate_cells_out(const rc_config* c, const rc_chrom* ch, const rc_barriers* b,
+                          uint64_t nlefs, uint64_t cell_begin, uint64_t cell_end,
+                          const uint64_t* targets, uint64_t target_epochs, rc_band* band,
+                          int nthreads, rc_cell_stats* stats, rc_outputs* out) {
   if (nthreads < 1) nthreads = 1;
   uint64_t next = cell_begin;
   pthread_t* th = (pthread_t*)calloc((size_t)nthreads, sizeof(pthread_t));
   worker_args* args = (worker_args*)calloc((size_t)nthreads, sizeof(worker_args));
   for (int t = 0; t < nthreads; ++t) {
     args[t] = (worker_args){c, ch, b, nlefs, cell_end, target_epochs, targets, cell_begin, &next,
-                            band, stats, 0};
+                            band, stats, out, 0};
     pthread_create(&th[t], NULL, worker_main, &args[t]);
   }
   int err = 0;

@@ -162,6 +162,47 @@ int rc_simulate_cells(const rc_config* cfg, const rc_chrom* chrom, const rc_barr
                       const uint64_t* target_contacts, uint64_t target_epochs, rc_band* band,
                       int nthreads, rc_cell_stats* stats);
 
+/* ---- optional per-cell outputs ---- */
+/* One line of the model internal-state log (Simulation::dump_stats, simulation.cpp:1166-1224),
+ * taken after extrude and before release_lefs (simulation.cpp:1133-1145). */
+typedef struct rc_state_record {
+  uint64_t cell_id;
+  uint64_t epoch;
+  int32_t burnin; /* 1 while burn-in is running */
+  uint32_t num_active_lefs;
+  uint64_t num_barriers, active_barriers;
+  uint64_t stalls_rev, stalls_fwd, stalls_both;
+  uint64_t lef_bar, primary, secondary;
+  double avg_loop_size;
+} rc_state_record;
+
+/* occupancy: u64[ceil(chrom size / bin)] summed over cells (register_1d_lef_occupancy,
+ * register_contacts.cpp:200-232), NULL when --track-1d-lef-position is off. log: records
+ * appended in any cell order (slot = atomic counter); log_n counts every record produced, so
+ * log_n > log_cap means the buffer was too small. NULL when the state log is off. */
+typedef struct rc_outputs {
+  uint64_t* occupancy;
+  rc_state_record* log;
+  uint64_t log_cap;
+  uint64_t log_n;
+} rc_outputs;
+
+/* dump_stats' counts over one epoch's state: n active LEFs, their collisions (reference
+ * encoding) and nbar barrier states (1 = active). cell_id, epoch and burnin are left to the
+ * caller. */
+void rc_count_state(const uint64_t* rev, const uint64_t* fwd, size_t n, const uint64_t* rev_coll,
+                    const uint64_t* fwd_coll, const uint8_t* barrier_state, size_t nbar,
+                    rc_state_record* out);
+int rc_simulate_cell_out(const rc_config* cfg, const rc_chrom* chrom, const rc_barriers* barriers,
+                         uint64_t nlefs, uint64_t cell_id, uint64_t target_contacts,
+                         uint64_t target_epochs, rc_band* band, rc_cell_stats* stats,
+                         uint64_t* epoch_hash, uint64_t max_trace, rc_outputs* out);
+int rc_simulate_cells_out(const rc_config* cfg, const rc_chrom* chrom,
+                          const rc_barriers* barriers, uint64_t nlefs, uint64_t cell_begin,
+                          uint64_t cell_end, const uint64_t* target_contacts,
+                          uint64_t target_epochs, rc_band* band, int nthreads,
+                          rc_cell_stats* stats, rc_outputs* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -100,6 +100,37 @@ class RcBand(C.Structure):
     ]
 
 
+class RcStateRecord(C.Structure):
+    _fields_ = [
+        ("cell_id", C.c_uint64),
+        ("epoch", C.c_uint64),
+        ("burnin", C.c_int32),
+        ("num_active_lefs", C.c_uint32),
+        ("num_barriers", C.c_uint64),
+        ("active_barriers", C.c_uint64),
+        ("stalls_rev", C.c_uint64),
+        ("stalls_fwd", C.c_uint64),
+        ("stalls_both", C.c_uint64),
+        ("lef_bar", C.c_uint64),
+        ("primary", C.c_uint64),
+        ("secondary", C.c_uint64),
+        ("avg_loop_size", C.c_double),
+    ]
+
+
+STATE_DTYPE = np.dtype([(n, {C.c_uint64: np.uint64, C.c_int32: np.int32, C.c_uint32: np.uint32,
+                             C.c_double: np.float64}[t]) for n, t in RcStateRecord._fields_])
+
+
+class RcOutputs(C.Structure):
+    _fields_ = [
+        ("occupancy", C.POINTER(C.c_uint64)),
+        ("log", C.POINTER(RcStateRecord)),
+        ("log_cap", C.c_uint64),
+        ("log_n", C.c_uint64),
+    ]
+
+
 class RcRng(C.Structure):
     _fields_ = [("s", C.c_uint64 * 4), ("draws", C.c_uint64)]
 
@@ -163,6 +194,15 @@ def lib():
                                         C.POINTER(RcBarriers), C.c_uint64, C.c_uint64, C.c_uint64,
                                         U64P, C.c_uint64, C.POINTER(RcBand), C.c_int,
                                         C.POINTER(RcCellStats)]
+        L.rc_count_state.restype = None
+        L.rc_count_state.argtypes = [U64P, U64P, C.c_size_t, U64P, U64P, U8P, C.c_size_t,
+                                     C.POINTER(RcStateRecord)]
+        L.rc_simulate_cells_out.restype = C.c_int
+        L.rc_simulate_cells_out.argtypes = [C.POINTER(RcConfig), C.POINTER(RcChrom),
+                                            C.POINTER(RcBarriers), C.c_uint64, C.c_uint64,
+                                            C.c_uint64, U64P, C.c_uint64, C.POINTER(RcBand),
+                                            C.c_int, C.POINTER(RcCellStats),
+                                            C.POINTER(RcOutputs)]
         _lib = L
     return _lib
 
@@ -211,8 +251,12 @@ def rng(seed: int) -> RcRng:
 
 def simulate_cells(cfg: dict, chrom, barriers: Barriers, nlefs: int, cell_begin: int,
                    cell_end: int, targets: np.ndarray, target_epochs: int, nrows: int,
-                   ncols: int, nthreads: int = 1, band: np.ndarray | None = None):
-    """Returns (band u32[nrows*ncols+1], missed, per-cell stats array)."""
+                   ncols: int, nthreads: int = 1, band: np.ndarray | None = None,
+                   occupancy: np.ndarray | None = None, state_log_cap: int = 0):
+    """Returns (band u32[nrows*ncols+1], missed, per-cell stats array); with `occupancy`
+    (u64[ceil(size / bin)], summed into) the 1D LEF occupancy is tracked, and with
+    state_log_cap > 0 a fourth item is returned: the per-epoch state records, sorted by
+    (cell, epoch), as a structured array (raises if more than state_log_cap were produced)."""
     if band is None:
         band = np.zeros(nrows * ncols + 1, dtype=np.uint32)
     rb = RcBand(nrows, ncols, ptr(band, C.c_uint32), 0)
@@ -220,14 +264,46 @@ def simulate_cells(cfg: dict, chrom, barriers: Barriers, nlefs: int, cell_begin:
     stats = (RcCellStats * max(ncell, 1))()
     targets = np.ascontiguousarray(targets, dtype=np.uint64)
     c = make_config(cfg)
-    rcode = lib().rc_simulate_cells(C.byref(c), C.byref(chrom), C.byref(barriers.c), nlefs,
-                                    cell_begin, cell_end, ptr(targets, C.c_uint64),
-                                    target_epochs, C.byref(rb), nthreads, stats)
+    out = None
+    log = None
+    if occupancy is not None or state_log_cap:
+        out = RcOutputs()
+        if occupancy is not None:
+            assert occupancy.dtype == np.uint64 and occupancy.flags.c_contiguous
+            out.occupancy = ptr(occupancy, C.c_uint64)
+        if state_log_cap:
+            log = (RcStateRecord * state_log_cap)()
+            out.log = C.cast(log, C.POINTER(RcStateRecord))
+            out.log_cap = state_log_cap
+    rcode = lib().rc_simulate_cells_out(C.byref(c), C.byref(chrom), C.byref(barriers.c), nlefs,
+                                        cell_begin, cell_end, ptr(targets, C.c_uint64),
+                                        target_epochs, C.byref(rb), nthreads, stats,
+                                        C.byref(out) if out is not None else None)
     if rcode != 0:
         raise RuntimeError("oracle simulation failed")
     st = np.array([(s.epochs, s.burnin_epochs, s.contacts, s.draws) for s in stats[:ncell]],
                   dtype=np.uint64).reshape(-1, 4)
-    return band, int(rb.missed), st
+    if not state_log_cap:
+        return band, int(rb.missed), st
+    if out.log_n > state_log_cap:
+        raise RuntimeError(f"state log needs {out.log_n} records, capacity {state_log_cap}")
+    recs = np.frombuffer(log, dtype=STATE_DTYPE, count=out.log_n).copy()
+    recs = recs[np.lexsort((recs["epoch"], recs["cell_id"]))]
+    return band, int(rb.missed), st, recs
+
+
+def count_state(rev, fwd, rev_coll, fwd_coll, barrier_state):
+    """dump_stats' counts for one epoch's state (rc_count_state)."""
+    rev = np.ascontiguousarray(rev, np.uint64)
+    fwd = np.ascontiguousarray(fwd, np.uint64)
+    rcol = np.ascontiguousarray(rev_coll, np.uint64)
+    fcol = np.ascontiguousarray(fwd_coll, np.uint64)
+    bst = np.ascontiguousarray(barrier_state, np.uint8)
+    r = RcStateRecord()
+    lib().rc_count_state(ptr(rev, C.c_uint64), ptr(fwd, C.c_uint64), len(rev),
+                         ptr(rcol, C.c_uint64), ptr(fcol, C.c_uint64),
+                         ptr(bst, C.c_uint8) if len(bst) else None, len(bst), C.byref(r))
+    return {n: getattr(r, n) for n, _ in RcStateRecord._fields_}
 
 
 def simulate_cell_trace(cfg: dict, chrom, barriers: Barriers, nlefs: int, cell: int,

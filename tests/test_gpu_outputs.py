@@ -1,0 +1,172 @@
+"""The optional per-cell outputs on the GPU, through the C-ABI: the 1D LEF occupancy track
+(register_1d_lef_occupancy, register_contacts.cpp:200-232) and the model internal-state log
+(Simulation::dump_stats, simulation.cpp:1166-1224). Deterministic mode matches the oracle bit
+for bit; fast mode matches it in distribution. Also the per-launch figures of a run."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+from modlefork_amd import _lib as L
+from modlefork_amd import simulation as S
+from modlefork_amd import workloads as W
+from oracle import config as OC
+from oracle import refcpu as R
+
+pytestmark = pytest.mark.gpu
+
+OUT_OPTS = dict(track_1d_lef_position=1, log_model_internal_state=1)
+FIELDS = ("epoch", "burnin", "num_active_lefs", "num_barriers", "active_barriers", "stalls_rev",
+          "stalls_fwd", "stalls_both", "lef_bar", "primary", "secondary", "avg_loop_size")
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = S.Context(0)
+    yield c
+    c.close()
+
+
+def tiny(size=2_000_000, ncells=8, nbar=40, seed=3):
+    rng = np.random.default_rng(seed)
+    starts = np.sort(rng.choice(size - 1, nbar, replace=False))
+    recs = [W.BedRecord("chrT", int(s), int(s) + 1, float(o), "+" if d else "-")
+            for s, d, o in zip(starts, rng.random(nbar) < 0.5, rng.uniform(0.5, 0.95, nbar))]
+    return W.Workload("tiny", [("chrT", size)], recs,
+                      dict(bin_size=5000, diagonal_width=1_000_000, num_cells=ncells, seed=seed))
+
+
+def oracle_outputs(wl, nthreads=8):
+    name, size = wl.chroms[0]
+    c = OC.transform(wl.options)
+    rc = OC.to_rc(c)
+    bars = OC.prepare_barriers(wl.barriers_of(name), 0, size, c)
+    nl = R.lib().rc_num_lefs(c["number_of_lefs_per_mbp"], size)
+    ncols = R.lib().rc_band_ncols(c["bin_size"], size)
+    nrows = R.lib().rc_band_nrows(c["diagonal_width"], c["bin_size"], size)
+    ncells = wl.options["num_cells"]
+    t = np.zeros(ncells, np.uint64)
+    cfg = R.make_config(rc)
+    te = R.lib().rc_plan_targets(C.byref(cfg), ncells, nl, nrows * ncols,
+                                 t.ctypes.data_as(C.POINTER(C.c_uint64)))
+    ch = R.make_chrom(name, size, 0, size)
+    occ = np.zeros((size + c["bin_size"] - 1) // c["bin_size"], np.uint64)
+    band, missed, st, recs = R.simulate_cells(rc, ch, bars, nl, 0, ncells, t, te, nrows, ncols,
+                                              nthreads, occupancy=occ,
+                                              state_log_cap=4 * int(t.sum()) + 400_000)
+    return band, missed, st, occ, recs
+
+
+def device_outputs(ctx, wl, mode):
+    sim = S.Simulation(dict(wl.options, **OUT_OPTS), ctx)
+    out, stats = sim.run_simulate(wl, mode)
+    name = wl.chroms[0][0]
+    return out[name][0], out[name][1], stats, sim.lef_1d_occupancy[name], sim.state_log, sim
+
+
+def test_deterministic_outputs_bit_exact(ctx):
+    wl = tiny()
+    say("outputs det: oracle")
+    band, missed, st, occ, recs = oracle_outputs(wl)
+    say("outputs det: device")
+    dband, dmissed, stats, docc, drecs, sim = device_outputs(ctx, wl, L.MODE_DETERMINISTIC)
+    # tracking consumes draws after the contacts: the band must still match the oracle's run
+    # with tracking on
+    assert dmissed == missed and np.array_equal(dband, band)
+    assert np.array_equal(docc, occ), int(np.count_nonzero(docc != occ))
+    assert int(docc.sum()) == 2 * int(st[:, 2].sum())
+    assert len(drecs) == len(recs) == int(st[:, 0].sum())
+    assert np.array_equal(drecs["cell_id"], recs["cell_id"])
+    for f in FIELDS:
+        assert np.array_equal(drecs[f], recs[f]), f
+    # the reference's TSV layout: header plus one line per record, 14 columns
+    lines = sim.state_log_tsv.splitlines()
+    assert lines[0].split("\t")[0] == "task_id" and len(lines) == len(recs) + 1
+    assert all(len(x.split("\t")) == 14 for x in lines[1:])
+
+
+def test_state_log_overflow_reruns_with_room(ctx):
+    wl = tiny(ncells=4)
+    sim = S.Simulation(dict(wl.options, log_model_internal_state=1), ctx)
+    names, chroms, bars, plans = sim.prepare(wl)
+    job = S.Job(ctx, sim.params, chroms, bars, plans, L.MODE_DETERMINISTIC)
+    job.set_state_log(10)
+    st = job.run()
+    recs, need = job.state_log()
+    assert recs is None and need == int(st.epochs)
+    job.set_state_log(need)
+    st2 = job.run()
+    recs, n2 = job.state_log()
+    assert n2 == need and len(recs) == need and int(st2.epochs) == need
+    job.close()
+
+
+def _sampling_means(recs):
+    s = recs[recs["burnin"] == 0]
+    n = s["num_active_lefs"].astype(np.float64)
+    out = {k: s[k] / n for k in ("stalls_rev", "stalls_fwd", "stalls_both", "lef_bar", "primary",
+                                 "secondary")}
+    out["avg_loop_size"] = s["avg_loop_size"]
+    out["barrier_occupancy"] = s["active_barriers"] / s["num_barriers"]
+    cells = s["cell_id"]
+    res = {}
+    for k, v in out.items():
+        per_cell = np.array([v[cells == c].mean() for c in np.unique(cells)])
+        res[k] = (float(v.mean()), float(per_cell.std() / np.sqrt(len(per_cell))))
+    return res
+
+
+def test_fast_outputs_match_the_oracle_in_distribution(ctx):
+    """Small configuration (20 Mbp, 400 LEFs, 200 barriers), 64 cells each side. Oracle split
+    halves agree within 0.5 % on these means and at 0.97 on the 25-kb occupancy profile; fast
+    mode measured within 0.8 % and at 0.98."""
+    wl = W.small()
+    say("outputs fast: oracle")
+    _, _, st, occ, recs = oracle_outputs(wl)
+    say("outputs fast: device")
+    _, _, stats, docc, drecs, _ = device_outputs(ctx, wl, L.MODE_FAST)
+    assert int(docc.sum()) == 2 * int(stats.contacts)
+    assert len(drecs) == int(stats.epochs)
+    for c in np.unique(drecs["cell_id"]):
+        e = drecs[drecs["cell_id"] == c]["epoch"]
+        assert np.array_equal(e, np.arange(len(e)))
+    assert np.all(drecs["stalls_both"] <= np.minimum(drecs["stalls_rev"], drecs["stalls_fwd"]))
+    a = occ[:4000].reshape(-1, 5).sum(1).astype(np.float64)
+    b = docc[:4000].reshape(-1, 5).sum(1).astype(np.float64)
+    r = float(np.corrcoef(a, b)[0, 1])
+    say(f"  occupancy profile correlation {r:.4f}")
+    assert r > 0.9
+    mo, md = _sampling_means(recs), _sampling_means(drecs)
+    bad = []
+    for k in mo:
+        (vo, so), (vd, sd) = mo[k], md[k]
+        tol = max(0.04 * abs(vo), 6.0 * np.hypot(so, sd))
+        say(f"  {k:18s} oracle {vo:.5g} fast {vd:.5g} (tol {tol:.3g})")
+        if abs(vd - vo) > tol:
+            bad.append(k)
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("mode", [L.MODE_FAST, L.MODE_DETERMINISTIC])
+def test_launch_stats_add_up(ctx, mode):
+    # fast mode: uneven chromosomes, so cells split over several size tiers; deterministic mode
+    # runs one lane per cell, so it gets small chromosomes
+    wl = W.genome(ncells=4, nchroms=6) if mode == L.MODE_FAST else tiny(ncells=8)
+    sim = S.Simulation(wl.options, ctx)
+    names, chroms, bars, plans = sim.prepare(wl)
+    job = S.Job(ctx, sim.params, chroms, bars, plans, mode)
+    st = job.run()
+    ls = job.launch_stats()
+    say(f"  mode {mode}: launches {[(x.threads, x.grid, x.cells, round(x.ms, 2)) for x in ls]}")
+    assert sum(x.cells for x in ls) == st.cells
+    assert sum(x.contacts for x in ls) == st.contacts
+    assert sum(x.epochs for x in ls) == st.epochs
+    assert sum(x.barriers for x in ls) == sum(len(b) * len(p.targets) for b, p in zip(bars, plans))
+    assert all(x.ms > 0 for x in ls) and sum(x.ms for x in ls) <= st.kernel_ms * 1.01 + 0.1
+    job.close()
