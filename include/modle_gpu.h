@@ -319,8 +319,13 @@ int mgpu_debug_philox(mgpu_ctx* ctx, const uint32_t* in, uint64_t n, uint32_t* o
  * out (n entries, may be null) receives the totals, which are then cleared. Phases: 0 stop test
  * and burn-in, 1 bind, 2-3 rank rev/fwd, 4 contacts, 5 moves, 6 adjust, 7 barrier states and
  * chromosome ends, 8 LEF-BAR, 9 primary LEF-LEF, 10 move corrections, 11-12 secondary rev/fwd
- * with their bypass fix-ups, 14 extrude and release (13 and 15 unused). */
+ * scans, 13 bypass fix-ups of both sides, 14 extrude and release (15 unused). */
 int mgpu_debug_phase_times(mgpu_ctx* ctx, int enable, uint64_t* out, uint64_t n);
+/* Fast mode's bypass fix-ups (fix_secondary_lef_lef_collisions,
+ * simulation_detect_collisions.cpp:520-647) run in parallel over runs of consecutive ranks;
+ * serial != 0 makes later runs of the job walk each side's list on one lane in the reference
+ * loop's order instead. Results must not change: exposed to check exactly that. */
+int mgpu_debug_fast_serial_fixups(mgpu_job* job, int serial);
 /* Per-epoch state hash of one cell in deterministic mode (for locating a first divergence). */
 int mgpu_debug_cell_trace(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom* chrom,
                           const mgpu_barriers* barriers, uint64_t nlefs, uint64_t cell_id,

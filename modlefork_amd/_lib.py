@@ -213,7 +213,7 @@ FP = C.POINTER(C.c_float)
 
 PHASE_NAMES = ["stop test and burn-in", "bind", "rank rev", "rank fwd", "contacts", "moves",
                "adjust", "barrier states and ends", "LEF-BAR", "primary LEF-LEF",
-               "move corrections", "secondary rev", "secondary fwd", "unused",
+               "move corrections", "secondary rev", "secondary fwd", "bypass fix-ups",
                "extrude and release", "unused"]
 
 # (name, restype, argtypes) for every entry point of include/modle_gpu.h
@@ -264,6 +264,7 @@ SIGNATURES = [
                                    C.c_double, C.c_uint64, U64P]),
     ("mgpu_debug_philox", C.c_int, [C.c_void_p, U32P, C.c_uint64, U32P]),
     ("mgpu_debug_phase_times", C.c_int, [C.c_void_p, C.c_int, U64P, C.c_uint64]),
+    ("mgpu_debug_fast_serial_fixups", C.c_int, [C.c_void_p, C.c_int]),
     ("mgpu_context_stream", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int)]),
     ("mgpu_sg_config_default", C.c_int, [C.POINTER(MgpuSgConfig)]),
     ("mgpu_sg_create", C.c_int, [C.c_void_p, C.POINTER(MgpuSgConfig), C.POINTER(C.c_void_p)]),

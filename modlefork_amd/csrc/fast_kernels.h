@@ -68,6 +68,9 @@ hipError_t launch_zero_words(hipStream_t stream, void* p, uint64_t bytes);
 hipError_t fast_job_prepare(FastJob* job);
 // Per-epoch state records of later runs go to log (log.rec == nullptr: none).
 void fast_job_set_log(FastJob* job, const StateLog& log);
+// serial != 0: bypass fix-ups walk their whole list on one lane, in the reference loop's order
+// (a check of the parallel runs, which must give identical results).
+void fast_job_set_serial_fixups(FastJob* job, int serial);
 // Philox4x32-10 of n (counter[4], key[2]) tuples: in holds 6 words per tuple, out 4.
 hipError_t launch_philox_kat(hipStream_t stream, const uint32_t* d_in, uint32_t* d_out, int n);
 

@@ -414,57 +414,86 @@ __device__ uint32_t secondary_side(const FastParams& P, const Rng& R, uint32_t e
   return S.navoid;
 }
 
-// fix_secondary_lef_lef_collisions (simulation_detect_collisions.cpp:520-647) for the rev units
-// listed in ascending rank order: the bypassing unit swaps places with the one it passed.
-__device__ void fix_avoided_rev(Cell& c, const FastChrom& ch, const uint32_t* list, uint32_t cnt) {
-  for (uint32_t a = 0; a < cnt; ++a) {
-    const uint32_t k = list[a];
-    const uint32_t r2 = c.rr[k];
-    if (k == 0 || ck_kind(c.rc[r2]) != kCkAvoided) continue;
-    const uint32_t r1 = c.rr[k - 1];
-    const uint32_t p1t = c.rev[r1] - c.rm[r1];
-    c.rm[r2] = c.rev[r2] > p1t + 1 ? c.rev[r2] - (p1t + 1) : 0;
-    c.rc[r2] = ck(kCkSecondary, r1);
-    const uint32_t a1 = c.rev[r1], b1 = c.rev[r2];
-    c.rev[r1] = min(c.fwd[r1], b1);
-    c.rev[r2] = min(c.fwd[r2], a1);
-    uint32_t t = c.rc[r1];
-    c.rc[r1] = c.rc[r2];
-    c.rc[r2] = t;
-    t = c.rm[r1];
-    c.rm[r1] = c.rm[r2];
-    c.rm[r2] = t;
-    c.rr[k - 1] = (uint16_t)r2;
-    c.rr[k] = (uint16_t)r1;
-    c.rm[r2] = min(c.rev[r2] - ch.start, c.rm[r2]);
-    c.rm[r1] = min(c.rev[r1] - ch.start, c.rm[r1]);
-  }
+// fix_secondary_lef_lef_collisions (simulation_detect_collisions.cpp:520-647) for the rev unit
+// at rank k: the bypassing unit swaps places with the one it passed.
+__device__ inline void fix_one_rev(Cell& c, const FastChrom& ch, uint32_t k) {
+  const uint32_t r2 = c.rr[k];
+  if (k == 0 || ck_kind(c.rc[r2]) != kCkAvoided) return;
+  const uint32_t r1 = c.rr[k - 1];
+  const uint32_t p1t = c.rev[r1] - c.rm[r1];
+  c.rm[r2] = c.rev[r2] > p1t + 1 ? c.rev[r2] - (p1t + 1) : 0;
+  c.rc[r2] = ck(kCkSecondary, r1);
+  const uint32_t a1 = c.rev[r1], b1 = c.rev[r2];
+  c.rev[r1] = min(c.fwd[r1], b1);
+  c.rev[r2] = min(c.fwd[r2], a1);
+  uint32_t t = c.rc[r1];
+  c.rc[r1] = c.rc[r2];
+  c.rc[r2] = t;
+  t = c.rm[r1];
+  c.rm[r1] = c.rm[r2];
+  c.rm[r2] = t;
+  c.rr[k - 1] = (uint16_t)r2;
+  c.rr[k] = (uint16_t)r1;
+  c.rm[r2] = min(c.rev[r2] - ch.start, c.rm[r2]);
+  c.rm[r1] = min(c.rev[r1] - ch.start, c.rm[r1]);
 }
 
-// The same for fwd units; list holds descending ranks, processed from the lowest rank up.
-__device__ void fix_avoided_fwd(Cell& c, const FastChrom& ch, uint32_t n, const uint32_t* list,
-                                uint32_t cnt) {
-  for (uint32_t a = cnt; a-- > 0;) {
-    const uint32_t k = list[a];
-    const uint32_t f1 = c.fr[k];
-    if (k + 1 >= n || ck_kind(c.fc[f1]) != kCkAvoided) continue;
-    const uint32_t f2 = c.fr[k + 1];
-    const uint32_t p2t = c.fwd[f2] + c.fm[f2];
-    c.fm[f1] = p2t > c.fwd[f1] + 1 ? p2t - (c.fwd[f1] + 1) : 0;
-    c.fc[f1] = ck(kCkSecondary, f2);
-    const uint32_t a1 = c.fwd[f1], b1 = c.fwd[f2];
-    c.fwd[f1] = max(c.rev[f1], b1);
-    c.fwd[f2] = max(c.rev[f2], a1);
-    uint32_t t = c.fc[f1];
-    c.fc[f1] = c.fc[f2];
-    c.fc[f2] = t;
-    t = c.fm[f1];
-    c.fm[f1] = c.fm[f2];
-    c.fm[f2] = t;
-    c.fr[k] = (uint16_t)f2;
-    c.fr[k + 1] = (uint16_t)f1;
-    c.fm[f2] = min(ch.end - 1 - c.fwd[f2], c.fm[f2]);
-    c.fm[f1] = min(ch.end - 1 - c.fwd[f1], c.fm[f1]);
+// The same for the fwd unit at rank k, which swaps with the unit at rank k + 1.
+__device__ inline void fix_one_fwd(Cell& c, const FastChrom& ch, uint32_t n, uint32_t k) {
+  const uint32_t f1 = c.fr[k];
+  if (k + 1 >= n || ck_kind(c.fc[f1]) != kCkAvoided) return;
+  const uint32_t f2 = c.fr[k + 1];
+  const uint32_t p2t = c.fwd[f2] + c.fm[f2];
+  c.fm[f1] = p2t > c.fwd[f1] + 1 ? p2t - (c.fwd[f1] + 1) : 0;
+  c.fc[f1] = ck(kCkSecondary, f2);
+  const uint32_t a1 = c.fwd[f1], b1 = c.fwd[f2];
+  c.fwd[f1] = max(c.rev[f1], b1);
+  c.fwd[f2] = max(c.rev[f2], a1);
+  uint32_t t = c.fc[f1];
+  c.fc[f1] = c.fc[f2];
+  c.fc[f2] = t;
+  t = c.fm[f1];
+  c.fm[f1] = c.fm[f2];
+  c.fm[f2] = t;
+  c.fr[k] = (uint16_t)f2;
+  c.fr[k + 1] = (uint16_t)f1;
+  c.fm[f2] = min(ch.end - 1 - c.fwd[f2], c.fm[f2]);
+  c.fm[f1] = min(ch.end - 1 - c.fwd[f1], c.fm[f1]);
+}
+
+// The fix-ups of one side, in the reference's order: ascending rank. list holds the bypassing
+// units' ranks in chain order (ascending for rev, descending for fwd). A fix-up at rank k touches
+// ranks k - 1 and k (rev) or k and k + 1 (fwd) only, so fix-ups depend on each other only along
+// runs of consecutive ranks: each run goes to one lane, which walks it in ascending rank, and
+// distinct runs (separated by at least one rank) proceed in parallel. serial != 0 walks the
+// whole list on lane 0 instead (the order of the reference loop, kept to check the runs against).
+template <int NT>
+__device__ void fix_avoided(Cell& c, const FastChrom& ch, uint32_t n, const uint32_t* list,
+                            uint32_t cnt, bool rev_side, bool serial) {
+  if (serial) {
+    if (threadIdx.x == 0) {
+      if (rev_side) {
+        for (uint32_t a = 0; a < cnt; ++a) fix_one_rev(c, ch, list[a]);
+      } else {
+        for (uint32_t a = cnt; a-- > 0;) fix_one_fwd(c, ch, n, list[a]);
+      }
+    }
+    return;
+  }
+  for (uint32_t a = threadIdx.x; a < cnt; a += NT) {
+    if (rev_side) {
+      if (a != 0 && list[a - 1] + 1 == list[a]) continue;  // not the lowest rank of its run
+      for (uint32_t b = a;; ++b) {
+        fix_one_rev(c, ch, list[b]);
+        if (b + 1 >= cnt || list[b + 1] != list[b] + 1) break;
+      }
+    } else {
+      if (a + 1 < cnt && list[a + 1] + 1 == list[a]) continue;
+      for (uint32_t b = a;; --b) {
+        fix_one_fwd(c, ch, n, list[b]);
+        if (b == 0 || list[b - 1] != list[b] + 1) break;
+      }
+    }
   }
 }
 
@@ -968,16 +997,18 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     // process_collisions (simulation.cpp:925-955).
     {
       const uint32_t na = secondary_side<NT>(P, R, epoch, c, S, n, true, max(1u, n5));
-      if (threadIdx.x == 0 && na) fix_avoided_rev(c, ch, c.scratch, na);
+      mark(P, S, 11);
+      if (na) fix_avoided<NT>(c, ch, n, c.scratch, na, true, P.serial_fixups);
       __syncthreads();
+      mark(P, S, 13);
     }
-    mark(P, S, 11);
     {
       const uint32_t na = secondary_side<NT>(P, R, epoch, c, S, n, false, (n3 ? n3 - 1 : 0) + 1);
-      if (threadIdx.x == 0 && na) fix_avoided_fwd(c, ch, n, c.scratch, na);
+      mark(P, S, 12);
+      if (na) fix_avoided<NT>(c, ch, n, c.scratch, na, false, P.serial_fixups);
       __syncthreads();
+      mark(P, S, 13);
     }
-    mark(P, S, 12);
 
     if constexpr (kOut) {
       if (P.log.rec) log_state<NT>(P, ch, c, S, n, epoch, bdone, item);
@@ -1486,6 +1517,10 @@ int fast_job_autotune(hipStream_t stream, FastJob* job, const WorkItem* d_items,
 
 void fast_job_set_log(FastJob* job, const StateLog& log) {
   if (auto* impl = static_cast<FastImpl*>(job->impl)) impl->P.log = log;
+}
+
+void fast_job_set_serial_fixups(FastJob* job, int serial) {
+  if (auto* impl = static_cast<FastImpl*>(job->impl)) impl->P.serial_fixups = serial ? 1u : 0u;
 }
 
 hipError_t fast_job_prepare(FastJob* job) {

@@ -115,6 +115,7 @@ struct FastParams {
   double target_contact_density;
   uint32_t timers;  // nonzero: lane 0 accumulates wall-clock ticks per epoch phase
   StateLog log;     // per-epoch state records (rec == nullptr: off)
+  uint32_t serial_fixups;  // nonzero: bypass fix-ups walk their list on one lane (check only)
 };
 
 struct FastChrom {

@@ -1232,6 +1232,14 @@ int mgpu_debug_phase_times(mgpu_ctx* ctx, int enable, uint64_t* out, uint64_t n)
   return MGPU_OK;
 }
 
+int mgpu_debug_fast_serial_fixups(mgpu_job* job, int serial) {
+  if (!job) return fail(nullptr, MGPU_EINVAL, "null job");
+  if (job->mode != MGPU_MODE_FAST) return fail(job->ctx, MGPU_EINVAL, "not a fast-mode job");
+  drop_graph(job);
+  fast_job_set_serial_fixups(&job->fast, serial);
+  return MGPU_OK;
+}
+
 int mgpu_debug_philox(mgpu_ctx* ctx, const uint32_t* in, uint64_t n, uint32_t* out) {
   if (!ctx || !in || !out) return fail(ctx, MGPU_EINVAL, "null argument");
   if (n == 0) return MGPU_OK;

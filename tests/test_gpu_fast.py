@@ -100,6 +100,36 @@ def test_fast_mode_band_does_not_depend_on_workgroup_size(ctx, threads):
     assert (sa.epochs, sa.burnin_epochs, sa.contacts) == (sb.epochs, sb.burnin_epochs, sb.contacts)
 
 
+def _job_bands(ctx, wl, serial):
+    sim = S.Simulation(wl.options, ctx)
+    names, chroms, bars, plans = sim.prepare(wl)
+    job = S.Job(ctx, sim.params, chroms, bars, plans, L.MODE_FAST)
+    ctx.check(L.lib().mgpu_debug_fast_serial_fixups(job.ptr, 1 if serial else 0))
+    st = job.run()
+    bands = [job.band(i) for i in range(len(chroms))]
+    job.close()
+    return bands, (st.epochs, st.burnin_epochs, st.contacts, st.missed)
+
+
+@pytest.mark.parametrize("case", ["bypass-0.5", "genome-slice"])
+def test_parallel_bypass_fixups_match_the_serial_walk(ctx, case):
+    """fix_secondary_lef_lef_collisions in parallel over runs of consecutive ranks against the
+    reference loop's order on one lane: identical bands and epoch counts. A bypass probability
+    of 0.5 makes long runs of adjacent bypassing units; the genome slice has 5 000-LEF cells."""
+    if case == "bypass-0.5":
+        wl = W.small(ncells=8)
+        wl.options["probability_of_extrusion_unit_bypass"] = 0.5
+    else:
+        wl = W.genome(ncells=2, nchroms=3)
+    say(f"fix-ups {case}: serial")
+    a, sa = _job_bands(ctx, wl, True)
+    say(f"fix-ups {case}: parallel")
+    b, sb = _job_bands(ctx, wl, False)
+    assert sa == sb
+    for (ba, ma), (bb, mb) in zip(a, b):
+        assert ma == mb and np.array_equal(ba, bb)
+
+
 def test_working_set_in_memory_gives_the_lds_band(ctx):
     """The same cells with their working sets in a device-memory slab (the layout chromosomes too
     large for LDS use) instead of LDS: identical band and statistics."""
