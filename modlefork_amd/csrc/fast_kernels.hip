@@ -80,6 +80,33 @@ __device__ uint64_t block_sum_u64(uint64_t v, uint64_t* sh) {
   return t;
 }
 
+// Three sums over the block with one pair of barriers; sh holds 3 * NT / 64 entries.
+template <int NT>
+__device__ void block_sum3_u64(uint64_t& a, uint64_t& b, uint64_t& c, uint64_t* sh) {
+  constexpr int NW = NT / 64;
+  for (int d = 32; d >= 1; d >>= 1) {
+    a += __shfl_xor(a, d, 64);
+    b += __shfl_xor(b, d, 64);
+    c += __shfl_xor(c, d, 64);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    sh[w] = a;
+    sh[NW + w] = b;
+    sh[2 * NW + w] = c;
+  }
+  __syncthreads();
+  a = lane < NW ? sh[lane] : 0;
+  b = lane < NW ? sh[NW + lane] : 0;
+  c = lane < NW ? sh[2 * NW + lane] : 0;
+  for (int d = NW / 2; d >= 1; d >>= 1) {
+    a += __shfl_xor(a, d, 64);
+    b += __shfl_xor(b, d, 64);
+    c += __shfl_xor(c, d, 64);
+  }
+  __syncthreads();
+}
+
 __device__ inline uint64_t pos_key(const uint32_t* pos, uint32_t lef) {
   return ((uint64_t)pos[lef] << 32) | lef;
 }
@@ -143,8 +170,8 @@ struct Shared {
   uint64_t item;
   uint64_t contacts;
   uint64_t burnin_epochs;
-  uint64_t shx[kFastMaxWaves];
-  uint32_t nact, hlen, burnin_done, done, nnew, nkept, n5, n3, stable, navoid;
+  uint64_t shx[3 * kFastMaxWaves];
+  uint32_t nact, hlen, hhead, burnin_done, done, nnew, nkept, n5, n3, stable, navoid;
 };
 
 // bitonic sort of keys[0, m) ascending, m a power of two
@@ -168,9 +195,14 @@ __device__ void bitonic(uint64_t* keys, uint32_t m) {
 }
 
 // Sorts LEF ids a[0, n) by (position, id) with odd-even transposition passes; returns after two
-// consecutive passes without a swap. Cheap on the nearly sorted orders extrusion leaves.
+// consecutive passes without a swap. Cheap on the nearly sorted orders extrusion leaves: one
+// check pass first, which usually finds the order sorted already.
 template <int NT>
 __device__ void odd_even_sort(uint32_t* a, uint32_t n, const uint32_t* pos) {
+  int bad = 0;
+  for (uint32_t i = threadIdx.x; i + 1 < n; i += NT)
+    bad |= pos_key(pos, a[i + 1]) < pos_key(pos, a[i]);
+  if (!__syncthreads_or(bad)) return;
   int quiet = 0;
   for (uint32_t pass = 0; quiet < 2; ++pass) {
     int swapped = 0;
@@ -183,6 +215,69 @@ __device__ void odd_even_sort(uint32_t* a, uint32_t n, const uint32_t* pos) {
       }
     }
     quiet = __syncthreads_or(swapped) ? 0 : quiet + 1;
+  }
+}
+
+// Sorts the k distinct keys of keys[0, k) ascending: by counting ranks (one lane per key, every
+// lane reading the same key at a time) when the block has a lane per key, else bitonic over the
+// next power of two (keys[k, m) padded with the largest key).
+template <int NT>
+__device__ void sort_new_keys(uint64_t* keys, uint32_t k) {
+  if (k <= NT && k <= 512) {
+    uint64_t mine = 0;
+    uint32_t r = 0;
+    if (threadIdx.x < k) {
+      mine = keys[threadIdx.x];
+      for (uint32_t j = 0; j < k; ++j) r += keys[j] < mine;
+    }
+    __syncthreads();
+    if (threadIdx.x < k) keys[r] = mine;
+    __syncthreads();
+    return;
+  }
+  uint32_t m = 1;
+  while (m < k) m <<= 1;
+  for (uint32_t i = k + threadIdx.x; i < m; i += NT) keys[i] = ~0ULL;
+  __syncthreads();
+  bitonic<NT>(keys, m);
+}
+
+// Merges the sorted kept LEF ids kept[0, nk) with the sorted new keys keys[0, k) into
+// out(i, lef) (a merge path: each lane owns a contiguous run of the kept order, finds where its
+// run starts among the new keys once, then walks). The new keys that fall before kept entry a
+// (and after a - 1) are written by a's lane, those after the last kept entry by the last lane.
+template <int NT, class Out>
+__device__ void merge_kept_new(const uint32_t* kept, uint32_t nk, const uint64_t* keys, uint32_t k,
+                               const uint32_t* pos, Out out) {
+  if (nk == 0) {
+    for (uint32_t b = threadIdx.x; b < k; b += NT) out(b, (uint32_t)keys[b]);
+    return;
+  }
+  const uint32_t chunk = (nk + NT - 1) / NT;
+  const uint32_t ab = threadIdx.x * chunk, ae = min(nk, ab + chunk);
+  if (ab >= ae) return;
+  // lo: new keys below kept[ab - 1] (0 for the first run)
+  uint32_t lo = 0;
+  if (ab > 0) {
+    const uint64_t key = pos_key(pos, kept[ab - 1]);
+    uint32_t hi = k;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (keys[mid] < key) lo = mid + 1;
+      else hi = mid;
+    }
+  }
+  for (uint32_t a = ab; a < ae; ++a) {
+    const uint32_t lef = kept[a];
+    const uint64_t key = pos_key(pos, lef);
+    while (lo < k && keys[lo] < key) {
+      out(lo + a, (uint32_t)keys[lo]);
+      ++lo;
+    }
+    out(a + lo, lef);
+  }
+  if (ae == nk) {
+    for (; lo < k; ++lo) out(lo + nk, (uint32_t)keys[lo]);
   }
 }
 
@@ -208,7 +303,7 @@ __device__ void rerank(uint16_t* ranks, const uint32_t* pos, uint32_t nact, uint
   }
   uint32_t nkept = S.nkept;
   // Extrusion keeps neighbours ordered almost always, but the merge below needs the kept order
-  // sorted exactly: odd-even transposition passes until two consecutive passes swap nothing.
+  // sorted exactly.
   odd_even_sort<NT>(tmp_a, nkept, pos);
   // new LEFs, merged batch by batch (one batch unless more than kNewCap were bound)
   const uint32_t batches = (nkept + kNewCap > nact || nact - nkept <= kNewCap)
@@ -226,40 +321,18 @@ __device__ void rerank(uint16_t* ranks, const uint32_t* pos, uint32_t nact, uint
     __syncthreads();
     const uint32_t k = S.nnew;
     if (k == 0) continue;
-    uint32_t m = 1;
-    while (m < k) m <<= 1;
-    for (uint32_t i = k + threadIdx.x; i < m; i += NT) keys[i] = ~0ULL;
-    __syncthreads();
-    bitonic<NT>(keys, m);
+    sort_new_keys<NT>(keys, k);
     // the last batch merges straight into the rank array (nothing reads it any more)
-    const bool last = bt + 1 == batches;
-    for (uint32_t a = threadIdx.x; a < nkept; a += NT) {
-      const uint64_t key = pos_key(pos, tmp_a[a]);
-      uint32_t lo = 0, hi = k;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (keys[mid] < key) lo = mid + 1;
-        else hi = mid;
-      }
-      if (last) ranks[a + lo] = (uint16_t)tmp_a[a];
-      else tmp_b[a + lo] = tmp_a[a];
-    }
-    for (uint32_t b = threadIdx.x; b < k; b += NT) {
-      const uint64_t key = keys[b];
-      uint32_t lo = 0, hi = nkept;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (pos_key(pos, tmp_a[mid]) < key) lo = mid + 1;
-        else hi = mid;
-      }
-      if (last) ranks[b + lo] = (uint16_t)key;
-      else tmp_b[b + lo] = (uint32_t)key;
-    }
-    __syncthreads();
-    nkept += k;
-    if (last) {
+    if (bt + 1 == batches) {
+      merge_kept_new<NT>(tmp_a, nkept, keys, k, pos,
+                         [&](uint32_t i, uint32_t lef) { ranks[i] = (uint16_t)lef; });
       written = true;
+      __syncthreads();
     } else {
+      merge_kept_new<NT>(tmp_a, nkept, keys, k, pos,
+                         [&](uint32_t i, uint32_t lef) { tmp_b[i] = lef; });
+      __syncthreads();
+      nkept += k;
       for (uint32_t i = threadIdx.x; i < nkept; i += NT) tmp_a[i] = tmp_b[i];
       __syncthreads();
     }
@@ -568,6 +641,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     S.nact = P.skip_burnin ? N : 0;
     S.burnin_done = P.skip_burnin ? 1 : 0;
     S.hlen = 0;
+    S.hhead = 0;
     S.contacts = 0;
     S.burnin_epochs = 0;
     S.done = 0;
@@ -613,9 +687,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
           sq_lo += x2 & 0xFFFFFFFFull;
           sq_hi += x2 >> 32;
         }
-        sx = block_sum_u64<NT>(sx, S.shx);
-        sq_lo = block_sum_u64<NT>(sq_lo, S.shx);
-        sq_hi = block_sum_u64<NT>(sq_hi, S.shx);
+        block_sum3_u64<NT>(sx, sq_lo, sq_hi, S.shx);
         const unsigned __int128 sxx = ((unsigned __int128)sq_hi << 32) + sq_lo;
         const unsigned __int128 num = (unsigned __int128)N * sxx - (unsigned __int128)sx * sx;
         const double num_d = (double)(uint64_t)(num >> 64) * 18446744073709551616.0 +
@@ -623,37 +695,42 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
         const double mean = (double)sx / (double)N;
         const double sd = sqrt(num_d / ((double)N * (double)N));
         const uint32_t cap = P.hist_len, w = P.window;
+        // the history is a ring: logical entry i (0 = oldest) sits at (hhead + i) % cap, so a
+        // full history drops its oldest entry without moving the others
         if (threadIdx.x == 0) {
           ++S.burnin_epochs;
+          uint32_t slot;
           if (S.hlen == cap) {
-            for (uint32_t i = 1; i < cap; ++i) {
-              c.avg[i - 1] = c.avg[i];
-              c.cfx[i - 1] = c.cfx[i];
-            }
-            --S.hlen;
+            slot = S.hhead;
+            S.hhead = S.hhead + 1 == cap ? 0 : S.hhead + 1;
+          } else {
+            slot = S.hlen++;
           }
-          c.avg[S.hlen] = mean;
-          c.cfx[S.hlen] = sd / mean;
-          ++S.hlen;
+          c.avg[slot] = mean;
+          c.cfx[slot] = sd / mean;
         }
         __syncthreads();
         uint32_t stable = 0;
         if (S.hlen == cap) {
-          // windows whose mean drops, for the CV and for the mean (evaluate_burnin :983-1025)
-          int32_t dc = 0, da = 0;
+          // windows whose mean drops, for the CV and for the mean (evaluate_burnin :983-1025);
+          // both counts (at most cap <= 4096) share one scan
+          const uint32_t head = S.hhead;
+          int32_t d2 = 0;
           for (uint32_t k = threadIdx.x; k + w + 1 <= cap - 1; k += NT) {
             double a1 = 0, a2 = 0, b1 = 0, b2 = 0;
+            uint32_t q = head + k >= cap ? head + k - cap : head + k;
             for (uint32_t j = 0; j < w; ++j) {
-              a1 += c.cfx[k + j];
-              a2 += c.cfx[k + 1 + j];
-              b1 += c.avg[k + j];
-              b2 += c.avg[k + 1 + j];
+              const uint32_t q1 = q + 1 == cap ? 0 : q + 1;
+              a1 += c.cfx[q];
+              a2 += c.cfx[q1];
+              b1 += c.avg[q];
+              b2 += c.avg[q1];
+              q = q1;
             }
-            dc += a1 > a2;
-            da += b1 > b2;
+            d2 += (a1 > a2 ? 1 : 0) + (b1 > b2 ? 1 << 16 : 0);
           }
-          const int32_t nc = block_exclusive<NT>(dc, OpAddI{}, 0, S.shi) + dc;
-          const int32_t na = block_exclusive<NT>(da, OpAddI{}, 0, S.shi) + da;
+          const int32_t tot = block_exclusive<NT>(d2, OpAddI{}, 0, S.shi) + d2;
+          const int32_t nc = tot & 0xFFFF, na = tot >> 16;
           if (threadIdx.x == NT - 1) {
             const double r1 = (double)nc / (double)((int32_t)(cap - w) - nc);
             const double r2 = (double)na / (double)((int32_t)(cap - w) - na);
