@@ -129,15 +129,16 @@ def surrogate_leg(ctx, npoints: int):
     if not os.path.exists(weights):
         log("no trained surrogate weights: sweep leg skipped")
         return None
-    net = SG.Surrogate(ctx, SG.default_config(max_batch=32))
-    SG.load_params(net, weights)
     layout = Sw.fixed_layout()
-    res, _ = Sw.run_sweep(ctx, net, layout, Sw.sample_points(npoints), log=log)
     nwin = len(Sw.window_starts(layout.size // 5000))
-    ms = net.predict_timed(32, 10) / 10
+    batch = 8 * nwin  # eight whole chromosomes per forward batch
+    net = SG.Surrogate(ctx, SG.default_config(max_batch=batch))
+    SG.load_params(net, weights)
+    res, _ = Sw.run_sweep(ctx, net, layout, Sw.sample_points(npoints), log=log)
+    ms = net.predict_timed(batch, 10) / 10
     flops = surrogate_flops(net.cfg)
-    tflops = flops * 32 / (ms / 1000.0) / 1e12
-    dev_rate = 32 / (ms / 1000.0) / nwin
+    tflops = flops * batch / (ms / 1000.0) / 1e12
+    dev_rate = batch / (ms / 1000.0) / nwin
     net.close()
     corr = np.array(res.correlations, dtype=np.float64)
     return {
@@ -149,6 +150,8 @@ def surrogate_leg(ctx, npoints: int):
         "judge_reliability": float(np.nanmean(res.reliabilities)),
         "passed": int(np.sum(corr >= 0.95)), "replaced": len(res.replaced),
         "surrogate_rejected": res.fell_back,
+        "sweep_wall_s": res.sweep_s, "surrogate_wall_s": res.surrogate_s,
+        "judge_wall_s": res.judge_s,
         "sim_matrices_per_s": res.sim_rate, "surrogate_matrices_per_s": res.surrogate_rate,
         "surrogate_device_matrices_per_s": dev_rate,
         "sweep_matrices_per_s": res.sweep_rate,

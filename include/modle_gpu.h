@@ -392,6 +392,17 @@ int mgpu_sg_train_step(mgpu_sg* s, const uint32_t* idx, uint64_t n, float lr, fl
                        float* loss, float* ms);
 /* Device time of `repeats` forward passes over the last uploaded batch of n samples. */
 int mgpu_sg_predict_timed(mgpu_sg* s, uint64_t n, uint64_t repeats, float* ms);
+/* Whole-chromosome predictions kept in device memory (no counterpart in the reference: the
+ * surrogate's sweep path). reserve: room for npoints bands of nbins bins (ndiag x nbins floats
+ * each), replacing an earlier reservation. predict: chromosomes first_point .. + npoints, each
+ * tiled by nwin windows whose inputs x holds as for mgpu_sg_forward (npoints * nwin samples) and
+ * whose outputs start at bins starts[0] = 0 < ... ; pixel (i, i + d) comes from the last window
+ * starting at or before i and is stored as the relative count expm1(y) / 10. read: copies bands
+ * to the host ([npoints][ndiag][nbins]). */
+int mgpu_sg_reserve_bands(mgpu_sg* s, uint64_t npoints, uint64_t nbins);
+int mgpu_sg_predict_chromosomes(mgpu_sg* s, const float* x, uint64_t first_point,
+                                uint64_t npoints, uint64_t nwin, const uint32_t* starts);
+int mgpu_sg_read_bands(mgpu_sg* s, uint64_t first_point, uint64_t npoints, float* out);
 /* One v_mfma_f32_32x32x16_bf16 of a (32 x 16) and b (16 x 32), bf16 row-major, into d (32 x 32
  * fp32 row-major) through the lane layout the surrogate kernels assume. */
 int mgpu_debug_mfma(mgpu_ctx* ctx, const uint16_t* a, const uint16_t* b, float* d);

@@ -279,6 +279,10 @@ SIGNATURES = [
     ("mgpu_sg_set_data", C.c_int, [C.c_void_p, FP, FP, C.c_uint64]),
     ("mgpu_sg_train_step", C.c_int, [C.c_void_p, U32P, C.c_uint64, C.c_float, C.c_float, FP, FP]),
     ("mgpu_sg_predict_timed", C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, FP]),
+    ("mgpu_sg_reserve_bands", C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64]),
+    ("mgpu_sg_predict_chromosomes", C.c_int, [C.c_void_p, FP, C.c_uint64, C.c_uint64, C.c_uint64,
+                                              U32P]),
+    ("mgpu_sg_read_bands", C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, FP]),
     ("mgpu_debug_mfma", C.c_int, [C.c_void_p, C.POINTER(C.c_uint16), C.POINTER(C.c_uint16), FP]),
     ("mgpu_debug_cell_trace", C.c_int, [C.c_void_p, C.POINTER(MgpuParams), C.POINTER(MgpuChrom),
                                         C.POINTER(MgpuBarriers), C.c_uint64, C.c_uint64, C.c_uint64,

@@ -59,6 +59,8 @@ hipError_t launch_adamw(hipStream_t s, float* p, float* m, float* v, const float
                         int64_t n, float lr, float b1, float b2, float eps, float wd, int step);
 hipError_t launch_to_bf16(hipStream_t s, const float* in, uint16_t* out, int64_t n);
 hipError_t launch_fill(hipStream_t s, float* p, float v, int64_t n);
+hipError_t launch_stitch_bands(hipStream_t s, const float* out, int npoints, int nwin,
+                               const uint32_t* starts, int ND, int Lout, int nbins, float* bands);
 hipError_t launch_gather_rows_u16(hipStream_t s, const uint16_t* src, const uint32_t* idx,
                                   uint16_t* dst, int nb, int64_t row);
 hipError_t launch_gather_rows_f32(hipStream_t s, const float* src, const uint32_t* idx, float* dst,

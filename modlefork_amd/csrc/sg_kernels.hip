@@ -428,6 +428,32 @@ __global__ void fill_kernel(float* p, float v, int64_t n) {
     p[e] = v;
 }
 
+// Stitches the windows of whole chromosomes into their bands. Window w of chromosome q (of
+// npoints in the batch, nwin windows each, starting at output bin starts[w]) predicted
+// out[q * nwin + w][d][j] = log1p(10 * relative count) for pixel (starts[w] + j, + d); pixel
+// (i, i + d) of the chromosome comes from the last window starting at or before i, and the band
+// receives the relative count expm1(y) / 10 (0 beyond the chromosome). bands[q][d][i], nbins per
+// diagonal, for q of the batch starting at bands.
+__global__ void stitch_bands_kernel(const float* out, int npoints, int nwin, const uint32_t* starts,
+                                    int ND, int Lout, int nbins, float* bands) {
+  const int64_t n = (int64_t)npoints * ND * nbins;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % nbins);
+    const int d = (int)((e / nbins) % ND);
+    const int q = (int)(e / ((int64_t)nbins * ND));
+    float v = 0.0f;
+    if (i + d < nbins) {
+      int w = 0;
+      while (w + 1 < nwin && (int)starts[w + 1] <= i) ++w;
+      const int j = i - (int)starts[w];
+      if (j >= 0 && j + d < Lout)
+        v = expm1f(out[(((int64_t)q * nwin + w) * ND + d) * Lout + j]) * 0.1f;
+    }
+    bands[e] = v;
+  }
+}
+
 // gathers batch rows idx[0, nb) of a [N][row] array into a dense [nb][row] array
 __global__ void gather_rows_u16_kernel(const uint16_t* src, const uint32_t* idx, uint16_t* dst,
                                        int nb, int64_t row) {
@@ -556,6 +582,14 @@ hipError_t launch_adamw(hipStream_t s, float* p, float* m, float* v, const float
 
 hipError_t launch_to_bf16(hipStream_t s, const float* in, uint16_t* out, int64_t n) {
   hipLaunchKernelGGL(to_bf16_kernel, dim3(grid_for(n)), dim3(256), 0, s, in, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_stitch_bands(hipStream_t s, const float* out, int npoints, int nwin,
+                               const uint32_t* starts, int ND, int Lout, int nbins, float* bands) {
+  const int64_t n = (int64_t)npoints * ND * nbins;
+  hipLaunchKernelGGL(stitch_bands_kernel, dim3(grid_for(n)), dim3(256), 0, s, out, npoints, nwin,
+                     starts, ND, Lout, nbins, bands);
   return hipGetLastError();
 }
 

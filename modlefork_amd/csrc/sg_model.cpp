@@ -69,6 +69,11 @@ struct mgpu_sg {
   float* data_y = nullptr;
   uint64_t ndata = 0;
   uint32_t* idx = nullptr;
+  // stitched chromosome bands of mgpu_sg_predict_chromosomes (relative counts, device memory)
+  float* bands = nullptr;
+  uint64_t bands_points = 0, bands_nbins = 0;
+  uint32_t* d_starts = nullptr;
+  uint64_t starts_cap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
 };
@@ -534,6 +539,8 @@ void mgpu_sg_destroy(mgpu_sg* s) {
   for (void* p : s->allocs) hipFree(p);
   if (s->data_x) hipFree(s->data_x);
   if (s->data_y) hipFree(s->data_y);
+  if (s->bands) hipFree(s->bands);
+  if (s->d_starts) hipFree(s->d_starts);
   if (s->ev0) hipEventDestroy(s->ev0);
   if (s->ev1) hipEventDestroy(s->ev1);
   delete s;
@@ -685,6 +692,66 @@ int mgpu_sg_adamw_step(mgpu_sg* s, const float* grads, float lr, float weight_de
   SGCHK(s, launch_adamw(s->stream, s->p32, s->m32, s->v32, s->g32, s->p16, (int64_t)s->nparam, lr,
                         0.9f, 0.999f, 1e-8f, weight_decay, s->step));
   SGCHK(s, hipStreamSynchronize(s->stream));
+  return MGPU_OK;
+}
+
+int mgpu_sg_reserve_bands(mgpu_sg* s, uint64_t npoints, uint64_t nbins) {
+  if (!s) return sg_fail(s, MGPU_EINVAL, "null argument");
+  if (npoints == 0 || nbins == 0) return sg_fail(s, MGPU_EINVAL, "empty band reservation");
+  if (s->bands) SGCHK(s, hipFree(s->bands));
+  s->bands = nullptr;
+  s->bands_points = s->bands_nbins = 0;
+  const uint64_t bytes = npoints * (uint64_t)s->ND * nbins * 4;
+  hipError_t e = hipMalloc(&s->bands, bytes);
+  if (e != hipSuccess)
+    return sg_fail(s, MGPU_ENOMEM, "band reservation of " + std::to_string(bytes) + " bytes: " +
+                                       hipGetErrorString(e));
+  s->bands_points = npoints;
+  s->bands_nbins = nbins;
+  return MGPU_OK;
+}
+
+int mgpu_sg_predict_chromosomes(mgpu_sg* s, const float* x, uint64_t first_point,
+                                uint64_t npoints, uint64_t nwin, const uint32_t* starts) {
+  if (!s || !x || !starts) return sg_fail(s, MGPU_EINVAL, "null argument");
+  if (!s->bands) return sg_fail(s, MGPU_EINVAL, "no band reservation (mgpu_sg_reserve_bands)");
+  if (nwin == 0 || nwin > (uint64_t)s->Bm)
+    return sg_fail(s, MGPU_EINVAL, "windows per chromosome must be in [1, max_batch]");
+  if (first_point + npoints > s->bands_points)
+    return sg_fail(s, MGPU_ERANGE, "points beyond the band reservation");
+  const uint64_t nbins = s->bands_nbins;
+  for (uint64_t w = 0; w < nwin; ++w)
+    if ((w && starts[w] < starts[w - 1]) || starts[w] >= nbins || starts[0] != 0)
+      return sg_fail(s, MGPU_EINVAL, "window starts must rise from 0 inside the chromosome");
+  if (nwin > s->starts_cap) {
+    if (s->d_starts) SGCHK(s, hipFree(s->d_starts));
+    s->d_starts = nullptr;
+    s->starts_cap = 0;
+    SGCHK(s, hipMalloc(&s->d_starts, nwin * 4));
+    s->starts_cap = nwin;
+  }
+  SGCHK(s, hipMemcpyAsync(s->d_starts, starts, nwin * 4, hipMemcpyHostToDevice, s->stream));
+  const uint64_t ppb = std::max<uint64_t>(1, (uint64_t)s->Bm / nwin);  // points per batch
+  const int64_t per_win = (int64_t)s->Lin * s->cfg.in_ch;
+  for (uint64_t p0 = 0; p0 < npoints; p0 += ppb) {
+    const uint64_t np = std::min<uint64_t>(ppb, npoints - p0);
+    const int B = (int)(np * nwin);
+    if (int rc = upload_inputs(s, x + p0 * nwin * per_win, B)) return rc;
+    if (int rc = forward(s, B)) return rc;
+    SGCHK(s, launch_stitch_bands(s->stream, s->out, (int)np, (int)nwin, s->d_starts, s->ND,
+                                 s->Lout, (int)nbins,
+                                 s->bands + (first_point + p0) * (uint64_t)s->ND * nbins));
+  }
+  SGCHK(s, hipStreamSynchronize(s->stream));
+  return MGPU_OK;
+}
+
+int mgpu_sg_read_bands(mgpu_sg* s, uint64_t first_point, uint64_t npoints, float* out) {
+  if (!s || !out) return sg_fail(s, MGPU_EINVAL, "null argument");
+  if (first_point + npoints > s->bands_points)
+    return sg_fail(s, MGPU_ERANGE, "points beyond the band reservation");
+  const uint64_t per = (uint64_t)s->ND * s->bands_nbins;
+  SGCHK(s, hipMemcpy(out, s->bands + first_point * per, npoints * per * 4, hipMemcpyDeviceToHost));
   return MGPU_OK;
 }
 

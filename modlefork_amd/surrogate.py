@@ -80,6 +80,31 @@ class Surrogate:
         self._check(L.lib().mgpu_sg_forward(self.ptr, _f(x), n, _f(y)))
         return y
 
+    def reserve_bands(self, npoints: int, nbins: int):
+        """Device memory for npoints whole-chromosome bands of nbins bins."""
+        self._check(L.lib().mgpu_sg_reserve_bands(self.ptr, int(npoints), int(nbins)))
+        self._bands_nbins = int(nbins)
+
+    def predict_chromosomes(self, x: np.ndarray, first_point: int, nwin: int, starts) -> None:
+        """Predicts chromosomes first_point .. from their windows' inputs x
+        ([npoints * nwin][in_len][in_ch], window k of a point starting at output bin starts[k]);
+        the stitched relative-count bands stay in device memory (read_bands copies them)."""
+        x = np.ascontiguousarray(x, np.float32)
+        self._check_x(x)
+        if x.shape[0] % nwin:
+            raise ValueError("inputs must hold whole chromosomes (a multiple of nwin windows)")
+        st = np.ascontiguousarray(starts, np.uint32)
+        if len(st) != nwin:
+            raise ValueError("one start per window expected")
+        self._check(L.lib().mgpu_sg_predict_chromosomes(
+            self.ptr, _f(x), int(first_point), x.shape[0] // nwin, int(nwin),
+            st.ctypes.data_as(C.POINTER(C.c_uint32))))
+
+    def read_bands(self, first_point: int, npoints: int) -> np.ndarray:
+        out = np.zeros((npoints, int(self.cfg.ndiag), self._bands_nbins), np.float32)
+        self._check(L.lib().mgpu_sg_read_bands(self.ptr, int(first_point), int(npoints), _f(out)))
+        return out
+
     def gradients(self, x: np.ndarray, y: np.ndarray):
         x = np.ascontiguousarray(x, np.float32)
         y = np.ascontiguousarray(y, np.float32)

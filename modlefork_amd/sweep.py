@@ -20,7 +20,9 @@ diagonals is the score; the uncorrected correlation is reported beside it.
 """
 from __future__ import annotations
 
+import threading
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 
 import numpy as np
@@ -44,19 +46,32 @@ def _pearson(a, b):
     return float((a * b).sum() / den) if den > 0 else 0.0
 
 
+_COARSE_INDEX = {}
+
+
+def _coarse_index(nd: int, n: int, f: int):
+    """Flat coarse-pixel index D * (n // f) + I of every fine pixel (i, i + d) of a [nd][n] band,
+    and the mask of the fine pixels that land in a coarse pixel of the [nd // f][n // f] band."""
+    key = (nd, n, f)
+    if key not in _COARSE_INDEX:
+        ncd, nc = nd // f, n // f
+        d = np.arange(nd)[:, None]
+        i = np.arange(n)[None, :]
+        I = i // f
+        D = (i + d) // f - I
+        ok = (D < ncd) & (I < nc) & ((i + d) < nc * f)
+        _COARSE_INDEX[key] = ((D * nc + I)[ok].astype(np.int64), ok)
+    return _COARSE_INDEX[key]
+
+
 def coarse_band(counts: np.ndarray, f: int = COARSE) -> np.ndarray:
     """[ND][N] counts of pixels (i, i + d) -> [ND // f][N // f] coarse counts of coarse pixels
     (I, I + D) (fine pixels whose row and column fall in coarse bins I and I + D)."""
     nd, n = counts.shape
     ncd, nc = nd // f, n // f
-    out = np.zeros((ncd, nc), np.float64)
-    d = np.arange(nd)[:, None]
-    i = np.arange(n)[None, :]
-    I = i // f
-    D = (i + d) // f - I
-    ok = (D < ncd) & (I < nc) & ((i + d) < nc * f)
-    np.add.at(out, (D[ok], np.broadcast_to(I, counts.shape)[ok]), counts[ok])
-    return out
+    idx, ok = _coarse_index(nd, n, f)
+    out = np.bincount(idx, weights=np.asarray(counts, np.float64)[ok], minlength=ncd * nc)
+    return out.reshape(ncd, nc)
 
 
 def diag_correlations(pred: np.ndarray, sim: np.ndarray, f: int = COARSE,
@@ -159,6 +174,44 @@ def point_inputs(layout: Layout, p: Point) -> np.ndarray:
                      for s in window_starts(nbins)])
 
 
+def points_inputs(layout: Layout, points) -> np.ndarray:
+    """point_inputs for many points at once, [npoints * nwin][IN_BINS][6] (the same values):
+    the count and direction channels come from the fixed layout, the occupancy channel from one
+    scatter per point, and each window is a slice of the chromosome's tracks padded by the
+    context on both sides."""
+    nbins = layout.size // Wn.BIN
+    starts = window_starts(nbins)
+    npts = len(points)
+    pad = Wn.CTX_BINS
+    L = nbins + 2 * pad + Wn.IN_BINS
+    bins = np.array([(r.start + r.end) // 2 // Wn.BIN for r in layout.records], np.int64)
+    sign = np.array([1.0 if r.strand == "+" else -1.0 for r in layout.records], np.float32)
+    base = np.array([r.score for r in layout.records], np.float64)
+    cnt = np.zeros(L, np.float32)
+    dirn = np.zeros(L, np.float32)
+    inside = np.zeros(L, np.float32)
+    np.add.at(cnt, bins + pad, 1.0)
+    np.add.at(dirn, bins + pad, sign)
+    inside[pad:pad + nbins] = 1.0
+    scale = np.array([p.occ_scale for p in points], np.float64)
+    occ_b = np.minimum(np.maximum(base[None, :] * scale[:, None], 1e-6), 1.0).astype(np.float32)
+    occ = np.zeros((npts, L), np.float32)
+    for b in range(len(bins)):  # bins may repeat: accumulate barrier by barrier, in record order
+        occ[:, bins[b] + pad] += occ_b[:, b]
+    s0 = np.array([Wn.scalar_channels(p.density, p.processivity)[0] for p in points], np.float32)
+    s1 = np.array([Wn.scalar_channels(p.density, p.processivity)[1] for p in points], np.float32)
+    x = np.empty((npts, len(starts), Wn.IN_BINS, Wn.IN_CH), np.float32)
+    for k, s in enumerate(starts):
+        sl = slice(s, s + Wn.IN_BINS)  # window bins s - CTX .. in padded coordinates
+        x[:, k, :, 0] = cnt[sl]
+        x[:, k, :, 1] = dirn[sl]
+        x[:, k, :, 2] = occ[:, sl]
+        x[:, k, :, 3] = inside[sl]
+        x[:, k, :, 4] = s0[:, None]
+        x[:, k, :, 5] = s1[:, None]
+    return x.reshape(npts * len(starts), Wn.IN_BINS, Wn.IN_CH)
+
+
 def stitch(pred_rel: np.ndarray, nbins: int) -> np.ndarray:
     """[ND][nbins] relative counts of a chromosome from its windows' predictions: pixel (i, i + d)
     comes from the last window starting at or before i whose output holds both ends."""
@@ -182,11 +235,23 @@ def sweep_options(p: Point, contact_density: float, seed: int, diagonal_width: i
 
 
 def simulate_points(ctx, layout: Layout, points, ncells: int = 512, contact_density: float = 30.0,
-                    seed: int = 0, diagonal_width: int = 3_000_000, chunk: int = 64):
+                    seed: int = 0, diagonal_width: int = 3_000_000, chunk: int = 64,
+                    stored_diagonals: int = 0):
     """Fast-mode simulation of each point (one parameter block per point); returns the [ND][nbins]
-    count bands and the device seconds."""
+    count bands and the device seconds.
+
+    stored_diagonals > 0 keeps only that many diagonals of each band on the device: the band is
+    declared that wide and the contact density scaled by (full rows / stored rows), so the
+    per-cell contact targets, and with them every cell's trajectory and every contact, are those
+    of the full-width run; contacts beyond the stored diagonals are counted as missed. The bands
+    returned (NDIAG diagonals) are the same, with less to clear and copy back."""
     bands, dev_s = [], 0.0
     nbins = layout.size // Wn.BIN
+    if stored_diagonals:
+        full_rows = min(-(-diagonal_width // Wn.BIN), nbins)
+        rows = min(stored_diagonals, full_rows)
+        contact_density = contact_density * full_rows / rows
+        diagonal_width = rows * Wn.BIN
     for c0 in range(0, len(points), chunk):
         part = points[c0:c0 + chunk]
         params, chroms, bars, plans = [], [], [], []
@@ -216,36 +281,85 @@ class SweepResult:
     reliabilities: list     # the judge simulation's own reliability
     replaced: list
     fell_back: bool
-    surrogate_s: float
-    judge_s: float
-    sim_rate: float         # simulated matrices per second (judge runs)
-    surrogate_rate: float   # predicted matrices per second (inference only)
-    sweep_rate: float       # matrices per second of the whole sweep (inference + judge)
+    surrogate_s: float      # inputs, inference and stitching of every point (wall)
+    judge_s: float          # the judge's simulations (wall, alongside the surrogate)
+    sweep_s: float          # the whole sweep, judge and fallback included (wall)
+    sim_rate: float         # plain simulation, matrices per second, measured on its own
+    surrogate_rate: float   # surrogate matrices per second (inputs, inference, stitching)
+    sweep_rate: float       # matrices per second of the whole sweep
+
+
+def plain_simulation_rate(ctx, layout: Layout, points, ncells: int = 512,
+                          contact_density: float = 30.0, seed: int = 17) -> float:
+    """Matrices per second of the plain method on its own: every point simulated with the judge's
+    cells and contact density, the 100 diagonals the sweep delivers kept on the device."""
+    t0 = time.perf_counter()
+    simulate_points(ctx, layout, points, ncells, contact_density, seed=seed,
+                    stored_diagonals=Wn.NDIAG)
+    return len(points) / (time.perf_counter() - t0)
 
 
 def run_sweep(ctx, net, layout: Layout, points, judge_frac: float = 0.05, threshold: float = 0.95,
-              seed: int = 3, ncells: int = 512, contact_density: float = 30.0, log=print):
+              seed: int = 3, ncells: int = 512, contact_density: float = 30.0, log=print,
+              rate_sample: int = 64, chunk_points: int = 256):
+    """The judge simulates on its own context (its own HIP stream) in a host thread while the
+    surrogate predicts every point; a judged point whose score is below the threshold is replaced
+    by its simulation. The plain simulation's rate is then measured on its own on rate_sample
+    judged points (0: not measured, sim_rate nan).
+
+    Returns (SweepResult, replaced bands {point: band}): every other point's band is the
+    surrogate's, in the network's device memory (net.read_bands)."""
+    t_start = time.perf_counter()
     nbins = layout.size // Wn.BIN
     rng = np.random.default_rng(seed)
     nj = max(1, int(round(judge_frac * len(points))))
     judged = sorted(rng.choice(len(points), size=nj, replace=False).tolist())
-    # surrogate: every point, windows batched across points
-    xs = [point_inputs(layout, p) for p in points]
-    nwin = xs[0].shape[0]
-    t0 = time.perf_counter()
-    pred = net.forward(np.concatenate(xs))
-    t_sur = time.perf_counter() - t0
-    rel = Wn.denormalise(pred).reshape(len(points), nwin, Wn.NDIAG, Wn.OUT_BINS)
-    bands = [stitch(rel[k], nbins) for k in range(len(points))]
-    # judge: each judged point simulated as two independent halves of its cells
-    t0 = time.perf_counter()
     jp = [points[k] for k in judged]
-    half_a, _ = simulate_points(ctx, layout, jp, ncells // 2, contact_density / 2, seed=seed)
-    half_b, _ = simulate_points(ctx, layout, jp, ncells // 2, contact_density / 2, seed=seed + 1)
-    t_judge = time.perf_counter() - t0
+    out = {}
+
+    def judge_worker():
+        # each judged point simulated as two independent halves of its cells
+        jctx = S.Context(0)
+        try:
+            t0 = time.perf_counter()
+            out["a"], _ = simulate_points(jctx, layout, jp, ncells // 2, contact_density / 2,
+                                          seed=seed, stored_diagonals=Wn.NDIAG)
+            out["b"], _ = simulate_points(jctx, layout, jp, ncells // 2, contact_density / 2,
+                                          seed=seed + 1, stored_diagonals=Wn.NDIAG)
+            out["t"] = time.perf_counter() - t0
+        except BaseException as e:  # re-raised by the sweep thread
+            out["err"] = e
+        finally:
+            jctx.close()
+
+    th = threading.Thread(target=judge_worker, name="sweep-judge")
+    th.start()
+    # surrogate: every point, its windows' inputs built in chunks of points, predictions stitched
+    # into whole-chromosome bands on the device (they stay there; judged points are read back)
+    starts = np.asarray(window_starts(nbins), np.uint32)
+    t0 = time.perf_counter()
+    try:
+        net.reserve_bands(len(points), nbins)
+        # inputs of the next chunk are built on a host thread while the device predicts this one
+        chunks = list(range(0, len(points), chunk_points))
+        with ThreadPoolExecutor(max_workers=1) as ex:
+            nxt = ex.submit(points_inputs, layout, points[0:chunk_points])
+            for c0 in chunks:
+                x = nxt.result()
+                if c0 + chunk_points < len(points):
+                    nxt = ex.submit(points_inputs, layout,
+                                    points[c0 + chunk_points:c0 + 2 * chunk_points])
+                net.predict_chromosomes(x, c0, len(starts), starts)
+    finally:
+        t_sur = time.perf_counter() - t0
+        th.join()
+    if "err" in out:
+        raise out["err"]
+    half_a, half_b, t_judge = out["a"], out["b"], out["t"]
     corrs, raws, rels, replaced = [], [], [], []
+    bands = {}  # points whose band is the simulation's, not the surrogate's
     for k, a, b in zip(judged, half_a, half_b):
-        c, r, q = judge(bands[k], a, b)
+        c, r, q = judge(net.read_bands(k, 1)[0].astype(np.float64), a, b)
         corrs.append(c)
         raws.append(r)
         rels.append(q)
@@ -253,12 +367,24 @@ def run_sweep(ctx, net, layout: Layout, points, judge_frac: float = 0.05, thresh
             replaced.append(k)
             bands[k] = a + b
     fell_back = len(replaced) > 0.2 * len(judged)
+    if fell_back:
+        # the surrogate is not trusted for this sweep: every point falls back to simulation
+        js = set(judged)
+        rest = [k for k in range(len(points)) if k not in js]
+        sims, _ = simulate_points(ctx, layout, [points[k] for k in rest], ncells, contact_density,
+                                  seed=seed, stored_diagonals=Wn.NDIAG)
+        for k, b in zip(rest, sims):
+            bands[k] = b
+    t_sweep = time.perf_counter() - t_start
     log(f"sweep: {len(points)} points, {nj} judged, corrected correlation "
         f"{np.nanmean(corrs):.4f} (raw {np.nanmean(raws):.4f}, judge reliability "
         f"{np.nanmean(rels):.4f}), {len(replaced)} replaced"
-        f"{', surrogate rejected' if fell_back else ''}")
+        f"{', surrogate rejected' if fell_back else ''}; surrogate {t_sur:.1f} s, judge "
+        f"{t_judge:.1f} s, sweep {t_sweep:.1f} s")
+    sim_rate = (plain_simulation_rate(ctx, layout, jp[:rate_sample], ncells, contact_density)
+                if rate_sample else float("nan"))
     return SweepResult(len(points), judged, corrs, raws, rels, replaced, fell_back, t_sur, t_judge,
-                       nj / t_judge, len(points) / t_sur, len(points) / (t_sur + t_judge)), bands
+                       t_sweep, sim_rate, len(points) / t_sur, len(points) / t_sweep), bands
 
 
 def main(argv=None):
@@ -286,16 +412,16 @@ def main(argv=None):
         print(msg, flush=True)
 
     ctx = S.Context(0)
-    net = SG.Surrogate(ctx, SG.default_config(max_batch=32))
-    SG.load_params(net, a.weights)
     layout = fixed_layout()
+    nwin = len(window_starts(layout.size // Wn.BIN))
+    net = SG.Surrogate(ctx, SG.default_config(max_batch=8 * nwin))
+    SG.load_params(net, a.weights)
     points = sample_points(a.points)
     res, _ = run_sweep(ctx, net, layout, points, a.judge_frac, a.threshold, ncells=a.cells,
                        contact_density=a.density, log=log)
-    # device-only inference rate: one batch of 32 windows
-    nwin = len(window_starts(layout.size // Wn.BIN))
-    ms = net.predict_timed(32, 10) / 10
-    dev_rate = 32 / (ms / 1000.0) / nwin
+    # device-only inference rate: one batch of eight chromosomes' windows
+    ms = net.predict_timed(8 * nwin, 10) / 10
+    dev_rate = 8 * nwin / (ms / 1000.0) / nwin
     out = {"points": res.npoints, "judged": len(res.judged), "threshold": a.threshold,
            "corr_mean": float(np.nanmean(res.correlations)),
            "corr_min": float(np.nanmin(res.correlations)),
@@ -304,6 +430,7 @@ def main(argv=None):
            "judge_reliability_mean": float(np.nanmean(res.reliabilities)),
            "replaced": len(res.replaced), "surrogate_rejected": res.fell_back,
            "surrogate_wall_s": res.surrogate_s, "judge_wall_s": res.judge_s,
+           "sweep_wall_s": res.sweep_s,
            "sim_matrices_per_s": res.sim_rate, "surrogate_matrices_per_s": res.surrogate_rate,
            "surrogate_device_matrices_per_s": dev_rate,
            "sweep_matrices_per_s": res.sweep_rate,

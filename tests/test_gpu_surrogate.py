@@ -5,6 +5,7 @@ fp32 oracle run on the same bf16-rounded weights and inputs, within a relative e
 of the output range; gradients tensor by tensor by cosine similarity (> 0.99) and relative
 norm error (< 12 %: the first layer's gradient passes through every bf16 activation of the
 network); AdamW exactly up to fp32 rounding."""
+import os
 import sys
 
 import numpy as np
@@ -175,3 +176,43 @@ def test_trained_surrogate_gains_over_simulation(ctx):
     net.close()
     assert np.nanmean(scores) >= 0.9
     assert speed >= 20
+
+
+def test_narrow_judge_bands_equal_the_full_width_run(ctx):
+    """The sweep's judge keeps only the 100 diagonals it compares (band declared 100 bins wide,
+    contact density scaled so the per-cell targets are unchanged): the simulation is the
+    full-width one, so those diagonals match it exactly."""
+    from modlefork_amd import sweep as Sw
+
+    layout = Sw.fixed_layout()
+    pts = Sw.sample_points(3, seed=77)
+    full, _ = Sw.simulate_points(ctx, layout, pts, 32, 30.0, seed=5)
+    narrow, _ = Sw.simulate_points(ctx, layout, pts, 32, 30.0, seed=5, stored_diagonals=100)
+    for f, n in zip(full, narrow):
+        assert f.shape == n.shape and np.array_equal(f, n)
+
+
+def test_device_stitched_bands_match_host_stitching(ctx):
+    """mgpu_sg_predict_chromosomes (forward, then stitching and expm1 / 10 on the device, bands
+    kept in HBM) against mgpu_sg_forward + the host's denormalise and stitch."""
+    from modlefork_amd import sweep as Sw
+    from modlefork_amd import windows as Wn
+
+    weights = os.path.join(os.path.dirname(SG.__file__), "data", "surrogate_weights.npy")
+    net = SG.Surrogate(ctx, SG.default_config(max_batch=32))
+    SG.load_params(net, weights)
+    layout = Sw.fixed_layout()
+    pts = Sw.sample_points(5, seed=31)
+    nbins = layout.size // Wn.BIN
+    x = Sw.points_inputs(layout, pts)
+    starts = np.asarray(Sw.window_starts(nbins), np.uint32)
+    nwin = len(starts)
+    rel = Wn.denormalise(net.forward(x)).reshape(len(pts), nwin, Wn.NDIAG, Wn.OUT_BINS)
+    want = np.stack([Sw.stitch(rel[k], nbins) for k in range(len(pts))])
+    net.reserve_bands(len(pts) + 2, nbins)
+    net.predict_chromosomes(x[:2 * nwin], 0, nwin, starts)
+    net.predict_chromosomes(x[2 * nwin:], 2, nwin, starts)
+    got = net.read_bands(0, len(pts)).astype(np.float64)
+    net.close()
+    assert got.shape == want.shape
+    assert np.allclose(got, want, rtol=2e-6, atol=1e-7), float(np.abs(got - want).max())

@@ -74,3 +74,12 @@ def test_track_channels():
     assert x[8, 0] == 1 and x[8, 1] == 1
     assert list(x[:, 3]) == [0, 0] + [1] * 8 + [0, 0]
     assert np.all(x[:, 4] == 0) and np.all(x[:, 5] == 0)
+
+
+def test_batched_point_inputs_equal_the_per_point_tracks():
+    layout = Sw.fixed_layout()
+    pts = Sw.sample_points(5, seed=9)
+    want = np.concatenate([Sw.point_inputs(layout, p) for p in pts])
+    got = Sw.points_inputs(layout, pts)
+    assert got.shape == want.shape
+    assert np.array_equal(got, want)
