@@ -108,21 +108,164 @@ __global__ void __launch_bounds__(256)
           acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[r].v, Bv[c].v, acc[r][c], 0, 0, 0);
     }
   }
-  // epilogue: v = acc + bias; relu; out1 = v; out2 = res + v (all bf16)
-  for (int c = 0; c < 2; ++c) {
-    const int co = n0 + 32 * c + (lane & 31);
-    const float bias = a.bias ? a.bias[co] : 0.0f;
-    for (int r = 0; r < 2; ++r) {
+  // epilogue: v = acc + bias; relu; out1 = v; out2 = res + v (all bf16), staged through LDS 32
+  // positions at a time so residual reads and output writes are 16-byte vectors per lane
+  constexpr int kEpStride = 64 + 4;
+  __shared__ __align__(16) float stage_all[kConvWaves * 32 * kEpStride];
+  float* stage = stage_all + wave * 32 * kEpStride;
+  for (int r = 0; r < 2; ++r) {
+    for (int c = 0; c < 2; ++c) {
+      const int cl = 32 * c + (lane & 31);
+      const float bias = a.bias ? a.bias[n0 + cl] : 0.0f;
       for (int k = 0; k < 16; ++k) {
-        const int64_t m = m0 + 32 * r + acc_row(k, lane);
-        if (m >= npos) continue;
         float v = acc[r][c][k] + bias;
         if (a.relu) v = v > 0.0f ? v : 0.0f;
-        const int64_t o = m * a.Cout + co;
-        if (a.out1) a.out1[o] = f2bf(v);
-        if (a.out2) a.out2[o] = f2bf((a.res ? bf2f(a.res[o]) : 0.0f) + v);
+        stage[acc_row(k, lane) * kEpStride + cl] = v;
       }
     }
+    __syncthreads();
+    for (int e = lane; e < 32 * 8; e += 64) {
+      const int pos = e >> 3, q = e & 7;
+      const int64_t m = m0 + 32 * r + pos;
+      if (m < npos) {
+        const int64_t o = m * a.Cout + n0 + 8 * q;
+        const float* sv = stage + pos * kEpStride + 8 * q;
+        Pack8 res;
+        res.u = a.res && a.out2 ? *reinterpret_cast<const uint4*>(a.res + o) : make_uint4(0, 0, 0, 0);
+        Pack8 o1, o2;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          o1.h[k] = f2bf(sv[k]);
+          o2.h[k] = f2bf(bf2f(res.h[k]) + sv[k]);
+        }
+        if (a.out1) *reinterpret_cast<uint4*>(a.out1 + o) = o1.u;
+        if (a.out2) *reinterpret_cast<uint4*>(a.out2 + o) = o2.u;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// 3 x 3 convolution, 64 -> 64 channels, on a D x I grid: the head's layer shape
+// ---------------------------------------------------------------------------------------------
+// One workgroup computes a tile of 4 rows (d) x 64 columns (i) of one image, all 64 output
+// channels; wave w owns row w (64 positions, 2 x 2 MFMA tiles). The input patch the tile reads
+// (6 x 66 positions, zero outside the image) is staged once in LDS, so each input position is
+// fetched from memory once per tile instead of once per tap and wave. Positions are padded to
+// 144 bytes: 16 consecutive positions then start 36 banks apart and a 16-byte read per lane is
+// free of bank conflicts.
+constexpr int kT3Rows = 4, kT3Cols = 64, kT3C = 64;
+constexpr int kT3PRows = kT3Rows + 2, kT3PCols = kT3Cols + 2;
+constexpr int kT3Stride = kT3C + 8;  // bf16 per patch position
+
+__global__ void __launch_bounds__(256) conv3x3_c64_kernel(ConvArgs a) {
+  __shared__ __align__(16) uint16_t patch[kT3PRows * kT3PCols * kT3Stride];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i0 = blockIdx.x * kT3Cols, d0 = blockIdx.y * kT3Rows, b = blockIdx.z;
+  const uint16_t* X = a.X + (int64_t)b * a.D * a.I * kT3C;
+  // stage the patch: 16-byte chunks, eight per position
+  for (int e = threadIdx.x; e < kT3PRows * kT3PCols * 8; e += 256) {
+    const int q = e & 7, pos = e >> 3;
+    const int pr = pos / kT3PCols, pc = pos % kT3PCols;
+    const int d = d0 + pr - 1, i = i0 + pc - 1;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (d >= 0 && d < a.D && i >= 0 && i < a.I)
+      v = *reinterpret_cast<const uint4*>(X + ((int64_t)d * a.I + i) * kT3C + 8 * q);
+    *reinterpret_cast<uint4*>(patch + pos * kT3Stride + 8 * q) = v;
+  }
+  __syncthreads();
+  f32x16 acc[2][2];
+  for (int r = 0; r < 2; ++r)
+    for (int c = 0; c < 2; ++c) acc[r][c] = f32x16{};
+  const int kofs = 8 * (lane >> 5);
+  // weight fragments of one tap (4 k-steps x 2 column tiles), loaded a tap ahead of their use
+  // so the global-load latency overlaps the previous tap's MFMAs
+  const uint16_t* wbase[2];
+  for (int c = 0; c < 2; ++c) wbase[c] = a.W + (int64_t)(32 * c + (lane & 31)) * 9 * kT3C + kofs;
+  uint4 wn[4][2];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    for (int c = 0; c < 2; ++c) wn[j][c] = *reinterpret_cast<const uint4*>(wbase[c] + 16 * j);
+#pragma unroll 1
+  for (int t = 0; t < 9; ++t) {
+    uint4 wc[4][2];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      for (int c = 0; c < 2; ++c) wc[j][c] = wn[j][c];
+    if (t + 1 < 9) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        for (int c = 0; c < 2; ++c)
+          wn[j][c] = *reinterpret_cast<const uint4*>(wbase[c] + (t + 1) * kT3C + 16 * j);
+    }
+    const int dd = t / 3, di = t % 3;  // patch offsets of tap t (the patch starts at -1, -1)
+    const uint16_t* arow[2];
+    for (int r = 0; r < 2; ++r)
+      arow[r] = patch + ((wave + dd) * kT3PCols + 32 * r + (lane & 31) + di) * kT3Stride + kofs;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      Pack8 A[2], Bv[2];
+      for (int r = 0; r < 2; ++r) A[r].u = *reinterpret_cast<const uint4*>(arow[r] + 16 * j);
+      for (int c = 0; c < 2; ++c) Bv[c].u = wc[j][c];
+      for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 2; ++c)
+          acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[r].v, Bv[c].v, acc[r][c], 0, 0, 0);
+    }
+  }
+  // epilogue as in conv_fwd_kernel (v = acc + bias; relu; out1 = v; out2 = res + v), through
+  // LDS: each wave stages 32 positions x 64 channels of fp32 at a time, then reads them back a
+  // channel octet per lane, so the residual reads and the output writes are 16-byte vectors
+  // covering whole positions. A residual that is the layer's own input is read from the patch
+  // (its centre rows) before the staging overwrites it.
+  const bool res_in_patch = a.res == a.X;
+  uint4 rp[2][4];
+  if (res_in_patch) {
+    for (int r = 0; r < 2; ++r)
+      for (int u = 0; u < 4; ++u) {
+        const int e = lane + 64 * u, pos = e >> 3, q = e & 7;
+        rp[r][u] = *reinterpret_cast<const uint4*>(
+            patch + ((wave + 1) * kT3PCols + 32 * r + pos + 1) * kT3Stride + 8 * q);
+      }
+  }
+  __syncthreads();  // every wave is done with the patch
+  constexpr int kEpStride = kT3C + 4;  // floats per staged position
+  float* stage = reinterpret_cast<float*>(patch) + wave * 32 * kEpStride;
+  const int d = d0 + wave;
+  const int64_t row = ((int64_t)b * a.D + (d < a.D ? d : 0)) * a.I;
+  for (int r = 0; r < 2; ++r) {
+    for (int c = 0; c < 2; ++c) {
+      const int co = 32 * c + (lane & 31);
+      const float bias = a.bias ? a.bias[co] : 0.0f;
+      for (int k = 0; k < 16; ++k) {
+        float v = acc[r][c][k] + bias;
+        if (a.relu) v = v > 0.0f ? v : 0.0f;
+        stage[acc_row(k, lane) * kEpStride + co] = v;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = lane + 64 * u, pos = e >> 3, q = e & 7;
+      const int i = i0 + 32 * r + pos;
+      if (d < a.D && i < a.I) {
+        const int64_t o = (row + i) * kT3C + 8 * q;
+        const float* sv = stage + pos * kEpStride + 8 * q;
+        Pack8 res;
+        if (!a.res || !a.out2) res.u = make_uint4(0, 0, 0, 0);
+        else if (res_in_patch) res.u = rp[r][u];
+        else res.u = *reinterpret_cast<const uint4*>(a.res + o);
+        Pack8 o1, o2;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          o1.h[k] = f2bf(sv[k]);
+          o2.h[k] = f2bf(bf2f(res.h[k]) + sv[k]);
+        }
+        if (a.out1) *reinterpret_cast<uint4*>(a.out1 + o) = o1.u;
+        if (a.out2) *reinterpret_cast<uint4*>(a.out2 + o) = o2.u;
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -257,6 +400,36 @@ __global__ void pair_fwd_kernel(const uint16_t* P, const uint16_t* Q, const floa
   }
 }
 
+// The same with eight channels per lane (16-byte loads and stores) and 32-bit indexing:
+// blockIdx.y = b * ND + d, x covers (i, channel octet); C % 8 == 0.
+__global__ void pair_fwd8_kernel(const uint16_t* P, const uint16_t* Q, const float* E, uint16_t* Z,
+                                 int ND, int L, int C) {
+  const int bd = blockIdx.y;
+  const int d = bd % ND, b = bd / ND;
+  const int oct = C / 8;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= L * oct) return;
+  const int i = t / oct, q = t % oct;
+  const int64_t zo = ((int64_t)bd * L + i) * C + 8 * q;
+  Pack8 out;
+  if (i + d < L) {
+    Pack8 pv, qv;
+    pv.u = *reinterpret_cast<const uint4*>(P + ((int64_t)b * L + i) * C + 8 * q);
+    qv.u = *reinterpret_cast<const uint4*>(Q + ((int64_t)b * L + i + d) * C + 8 * q);
+    const float4 e0 = *reinterpret_cast<const float4*>(E + d * C + 8 * q);
+    const float4 e1 = *reinterpret_cast<const float4*>(E + d * C + 8 * q + 4);
+    const float ev[8] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float v = bf2f(pv.h[k]) + bf2f(qv.h[k]) + ev[k];
+      out.h[k] = f2bf(v > 0.0f ? v : 0.0f);
+    }
+  } else {
+    out.u = make_uint4(0, 0, 0, 0);
+  }
+  *reinterpret_cast<uint4*>(Z + zo) = out.u;
+}
+
 // gs = gz * [z > 0]; dP[b][i][c] = sum_d gs[b][d][i][c]; dQ[b][j][c] = sum_d gs[b][d][j - d][c]
 __global__ void pair_bwd_pq_kernel(const uint16_t* GZ, const uint16_t* Z, uint16_t* dP, uint16_t* dQ,
                                    int B, int ND, int L, int C) {
@@ -319,6 +492,39 @@ __global__ void out_fwd_kernel(const uint16_t* Z, const float* V, const float* c
     }
     out[p] = s;
   }
+}
+
+// The same with eight lanes per position, each reading one channel octet (16 bytes) of every
+// tap, so a wave's loads cover eight neighbouring positions contiguously; the octets' partial
+// sums meet in a butterfly. Weights V [9][C] are staged in LDS. C == 64.
+__global__ void __launch_bounds__(256) out_fwd8_kernel(const uint16_t* Z, const float* V,
+                                                       const float* c0, float* out, int B, int D,
+                                                       int I) {
+  constexpr int C = 64;
+  __shared__ float Vs[9 * C];
+  for (int k = threadIdx.x; k < 9 * C; k += blockDim.x) Vs[k] = V[k];
+  __syncthreads();
+  const int64_t npos = (int64_t)B * D * I;
+  const int q = threadIdx.x & 7;
+  const int64_t p = (int64_t)blockIdx.x * (blockDim.x / 8) + (threadIdx.x >> 3);
+  float s = 0.0f;
+  if (p < npos) {
+    const int i = (int)(p % I), d = (int)((p / I) % D);
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      const int d2 = d + t / 3 - 1, i2 = i + t % 3 - 1;
+      if (d2 < 0 || d2 >= D || i2 < 0 || i2 >= I) continue;
+      Pack8 v;
+      v.u = *reinterpret_cast<const uint4*>(Z + (p + (int64_t)(t / 3 - 1) * I + (t % 3 - 1)) * C +
+                                            8 * q);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) s += Vs[t * C + 8 * q + k] * bf2f(v.h[k]);
+    }
+  }
+  s += __shfl_xor(s, 1, 64);
+  s += __shfl_xor(s, 2, 64);
+  s += __shfl_xor(s, 4, 64);
+  if (p < npos && q == 0) out[p] = s + c0[0];
 }
 
 // loss: masked squared error; g[p] = scale * (out - y) on valid pixels (i + d < I), 0 elsewhere;
@@ -485,6 +691,13 @@ hipError_t launch_mfma_probe(hipStream_t s, const uint16_t* A, const uint16_t* B
 
 hipError_t launch_conv_fwd(hipStream_t s, const ConvArgs& a) {
   if (a.Cin % 16 || a.Cout % 64 || a.KD < 1 || a.KI < 1 || !a.X || !a.W) return hipErrorInvalidValue;
+  if (a.KD == 3 && a.KI == 3 && a.dil <= 1 && a.Cin == kT3C && a.Cout == kT3C && a.B <= 65535 &&
+      a.D <= 65535 * kT3Rows) {
+    const dim3 grid((unsigned)((a.I + kT3Cols - 1) / kT3Cols), (unsigned)((a.D + kT3Rows - 1) / kT3Rows),
+                    (unsigned)a.B);
+    hipLaunchKernelGGL(conv3x3_c64_kernel, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+  }
   const int64_t npos = (int64_t)a.B * a.D * a.I;
   const dim3 grid((unsigned)((npos + kConvRows - 1) / kConvRows), (unsigned)(a.Cout / 64));
   hipLaunchKernelGGL(conv_fwd_kernel, grid, dim3(256), 0, s, a);
@@ -525,6 +738,12 @@ hipError_t launch_crop(hipStream_t s, const uint16_t* in, uint16_t* out, int B, 
 
 hipError_t launch_pair_fwd(hipStream_t s, const uint16_t* P, const uint16_t* Q, const float* E,
                            uint16_t* Z, int B, int ND, int L, int C) {
+  if (C % 8 == 0 && (int64_t)B * ND <= 65535) {
+    const int per = L * (C / 8);
+    hipLaunchKernelGGL(pair_fwd8_kernel, dim3((per + 255) / 256, B * ND), dim3(256), 0, s, P, Q, E,
+                       Z, ND, L, C);
+    return hipGetLastError();
+  }
   const int64_t n = (int64_t)B * ND * L * C;
   hipLaunchKernelGGL(pair_fwd_kernel, dim3(grid_for(n)), dim3(256), 0, s, P, Q, E, Z, B, ND, L, C);
   return hipGetLastError();
@@ -543,6 +762,12 @@ hipError_t launch_pair_bwd(hipStream_t s, const uint16_t* GZ, const uint16_t* Z,
 hipError_t launch_out_fwd(hipStream_t s, const uint16_t* Z, const float* V, const float* c0,
                           float* out, int B, int D, int I, int C) {
   if (C % 8) return hipErrorInvalidValue;
+  if (C == 64) {
+    const int64_t npos = (int64_t)B * D * I;
+    hipLaunchKernelGGL(out_fwd8_kernel, dim3((unsigned)((npos + 31) / 32)), dim3(256), 0, s, Z, V,
+                       c0, out, B, D, I);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(out_fwd_kernel, dim3(grid_for((int64_t)B * D * I)), dim3(256), 0, s, Z, V, c0,
                      out, B, D, I, C);
   return hipGetLastError();

@@ -177,7 +177,9 @@ int64_t valid_pixels(const mgpu_sg* s) {
 }
 
 // ---- forward over the first B samples of s->x ----
-int forward(mgpu_sg* s, int B) {
+// keep_act: also store the post-ReLU activations (s->r, s->r2) that backward needs; inference
+// leaves them out and saves their writes.
+int forward(mgpu_sg* s, int B, bool keep_act = true) {
   hipStream_t st = s->stream;
   const uint16_t* w16 = s->p16;
   ConvArgs a{};
@@ -200,7 +202,7 @@ int forward(mgpu_sg* s, int B) {
       a.out2 = nullptr;
     } else {
       a.res = s->h[l];
-      a.out1 = s->r[l];
+      a.out1 = keep_act ? s->r[l] : nullptr;
       a.out2 = s->h[l + 1];
     }
     SGCHK(s, launch_conv_fwd(st, a));
@@ -236,7 +238,7 @@ int forward(mgpu_sg* s, int B) {
     hz.W = w16 + s->head_w[j].off;
     hz.bias = s->p32 + s->head_b[j].off;
     hz.res = s->z[j];
-    hz.out1 = s->r2[j];
+    hz.out1 = keep_act ? s->r2[j] : nullptr;
     hz.out2 = s->z[j + 1];
     SGCHK(s, launch_conv_fwd(st, hz));
   }
@@ -588,7 +590,7 @@ int mgpu_sg_forward(mgpu_sg* s, const float* x, uint64_t n, float* y) {
   for (uint64_t b0 = 0; b0 < n; b0 += s->Bm) {
     const int B = (int)std::min<uint64_t>(s->Bm, n - b0);
     if (int rc = upload_inputs(s, x + b0 * s->Lin * s->cfg.in_ch, B)) return rc;
-    if (int rc = forward(s, B)) return rc;
+    if (int rc = forward(s, B, false)) return rc;
     SGCHK(s, hipMemcpyAsync(y + b0 * per, s->out, (size_t)B * per * 4, hipMemcpyDeviceToHost,
                             s->stream));
     SGCHK(s, hipStreamSynchronize(s->stream));
@@ -737,7 +739,7 @@ int mgpu_sg_predict_chromosomes(mgpu_sg* s, const float* x, uint64_t first_point
     const uint64_t np = std::min<uint64_t>(ppb, npoints - p0);
     const int B = (int)(np * nwin);
     if (int rc = upload_inputs(s, x + p0 * nwin * per_win, B)) return rc;
-    if (int rc = forward(s, B)) return rc;
+    if (int rc = forward(s, B, false)) return rc;
     SGCHK(s, launch_stitch_bands(s->stream, s->out, (int)np, (int)nwin, s->d_starts, s->ND,
                                  s->Lout, (int)nbins,
                                  s->bands + (first_point + p0) * (uint64_t)s->ND * nbins));
@@ -760,7 +762,7 @@ int mgpu_sg_predict_timed(mgpu_sg* s, uint64_t n, uint64_t repeats, float* ms) {
   if (n == 0 || n > (uint64_t)s->Bm) return sg_fail(s, MGPU_EINVAL, "n must be in [1, max_batch]");
   SGCHK(s, hipEventRecord(s->ev0, s->stream));
   for (uint64_t r = 0; r < repeats; ++r)
-    if (int rc = forward(s, (int)n)) return rc;
+    if (int rc = forward(s, (int)n, false)) return rc;
   SGCHK(s, hipEventRecord(s->ev1, s->stream));
   SGCHK(s, hipEventSynchronize(s->ev1));
   SGCHK(s, hipEventElapsedTime(ms, s->ev0, s->ev1));

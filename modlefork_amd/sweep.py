@@ -22,7 +22,6 @@ from __future__ import annotations
 
 import threading
 import time
-from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 
 import numpy as np
@@ -281,12 +280,14 @@ class SweepResult:
     reliabilities: list     # the judge simulation's own reliability
     replaced: list
     fell_back: bool
-    surrogate_s: float      # inputs, inference and stitching of every point (wall)
-    judge_s: float          # the judge's simulations (wall, alongside the surrogate)
+    surrogate_s: float      # inference and stitching of every point, plus any input building
+                            # the judge did not hide (wall)
+    judge_s: float          # the judge's simulations (wall; inputs are built meanwhile)
     sweep_s: float          # the whole sweep, judge and fallback included (wall)
     sim_rate: float         # plain simulation, matrices per second, measured on its own
     surrogate_rate: float   # surrogate matrices per second (inputs, inference, stitching)
     sweep_rate: float       # matrices per second of the whole sweep
+    slots: np.ndarray       # device band slot of each point (net.read_bands)
 
 
 def plain_simulation_rate(ctx, layout: Layout, points, ncells: int = 512,
@@ -295,38 +296,61 @@ def plain_simulation_rate(ctx, layout: Layout, points, ncells: int = 512,
     cells and contact density, the 100 diagonals the sweep delivers kept on the device."""
     t0 = time.perf_counter()
     simulate_points(ctx, layout, points, ncells, contact_density, seed=seed,
-                    stored_diagonals=Wn.NDIAG)
+                    stored_diagonals=Wn.NDIAG, chunk=len(points))
     return len(points) / (time.perf_counter() - t0)
 
 
 def run_sweep(ctx, net, layout: Layout, points, judge_frac: float = 0.05, threshold: float = 0.95,
               seed: int = 3, ncells: int = 512, contact_density: float = 30.0, log=print,
               rate_sample: int = 64, chunk_points: int = 256):
-    """The judge simulates on its own context (its own HIP stream) in a host thread while the
-    surrogate predicts every point; a judged point whose score is below the threshold is replaced
-    by its simulation. The plain simulation's rate is then measured on its own on rate_sample
-    judged points (0: not measured, sim_rate nan).
+    """Predicts every point with the surrogate and judges a seeded judge_frac of them by
+    simulation; a judged point whose score is below the threshold is replaced by its simulation,
+    and more than a fifth failing rejects the surrogate for the sweep. The plain simulation's rate
+    is then measured on its own on rate_sample judged points (0: not measured, sim_rate nan).
 
-    Returns (SweepResult, replaced bands {point: band}): every other point's band is the
-    surrogate's, in the network's device memory (net.read_bands)."""
+    Order of work: the judged points are predicted first; the judge then simulates them on its
+    own context (HIP stream) in a host thread while the host builds every other point's inputs;
+    after it, the device predicts the other points while a host thread scores the judged ones.
+    (Running predictions beside the judge's persistent launches only makes them compete for the
+    CUs.)
+
+    Returns (SweepResult, replaced bands {point: band}). Every other point's band is the
+    surrogate's and stays in the network's device memory: net.read_bands(result.slots[k], 1)."""
     t_start = time.perf_counter()
     nbins = layout.size // Wn.BIN
     rng = np.random.default_rng(seed)
     nj = max(1, int(round(judge_frac * len(points))))
     judged = sorted(rng.choice(len(points), size=nj, replace=False).tolist())
     jp = [points[k] for k in judged]
+    js = set(judged)
+    rest = [k for k in range(len(points)) if k not in js]
+    order = judged + rest  # device band slot s holds point order[s]
+    slots = np.empty(len(points), np.int64)
+    slots[np.asarray(order)] = np.arange(len(points))
+    starts = np.asarray(window_starts(nbins), np.uint32)
+    nwin = len(starts)
+
+    # 1. the judged points' predictions, read back for scoring
+    t0 = time.perf_counter()
+    net.reserve_bands(len(points), nbins)
+    net.predict_chromosomes(points_inputs(layout, jp), 0, nwin, starts)
+    pred_j = net.read_bands(0, nj)
+    t_sur = time.perf_counter() - t0
+
+    # 2. the judge on the device, the other points' inputs on the host
     out = {}
 
     def judge_worker():
-        # each judged point simulated as two independent halves of its cells
+        # each judged point simulated as two independent halves of its cells, one job per half
+        # (a single persistent launch over every judged point's cells)
         jctx = S.Context(0)
         try:
-            t0 = time.perf_counter()
+            t1 = time.perf_counter()
             out["a"], _ = simulate_points(jctx, layout, jp, ncells // 2, contact_density / 2,
-                                          seed=seed, stored_diagonals=Wn.NDIAG)
+                                          seed=seed, stored_diagonals=Wn.NDIAG, chunk=len(jp))
             out["b"], _ = simulate_points(jctx, layout, jp, ncells // 2, contact_density / 2,
-                                          seed=seed + 1, stored_diagonals=Wn.NDIAG)
-            out["t"] = time.perf_counter() - t0
+                                          seed=seed + 1, stored_diagonals=Wn.NDIAG, chunk=len(jp))
+            out["t"] = time.perf_counter() - t1
         except BaseException as e:  # re-raised by the sweep thread
             out["err"] = e
         finally:
@@ -334,32 +358,42 @@ def run_sweep(ctx, net, layout: Layout, points, judge_frac: float = 0.05, thresh
 
     th = threading.Thread(target=judge_worker, name="sweep-judge")
     th.start()
-    # surrogate: every point, its windows' inputs built in chunks of points, predictions stitched
-    # into whole-chromosome bands on the device (they stay there; judged points are read back)
-    starts = np.asarray(window_starts(nbins), np.uint32)
     t0 = time.perf_counter()
     try:
-        net.reserve_bands(len(points), nbins)
-        # inputs of the next chunk are built on a host thread while the device predicts this one
-        chunks = list(range(0, len(points), chunk_points))
-        with ThreadPoolExecutor(max_workers=1) as ex:
-            nxt = ex.submit(points_inputs, layout, points[0:chunk_points])
-            for c0 in chunks:
-                x = nxt.result()
-                if c0 + chunk_points < len(points):
-                    nxt = ex.submit(points_inputs, layout,
-                                    points[c0 + chunk_points:c0 + 2 * chunk_points])
-                net.predict_chromosomes(x, c0, len(starts), starts)
+        xs = [(nj + c0, points_inputs(layout, [points[k] for k in rest[c0:c0 + chunk_points]]))
+              for c0 in range(0, len(rest), chunk_points)]
     finally:
-        t_sur = time.perf_counter() - t0
+        t_inputs = time.perf_counter() - t0
         th.join()
     if "err" in out:
         raise out["err"]
     half_a, half_b, t_judge = out["a"], out["b"], out["t"]
+    t_sur += max(0.0, t_inputs - t_judge)  # input building the judge did not hide
+
+    # 3. the other points on the device, the judged points' scores on the host
+    scores = {}
+
+    def score_worker():
+        try:
+            scores["v"] = [judge(pred_j[j].astype(np.float64), a, b)
+                           for j, (a, b) in enumerate(zip(half_a, half_b))]
+        except BaseException as e:
+            scores["err"] = e
+
+    sc = threading.Thread(target=score_worker, name="sweep-score")
+    sc.start()
+    t0 = time.perf_counter()
+    try:
+        for slot, x in xs:
+            net.predict_chromosomes(x, slot, nwin, starts)
+    finally:
+        t_sur += time.perf_counter() - t0
+        sc.join()
+    if "err" in scores:
+        raise scores["err"]
     corrs, raws, rels, replaced = [], [], [], []
     bands = {}  # points whose band is the simulation's, not the surrogate's
-    for k, a, b in zip(judged, half_a, half_b):
-        c, r, q = judge(net.read_bands(k, 1)[0].astype(np.float64), a, b)
+    for k, (c, r, q), a, b in zip(judged, scores["v"], half_a, half_b):
         corrs.append(c)
         raws.append(r)
         rels.append(q)
@@ -369,8 +403,6 @@ def run_sweep(ctx, net, layout: Layout, points, judge_frac: float = 0.05, thresh
     fell_back = len(replaced) > 0.2 * len(judged)
     if fell_back:
         # the surrogate is not trusted for this sweep: every point falls back to simulation
-        js = set(judged)
-        rest = [k for k in range(len(points)) if k not in js]
         sims, _ = simulate_points(ctx, layout, [points[k] for k in rest], ncells, contact_density,
                                   seed=seed, stored_diagonals=Wn.NDIAG)
         for k, b in zip(rest, sims):
@@ -384,7 +416,8 @@ def run_sweep(ctx, net, layout: Layout, points, judge_frac: float = 0.05, thresh
     sim_rate = (plain_simulation_rate(ctx, layout, jp[:rate_sample], ncells, contact_density)
                 if rate_sample else float("nan"))
     return SweepResult(len(points), judged, corrs, raws, rels, replaced, fell_back, t_sur, t_judge,
-                       t_sweep, sim_rate, len(points) / t_sur, len(points) / t_sweep), bands
+                       t_sweep, sim_rate, len(points) / t_sur, len(points) / t_sweep,
+                       slots), bands
 
 
 def main(argv=None):
