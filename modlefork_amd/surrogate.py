@@ -258,14 +258,34 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--out", required=True, help="weights (.npy)")
+    ap.add_argument("--out", default=None, help="weights (.npy); required for training")
     ap.add_argument("--summary", default=None, help="JSON summary")
+    ap.add_argument("--time-forward", type=int, default=0, metavar="WINDOWS",
+                    help="only time forward passes over a batch of WINDOWS windows with the "
+                         "shipped weights (one untimed pass, then --repeats timed ones)")
+    ap.add_argument("--repeats", type=int, default=20)
     a = ap.parse_args(argv)
 
     def log(msg):
         print(msg, flush=True)
 
     ctx = S.Context(0)
+    if a.time_forward:
+        import os
+
+        net = Surrogate(ctx, default_config(max_batch=a.time_forward))
+        load_params(net, os.path.join(os.path.dirname(__file__), "data", "surrogate_weights.npy"))
+        cfg = net.cfg
+        x = np.random.default_rng(0).standard_normal(
+            (a.time_forward, int(cfg.in_len), int(cfg.in_ch))).astype(np.float32)
+        net.forward(x)
+        ms = net.predict_timed(a.time_forward, a.repeats) / a.repeats
+        log(f"forward of {a.time_forward} windows: {ms:.2f} ms")
+        net.close()
+        ctx.close()
+        return
+    if not a.out:
+        ap.error("--out is required for training")
     t0 = time.perf_counter()
     log(f"simulating {a.chromosomes} training + {a.val} validation chromosomes, {a.cells} cells")
     x, y, _, dev_tr = make_dataset(ctx, a.chromosomes, a.seed, a.cells, a.density, log=log)
