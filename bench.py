@@ -25,6 +25,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# L2 atomics: 16 channels per XCD x 8 XCDs, one 32-bit atomic per channel per 2.4 GHz clock
+ATOMIC_PEAK_PER_S = 128 * 2.4e9
 
 
 def log(msg):
@@ -264,6 +266,23 @@ def main():
         if prof.get("launch_shape") == shape:
             traffic = prof.get("hbm_bytes_per_launch", {}).get(str(d.threads))
 
+    # LDS and atomic figures of the dominant launch from the committed rocprof counter runs
+    # (taken for one launch shape; other shapes report none)
+    lds = atomics = None
+    cfile = os.path.join(ROOT, "profiles", f"counters_{args.mode}_{args.workload}.json")
+    if os.path.exists(cfile):
+        with open(cfile) as fh:
+            cprof = json.load(fh)
+        dv = cprof.get("per_launch", {}).get(str(d.threads), {}).get("derived")
+        if cprof.get("launch_shape") == shape and dv:
+            lds = {"util": dv["lds_util"], "bank_conflict_ratio": dv["lds_bank_conflict_ratio"],
+                   "instructions": dv["lds_instructions"], "valu_issue_util": dv["valu_issue_util"],
+                   "wait_fraction": dv["wait_fraction"],
+                   "peak": "LDS busy on every cycle of all 256 CUs (SQ_LDS_IDX_ACTIVE)"}
+            atomics = {"per_s": dv["atomics_per_s"], "count": dv["atomics"],
+                       "peak_per_s": ATOMIC_PEAK_PER_S,
+                       "frac": dv["atomics_per_s"] / ATOMIC_PEAK_PER_S}
+
     cpu = None
     if not args.no_cpu:
         nthreads = args.cpu_threads or min(os.cpu_count() or 1, 64)
@@ -302,6 +321,8 @@ def main():
                      "all_launches": {"algorithmic_bytes": all_bytes,
                                       "ms": float(sum(x.ms for x in ls)),
                                       "achieved": all_bytes / (sum(x.ms for x in ls) / 1000.0) / 1e9}},
+        "lds": lds,
+        "atomics": atomics,
         "cpu_baseline": cpu,
     }
     if cpu:
