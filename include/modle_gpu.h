@@ -266,20 +266,22 @@ int mgpu_job_read_state_log(mgpu_job* job, mgpu_state_record* out, uint64_t cap,
 /* ---- launch tuning (no counterpart in the reference: it sizes the device launch) ----
  * Fast mode runs its cells in persistent launches. By default (threads_per_cell 0) it makes one
  * launch per size tier: cells whose working set lets 1, 2 or 4 of them share one CU's LDS run at
- * 1024, 512 or 256 lanes per cell (16 waves per CU in every tier), cells too large for LDS run
- * from device memory at 512 lanes; tier_threads[k] (set, when nonzero) overrides tier k's size.
+ * 1024, 512 or 256 lanes per cell (16 waves per CU in every tier); cells too large for LDS whose
+ * positions, ranks and barrier states fit there run hybrid at 1024 lanes (the per-epoch
+ * temporaries in device memory); larger cells run wholly from device memory at 512 lanes;
+ * tier_threads[k] (set, when nonzero) overrides tier k's size.
  * threads_per_cell 64, 128, 256, 512 or 1024 instead runs every cell in one launch of that size.
  * global_working_set: 1 keeps every cell's working set in device memory, in one launch (0 or the
  * chosen threads_per_cell: 512 lanes by default). Bands, missed counts and statistics do not
- * depend on the tuning. Get only: launches (the launches the next run makes, at most 4), their
+ * depend on the tuning. Get only: launches (the launches the next run makes, at most 5), their
  * sizes in tier_threads and grids in tier_grid, and grid (the largest of them). */
 typedef struct mgpu_tuning {
   uint32_t threads_per_cell;
   uint32_t grid;
   uint32_t global_working_set;
   uint32_t launches;
-  uint32_t tier_threads[4];
-  uint32_t tier_grid[4];
+  uint32_t tier_threads[5];
+  uint32_t tier_grid[5];
 } mgpu_tuning;
 int mgpu_job_set_tuning(mgpu_job* job, const mgpu_tuning* t);
 int mgpu_job_get_tuning(mgpu_job* job, mgpu_tuning* t);

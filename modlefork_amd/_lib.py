@@ -166,8 +166,8 @@ class MgpuTuning(C.Structure):
         ("grid", C.c_uint32),
         ("global_working_set", C.c_uint32),
         ("launches", C.c_uint32),
-        ("tier_threads", C.c_uint32 * 4),
-        ("tier_grid", C.c_uint32 * 4),
+        ("tier_threads", C.c_uint32 * 5),
+        ("tier_grid", C.c_uint32 * 5),
     ]
 
 

@@ -162,6 +162,44 @@ __device__ inline void carve(Cell& c, uint8_t* base, uint32_t n, uint32_t nbar, 
   c.avg = (double*)take(8ull * (hist + 1));
 }
 
+// Working sets too large for LDS keep the arrays every phase gathers from (positions, ranks,
+// barrier states, burn-in history) in LDS and the per-epoch temporaries (moves, collisions,
+// scratch) in the workgroup's slab of device memory.
+__device__ __host__ inline uint64_t hybrid_lds_bytes(uint32_t n, uint32_t nbar, uint32_t hist) {
+  return align16(4ull * n) * 2 + align16(2ull * n) * 2 + align16(nbar) +
+         align16(8ull * (hist + 1)) * 2;
+}
+
+__device__ __host__ inline uint64_t hybrid_slab_bytes(uint32_t n) {
+  return align16(4ull * n) * 4 + align16(scratch_bytes(n));
+}
+
+__device__ inline void carve_hybrid(Cell& c, uint8_t* lds, uint8_t* slab, uint32_t n, uint32_t nbar,
+                                    uint32_t hist) {
+  uint8_t* p = lds;
+  uint8_t* q = slab;
+  auto take = [](uint8_t*& at, uint64_t bytes) {
+    uint8_t* r = at;
+    at += align16(bytes);
+    return r;
+  };
+  c.rev = (uint32_t*)take(p, 4ull * n);
+  c.fwd = (uint32_t*)take(p, 4ull * n);
+  c.rr = (uint16_t*)take(p, 2ull * n);
+  c.fr = (uint16_t*)take(p, 2ull * n);
+  c.bst = take(p, nbar);
+  c.cfx = (double*)take(p, 8ull * (hist + 1));
+  c.avg = (double*)take(p, 8ull * (hist + 1));
+  c.rm = (uint32_t*)take(q, 4ull * n);
+  c.fm = (uint32_t*)take(q, 4ull * n);
+  c.rc = (uint32_t*)take(q, 4ull * n);
+  c.fc = (uint32_t*)take(q, 4ull * n);
+  c.scratch = (uint32_t*)take(q, scratch_bytes(n));
+}
+
+// where a launch keeps its cells' working sets
+constexpr int kStoreSlab = 0, kStoreLds = 1, kStoreHybrid = 2;
+
 struct Shared {
   int32_t shi[kFastMaxWaves];
   uint32_t shu[kFastMaxWaves];
@@ -1116,13 +1154,13 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     for (int k = 0; k < kFastPhases; ++k) atomicAdd(&g_fast_phase[k], (unsigned long long)S.phase[k]);
 }
 
-// Persistent workers: each workgroup takes cells off the queue until it is empty. kLds selects
-// where the working set lives (LDS, or a per-workgroup slab in global memory); it is a template
-// argument so the LDS variant addresses its arrays with LDS instructions. Every size is compiled
+// Persistent workers: each workgroup takes cells off the queue until it is empty. kStore selects
+// where the working set lives (LDS, a per-workgroup slab in global memory, or the hybrid of the
+// two); it is a template argument so arrays in LDS are addressed with LDS instructions. Every size is compiled
 // for four waves per SIMD (at most 128 VGPRs), so 16 waves share a CU at every workgroup size
 // whose cells fit 1024 / NT to a CU: the epoch loop waits on LDS and atomics, and the waves
 // of other workgroups hide that latency.
-template <int NT, bool kLds, bool kOut>
+template <int NT, int kStore, bool kOut>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
     fast_cells_kernel(FastParams P, const FastChrom* chroms, const WorkItem* items,
                       const uint32_t* order, uint64_t nitems, unsigned long long* next,
@@ -1139,10 +1177,13 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
     const WorkItem w = items[it];
     const FastChrom ch = chroms[w.chrom];
     Cell c;
-    if constexpr (kLds) {
+    if constexpr (kStore == kStoreLds) {
       carve(c, lds, ch.nlefs, ch.nbar, P.hist_len);
-    } else {
+    } else if constexpr (kStore == kStoreSlab) {
       carve(c, slab + (uint64_t)blockIdx.x * slab_stride, ch.nlefs, ch.nbar, P.hist_len);
+    } else {
+      carve_hybrid(c, lds, slab + (uint64_t)blockIdx.x * slab_stride, ch.nlefs, ch.nbar,
+                   P.hist_len);
     }
     fast_cell<NT, kOut>(P, ch, c, S, w.seed, w.target,
                         cell_stats ? cell_stats + 4 * it : nullptr, (uint32_t)it);
@@ -1184,6 +1225,7 @@ struct Tier {
   uint32_t threads = 0;    // workgroup size
   uint32_t per_cu = 1;     // workgroups one CU holds by LDS (2 for device-memory working sets)
   uint64_t shmem = 0;      // dynamic LDS per workgroup; 0: working sets in the slab
+  int store = kStoreLds;   // kStoreLds, kStoreSlab or kStoreHybrid
   uint64_t begin = 0, count = 0;  // cells order[begin, begin + count)
 };
 
@@ -1228,6 +1270,7 @@ Tier single_launch(const FastImpl* impl) {
   const bool lds = impl->fits_lds && !impl->global;
   t.threads = impl->threads ? impl->threads : kFastDefaultThreads;
   t.shmem = lds ? impl->max_bytes : 0;
+  t.store = lds ? kStoreLds : kStoreSlab;
   t.per_cu = lds ? (uint32_t)std::max<uint64_t>(1, kLdsTotal / (impl->max_bytes + kLdsReserve)) : 2u;
   t.begin = 0;
   t.count = impl->nitems;
@@ -1238,14 +1281,18 @@ bool tiered(const FastImpl* impl) { return impl->threads == 0 && !impl->global; 
 
 template <int NT, bool kOut>
 hipError_t launch_fast_out(hipStream_t stream, FastImpl* impl, const WorkItem* d_items,
-                           uint64_t* d_cell_stats, uint32_t grid, uint64_t shmem,
+                           uint64_t* d_cell_stats, uint32_t grid, uint64_t shmem, int store,
                            const uint32_t* order, uint64_t nitems) {
-  if (shmem)
-    hipLaunchKernelGGL((fast_cells_kernel<NT, true, kOut>), dim3(grid), dim3(NT), shmem, stream,
-                       impl->P, impl->d_chroms, d_items, order, nitems, impl->d_next, nullptr,
-                       (uint64_t)0, d_cell_stats);
+  if (store == kStoreLds)
+    hipLaunchKernelGGL((fast_cells_kernel<NT, kStoreLds, kOut>), dim3(grid), dim3(NT), shmem,
+                       stream, impl->P, impl->d_chroms, d_items, order, nitems, impl->d_next,
+                       nullptr, (uint64_t)0, d_cell_stats);
+  else if (store == kStoreHybrid)
+    hipLaunchKernelGGL((fast_cells_kernel<NT, kStoreHybrid, kOut>), dim3(grid), dim3(NT), shmem,
+                       stream, impl->P, impl->d_chroms, d_items, order, nitems, impl->d_next,
+                       impl->d_slab, impl->slab_stride, d_cell_stats);
   else
-    hipLaunchKernelGGL((fast_cells_kernel<NT, false, kOut>), dim3(grid), dim3(NT), 0, stream,
+    hipLaunchKernelGGL((fast_cells_kernel<NT, kStoreSlab, kOut>), dim3(grid), dim3(NT), 0, stream,
                        impl->P, impl->d_chroms, d_items, order, nitems, impl->d_next,
                        impl->d_slab, impl->slab_stride, d_cell_stats);
   return hipGetLastError();
@@ -1253,13 +1300,13 @@ hipError_t launch_fast_out(hipStream_t stream, FastImpl* impl, const WorkItem* d
 
 template <int NT>
 hipError_t launch_fast(hipStream_t stream, FastImpl* impl, const WorkItem* d_items,
-                       uint64_t* d_cell_stats, uint32_t grid, uint64_t shmem,
+                       uint64_t* d_cell_stats, uint32_t grid, uint64_t shmem, int store,
                        const uint32_t* order, uint64_t nitems) {
   if (impl->occupancy || impl->P.log.rec)
-    return launch_fast_out<NT, true>(stream, impl, d_items, d_cell_stats, grid, shmem, order,
-                                     nitems);
-  return launch_fast_out<NT, false>(stream, impl, d_items, d_cell_stats, grid, shmem, order,
-                                    nitems);
+    return launch_fast_out<NT, true>(stream, impl, d_items, d_cell_stats, grid, shmem, store,
+                                     order, nitems);
+  return launch_fast_out<NT, false>(stream, impl, d_items, d_cell_stats, grid, shmem, store,
+                                    order, nitems);
 }
 
 // Launches the persistent kernel with tier t's shape over order[0, nitems) (the work counter is
@@ -1271,11 +1318,11 @@ hipError_t launch_tier(hipStream_t stream, FastImpl* impl, const Tier& t, const 
   if (e != hipSuccess) return e;
   const uint32_t grid = launch_grid(impl, t.threads, t.per_cu, nitems);
   switch (t.threads) {
-    case 64: return launch_fast<64>(stream, impl, d_items, d_cell_stats, grid, t.shmem, order, nitems);
-    case 128: return launch_fast<128>(stream, impl, d_items, d_cell_stats, grid, t.shmem, order, nitems);
-    case 256: return launch_fast<256>(stream, impl, d_items, d_cell_stats, grid, t.shmem, order, nitems);
-    case 512: return launch_fast<512>(stream, impl, d_items, d_cell_stats, grid, t.shmem, order, nitems);
-    case 1024: return launch_fast<1024>(stream, impl, d_items, d_cell_stats, grid, t.shmem, order, nitems);
+    case 64: return launch_fast<64>(stream, impl, d_items, d_cell_stats, grid, t.shmem, t.store, order, nitems);
+    case 128: return launch_fast<128>(stream, impl, d_items, d_cell_stats, grid, t.shmem, t.store, order, nitems);
+    case 256: return launch_fast<256>(stream, impl, d_items, d_cell_stats, grid, t.shmem, t.store, order, nitems);
+    case 512: return launch_fast<512>(stream, impl, d_items, d_cell_stats, grid, t.shmem, t.store, order, nitems);
+    case 1024: return launch_fast<1024>(stream, impl, d_items, d_cell_stats, grid, t.shmem, t.store, order, nitems);
     default: return hipErrorInvalidValue;
   }
 }
@@ -1284,12 +1331,16 @@ hipError_t launch_tier(hipStream_t stream, FastImpl* impl, const Tier& t, const 
 // launch changes kernel attributes (launches may be recorded into a graph).
 template <int NT>
 hipError_t set_lds_limit() {
-  hipError_t e =
-      hipFuncSetAttribute(reinterpret_cast<const void*>(fast_cells_kernel<NT, true, false>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsLimit);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(fast_cells_kernel<NT, true, true>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsLimit);
+  const void* fns[] = {reinterpret_cast<const void*>(fast_cells_kernel<NT, kStoreLds, false>),
+                       reinterpret_cast<const void*>(fast_cells_kernel<NT, kStoreLds, true>),
+                       reinterpret_cast<const void*>(fast_cells_kernel<NT, kStoreHybrid, false>),
+                       reinterpret_cast<const void*>(fast_cells_kernel<NT, kStoreHybrid, true>)};
+  for (const void* f : fns) {
+    const hipError_t e =
+        hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsLimit);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t set_lds_limits() {
@@ -1302,11 +1353,15 @@ hipError_t set_lds_limits() {
 }
 
 // Size tiers: a cell whose working set leaves room for k = 1, 2 or 4 of them in one CU's LDS
-// runs in the tier of that k, at 1024 / k lanes, so every tier keeps 16 waves on each CU; cells
-// too large for LDS form a tier of their own that works from the slab (two workgroups of
-// kFastDefaultThreads per CU). Tier of chromosome c: 0 slab, 1 k = 1, 2 k = 2, 3 k = 4.
-uint32_t tier_of(uint64_t bytes) {
-  if (bytes > kLdsLimit) return 0;
+// runs in the tier of that k, at 1024 / k lanes, so every tier keeps 16 waves on each CU. A cell
+// too large for LDS whose positions, ranks and barrier states fit there runs hybrid (those in
+// LDS, the per-epoch temporaries in the slab; one workgroup of 1024 lanes per CU); larger cells
+// work wholly from the slab (two workgroups of kFastDefaultThreads per CU). Tier of chromosome
+// c: 0 slab, 1 k = 1, 2 k = 2, 3 k = 4, 4 hybrid.
+constexpr uint32_t kTiers = 5;
+
+uint32_t tier_of(uint64_t bytes, uint64_t hybrid_bytes) {
+  if (bytes > kLdsLimit) return hybrid_bytes <= kLdsLimit ? 4 : 0;
   const uint64_t k = kLdsTotal / (bytes + kLdsReserve);
   return k >= 4 ? 3 : (k >= 2 ? 2 : 1);
 }
@@ -1316,9 +1371,15 @@ Tier tier_shape(uint32_t tier) {
   if (tier == 0) {
     t.threads = kFastDefaultThreads;
     t.per_cu = 2;
+    t.store = kStoreSlab;
+  } else if (tier == 4) {
+    t.threads = 1024;
+    t.per_cu = 1;
+    t.store = kStoreHybrid;
   } else {
     t.per_cu = 1u << (tier - 1);
     t.threads = 1024 / t.per_cu;
+    t.store = kStoreLds;
   }
   return t;
 }
@@ -1461,7 +1522,9 @@ int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& dchroms,
   // work order: by size tier, longest cells first within a tier (chromosomes with more LEFs
   // take longer per epoch)
   std::vector<uint32_t> tier(nch);
-  for (uint64_t c = 0; c < nch; ++c) tier[c] = tier_of(impl->chroms[c].lds_bytes);
+  for (uint64_t c = 0; c < nch; ++c)
+    tier[c] = tier_of(impl->chroms[c].lds_bytes,
+                      hybrid_lds_bytes(impl->chroms[c].nlefs, impl->chroms[c].nbar, P.hist_len));
   std::vector<uint32_t> order(items.size());
   for (uint32_t i = 0; i < order.size(); ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
@@ -1469,14 +1532,16 @@ int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& dchroms,
     if (tier[ca] != tier[cb]) return tier[ca] < tier[cb];
     return dchroms[ca].nlefs > dchroms[cb].nlefs;
   });
-  for (uint32_t k = 0; k < 4; ++k) {
+  for (uint32_t k = 0; k < kTiers; ++k) {
     Tier t = tier_shape(k);
     uint64_t bytes = 0;
     for (uint64_t i = 0; i < order.size(); ++i) {
       const uint32_t c = items[order[i]].chrom;
       if (tier[c] != k) continue;
       if (t.count++ == 0) t.begin = i;
-      bytes = std::max<uint64_t>(bytes, impl->chroms[c].lds_bytes);
+      const FastChrom& f = impl->chroms[c];
+      bytes = std::max<uint64_t>(bytes, k == 4 ? hybrid_lds_bytes(f.nlefs, f.nbar, P.hist_len)
+                                               : f.lds_bytes);
     }
     t.shmem = k == 0 ? 0 : bytes;
     if (t.count) impl->tiers.push_back(t);
@@ -1682,8 +1747,8 @@ int fast_job_set_tier_threads(FastJob* job, uint32_t tier, uint32_t threads) {
 }
 
 uint32_t fast_job_grid(const FastJob* job) {
-  uint32_t g[4] = {};
-  const uint32_t n = fast_job_tiers(job, nullptr, g, nullptr, 4);
+  uint32_t g[5] = {};
+  const uint32_t n = fast_job_tiers(job, nullptr, g, nullptr, 5);
   uint32_t m = 0;
   for (uint32_t k = 0; k < n; ++k) m = std::max(m, g[k]);
   return m;

@@ -23,7 +23,7 @@ struct mgpu_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipEvent_t lev[5] = {};  // around each launch of a run (fast mode launches by size tier)
+  hipEvent_t lev[6] = {};  // around each launch of a run (fast mode launches by size tier)
   std::string err;
   int num_cus = 0;
   int phase_timers = 0;
@@ -806,11 +806,11 @@ int mgpu_job_run(mgpu_job* job, mgpu_stats* stats) {
     HIPCHK(ctx, hipGraphLaunch(job->graph_exec, ctx->stream));
   } else {
     nlaunch = job->mode == MGPU_MODE_FAST
-                  ? fast_job_tiers(&job->fast, nullptr, nullptr, nullptr, 4)
+                  ? fast_job_tiers(&job->fast, nullptr, nullptr, nullptr, 5)
                   : 1;
     HIPCHK(ctx, enqueue_clears(job, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    HIPCHK(ctx, enqueue_simulation(job, ctx->stream, ctx->lev, 5));
+    HIPCHK(ctx, enqueue_simulation(job, ctx->stream, ctx->lev, 6));
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -848,8 +848,8 @@ int mgpu_job_set_tuning(mgpu_job* job, const mgpu_tuning* t) {
   if (fast_job_set_threads(&job->fast, t->threads_per_cell) != 0)
     return fail(job->ctx, MGPU_EINVAL, "%s", fast_last_error());
   if (t->threads_per_cell == 0) {
-    const uint32_t n = fast_job_tiers(&job->fast, nullptr, nullptr, nullptr, 4);
-    for (uint32_t k = 0; k < n && k < 4; ++k)
+    const uint32_t n = fast_job_tiers(&job->fast, nullptr, nullptr, nullptr, 5);
+    for (uint32_t k = 0; k < n && k < 5; ++k)
       if (t->tier_threads[k] && fast_job_set_tier_threads(&job->fast, k, t->tier_threads[k]) != 0)
         return fail(job->ctx, MGPU_EINVAL, "%s", fast_last_error());
   }
@@ -879,7 +879,7 @@ int mgpu_job_get_tuning(mgpu_job* job, mgpu_tuning* t) {
     t->threads_per_cell = fast_job_threads(&job->fast);
     t->grid = fast_job_grid(&job->fast);
     t->global_working_set = (uint32_t)fast_job_global_storage(&job->fast);
-    t->launches = fast_job_tiers(&job->fast, t->tier_threads, t->tier_grid, nullptr, 4);
+    t->launches = fast_job_tiers(&job->fast, t->tier_threads, t->tier_grid, nullptr, 5);
   } else {
     t->threads_per_cell = 64;
     t->grid = job->grid;
@@ -906,12 +906,12 @@ int mgpu_job_launch_stats(mgpu_job* job, mgpu_launch_stats* out, uint32_t max, u
   if (!job || !n) return fail(job ? job->ctx : nullptr, MGPU_EINVAL, "null argument");
   mgpu_ctx* ctx = job->ctx;
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  uint32_t threads[4] = {64, 0, 0, 0}, grid[4] = {job->grid, 0, 0, 0};
-  uint64_t cells[4] = {job->nitems, 0, 0, 0}, begin[4] = {0, 0, 0, 0};
+  uint32_t threads[5] = {64, 0, 0, 0, 0}, grid[5] = {job->grid, 0, 0, 0, 0};
+  uint64_t cells[5] = {job->nitems, 0, 0, 0, 0}, begin[5] = {0, 0, 0, 0, 0};
   uint32_t k = 1;
   const uint32_t* order = nullptr;
   if (job->mode == MGPU_MODE_FAST) {
-    k = fast_job_tiers(&job->fast, threads, grid, cells, 4, begin);
+    k = fast_job_tiers(&job->fast, threads, grid, cells, 5, begin);
     order = fast_job_order(&job->fast);
   }
   *n = k;

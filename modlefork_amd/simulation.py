@@ -183,7 +183,7 @@ class Job:
         otherwise one launch of that size over every cell."""
         t = self.tuning()
         t.threads_per_cell = threads_per_cell
-        for k in range(4):
+        for k in range(5):
             t.tier_threads[k] = int(tier_threads[k]) if tier_threads and k < len(tier_threads) else 0
         if global_working_set is not None:
             t.global_working_set = 1 if global_working_set else 0
@@ -217,10 +217,10 @@ class Job:
 
     def launch_stats(self):
         """[MgpuLaunchStats] of the last uncaptured run, one per launch."""
-        out = (L.MgpuLaunchStats * 4)()
+        out = (L.MgpuLaunchStats * 5)()
         n = C.c_uint32()
-        self.ctx.check(L.lib().mgpu_job_launch_stats(self.ptr, out, 4, C.byref(n)))
-        return [out[k] for k in range(min(n.value, 4))]
+        self.ctx.check(L.lib().mgpu_job_launch_stats(self.ptr, out, 5, C.byref(n)))
+        return [out[k] for k in range(min(n.value, 5))]
 
     def occupancy(self, c: int) -> np.ndarray:
         """Chromosome c's 1D LEF occupancy of the last run (u64 per bin); the job must have been

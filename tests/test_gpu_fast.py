@@ -175,11 +175,12 @@ def test_captured_graph_replays_match_eager_runs(ctx, mode):
 
 
 def tier_workload(ncells=4):
-    """Four chromosomes at 40 LEFs/Mbp whose working sets need device memory (5200 LEFs) or let
-    1 (4000), 2 (2000) and 4 (800) of them share one CU's LDS: one of each launch tier."""
+    """Five chromosomes at 40 LEFs/Mbp whose working sets need device memory wholly (13 600
+    LEFs) or for their per-epoch temporaries only (5 200), or let 1 (4000), 2 (2000) and 4 (800)
+    of them share one CU's LDS: one of each launch tier."""
     rng = np.random.default_rng(5)
-    chroms = [("chrA", 130_000_000), ("chrB", 100_000_000), ("chrC", 50_000_000),
-              ("chrD", 20_000_000)]
+    chroms = [("chrE", 340_000_000), ("chrA", 130_000_000), ("chrB", 100_000_000),
+              ("chrC", 50_000_000), ("chrD", 20_000_000)]
     bars = []
     for name, size in chroms:
         bars += W._uniform_barriers(rng, name, size, 100, lambda n: rng.uniform(0.5, 0.95, n))
@@ -190,15 +191,16 @@ def tier_workload(ncells=4):
 
 def test_size_tiers_give_the_single_launch_band(ctx):
     """The default run makes one launch per size tier (device memory, then 1, 2 and 4 cells per
-    CU); one launch of every cell from device memory gives the same bands and statistics, and so
-    do replays of the tiered run captured into a graph."""
+    CU, then the hybrid of LDS and device memory); one launch of every cell from device memory
+    gives the same bands and statistics, and so do replays of the tiered run captured into a
+    graph."""
     wl = tier_workload()
     sim = S.Simulation(wl.options, ctx)
     names, chroms, bars, plans = sim.prepare(wl)
     job = S.Job(ctx, sim.params, chroms, bars, plans, L.MODE_FAST)
     launches = job.launches()
     say(f"tier launches (threads, grid): {launches}")
-    assert [t for t, _ in launches] == [512, 1024, 512, 256]
+    assert [t for t, _ in launches] == [512, 1024, 512, 256, 1024]
     st = job.run()
     tiered = [job.band(i) for i in range(len(names))]
     job.set_tuning(512)
