@@ -211,7 +211,7 @@ class MgpuSgConfig(C.Structure):
 
 FP = C.POINTER(C.c_float)
 
-PHASE_NAMES = ["stop test and burn-in", "bind", "rank rev", "rank fwd", "contacts", "moves",
+PHASE_NAMES = ["stop test and burn-in", "bind", "rank", "rank fwd (batched)", "contacts", "moves",
                "adjust", "barrier states and ends", "LEF-BAR", "primary LEF-LEF",
                "move corrections", "secondary rev", "secondary fwd", "bypass fix-ups",
                "extrude and release", "unused"]

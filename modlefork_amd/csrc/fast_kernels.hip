@@ -24,6 +24,9 @@ struct OpMinI {
 struct OpAddI {
   __device__ int32_t operator()(int32_t a, int32_t b) const { return a + b; }
 };
+struct OpAddU {
+  __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a + b; }
+};
 struct OpMaxU {
   __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a > b ? a : b; }
 };
@@ -319,27 +322,26 @@ __device__ void merge_kept_new(const uint32_t* kept, uint32_t nk, const uint64_t
   }
 }
 
-// Re-ranks one unit side. ranks[0, nact_old) hold last epoch's order; LEFs flagged in is_new
-// were (re)bound this epoch and are merged in; tmp_a / tmp_b are n-sized scratch arrays.
-template <int NT>
-__device__ void rerank(uint16_t* ranks, const uint32_t* pos, uint32_t nact, uint32_t nact_old,
-                       const uint32_t* is_new, uint32_t* tmp_a, uint32_t* tmp_b, uint64_t* keys,
-                       Shared& S) {
-  // kept order: last epoch's entries that were not rebound, compacted stably
-  {
-    const uint32_t c = (nact_old + NT - 1) / NT;
-    const uint32_t b = threadIdx.x * c, e = min(nact_old, b + c);
-    int32_t cnt = 0;
-    for (uint32_t i = b; i < e; ++i) cnt += is_new[ranks[i]] == 0;
-    int32_t off = block_exclusive<NT>(cnt, OpAddI{}, 0, S.shi);
-    for (uint32_t i = b; i < e; ++i) {
-      const uint32_t l = ranks[i];
-      if (is_new[l] == 0) tmp_a[off++] = l;
-    }
-    if (threadIdx.x == NT - 1) S.nkept = (uint32_t)off;
-    __syncthreads();
+static_assert(kFastPhases == kFastPhaseCount, "phase count of the timers and of the host API");
+__device__ unsigned long long g_fast_phase[kFastPhases];
+
+// Phase timers: called by every lane right after the barrier that ends a phase; lane 0 adds the
+// wall-clock ticks since the previous mark to the phase's total.
+__device__ inline void mark(const FastParams& P, Shared& S, int ph) {
+  if (P.timers && threadIdx.x == 0) {
+    const uint64_t t = wall_clock64();
+    S.phase[ph] += t - S.tick;
+    S.tick = t;
   }
-  uint32_t nkept = S.nkept;
+}
+
+// Merges the LEFs flagged in is_new into one unit side's order: tmp_a[0, nkept) holds the kept
+// entries of last epoch's order (compacted stably); the result goes to ranks[0, nact). More
+// than kNewCap new LEFs are merged in batches through tmp_b (n-sized scratch).
+template <int NT>
+__device__ void merge_new(uint16_t* ranks, const uint32_t* pos, uint32_t nact, uint32_t nkept,
+                          const uint32_t* is_new, uint32_t* tmp_a, uint32_t* tmp_b,
+                          uint64_t* keys, Shared& S) {
   // Extrusion keeps neighbours ordered almost always, but the merge below needs the kept order
   // sorted exactly.
   odd_even_sort<NT>(tmp_a, nkept, pos);
@@ -381,17 +383,72 @@ __device__ void rerank(uint16_t* ranks, const uint32_t* pos, uint32_t nact, uint
   }
 }
 
-static_assert(kFastPhases == kFastPhaseCount, "phase count of the timers and of the host API");
-__device__ unsigned long long g_fast_phase[kFastPhases];
-
-// Phase timers: called by every lane right after the barrier that ends a phase; lane 0 adds the
-// wall-clock ticks since the previous mark to the phase's total.
-__device__ inline void mark(const FastParams& P, Shared& S, int ph) {
-  if (P.timers && threadIdx.x == 0) {
-    const uint64_t t = wall_clock64();
-    S.phase[ph] += t - S.tick;
-    S.tick = t;
+// Re-ranks both unit sides (rank_lefs for rev and fwd units). ranks[0, nact_old) hold last
+// epoch's orders; LEFs flagged in c.rm were (re)bound this epoch. Both kept orders are
+// compacted in one pass (their counts packed into one scan). A LEF bound this epoch sits at one
+// position on both sides (rev == fwd), so the new LEFs have the same keys on both sides: they
+// are collected and sorted once and merged into both orders in one pass. Epochs that bind more
+// than kNewCap LEFs (the first ones) merge each side in batches instead.
+template <int NT>
+__device__ void rerank_both(Cell& c, uint32_t nact, uint32_t nact_old, uint64_t* keys,
+                            Shared& S, const FastParams& P) {
+  const uint32_t* is_new = c.rm;
+  uint32_t* kr = c.rc;
+  uint32_t* kf = c.fc;
+  {
+    const uint32_t ch = (nact_old + NT - 1) / NT;
+    const uint32_t b = threadIdx.x * ch, e = min(nact_old, b + ch);
+    // kept counts of this lane's chunk: rev side in the low 16 bits, fwd side in the high 16
+    // (at most 65535 LEFs per cell, so the low half cannot carry into the high one)
+    uint32_t cnt = 0;
+    for (uint32_t i = b; i < e; ++i)
+      cnt += (is_new[c.rr[i]] == 0 ? 1u : 0u) + (is_new[c.fr[i]] == 0 ? 0x10000u : 0u);
+    const uint32_t off = block_exclusive<NT>(cnt, OpAddU{}, 0u, S.shu);
+    uint32_t orv = off & 0xFFFFu, ofw = off >> 16;
+    for (uint32_t i = b; i < e; ++i) {
+      const uint32_t lr = c.rr[i], lf = c.fr[i];
+      if (is_new[lr] == 0) kr[orv++] = lr;
+      if (is_new[lf] == 0) kf[ofw++] = lf;
+    }
+    if (threadIdx.x == NT - 1) S.nkept = orv;  // the same LEFs are kept on both sides
+    if (threadIdx.x == 0) S.nnew = 0;
+    __syncthreads();
   }
+  const uint32_t nkept = S.nkept;
+  if (nact - nkept > kNewCap) {
+    merge_new<NT>(c.rr, c.rev, nact, nkept, is_new, kr, c.fm, keys, S);
+    mark(P, S, 2);
+    merge_new<NT>(c.fr, c.fwd, nact, nkept, is_new, kf, c.fm, keys, S);
+    mark(P, S, 3);
+    return;
+  }
+  // both kept orders must be sorted exactly for the merge; one check pass covers both
+  int bad = 0;
+  for (uint32_t i = threadIdx.x; i + 1 < nkept; i += NT)
+    bad |= (pos_key(c.rev, kr[i + 1]) < pos_key(c.rev, kr[i])) |
+           (pos_key(c.fwd, kf[i + 1]) < pos_key(c.fwd, kf[i]));
+  for (uint32_t l = threadIdx.x; l < nact; l += NT) {
+    if (is_new[l]) keys[atomicAdd(&S.nnew, 1u)] = pos_key(c.rev, l);
+  }
+  if (__syncthreads_or(bad)) {
+    odd_even_sort<NT>(kr, nkept, c.rev);
+    odd_even_sort<NT>(kf, nkept, c.fwd);
+  }
+  const uint32_t k = S.nnew;
+  if (k == 0) {
+    for (uint32_t i = threadIdx.x; i < nact; i += NT) {
+      c.rr[i] = (uint16_t)kr[i];
+      c.fr[i] = (uint16_t)kf[i];
+    }
+  } else {
+    sort_new_keys<NT>(keys, k);
+    merge_kept_new<NT>(kr, nkept, keys, k, c.rev,
+                       [&](uint32_t i, uint32_t lef) { c.rr[i] = (uint16_t)lef; });
+    merge_kept_new<NT>(kf, nkept, keys, k, c.fwd,
+                       [&](uint32_t i, uint32_t lef) { c.fr[i] = (uint16_t)lef; });
+  }
+  __syncthreads();
+  mark(P, S, 2);
 }
 
 __device__ inline float gev_noise(const FastParams& P, uint32_t x) {
@@ -805,10 +862,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     mark(P, S, 1);
     // ---- ranks (rank_lefs, simulation.cpp:402-487) ----
     uint64_t* keys = reinterpret_cast<uint64_t*>(c.scratch);
-    rerank<NT>(c.rr, c.rev, n, nact_prev, c.rm, c.rc, c.fc, keys, S);
-    mark(P, S, 2);
-    rerank<NT>(c.fr, c.fwd, n, nact_prev, c.rm, c.rc, c.fc, keys, S);
-    mark(P, S, 3);
+    rerank_both<NT>(c, n, nact_prev, keys, S, P);
     nact_prev = n;
 
     // ---- contacts (sample_and_register_contacts, register_contacts.cpp:94-198) ----
