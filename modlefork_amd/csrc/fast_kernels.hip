@@ -51,21 +51,38 @@ __device__ T block_exclusive(T v, Op op, T ident, T* sh) {
   return op(pre, ex);
 }
 
-// In-place inclusive scan of a[0, n) (reverse: from index n-1 down to 0). Each lane owns a
-// contiguous chunk of the scan order.
-template <int NT, class T, class Op>
-__device__ void block_scan(T* a, int n, Op op, T ident, bool reverse, T* sh) {
-  const int c = (n + NT - 1) / NT;
-  const int b = threadIdx.x * c, e = min(n, b + c);
-  T acc = ident;
-  for (int i = b; i < e; ++i) acc = op(acc, a[reverse ? n - 1 - i : i]);
-  acc = block_exclusive<NT>(acc, op, ident, sh);
-  for (int i = b; i < e; ++i) {
-    const int k = reverse ? n - 1 - i : i;
-    acc = op(acc, a[k]);
-    a[k] = acc;
+// Exclusive scans of two values per lane across the block (a with opa, b with opb) sharing one
+// pair of barriers; sh holds 2 * NT / 64 entries.
+template <int NT, class T, class OpA, class OpB>
+__device__ void block_exclusive2(T& a, T& b, OpA opa, OpB opb, T ia, T ib, T* sh) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  T xa = a, xb = b;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const T ya = __shfl_up(xa, d, 64), yb = __shfl_up(xb, d, 64);
+    if (lane >= d) {
+      xa = opa(xa, ya);
+      xb = opb(xb, yb);
+    }
+  }
+  if (lane == 63) {
+    sh[w] = xa;
+    sh[NT / 64 + w] = xb;
   }
   __syncthreads();
+  T pa = ia, pb = ib;
+  for (int k = 0; k < w; ++k) {
+    pa = opa(pa, sh[k]);
+    pb = opb(pb, sh[NT / 64 + k]);
+  }
+  T ea = __shfl_up(xa, 1, 64), eb = __shfl_up(xb, 1, 64);
+  if (lane == 0) {
+    ea = ia;
+    eb = ib;
+  }
+  __syncthreads();
+  a = opa(pa, ea);
+  b = opb(pb, eb);
 }
 
 // Sum over the block: a butterfly within each wave, then every wave reduces the NT / 64 wave
@@ -945,36 +962,43 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     }
     __syncthreads();
     mark(P, S, 5);
-    // adjust (simulation.cpp:342-399): rev targets must fall strictly along the rev order,
-    // fwd targets must rise strictly along the fwd order; then clamp to the chromosome.
+    // adjust (simulation.cpp:342-399): rev targets must fall strictly along the rev order and
+    // fwd targets rise strictly along the fwd order: T[k] = target(rr[k]) + k takes a reverse
+    // prefix min, Tf[k] = target(fr[k]) - k a prefix max, and the targets are then clamped to
+    // the chromosome. Both sides share one scan; each lane scans a contiguous run of ranks (from
+    // the top for rev) and applies the prefix to the moves as it goes.
     {
       int32_t* T = reinterpret_cast<int32_t*>(c.scratch);
+      int32_t* Tf = reinterpret_cast<int32_t*>(c.fc);  // collisions are cleared below
       for (uint32_t k = threadIdx.x; k < n; k += NT) {
-        const uint32_t l = c.rr[k];
+        const uint32_t l = c.rr[k], l2 = c.fr[k];
         T[k] = (int32_t)c.rev[l] - (int32_t)c.rm[l] + (int32_t)k;
+        Tf[k] = (int32_t)c.fwd[l2] + (int32_t)c.fm[l2] - (int32_t)k;
       }
       __syncthreads();
-      block_scan<NT>(T, (int)n, OpMinI{}, INT32_MAX, true, S.shi);
-      for (uint32_t k = threadIdx.x; k < n; k += NT) {
+      const uint32_t chunk = (n + NT - 1) / NT;
+      const uint32_t ib = threadIdx.x * chunk, ie = min(n, ib + chunk);
+      int32_t ar = INT32_MAX, af = INT32_MIN;
+      for (uint32_t i = ib; i < ie; ++i) {
+        ar = min(ar, T[n - 1 - i]);
+        af = max(af, Tf[i]);
+      }
+      block_exclusive2<NT>(ar, af, OpMinI{}, OpMaxI{}, INT32_MAX, INT32_MIN,
+                           reinterpret_cast<int32_t*>(S.shx));
+      for (uint32_t i = ib; i < ie; ++i) {
+        const uint32_t k = n - 1 - i;
+        ar = min(ar, T[k]);
         const uint32_t l = c.rr[k];
-        int32_t t = T[k] - (int32_t)k;
+        int32_t t = ar - (int32_t)k;
         if (t < (int32_t)ch.start) t = (int32_t)ch.start;
         c.rm[l] = c.rev[l] - (uint32_t)t;
-      }
-      __syncthreads();
-      for (uint32_t k = threadIdx.x; k < n; k += NT) {
-        const uint32_t l = c.fr[k];
-        T[k] = (int32_t)c.fwd[l] + (int32_t)c.fm[l] - (int32_t)k;
-      }
-      __syncthreads();
-      block_scan<NT>(T, (int)n, OpMaxI{}, INT32_MIN, false, S.shi);
-      for (uint32_t k = threadIdx.x; k < n; k += NT) {
-        const uint32_t l = c.fr[k];
-        int32_t t = T[k] + (int32_t)k;
-        if (t > (int32_t)(ch.end - 1)) t = (int32_t)(ch.end - 1);
-        c.fm[l] = (uint32_t)t - c.fwd[l];
         c.rc[k] = 0;
-        c.fc[k] = 0;
+        af = max(af, Tf[i]);
+        const uint32_t l2 = c.fr[i];
+        int32_t t2 = af + (int32_t)i;
+        if (t2 > (int32_t)(ch.end - 1)) t2 = (int32_t)(ch.end - 1);
+        c.fm[l2] = (uint32_t)t2 - c.fwd[l2];
+        c.fc[i] = 0;  // Tf[i] was last read just above, by this lane
       }
       __syncthreads();
     }
@@ -987,7 +1011,6 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
       if (st == 0 && d.x >= ch.bar_t_inactive[b]) c.bst[b] = 1;
       else if (st == 1 && d.x >= ch.bar_t_active[b]) c.bst[b] = 0;
     }
-
     // ---- units at the chromosome ends (simulation_detect_collisions.cpp:30-125) ----
     if (threadIdx.x == 0) {
       const uint32_t first_fwd = c.fwd[c.fr[0]];
@@ -1023,20 +1046,22 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
       S.n5 = n5;
       S.n3 = n3;
     }
+    // fwd positions in rank order for the first LEF-BAR pass, filled beside lane 0's walk
+    uint32_t* sorted = c.scratch;
+    for (uint32_t k = threadIdx.x; k < n; k += NT) sorted[k] = c.fwd[c.fr[k]];
     __syncthreads();
     mark(P, S, 7);
     const uint32_t n5 = S.n5, n3 = S.n3;
 
     // ---- LEF-BAR (simulation_detect_collisions.cpp:128-252): per active barrier, the nearest
     // unit moving towards it; the closest blocking barrier wins ----
-    uint32_t* sorted = c.scratch;
     // fwd side first, so the rev positions in rank order stay in `sorted` for the primary step
     for (int it = 0; it < 2; ++it) {
       const int side = 1 - it;
-      const uint16_t* ranks = side == 0 ? c.rr : c.fr;
-      const uint32_t* pos = side == 0 ? c.rev : c.fwd;
-      for (uint32_t k = threadIdx.x; k < n; k += NT) sorted[k] = pos[ranks[k]];
-      __syncthreads();
+      if (side == 0) {  // the fwd pass ended with a barrier, so `sorted` is free
+        for (uint32_t k = threadIdx.x; k < n; k += NT) sorted[k] = c.rev[c.rr[k]];
+        __syncthreads();
+      }
       const uint32_t rev_lo = n5 ? n5 - 1 : 0;
       const uint32_t fwd_top = n - (n3 ? n3 - 1 : 0);
       for (uint32_t b = threadIdx.x; b < NB; b += NT) {
