@@ -321,8 +321,8 @@ int mgpu_debug_philox(mgpu_ctx* ctx, const uint32_t* in, uint64_t n, uint32_t* o
  * out (n entries, may be null) receives the totals, which are then cleared. Phases: 0 stop test
  * and burn-in, 1 bind, 2 rank (both sides; the rev side alone in epochs that bind more than
  * 1024 LEFs), 3 rank fwd in those epochs, 4 contacts, 5 moves, 6 adjust, 7 barrier states
- * and chromosome ends, 8 LEF-BAR, 9 primary LEF-LEF, 10 move corrections, 11-12 secondary
- * rev/fwd scans, 13 bypass fix-ups of both sides, 14 extrude and release (15 unused). */
+ * and chromosome ends, 8 LEF-BAR, 9 primary LEF-LEF, 10 move corrections, 11 secondary scans
+ * of both sides, 13 bypass fix-ups of both sides, 14 extrude and release (12 and 15 unused). */
 int mgpu_debug_phase_times(mgpu_ctx* ctx, int enable, uint64_t* out, uint64_t n);
 /* Fast mode's bypass fix-ups (fix_secondary_lef_lef_collisions,
  * simulation_detect_collisions.cpp:520-647) run in parallel over runs of consecutive ranks;

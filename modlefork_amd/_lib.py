@@ -213,7 +213,7 @@ FP = C.POINTER(C.c_float)
 
 PHASE_NAMES = ["stop test and burn-in", "bind", "rank", "rank fwd (batched)", "contacts", "moves",
                "adjust", "barrier states and ends", "LEF-BAR", "primary LEF-LEF",
-               "move corrections", "secondary rev", "secondary fwd", "bypass fix-ups",
+               "move corrections", "secondary (both sides)", "unused", "bypass fix-ups",
                "extrude and release", "unused"]
 
 # (name, restype, argtypes) for every entry point of include/modle_gpu.h

@@ -229,7 +229,8 @@ struct Shared {
   uint64_t contacts;
   uint64_t burnin_epochs;
   uint64_t shx[3 * kFastMaxWaves];
-  uint32_t nact, hlen, hhead, burnin_done, done, nnew, nkept, n5, n3, stable, navoid;
+  uint32_t nact, hlen, hhead, burnin_done, done, nnew, nkept, n5, n3, stable, navoid,
+      navoid_fwd;
 };
 
 // bitonic sort of keys[0, m) ascending, m a power of two
@@ -504,99 +505,133 @@ __device__ inline uint32_t poisson(const Rng& R, double mean, uint32_t tag) {
   return k;
 }
 
-// One side of the secondary LEF-LEF pass. Chain index j runs along the direction in which
-// stalls propagate (rev: rank order; fwd: reverse rank order). Seeds are units already stalled;
-// unit j behind seed s stalls when every unit in (s, j] reaches its predecessor's target and
-// keeps its collision (does not bypass). X[j] packs (s + 1) << 16 | (last failure + 1), so a
-// single prefix-max yields the last failure inside j's own chain.
-// Each lane owns a contiguous run of j; the two prefix-max scans are fused with the passes
-// around them: seed flags are formed in the first scan's reduction, failure flags while its
-// prefix is applied, and the moves are corrected while the second scan's prefix is applied.
-// Returns the number of units that bypassed, listed (chain order) at the start of c.scratch.
+// The secondary LEF-LEF pass of both sides at once (a side's pass reads and writes only that
+// side's ranks, positions, moves and collisions). Chain index j runs along the direction in
+// which stalls propagate (rev: rank order; fwd: reverse rank order). Seeds are units already
+// stalled; unit j behind seed s stalls when every unit in (s, j] reaches its predecessor's
+// target and keeps its collision (does not bypass). w(j) = (s + 1) << 16 | (last failure + 1)
+// packs both, so a single prefix max yields the last failure inside j's own chain.
+// Each lane owns the same contiguous run of j on both sides, and the two sides share every
+// scan's barriers. Seeds are re-read where needed (a unit is rewritten only at its own j, after
+// its seed test, and never when it is a seed) and only the 16-bit failure flags are stored, so
+// both sides fit in scratch (F: 2 x n u16). Returns the bypassing units' ranks in F[0, na[0])
+// (rev, chain order) and F[n, n + na[1]) (fwd, chain order).
 template <int NT>
-__device__ uint32_t secondary_side(const FastParams& P, const Rng& R, uint32_t epoch, Cell& c,
-                                   Shared& S, uint32_t n, bool rev_side, uint32_t lo_j) {
-  uint16_t* ranks = rev_side ? c.rr : c.fr;
-  uint32_t* coll = rev_side ? c.rc : c.fc;
-  uint32_t* mv = rev_side ? c.rm : c.fm;
-  const uint32_t* pos = rev_side ? c.rev : c.fwd;
-  uint32_t* X = c.scratch;
-  auto unit_of = [&](uint32_t j) -> uint32_t { return ranks[rev_side ? j : n - 1 - j]; };
-  auto target = [&](uint32_t l) -> int32_t {
-    return rev_side ? (int32_t)pos[l] - (int32_t)mv[l] : (int32_t)pos[l] + (int32_t)mv[l];
-  };
+__device__ void secondary_both(const FastParams& P, const Rng& R, uint32_t epoch, Cell& c,
+                               Shared& S, uint32_t n, uint32_t lo_rev, uint32_t lo_fwd,
+                               uint16_t* F, uint32_t* na) {
   const uint32_t chunk = (n + NT - 1) / NT;
   const uint32_t jb = threadIdx.x * chunk, je = min(n, jb + chunk);
-  // seeds: X[j] = j + 1 where unit j is already stalled
-  uint32_t m = 0;
-  for (uint32_t j = jb; j < je; ++j) {
-    const uint32_t v = ck_stalled(coll[unit_of(j)]) ? j + 1 : 0;
-    X[j] = v;
-    m = v > m ? v : m;
+  auto unit = [&](int sd, uint32_t j) -> uint32_t { return sd == 0 ? c.rr[j] : c.fr[n - 1 - j]; };
+  auto coll = [&](int sd) -> uint32_t* { return sd == 0 ? c.rc : c.fc; };
+  auto target = [&](int sd, uint32_t l) -> int32_t {
+    return sd == 0 ? (int32_t)c.rev[l] - (int32_t)c.rm[l] : (int32_t)c.fwd[l] + (int32_t)c.fm[l];
+  };
+  auto seed = [&](int sd, uint32_t j) -> uint32_t {
+    return ck_stalled(coll(sd)[unit(sd, j)]) ? j + 1 : 0u;
+  };
+  // the last seed of this lane's run, per side
+  uint32_t e0 = 0, e1 = 0;
+  for (uint32_t j = je; j-- > jb;) {
+    if (e0 == 0) e0 = seed(0, j);
+    if (e1 == 0) e1 = seed(1, j);
+    if (e0 && e1) break;
   }
-  uint32_t run = block_exclusive<NT>(m, OpMaxU{}, 0u, S.shu);
-  // chain start s1 (prefix max of the seeds) and the failure flag of each j, packed
-  uint32_t mf = 0;
+  uint32_t* shu2 = reinterpret_cast<uint32_t*>(S.shx);
+  block_exclusive2<NT>(e0, e1, OpMaxU{}, OpMaxU{}, 0u, 0u, shu2);
+  const uint32_t s0 = e0, s1p = e1;  // seed prefixes before this run, kept for the last pass
+  // failure flags; the packed w's prefix max over this run
+  uint32_t m0 = 0, m1 = 0;
   for (uint32_t j = jb; j < je; ++j) {
-    run = X[j] > run ? X[j] : run;
-    const uint32_t s1 = run;  // seed + 1, 0 if none
-    uint32_t fail = 0;
-    if (s1 != 0 && s1 - 1 != j) {
-      bool ok = j >= lo_j;
-      if (ok) {
-        const uint32_t l = unit_of(j);
-        const int32_t ts = target(unit_of(s1 - 1));
-        const int32_t dist = (int32_t)j - (int32_t)(s1 - 1);
-        const bool reach = rev_side ? target(l) <= ts + dist - 1 : target(l) >= ts - dist + 1;
-        // the bypass trial only for units that reach the one ahead of them
-        ok = reach && bern(R.draw(l, epoch, rev_side ? kSSecondary : kSSecondary + 16).x,
-                           P.t_bypass_keep);
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd) {
+      uint32_t& run = sd == 0 ? e0 : e1;
+      const uint32_t sj = seed(sd, j);
+      run = sj > run ? sj : run;
+      const uint32_t s1 = run;  // seed + 1, 0 if none
+      uint32_t fail = 0;
+      if (s1 != 0 && s1 - 1 != j) {
+        bool ok = j >= (sd == 0 ? lo_rev : lo_fwd);
+        if (ok) {
+          const uint32_t l = unit(sd, j);
+          const int32_t ts = target(sd, unit(sd, s1 - 1));
+          const int32_t dist = (int32_t)j - (int32_t)(s1 - 1);
+          const bool reach = sd == 0 ? target(0, l) <= ts + dist - 1 : target(1, l) >= ts - dist + 1;
+          // the bypass trial only for units that reach the one ahead of them
+          ok = reach && bern(R.draw(l, epoch, sd == 0 ? kSSecondary : kSSecondary + 16).x,
+                             P.t_bypass_keep);
+        }
+        fail = ok ? 0 : j + 1;
       }
-      fail = ok ? 0 : j + 1;
-    }
-    const uint32_t w = (s1 << 16) | fail;
-    X[j] = w;
-    mf = w > mf ? w : mf;
-  }
-  uint32_t run2 = block_exclusive<NT>(mf, OpMaxU{}, 0u, S.shu);
-  // corrections: X[j] becomes the prefix max; prev is the prefix max at j - 1
-  int32_t navoid = 0;
-  for (uint32_t j = jb; j < je; ++j) {
-    const uint32_t prev = run2;
-    run2 = X[j] > run2 ? X[j] : run2;
-    const uint32_t s1 = run2 >> 16;
-    if (s1 == 0 || s1 - 1 == j || j < lo_j) continue;
-    const uint32_t l = unit_of(j);
-    if (ck_stalled(coll[l])) continue;
-    const uint32_t s = s1 - 1;
-    const uint32_t lead = unit_of(j - 1);
-    const int32_t dist = (int32_t)j - (int32_t)s;
-    const int32_t ts = target(unit_of(s));
-    const uint32_t last_fail = run2 & 0xFFFFu;
-    if (last_fail == 0) {
-      // stalled one bp behind the unit ahead of it in the chain
-      const int32_t t = rev_side ? ts + dist : ts - dist;
-      const int32_t mm = rev_side ? (int32_t)pos[l] - t : t - (int32_t)pos[l];
-      mv[l] = mm > 0 ? (uint32_t)mm : 0u;
-      coll[l] = ck(kCkSecondary, lead);
-    } else if (last_fail == j + 1 && (prev & 0xFFFFu) == 0) {
-      const bool reach = rev_side ? target(l) <= ts + dist - 1 : target(l) >= ts - dist + 1;
-      if (reach) {
-        coll[l] = ck(kCkAvoided, lead);
-        ++navoid;
-      }
+      F[sd * n + j] = (uint16_t)fail;
+      const uint32_t w = (s1 << 16) | fail;
+      uint32_t& m = sd == 0 ? m0 : m1;
+      m = w > m ? w : m;
     }
   }
-  // bypassing units, compacted in chain order (the exclusive scan's barriers also end every
-  // lane's reads of X, which the list overwrites)
-  int32_t off = block_exclusive<NT>(navoid, OpAddI{}, 0, S.shi);
-  if (navoid) {
+  block_exclusive2<NT>(m0, m1, OpMaxU{}, OpMaxU{}, 0u, 0u, shu2);
+  // corrections; m is the prefix max of w through j, prev through j - 1
+  e0 = s0;
+  e1 = s1p;
+  uint32_t navoid = 0;  // rev count in the low 16 bits, fwd in the high 16
+  for (uint32_t j = jb; j < je; ++j) {
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd) {
+      uint32_t& run = sd == 0 ? e0 : e1;
+      uint32_t& m = sd == 0 ? m0 : m1;
+      const uint32_t sj = seed(sd, j);
+      run = sj > run ? sj : run;
+      const uint32_t prev = m;
+      const uint32_t w = (run << 16) | F[sd * n + j];
+      m = w > m ? w : m;
+      const uint32_t s1 = m >> 16;
+      if (s1 == 0 || s1 - 1 == j || j < (sd == 0 ? lo_rev : lo_fwd)) continue;
+      const uint32_t l = unit(sd, j);
+      uint32_t* cl = coll(sd);
+      if (ck_stalled(cl[l])) continue;
+      const uint32_t s = s1 - 1;
+      const uint32_t lead = unit(sd, j - 1);
+      const int32_t dist = (int32_t)j - (int32_t)s;
+      const int32_t ts = target(sd, unit(sd, s));
+      const uint32_t last_fail = m & 0xFFFFu;
+      if (last_fail == 0) {
+        // stalled one bp behind the unit ahead of it in the chain
+        if (sd == 0) {
+          const int32_t mm = (int32_t)c.rev[l] - (ts + dist);
+          c.rm[l] = mm > 0 ? (uint32_t)mm : 0u;
+        } else {
+          const int32_t mm = (ts - dist) - (int32_t)c.fwd[l];
+          c.fm[l] = mm > 0 ? (uint32_t)mm : 0u;
+        }
+        cl[l] = ck(kCkSecondary, lead);
+      } else if (last_fail == j + 1 && (prev & 0xFFFFu) == 0) {
+        const bool reach = sd == 0 ? target(0, l) <= ts + dist - 1 : target(1, l) >= ts - dist + 1;
+        if (reach) {
+          cl[l] = ck(kCkAvoided, lead);
+          navoid += sd == 0 ? 1u : 0x10000u;
+        }
+      }
+    }
+  }
+  // bypassing units, compacted in chain order (the scan's barriers also end every lane's reads
+  // of F, which the lists overwrite)
+  const uint32_t off = block_exclusive<NT>(navoid, OpAddU{}, 0u, S.shu);
+  uint32_t o0 = off & 0xFFFFu, o1 = off >> 16;
+  if (navoid & 0xFFFFu) {
     for (uint32_t j = jb; j < je; ++j)
-      if (ck_kind(coll[unit_of(j)]) == kCkAvoided) X[off++] = rev_side ? j : n - 1 - j;
+      if (ck_kind(c.rc[unit(0, j)]) == kCkAvoided) F[o0++] = (uint16_t)j;
   }
-  if (threadIdx.x == NT - 1) S.navoid = (uint32_t)off;
+  if (navoid >> 16) {
+    for (uint32_t j = jb; j < je; ++j)
+      if (ck_kind(c.fc[unit(1, j)]) == kCkAvoided) F[n + o1++] = (uint16_t)(n - 1 - j);
+  }
+  if (threadIdx.x == NT - 1) {
+    S.navoid = o0;
+    S.navoid_fwd = o1;
+  }
   __syncthreads();
-  return S.navoid;
+  na[0] = S.navoid;
+  na[1] = S.navoid_fwd;
 }
 
 // fix_secondary_lef_lef_collisions (simulation_detect_collisions.cpp:520-647) for the rev unit
@@ -653,7 +688,7 @@ __device__ inline void fix_one_fwd(Cell& c, const FastChrom& ch, uint32_t n, uin
 // distinct runs (separated by at least one rank) proceed in parallel. serial != 0 walks the
 // whole list on lane 0 instead (the order of the reference loop, kept to check the runs against).
 template <int NT>
-__device__ void fix_avoided(Cell& c, const FastChrom& ch, uint32_t n, const uint32_t* list,
+__device__ void fix_avoided(Cell& c, const FastChrom& ch, uint32_t n, const uint16_t* list,
                             uint32_t cnt, bool rev_side, bool serial) {
   if (serial) {
     if (threadIdx.x == 0) {
@@ -1186,21 +1221,22 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     mark(P, S, 10);
 
     // ---- secondary LEF-LEF (simulation_detect_collisions.cpp:404-518) ----
-    // The rev fix-ups touch only rev-side state (and read fwd positions, which the fwd pass
-    // does not change), so each side is fixed right after its own pass, rev first as in
-    // process_collisions (simulation.cpp:925-955).
+    // Both sides' passes first (neither reads what the other writes), then the fix-ups, rev
+    // first as in process_collisions (simulation.cpp:925-955): the rev fix-ups read fwd
+    // positions and the fwd fix-ups rev positions, as of the fix-ups before them.
     {
-      const uint32_t na = secondary_side<NT>(P, R, epoch, c, S, n, true, max(1u, n5));
+      uint16_t* F = reinterpret_cast<uint16_t*>(c.scratch);
+      uint32_t na[2];
+      secondary_both<NT>(P, R, epoch, c, S, n, max(1u, n5), (n3 ? n3 - 1 : 0) + 1, F, na);
       mark(P, S, 11);
-      if (na) fix_avoided<NT>(c, ch, n, c.scratch, na, true, P.serial_fixups);
-      __syncthreads();
-      mark(P, S, 13);
-    }
-    {
-      const uint32_t na = secondary_side<NT>(P, R, epoch, c, S, n, false, (n3 ? n3 - 1 : 0) + 1);
-      mark(P, S, 12);
-      if (na) fix_avoided<NT>(c, ch, n, c.scratch, na, false, P.serial_fixups);
-      __syncthreads();
+      if (na[0]) {
+        fix_avoided<NT>(c, ch, n, F, na[0], true, P.serial_fixups);
+        __syncthreads();
+      }
+      if (na[1]) {
+        fix_avoided<NT>(c, ch, n, F + n, na[1], false, P.serial_fixups);
+        __syncthreads();
+      }
       mark(P, S, 13);
     }
 
