@@ -234,6 +234,12 @@ struct Shared {
 };
 
 // bitonic sort of keys[0, m) ascending, m a power of two
+// LEF-BAR records a fwd barrier collision as kCkBarFwdTmp with the index mirrored, so that
+// atomicMax keeps the closest (lowest-index) barrier; this gives the plain kCkBar word.
+__device__ inline uint32_t fc_final(uint32_t v) {
+  return ck_kind(v) == kCkBarFwdTmp ? ck(kCkBar, kCkIdxMask - ck_idx(v)) : v;
+}
+
 template <int NT>
 __device__ void bitonic(uint64_t* keys, uint32_t m) {
   for (uint32_t k = 2; k <= m; k <<= 1) {
@@ -1132,14 +1138,12 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
       }
       __syncthreads();
     }
-    for (uint32_t i = threadIdx.x; i < n; i += NT) {
-      if (ck_kind(c.fc[i]) == kCkBarFwdTmp) c.fc[i] = ck(kCkBar, kCkIdxMask - ck_idx(c.fc[i]));
-    }
 
     // ---- primary LEF-LEF (simulation_detect_collisions.cpp:255-401): a fwd unit that is the
     // last one before a rev unit pairs with that rev unit ----
-    // `sorted` holds the rev positions in rank order (the LEF-BAR pass above ended with them)
-    __syncthreads();
+    // `sorted` holds the rev positions in rank order (the LEF-BAR pass above ended with them);
+    // fwd barrier collisions still carry the LEF-BAR encoding until the move corrections rewrite
+    // them, so they are read through fc_final
     mark(P, S, 8);
     if (n5 < n && n3 < n) {
       const uint32_t i2 = n - (n3 ? n3 - 1 : 0);
@@ -1170,7 +1174,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
         if (!(delta < c.rm[rl] + c.fm[fl])) continue;
         const U4 d = R.draw(k, epoch, kSPrimary);
         if (!bern(d.x, P.t_bypass_keep)) continue;
-        const uint32_t rv = c.rc[rl], fv = c.fc[fl];
+        const uint32_t rv = c.rc[rl], fv = fc_final(c.fc[fl]);
         const bool rocc = ck_stalled(rv), focc = ck_stalled(fv);
         // collision site (compute_lef_lef_collision_pos, simulation.cpp:514-542)
         const uint32_t rel = c.rm[rl] + c.fm[fl];
@@ -1193,29 +1197,33 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     mark(P, S, 9);
 
     // ---- move corrections (simulation_correct_moves.cpp:20-122) ----
+    // One pass: LEF i's barrier corrections, then i as the rev unit and as the fwd unit of a
+    // primary pair. The reference's three passes read a partner's barrier-corrected move only
+    // as a position (fwd + corrected fwd move = barrier - 1, rev - corrected rev move =
+    // barrier + 1), taken here from the barrier itself, so no pass waits for another. Primary
+    // pairs are one-to-one (one fwd unit per rev unit), so every move is written by one lane.
     for (uint32_t i = threadIdx.x; i < n; i += NT) {
-      if (ck_kind(c.rc[i]) == kCkBar) c.rm[i] = (c.rev[i] - ch.bar_pos[ck_idx(c.rc[i])]) - 1;
-      if (ck_kind(c.fc[i]) == kCkBar) c.fm[i] = (ch.bar_pos[ck_idx(c.fc[i])] - c.fwd[i]) - 1;
-    }
-    __syncthreads();
-    for (uint32_t r = threadIdx.x; r < n; r += NT) {
-      if (ck_kind(c.rc[r]) != kCkPrimary) continue;
-      const uint32_t f = ck_idx(c.rc[r]);
-      if (ck_kind(c.fc[f]) == kCkPrimary) {
-        const uint32_t rp = c.rev[r], fp = c.fwd[f];
-        const uint32_t rel = c.rm[r] + c.fm[f];
-        const uint32_t cpos = fp + (uint32_t)rint((double)c.fm[f] * ((double)(rp - fp) / (double)rel));
-        c.rm[r] = rp - (cpos == fp ? cpos + 1 : cpos);
-        c.fm[f] = (cpos == fp ? cpos : cpos - 1) - fp;
-      } else if (ck_kind(c.fc[f]) == kCkBar) {
-        c.rm[r] = c.rev[r] - (c.fwd[f] + c.fm[f]) - 1;
+      const uint32_t rci = c.rc[i], fci = fc_final(c.fc[i]);
+      c.fc[i] = fci;
+      if (ck_kind(rci) == kCkBar) c.rm[i] = (c.rev[i] - ch.bar_pos[ck_idx(rci)]) - 1;
+      if (ck_kind(fci) == kCkBar) c.fm[i] = (ch.bar_pos[ck_idx(fci)] - c.fwd[i]) - 1;
+      if (ck_kind(rci) == kCkPrimary) {
+        const uint32_t f = ck_idx(rci);
+        const uint32_t fcf = fc_final(c.fc[f]);
+        if (ck_kind(fcf) == kCkPrimary) {
+          const uint32_t rp = c.rev[i], fp = c.fwd[f];
+          const uint32_t rel = c.rm[i] + c.fm[f];
+          const uint32_t cpos = fp + (uint32_t)rint((double)c.fm[f] * ((double)(rp - fp) / (double)rel));
+          c.rm[i] = rp - (cpos == fp ? cpos + 1 : cpos);
+          c.fm[f] = (cpos == fp ? cpos : cpos - 1) - fp;
+        } else if (ck_kind(fcf) == kCkBar) {
+          c.rm[i] = c.rev[i] - (ch.bar_pos[ck_idx(fcf)] - 1) - 1;
+        }
       }
-    }
-    __syncthreads();
-    for (uint32_t f = threadIdx.x; f < n; f += NT) {
-      if (ck_kind(c.fc[f]) != kCkPrimary) continue;
-      const uint32_t r = ck_idx(c.fc[f]);
-      if (ck_kind(c.rc[r]) == kCkBar) c.fm[f] = (c.rev[r] - c.rm[r]) - c.fwd[f] - 1;
+      if (ck_kind(fci) == kCkPrimary) {
+        const uint32_t rcr = c.rc[ck_idx(fci)];
+        if (ck_kind(rcr) == kCkBar) c.fm[i] = (ch.bar_pos[ck_idx(rcr)] + 1) - c.fwd[i] - 1;
+      }
     }
     __syncthreads();
     mark(P, S, 10);
