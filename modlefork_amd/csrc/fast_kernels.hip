@@ -226,14 +226,19 @@ struct Shared {
   uint64_t phase[kFastPhases];
   uint64_t tick;
   uint64_t item;
-  uint64_t contacts;
   uint64_t burnin_epochs;
   uint64_t shx[3 * kFastMaxWaves];
-  uint32_t nact, hlen, hhead, burnin_done, done, nnew, nkept, n5, n3, stable, navoid,
-      navoid_fwd;
+  uint32_t nact, hlen, hhead, burnin_done, nnew, nkept, n5, n3, stable, navoid, navoid_fwd;
 };
 
 // bitonic sort of keys[0, m) ascending, m a power of two
+// A value every lane of the block holds equal, moved to scalar registers (lane 0's copy).
+__device__ inline uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // LEF-BAR records a fwd barrier collision as kCkBarFwdTmp with the index mirrored, so that
 // atomicMax keeps the closest (lowest-index) barrier; this gives the plain kCkBar word.
 __device__ inline uint32_t fc_final(uint32_t v) {
@@ -782,6 +787,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
                           uint64_t seed, uint64_t target, uint64_t* stats, uint32_t item) {
   const uint32_t N = ch.nlefs, NB = ch.nbar;
   const Rng R{(uint32_t)seed, (uint32_t)(seed >> 32)};
+  target = uniform64(target);
   for (uint32_t i = threadIdx.x; i < N; i += NT) {
     c.rev[i] = c.fwd[i] = kFreePos;
     c.rr[i] = c.fr[i] = (uint16_t)i;
@@ -795,26 +801,22 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     S.burnin_done = P.skip_burnin ? 1 : 0;
     S.hlen = 0;
     S.hhead = 0;
-    S.contacts = 0;
     S.burnin_epochs = 0;
-    S.done = 0;
     for (int k = 0; k < kFastPhases; ++k) S.phase[k] = 0;
     S.tick = P.timers ? wall_clock64() : 0;
   }
   __syncthreads();
   const uint32_t span = ch.end - ch.start;  // bind positions in [start, end - 1]
   uint32_t nact_prev = 0;
+  uint64_t contacts = 0;  // registered so far, the same in every lane (kept in scalar registers)
   uint32_t epoch = 0;
   for (;; ++epoch) {
     // ---- stop test and burn-in (simulation.cpp:1087-1098, 1027-1054) ----
-    if (threadIdx.x == 0) {
-      S.done = (P.target_contact_density >= 0 ? S.contacts < target
-                                              : (uint64_t)epoch - S.burnin_epochs < ch.target_epochs)
-                   ? 0
-                   : 1;
-    }
-    __syncthreads();
-    if (S.done) break;
+    // every lane holds the contact count and tests on its own (the same values everywhere);
+    // lane 0 changes burnin_epochs only after later barriers
+    if (P.target_contact_density >= 0 ? contacts >= target
+                                      : (uint64_t)epoch - S.burnin_epochs >= ch.target_epochs)
+      break;
     if (!S.burnin_done) {
       const bool activating = S.nact != N;
       __syncthreads();  // every lane has read nact before lane 0 changes it
@@ -926,7 +928,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     // ---- contacts (sample_and_register_contacts, register_contacts.cpp:94-198) ----
     if (bdone) {
       uint64_t want = ch.cpe;
-      if (P.target_contact_density > 0.0) want = min(want, target - S.contacts);
+      if (P.target_contact_density > 0.0) want = min(want, target - contacts);
       const uint32_t sp = ch.start + 1, ep = ch.end - 1;
       for (uint32_t s = threadIdx.x; s < want; s += NT) {
         for (uint32_t attempt = 0; attempt < 1024; ++attempt) {
@@ -981,11 +983,11 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
           }
         }
       }
-      __syncthreads();  // every lane has read the remaining target before it changes
-      if (threadIdx.x == 0) S.contacts += want;
-      __syncthreads();
+      // no barrier: the phases up to extrusion do not write the positions sampled here, and a
+      // cell that ends here meets the barrier after it before the next cell is set up
+      contacts = uniform64(contacts + want);
       mark(P, S, 4);
-      if (target != 0 && S.contacts >= target) break;
+      if (target != 0 && contacts >= target) break;
     }
 
     // ---- moves (generate_moves, simulation.cpp:265-323) ----
@@ -1270,7 +1272,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
   if (threadIdx.x == 0 && stats) {
     stats[0] = epoch;
     stats[1] = S.burnin_epochs;
-    stats[2] = S.contacts;
+    stats[2] = contacts;
     stats[3] = 0;
   }
   if (P.timers && threadIdx.x == 0)
