@@ -264,21 +264,21 @@ __device__ void bitonic(uint64_t* keys, uint32_t m) {
   }
 }
 
-// Sorts LEF ids a[0, n) by (position, id) with odd-even transposition passes; returns after two
-// consecutive passes without a swap. Cheap on the nearly sorted orders extrusion leaves: one
-// check pass first, which usually finds the order sorted already.
+// Sorts LEF ids a[0, n) by position with odd-even transposition passes (stable: equal
+// positions keep their order); returns after two consecutive passes without a swap. Cheap on
+// the nearly sorted orders extrusion leaves: one check pass first, which usually finds the
+// order sorted already.
 template <int NT>
 __device__ void odd_even_sort(uint32_t* a, uint32_t n, const uint32_t* pos) {
   int bad = 0;
-  for (uint32_t i = threadIdx.x; i + 1 < n; i += NT)
-    bad |= pos_key(pos, a[i + 1]) < pos_key(pos, a[i]);
+  for (uint32_t i = threadIdx.x; i + 1 < n; i += NT) bad |= pos[a[i + 1]] < pos[a[i]];
   if (!__syncthreads_or(bad)) return;
   int quiet = 0;
   for (uint32_t pass = 0; quiet < 2; ++pass) {
     int swapped = 0;
     for (uint32_t i = 2 * threadIdx.x + (pass & 1); i + 1 < n; i += 2 * NT) {
       const uint32_t x = a[i], y = a[i + 1];
-      if (pos_key(pos, y) < pos_key(pos, x)) {
+      if (pos[y] < pos[x]) {
         a[i] = y;
         a[i + 1] = x;
         swapped = 1;
@@ -312,35 +312,41 @@ __device__ void sort_new_keys(uint64_t* keys, uint32_t k) {
   bitonic<NT>(keys, m);
 }
 
-// Merges the sorted kept LEF ids kept[0, nk) with the sorted new keys keys[0, k) into
-// out(i, lef) (a merge path: each lane owns a contiguous run of the kept order, finds where its
-// run starts among the new keys once, then walks). The new keys that fall before kept entry a
-// (and after a - 1) are written by a's lane, those after the last kept entry by the last lane.
-template <int NT, class Out>
+// Merges the kept LEF ids kept[0, nk) (sorted by position) with the new keys keys[0, k)
+// (sorted by position, then id) into out(i, lef) (a merge path: each lane owns a contiguous run
+// of the kept order, finds where its run starts among the new keys once, then walks). At equal
+// positions the new LEFs go after the kept ones, or before them when kNewFirst. The new keys
+// that fall before kept entry a (and after a - 1) are written by a's lane, those after the
+// last kept entry by the last lane.
+template <int NT, bool kNewFirst, class Out>
 __device__ void merge_kept_new(const uint32_t* kept, uint32_t nk, const uint64_t* keys, uint32_t k,
                                const uint32_t* pos, Out out) {
   if (nk == 0) {
     for (uint32_t b = threadIdx.x; b < k; b += NT) out(b, (uint32_t)keys[b]);
     return;
   }
+  // whether new key q goes before a kept LEF at position p (true on a prefix of the keys)
+  auto before = [](uint64_t q, uint32_t p) {
+    return kNewFirst ? (uint32_t)(q >> 32) <= p : (uint32_t)(q >> 32) < p;
+  };
   const uint32_t chunk = (nk + NT - 1) / NT;
   const uint32_t ab = threadIdx.x * chunk, ae = min(nk, ab + chunk);
   if (ab >= ae) return;
-  // lo: new keys below kept[ab - 1] (0 for the first run)
+  // lo: new keys that go before kept[ab - 1] (0 for the first run)
   uint32_t lo = 0;
   if (ab > 0) {
-    const uint64_t key = pos_key(pos, kept[ab - 1]);
+    const uint32_t p = pos[kept[ab - 1]];
     uint32_t hi = k;
     while (lo < hi) {
       const uint32_t mid = (lo + hi) >> 1;
-      if (keys[mid] < key) lo = mid + 1;
+      if (before(keys[mid], p)) lo = mid + 1;
       else hi = mid;
     }
   }
   for (uint32_t a = ab; a < ae; ++a) {
     const uint32_t lef = kept[a];
-    const uint64_t key = pos_key(pos, lef);
-    while (lo < k && keys[lo] < key) {
+    const uint32_t p = pos[lef];
+    while (lo < k && before(keys[lo], p)) {
       out(lo + a, (uint32_t)keys[lo]);
       ++lo;
     }
@@ -366,8 +372,9 @@ __device__ inline void mark(const FastParams& P, Shared& S, int ph) {
 
 // Merges the LEFs flagged in is_new into one unit side's order: tmp_a[0, nkept) holds the kept
 // entries of last epoch's order (compacted stably); the result goes to ranks[0, nact). More
-// than kNewCap new LEFs are merged in batches through tmp_b (n-sized scratch).
-template <int NT>
+// than kNewCap new LEFs are merged in batches through tmp_b (n-sized scratch). kNewFirst: new
+// LEFs go before kept ones at equal positions (the fwd side).
+template <int NT, bool kNewFirst>
 __device__ void merge_new(uint16_t* ranks, const uint32_t* pos, uint32_t nact, uint32_t nkept,
                           const uint32_t* is_new, uint32_t* tmp_a, uint32_t* tmp_b,
                           uint64_t* keys, Shared& S) {
@@ -393,13 +400,13 @@ __device__ void merge_new(uint16_t* ranks, const uint32_t* pos, uint32_t nact, u
     sort_new_keys<NT>(keys, k);
     // the last batch merges straight into the rank array (nothing reads it any more)
     if (bt + 1 == batches) {
-      merge_kept_new<NT>(tmp_a, nkept, keys, k, pos,
-                         [&](uint32_t i, uint32_t lef) { ranks[i] = (uint16_t)lef; });
+      merge_kept_new<NT, kNewFirst>(tmp_a, nkept, keys, k, pos,
+                                    [&](uint32_t i, uint32_t lef) { ranks[i] = (uint16_t)lef; });
       written = true;
       __syncthreads();
     } else {
-      merge_kept_new<NT>(tmp_a, nkept, keys, k, pos,
-                         [&](uint32_t i, uint32_t lef) { tmp_b[i] = lef; });
+      merge_kept_new<NT, kNewFirst>(tmp_a, nkept, keys, k, pos,
+                                    [&](uint32_t i, uint32_t lef) { tmp_b[i] = lef; });
       __syncthreads();
       nkept += k;
       for (uint32_t i = threadIdx.x; i < nkept; i += NT) tmp_a[i] = tmp_b[i];
@@ -418,6 +425,11 @@ __device__ void merge_new(uint16_t* ranks, const uint32_t* pos, uint32_t nact, u
 // position on both sides (rev == fwd), so the new LEFs have the same keys on both sides: they
 // are collected and sorted once and merged into both orders in one pass. Epochs that bind more
 // than kNewCap LEFs (the first ones) merge each side in batches instead.
+// Ties (rank_lefs orders equal positions by binding epoch: earliest first on the rev side,
+// latest first on the fwd side): kept LEFs keep last epoch's order among themselves, and the
+// LEFs bound this epoch, the latest bound, go after them on the rev side and before them on the
+// fwd side. So a kept order with equal positions stays sorted, and the units stacked at a
+// chromosome end need no sorting pass.
 template <int NT>
 __device__ void rerank_both(Cell& c, uint32_t nact, uint32_t nact_old, uint64_t* keys,
                             Shared& S, const FastParams& P) {
@@ -445,17 +457,16 @@ __device__ void rerank_both(Cell& c, uint32_t nact, uint32_t nact_old, uint64_t*
   }
   const uint32_t nkept = S.nkept;
   if (nact - nkept > kNewCap) {
-    merge_new<NT>(c.rr, c.rev, nact, nkept, is_new, kr, c.fm, keys, S);
+    merge_new<NT, false>(c.rr, c.rev, nact, nkept, is_new, kr, c.fm, keys, S);
     mark(P, S, 2);
-    merge_new<NT>(c.fr, c.fwd, nact, nkept, is_new, kf, c.fm, keys, S);
+    merge_new<NT, true>(c.fr, c.fwd, nact, nkept, is_new, kf, c.fm, keys, S);
     mark(P, S, 3);
     return;
   }
   // both kept orders must be sorted exactly for the merge; one check pass covers both
   int bad = 0;
   for (uint32_t i = threadIdx.x; i + 1 < nkept; i += NT)
-    bad |= (pos_key(c.rev, kr[i + 1]) < pos_key(c.rev, kr[i])) |
-           (pos_key(c.fwd, kf[i + 1]) < pos_key(c.fwd, kf[i]));
+    bad |= (c.rev[kr[i + 1]] < c.rev[kr[i]]) | (c.fwd[kf[i + 1]] < c.fwd[kf[i]]);
   for (uint32_t l = threadIdx.x; l < nact; l += NT) {
     if (is_new[l]) keys[atomicAdd(&S.nnew, 1u)] = pos_key(c.rev, l);
   }
@@ -471,10 +482,10 @@ __device__ void rerank_both(Cell& c, uint32_t nact, uint32_t nact_old, uint64_t*
     }
   } else {
     sort_new_keys<NT>(keys, k);
-    merge_kept_new<NT>(kr, nkept, keys, k, c.rev,
-                       [&](uint32_t i, uint32_t lef) { c.rr[i] = (uint16_t)lef; });
-    merge_kept_new<NT>(kf, nkept, keys, k, c.fwd,
-                       [&](uint32_t i, uint32_t lef) { c.fr[i] = (uint16_t)lef; });
+    merge_kept_new<NT, false>(kr, nkept, keys, k, c.rev,
+                              [&](uint32_t i, uint32_t lef) { c.rr[i] = (uint16_t)lef; });
+    merge_kept_new<NT, true>(kf, nkept, keys, k, c.fwd,
+                             [&](uint32_t i, uint32_t lef) { c.fr[i] = (uint16_t)lef; });
   }
   __syncthreads();
   mark(P, S, 2);
