@@ -471,8 +471,27 @@ __device__ void rerank_both(Cell& c, uint32_t nact, uint32_t nact_old, uint64_t*
     if (is_new[l]) keys[atomicAdd(&S.nnew, 1u)] = pos_key(c.rev, l);
   }
   if (__syncthreads_or(bad)) {
-    odd_even_sort<NT>(kr, nkept, c.rev);
-    odd_even_sort<NT>(kf, nkept, c.fwd);
+    // odd-even transposition passes over both orders at once (stable), until two consecutive
+    // passes swap nothing on either side
+    int quiet = 0;
+    for (uint32_t pass = 0; quiet < 2; ++pass) {
+      int swapped = 0;
+      for (uint32_t i = 2 * threadIdx.x + (pass & 1); i + 1 < nkept; i += 2 * NT) {
+        const uint32_t x = kr[i], y = kr[i + 1];
+        if (c.rev[y] < c.rev[x]) {
+          kr[i] = y;
+          kr[i + 1] = x;
+          swapped = 1;
+        }
+        const uint32_t u = kf[i], v = kf[i + 1];
+        if (c.fwd[v] < c.fwd[u]) {
+          kf[i] = v;
+          kf[i + 1] = u;
+          swapped = 1;
+        }
+      }
+      quiet = __syncthreads_or(swapped) ? 0 : quiet + 1;
+    }
   }
   const uint32_t k = S.nnew;
   if (k == 0) {
