@@ -939,16 +939,22 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     mark(P, S, 0);
 
     // ---- bind (simulation_impl.hpp:28-89) ----
-    for (uint32_t i = threadIdx.x; i < n; i += NT) {
-      uint32_t flag = 0;
-      if (c.rev[i] == kFreePos) {
-        const U4 d = R.draw(i, epoch, kSBind);
-        c.rev[i] = c.fwd[i] = ch.start + ubounded(d.x, span);
-        flag = 1;
+    // rm holds the "bound this epoch" flags until moves are drawn. A LEF released last epoch was
+    // rebound, and flagged, right at its release (its bind draw is keyed by this epoch), so
+    // this pass only binds the LEFs activated this epoch (all of them in the first epoch); it is
+    // skipped once burn-in has activated every LEF.
+    if (n > nact_prev) {
+      for (uint32_t i = nact_prev + threadIdx.x; i < n; i += NT) {
+        uint32_t flag = 0;
+        if (c.rev[i] == kFreePos) {
+          const U4 d = R.draw(i, epoch, kSBind);
+          c.rev[i] = c.fwd[i] = ch.start + ubounded(d.x, span);
+          flag = 1;
+        }
+        c.rm[i] = flag;
       }
-      c.rm[i] = flag;  // rm holds the "bound this epoch" flags until moves are drawn
+      __syncthreads();
     }
-    __syncthreads();
     mark(P, S, 1);
     // ---- ranks (rank_lefs, simulation.cpp:402-487) ----
     uint64_t* keys = reinterpret_cast<uint64_t*>(c.scratch);
@@ -1293,7 +1299,15 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
                       (ck_kind(c.fc[i]) == kCkBar && ch.bar_dir[ck_idx(c.fc[i])] == 2);
         const double p = (h == 0 ? 1.0 : (h == 1 ? P.inv_soft : P.inv_hard)) * base;
         const U4 d = R.draw(i, epoch, kSRelease);
-        if (bern(d.x, thresh(p))) c.rev[i] = c.fwd[i] = kFreePos;
+        // a released LEF binds again at once, with the draw the next epoch's bind would make
+        // (simulate_one_cell rebinds every released LEF at the start of the next epoch)
+        uint32_t rebound = 0;
+        if (bern(d.x, thresh(p))) {
+          const U4 e = R.draw(i, epoch + 1, kSBind);
+          c.rev[i] = c.fwd[i] = ch.start + ubounded(e.x, span);
+          rebound = 1;
+        }
+        c.rm[i] = rebound;  // the next epoch's "bound this epoch" flag
       }
     }
     __syncthreads();
