@@ -837,6 +837,15 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
   }
   __syncthreads();
   const uint32_t span = ch.end - ch.start;  // bind positions in [start, end - 1]
+  // Position buckets for the LEF-BAR and primary searches, kept in the burn-in history's memory
+  // once burn-in is over: T[q] = the first rank whose position lies in bucket q or later
+  // (bucket = (position - start) >> bshift, T[nbk] = n), so a search narrows to one bucket.
+  uint16_t* const T = reinterpret_cast<uint16_t*>(c.cfx);
+  const uint32_t nbk_cap = (uint32_t)align16(8ull * (P.hist_len + 1)) - 1;  // cfx and avg, u16
+  uint32_t bshift = 0;
+  while (bshift < 31 && ((span - 1) >> bshift) + 1 > nbk_cap) ++bshift;
+  const uint32_t nbk = min(((span - 1) >> bshift) + 1, nbk_cap);
+  auto bucket = [&](uint32_t p) { return min((p - ch.start) >> bshift, nbk - 1); };
   uint32_t nact_prev = 0;
   uint64_t contacts = 0;  // registered so far, the same in every lane (kept in scalar registers)
   uint32_t epoch = 0;
@@ -1125,9 +1134,24 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
       S.n5 = n5;
       S.n3 = n3;
     }
-    // fwd positions in rank order for the first LEF-BAR pass, filled beside lane 0's walk
+    // fwd positions in rank order for the first LEF-BAR pass, filled beside lane 0's walk, with
+    // their bucket index once burn-in is over
     uint32_t* sorted = c.scratch;
-    for (uint32_t k = threadIdx.x; k < n; k += NT) sorted[k] = c.fwd[c.fr[k]];
+    const bool use_bk = bdone && n > 0;
+    auto fill_sorted = [&](const uint32_t* pos, const uint16_t* ranks) {
+      for (uint32_t k = threadIdx.x; k < n; k += NT) {
+        const uint32_t pk = pos[ranks[k]];
+        sorted[k] = pk;
+        if (use_bk) {
+          const uint32_t q = bucket(pk);
+          const uint32_t q0 = k == 0 ? 0u : bucket(pos[ranks[k - 1]]) + 1;
+          for (uint32_t b = q0; b <= q; ++b) T[b] = (uint16_t)k;
+          if (k == n - 1)
+            for (uint32_t b = q + 1; b <= nbk; ++b) T[b] = (uint16_t)n;
+        }
+      }
+    };
+    fill_sorted(c.fwd, c.fr);
     __syncthreads();
     mark(P, S, 7);
     const uint32_t n5 = S.n5, n3 = S.n3;
@@ -1137,8 +1161,8 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     // fwd side first, so the rev positions in rank order stay in `sorted` for the primary step
     for (int it = 0; it < 2; ++it) {
       const int side = 1 - it;
-      if (side == 0) {  // the fwd pass ended with a barrier, so `sorted` is free
-        for (uint32_t k = threadIdx.x; k < n; k += NT) sorted[k] = c.rev[c.rr[k]];
+      if (side == 0) {  // the fwd pass ended with a barrier, so `sorted` and T are free
+        fill_sorted(c.rev, c.rr);
         __syncthreads();
       }
       const uint32_t rev_lo = n5 ? n5 - 1 : 0;
@@ -1147,7 +1171,14 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
         if (!c.bst[b]) continue;
         const uint32_t bp = ch.bar_pos[b];
         if (side == 0) {
+          // first rank at or after rev_lo with a position beyond bp; it lies in bp's bucket
+          // range [T[q], T[q + 1]]
           uint32_t lo = rev_lo, hi = n;
+          if (use_bk) {
+            const uint32_t q = bucket(bp);
+            lo = max(lo, (uint32_t)T[q]);
+            hi = max(lo, (uint32_t)T[q + 1]);
+          }
           while (lo < hi) {
             const uint32_t mid = (lo + hi) >> 1;
             if (sorted[mid] <= bp) lo = mid + 1;
@@ -1160,7 +1191,13 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
           if (bern(d.x, ch.bar_dir[b] == 1 ? P.t_pblock_major : P.t_pblock_minor))
             atomicMax(&c.rc[l], ck(kCkBar, b));
         } else {
+          // first rank below fwd_top at or beyond bp, searched within bp's bucket range
           uint32_t lo = 0, hi = fwd_top;
+          if (use_bk) {
+            const uint32_t q = bucket(bp);
+            lo = min((uint32_t)T[q], fwd_top);
+            hi = min((uint32_t)T[q + 1], fwd_top);
+          }
           while (lo < hi) {
             const uint32_t mid = (lo + hi) >> 1;
             if (sorted[mid] < bp) lo = mid + 1;
@@ -1196,6 +1233,11 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
         const uint32_t fp = c.fwd[fl];
         if (k == k_begin) {
           uint32_t hi = n;
+          if (use_bk) {  // T indexes the rev positions (the last LEF-BAR fill)
+            const uint32_t q = bucket(fp);
+            lo = max(lo, (uint32_t)T[q]);
+            hi = max(lo, (uint32_t)T[q + 1]);
+          }
           while (lo < hi) {
             const uint32_t mid = (lo + hi) >> 1;
             if (sorted[mid] <= fp) lo = mid + 1;
