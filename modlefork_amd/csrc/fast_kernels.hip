@@ -1093,11 +1093,13 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     mark(P, S, 6);
 
     // ---- barrier states (extrusion_barriers.cpp:145-161) ----
+    // both thresholds are loaded ahead of the draw, so their latency hides behind it
     for (uint32_t b = threadIdx.x; b < NB; b += NT) {
+      const uint32_t t_in = ch.bar_t_inactive[b], t_act = ch.bar_t_active[b];
       const U4 d = R.draw(b, epoch, kSBarrier);
       const uint8_t st = c.bst[b];
-      if (st == 0 && d.x >= ch.bar_t_inactive[b]) c.bst[b] = 1;
-      else if (st == 1 && d.x >= ch.bar_t_active[b]) c.bst[b] = 0;
+      if (st == 0 && d.x >= t_in) c.bst[b] = 1;
+      else if (st == 1 && d.x >= t_act) c.bst[b] = 0;
     }
     // ---- units at the chromosome ends (simulation_detect_collisions.cpp:30-125) ----
     if (threadIdx.x == 0) {
@@ -1167,9 +1169,12 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
       }
       const uint32_t rev_lo = n5 ? n5 - 1 : 0;
       const uint32_t fwd_top = n - (n3 ? n3 - 1 : 0);
+      // the next barrier's position is loaded while this one is searched
+      uint32_t bp_next = threadIdx.x < NB ? ch.bar_pos[threadIdx.x] : 0u;
       for (uint32_t b = threadIdx.x; b < NB; b += NT) {
+        const uint32_t bp = bp_next;
+        if (b + NT < NB) bp_next = ch.bar_pos[b + NT];
         if (!c.bst[b]) continue;
-        const uint32_t bp = ch.bar_pos[b];
         if (side == 0) {
           // first rank at or after rev_lo with a position beyond bp; it lies in bp's bucket
           // range [T[q], T[q + 1]]
