@@ -266,8 +266,8 @@ __device__ void bitonic(uint64_t* keys, uint32_t m) {
 
 // Sorts LEF ids a[0, n) by position with odd-even transposition passes (stable: equal
 // positions keep their order); returns after two consecutive passes without a swap. Cheap on
-// the nearly sorted orders extrusion leaves: one check pass first, which usually finds the
-// order sorted already.
+// the nearly sorted orders extrusion leaves (a few adjacent inversions per epoch): one check
+// pass first, then a handful of passes. rerank_both runs the same passes on both sides at once.
 template <int NT>
 __device__ void odd_even_sort(uint32_t* a, uint32_t n, const uint32_t* pos) {
   int bad = 0;
