@@ -309,3 +309,20 @@ def test_fast_mode_genome_slice_runs(ctx):
     say(f"fast genome slice: {st.cells} cells, {st.kernel_ms:.1f} ms, epochs {st.epochs}")
     for name, (band, missed, plan) in out.items():
         assert int(band.sum()) + missed == int(plan.targets.sum())
+
+
+def test_fast_mode_matches_recorded_fingerprints(ctx):
+    """Fast-mode bands of three seeded workloads against the fingerprints recorded in
+    tests/golden/fast_fingerprints.json (SHA-256 of the bands and missed counts, and the epoch
+    and contact totals). Kernel changes meant to leave results alone must keep them; a change
+    of the random streams or the physics rewrites them with tests/golden/make_fast_fingerprints.py."""
+    import json
+
+    from golden import make_fast_fingerprints as F
+
+    with open(F.PATH) as fh:
+        want = json.load(fh)
+    for name, wl in F.cases().items():
+        say(f"fingerprint: {name}")
+        got = F.fingerprint(ctx, wl)
+        assert got == want[name], f"{name}: {got} != recorded {want[name]}"
