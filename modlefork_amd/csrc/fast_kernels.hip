@@ -571,13 +571,29 @@ __device__ void secondary_both(const FastParams& P, const Rng& R, uint32_t epoch
   auto seed = [&](int sd, uint32_t j) -> uint32_t {
     return ck_stalled(coll(sd)[unit(sd, j)]) ? j + 1 : 0u;
   };
-  // the last seed of this lane's run, per side
+  // the last seed of this lane's run, per side. A run of at most 32 units keeps its seed flags
+  // in a bit mask, which the later passes read instead of the collisions.
+  const bool use_mask = chunk <= 32;
+  uint32_t mask0 = 0, mask1 = 0;
   uint32_t e0 = 0, e1 = 0;
-  for (uint32_t j = je; j-- > jb;) {
-    if (e0 == 0) e0 = seed(0, j);
-    if (e1 == 0) e1 = seed(1, j);
-    if (e0 && e1) break;
+  if (use_mask) {
+    for (uint32_t j = jb; j < je; ++j) {
+      mask0 |= (seed(0, j) ? 1u : 0u) << (j - jb);
+      mask1 |= (seed(1, j) ? 1u : 0u) << (j - jb);
+    }
+    e0 = mask0 ? jb + (31u - (uint32_t)__clz((int)mask0)) + 1 : 0u;
+    e1 = mask1 ? jb + (31u - (uint32_t)__clz((int)mask1)) + 1 : 0u;
+  } else {
+    for (uint32_t j = je; j-- > jb;) {
+      if (e0 == 0) e0 = seed(0, j);
+      if (e1 == 0) e1 = seed(1, j);
+      if (e0 && e1) break;
+    }
   }
+  auto seed_at = [&](int sd, uint32_t j) -> uint32_t {
+    if (use_mask) return (((sd == 0 ? mask0 : mask1) >> (j - jb)) & 1u) ? j + 1 : 0u;
+    return seed(sd, j);
+  };
   uint32_t* shu2 = reinterpret_cast<uint32_t*>(S.shx);
   block_exclusive2<NT>(e0, e1, OpMaxU{}, OpMaxU{}, 0u, 0u, shu2);
   const uint32_t s0 = e0, s1p = e1;  // seed prefixes before this run, kept for the last pass
@@ -587,7 +603,7 @@ __device__ void secondary_both(const FastParams& P, const Rng& R, uint32_t epoch
 #pragma unroll
     for (int sd = 0; sd < 2; ++sd) {
       uint32_t& run = sd == 0 ? e0 : e1;
-      const uint32_t sj = seed(sd, j);
+      const uint32_t sj = seed_at(sd, j);
       run = sj > run ? sj : run;
       const uint32_t s1 = run;  // seed + 1, 0 if none
       uint32_t fail = 0;
@@ -620,16 +636,16 @@ __device__ void secondary_both(const FastParams& P, const Rng& R, uint32_t epoch
     for (int sd = 0; sd < 2; ++sd) {
       uint32_t& run = sd == 0 ? e0 : e1;
       uint32_t& m = sd == 0 ? m0 : m1;
-      const uint32_t sj = seed(sd, j);
+      const uint32_t sj = seed_at(sd, j);
       run = sj > run ? sj : run;
       const uint32_t prev = m;
       const uint32_t w = (run << 16) | F[sd * n + j];
       m = w > m ? w : m;
       const uint32_t s1 = m >> 16;
       if (s1 == 0 || s1 - 1 == j || j < (sd == 0 ? lo_rev : lo_fwd)) continue;
+      if (sj) continue;  // already stalled (its seed flag)
       const uint32_t l = unit(sd, j);
       uint32_t* cl = coll(sd);
-      if (ck_stalled(cl[l])) continue;
       const uint32_t s = s1 - 1;
       const uint32_t lead = unit(sd, j - 1);
       const int32_t dist = (int32_t)j - (int32_t)s;
