@@ -1355,17 +1355,19 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     // ---- extrude and release (simulation.cpp:489-512, 544-592) ----
     {
       const double base = bdone ? ch.prob_release : ch.prob_release_burnin;
+      // the three release thresholds (no, one soft, two or one hard barrier stall), once
+      const uint64_t t_rel[3] = {thresh(1.0 * base), thresh(P.inv_soft * base),
+                                 thresh(P.inv_hard * base)};
       for (uint32_t i = threadIdx.x; i < n; i += NT) {
         c.rev[i] -= c.rm[i];
         c.fwd[i] += c.fm[i];
         const int h = (ck_kind(c.rc[i]) == kCkBar && ch.bar_dir[ck_idx(c.rc[i])] == 1) +
                       (ck_kind(c.fc[i]) == kCkBar && ch.bar_dir[ck_idx(c.fc[i])] == 2);
-        const double p = (h == 0 ? 1.0 : (h == 1 ? P.inv_soft : P.inv_hard)) * base;
         const U4 d = R.draw(i, epoch, kSRelease);
         // a released LEF binds again at once, with the draw the next epoch's bind would make
         // (simulate_one_cell rebinds every released LEF at the start of the next epoch)
         uint32_t rebound = 0;
-        if (bern(d.x, thresh(p))) {
+        if (bern(d.x, h == 0 ? t_rel[0] : (h == 1 ? t_rel[1] : t_rel[2]))) {
           const U4 e = R.draw(i, epoch + 1, kSBind);
           c.rev[i] = c.fwd[i] = ch.start + ubounded(e.x, span);
           rebound = 1;
