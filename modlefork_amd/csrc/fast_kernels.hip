@@ -190,9 +190,6 @@ __device__ __host__ inline uint64_t hybrid_lds_bytes(uint32_t n, uint32_t nbar, 
          align16(8ull * (hist + 1)) * 2;
 }
 
-__device__ __host__ inline uint64_t hybrid_slab_bytes(uint32_t n) {
-  return align16(4ull * n) * 4 + align16(scratch_bytes(n));
-}
 
 __device__ inline void carve_hybrid(Cell& c, uint8_t* lds, uint8_t* slab, uint32_t n, uint32_t nbar,
                                     uint32_t hist) {
@@ -1785,7 +1782,7 @@ int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& dchroms,
   impl->order_host = order;
   if (e != hipSuccess) {
     g_fast_err = std::string("fast-mode upload failed: ") + hipGetErrorString(e);
-    hipFree(impl->d_mem);
+    (void)hipFree(impl->d_mem);
     delete impl;
     return -1;
   }
@@ -2000,8 +1997,8 @@ hipError_t fast_phase_times(uint64_t* out, int n) {
 void fast_job_destroy(FastJob* job) {
   auto* impl = static_cast<FastImpl*>(job->impl);
   if (!impl) return;
-  if (impl->d_mem) hipFree(impl->d_mem);
-  if (impl->d_slab_own) hipFree(impl->d_slab_own);
+  if (impl->d_mem) (void)hipFree(impl->d_mem);
+  if (impl->d_slab_own) (void)hipFree(impl->d_slab_own);
   delete impl;
   job->impl = nullptr;
 }

@@ -142,12 +142,12 @@ int mgpu_init(int device, mgpu_ctx** out) {
 
 void mgpu_destroy(mgpu_ctx* ctx) {
   if (!ctx) return;
-  hipSetDevice(ctx->device);
-  if (ctx->ev0) hipEventDestroy(ctx->ev0);
-  if (ctx->ev1) hipEventDestroy(ctx->ev1);
+  (void)hipSetDevice(ctx->device);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   for (auto e : ctx->lev)
-    if (e) hipEventDestroy(e);
-  if (ctx->stream) hipStreamDestroy(ctx->stream);
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
 
@@ -662,14 +662,14 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
   std::memcpy(stage.data() + off_items, items.data(), items.size() * sizeof(WorkItem));
   e = hipMemcpy(base, stage.data(), stage.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
-    hipFree(job->d_pool);
+    (void)hipFree(job->d_pool);
     delete job;
     return fail(ctx, MGPU_EDEVICE, "upload failed: %s", hipGetErrorString(e));
   }
   if (mode == MGPU_MODE_FAST) {
     int rc = fast_job_create(*p, job->chroms, barriers, release, items, &job->fast);
     if (rc != 0) {
-      hipFree(job->d_pool);
+      (void)hipFree(job->d_pool);
       delete job;
       return fail(ctx, MGPU_EDEVICE, "fast-mode setup failed: %s", fast_last_error());
     }
@@ -692,7 +692,7 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
     job->log_default = want;
     if (int rc = mgpu_job_set_state_log(job, want)) {
       fast_job_destroy(&job->fast);
-      hipFree(job->d_pool);
+      (void)hipFree(job->d_pool);
       delete job;
       return rc;
     }
@@ -755,8 +755,8 @@ hipError_t enqueue_simulation(mgpu_job* job, hipStream_t s, hipEvent_t* ev = nul
 }
 
 void drop_graph(mgpu_job* job) {
-  if (job->graph_exec) hipGraphExecDestroy(job->graph_exec);
-  if (job->graph) hipGraphDestroy(job->graph);
+  if (job->graph_exec) (void)hipGraphExecDestroy(job->graph_exec);
+  if (job->graph) (void)hipGraphDestroy(job->graph);
   job->graph_exec = nullptr;
   job->graph = nullptr;
 }
@@ -1030,11 +1030,11 @@ int mgpu_job_band_device_ptr(mgpu_job* job, uint64_t c, void** dptr, uint64_t* n
 
 void mgpu_job_destroy(mgpu_job* job) {
   if (!job) return;
-  hipSetDevice(job->ctx->device);
+  (void)hipSetDevice(job->ctx->device);
   drop_graph(job);
   fast_job_destroy(&job->fast);
-  if (job->d_pool) hipFree(job->d_pool);
-  if (job->d_log) hipFree(job->d_log);
+  if (job->d_pool) (void)hipFree(job->d_pool);
+  if (job->d_log) (void)hipFree(job->d_log);
   delete job;
 }
 
@@ -1157,7 +1157,7 @@ int mgpu_debug_step(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom* chrom
   if (e == hipSuccess) e = down(rr.data(), d_rr, n * 4);
   if (e == hipSuccess) e = down(fr.data(), d_fr, n * 4);
   if (e == hipSuccess) e = down(rng_state, d_rng, 32);
-  hipFree(d);
+  (void)hipFree(d);
   if (e != hipSuccess) return fail(ctx, MGPU_EDEVICE, "debug step failed: %s", hipGetErrorString(e));
   for (uint64_t i = 0; i < n; ++i) {
     rev_ranks[i] = rr[i];
@@ -1177,7 +1177,7 @@ int mgpu_debug_draws(mgpu_ctx* ctx, uint64_t seed, int kind, double a, double b,
   hipError_t e = launch_det_draws(ctx->stream, seed, kind, a, b, c, n, d);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e == hipSuccess) e = hipMemcpy(out, d, n * 8, hipMemcpyDeviceToHost);
-  hipFree(d);
+  (void)hipFree(d);
   if (e != hipSuccess) return fail(ctx, MGPU_EDEVICE, "debug draws failed: %s", hipGetErrorString(e));
   return MGPU_OK;
 }
@@ -1207,7 +1207,7 @@ int mgpu_debug_cell_trace(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom*
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e == hipSuccess && max_trace) e = hipMemcpy(trace, d_trace, max_trace * 8, hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(out_epochs, d_out, 32, hipMemcpyDeviceToHost);
-  if (d) hipFree(d);
+  if (d) (void)hipFree(d);
   mgpu_job_destroy(job);
   if (e != hipSuccess) return fail(ctx, MGPU_EDEVICE, "trace failed: %s", hipGetErrorString(e));
   return MGPU_OK;
@@ -1251,7 +1251,7 @@ int mgpu_debug_philox(mgpu_ctx* ctx, const uint32_t* in, uint64_t n, uint32_t* o
   if (e == hipSuccess) e = launch_philox_kat(ctx->stream, d, d + 6 * n, (int)n);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e == hipSuccess) e = hipMemcpy(out, d + 6 * n, n * 16, hipMemcpyDeviceToHost);
-  hipFree(d);
+  (void)hipFree(d);
   if (e != hipSuccess) return fail(ctx, MGPU_EDEVICE, "philox failed: %s", hipGetErrorString(e));
   return MGPU_OK;
 }

@@ -426,7 +426,7 @@ int mgpu_sg_create(mgpu_ctx* ctx, const mgpu_sg_config* cfg, mgpu_sg** out) {
   int device = 0;
   if (mgpu_context_stream(ctx, (void**)&stream, &device) != MGPU_OK)
     return sg_fail(nullptr, MGPU_EINVAL, "invalid context");
-  hipSetDevice(device);
+  (void)hipSetDevice(device);
   auto* s = new mgpu_sg();
   s->ctx = ctx;
   s->stream = stream;
@@ -538,13 +538,13 @@ int mgpu_sg_create(mgpu_ctx* ctx, const mgpu_sg_config* cfg, mgpu_sg** out) {
 
 void mgpu_sg_destroy(mgpu_sg* s) {
   if (!s) return;
-  for (void* p : s->allocs) hipFree(p);
-  if (s->data_x) hipFree(s->data_x);
-  if (s->data_y) hipFree(s->data_y);
-  if (s->bands) hipFree(s->bands);
-  if (s->d_starts) hipFree(s->d_starts);
-  if (s->ev0) hipEventDestroy(s->ev0);
-  if (s->ev1) hipEventDestroy(s->ev1);
+  for (void* p : s->allocs) (void)hipFree(p);
+  if (s->data_x) (void)hipFree(s->data_x);
+  if (s->data_y) (void)hipFree(s->data_y);
+  if (s->bands) (void)hipFree(s->bands);
+  if (s->d_starts) (void)hipFree(s->d_starts);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
   delete s;
 }
 
@@ -628,8 +628,8 @@ int mgpu_sg_gradients(mgpu_sg* s, const float* x, const float* y, uint64_t n, fl
 int mgpu_sg_set_data(mgpu_sg* s, const float* x, const float* y, uint64_t n) {
   if (!s || !x || !y || n == 0) return sg_fail(s, MGPU_EINVAL, "null argument or empty data");
   if (n > 0xFFFFFFFFull) return sg_fail(s, MGPU_ERANGE, "too many samples");
-  if (s->data_x) hipFree(s->data_x);
-  if (s->data_y) hipFree(s->data_y);
+  if (s->data_x) (void)hipFree(s->data_x);
+  if (s->data_y) (void)hipFree(s->data_y);
   s->data_x = nullptr;
   s->data_y = nullptr;
   s->ndata = 0;
@@ -774,7 +774,7 @@ int mgpu_debug_mfma(mgpu_ctx* ctx, const uint16_t* a, const uint16_t* b, float* 
   hipStream_t stream = nullptr;
   int device = 0;
   if (mgpu_context_stream(ctx, (void**)&stream, &device) != MGPU_OK) return MGPU_EINVAL;
-  hipSetDevice(device);
+  (void)hipSetDevice(device);
   uint8_t* buf = nullptr;
   if (hipMalloc(&buf, 1024 + 1024 + 4096) != hipSuccess) return MGPU_ENOMEM;
   hipError_t e = hipMemcpy(buf, a, 1024, hipMemcpyHostToDevice);
@@ -784,7 +784,7 @@ int mgpu_debug_mfma(mgpu_ctx* ctx, const uint16_t* a, const uint16_t* b, float* 
                           reinterpret_cast<uint16_t*>(buf + 1024), reinterpret_cast<float*>(buf + 2048));
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   if (e == hipSuccess) e = hipMemcpy(d, buf + 2048, 4096, hipMemcpyDeviceToHost);
-  hipFree(buf);
+  (void)hipFree(buf);
   return e == hipSuccess ? MGPU_OK : MGPU_EDEVICE;
 }
 
