@@ -235,9 +235,11 @@ def sweep_options(p: Point, contact_density: float, seed: int, diagonal_width: i
 
 def simulate_points(ctx, layout: Layout, points, ncells: int = 512, contact_density: float = 30.0,
                     seed: int = 0, diagonal_width: int = 3_000_000, chunk: int = 64,
-                    stored_diagonals: int = 0):
+                    stored_diagonals: int = 0, name_ids=None):
     """Fast-mode simulation of each point (one parameter block per point); returns the [ND][nbins]
-    count bands and the device seconds.
+    count bands and the device seconds. name_ids (one per point, optional) gives the index in a
+    point's chromosome name ("sweep<i>", which enters its cells' seeds) instead of its position
+    in `points`.
 
     stored_diagonals > 0 keeps only that many diagonals of each band on the device: the band is
     declared that wide and the contact density scaled by (full rows / stored rows), so the
@@ -255,8 +257,9 @@ def simulate_points(ctx, layout: Layout, points, ncells: int = 512, contact_dens
         part = points[c0:c0 + chunk]
         params, chroms, bars, plans = [], [], [], []
         for k, p in enumerate(part):
+            nid = c0 + k if name_ids is None else name_ids[c0 + k]
             prm = S.make_params(sweep_options(p, contact_density, seed, diagonal_width))
-            ch = S.chrom_view(f"sweep{c0 + k}", layout.size)
+            ch = S.chrom_view(f"sweep{nid}", layout.size)
             params.append(prm)
             chroms.append(ch)
             bars.append(S.prepare_barriers(prm, scaled_records(layout, p.occ_scale), 0, layout.size))
@@ -341,15 +344,15 @@ def run_sweep(ctx, net, layout: Layout, points, judge_frac: float = 0.05, thresh
     out = {}
 
     def judge_worker():
-        # each judged point simulated as two independent halves of its cells, one job per half
-        # (a single persistent launch over every judged point's cells)
+        # each judged point simulated as two independent halves of its cells, both halves of
+        # every judged point in one persistent launch (a half's cells take their seeds from its
+        # own chromosome name, so the halves draw independent streams)
         jctx = S.Context(0)
         try:
             t1 = time.perf_counter()
-            out["a"], _ = simulate_points(jctx, layout, jp, ncells // 2, contact_density / 2,
-                                          seed=seed, stored_diagonals=Wn.NDIAG, chunk=len(jp))
-            out["b"], _ = simulate_points(jctx, layout, jp, ncells // 2, contact_density / 2,
-                                          seed=seed + 1, stored_diagonals=Wn.NDIAG, chunk=len(jp))
+            both, _ = simulate_points(jctx, layout, jp + jp, ncells // 2, contact_density / 2,
+                                      seed=seed, stored_diagonals=Wn.NDIAG, chunk=2 * len(jp))
+            out["a"], out["b"] = both[:len(jp)], both[len(jp):]
             out["t"] = time.perf_counter() - t1
         except BaseException as e:  # re-raised by the sweep thread
             out["err"] = e

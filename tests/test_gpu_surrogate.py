@@ -192,6 +192,24 @@ def test_narrow_judge_bands_equal_the_full_width_run(ctx):
         assert f.shape == n.shape and np.array_equal(f, n)
 
 
+def test_judge_halves_in_one_launch_equal_separate_runs(ctx):
+    """The judge simulates both halves of its points in one launch, the second half under the
+    chromosome names that follow the first's: the bands equal two separate runs under those
+    names, bit for bit, and the two halves differ (independent cell streams)."""
+    from modlefork_amd import sweep as Sw
+
+    layout = Sw.fixed_layout()
+    pts = Sw.sample_points(3, seed=78)
+    a, _ = Sw.simulate_points(ctx, layout, pts, 32, 15.0, seed=5, stored_diagonals=100)
+    b, _ = Sw.simulate_points(ctx, layout, pts, 32, 15.0, seed=5, stored_diagonals=100,
+                              name_ids=[3, 4, 5])
+    both, _ = Sw.simulate_points(ctx, layout, pts + pts, 32, 15.0, seed=5, stored_diagonals=100,
+                                 chunk=6)
+    for x, y in zip(a + b, both):
+        assert x.shape == y.shape and np.array_equal(x, y)
+    assert not any(np.array_equal(x, y) for x, y in zip(a, b))
+
+
 def test_device_stitched_bands_match_host_stitching(ctx):
     """mgpu_sg_predict_chromosomes (forward, then stitching and expm1 / 10 on the device, bands
     kept in HBM) against mgpu_sg_forward + the host's denormalise and stitch."""
