@@ -7,8 +7,18 @@ kernel and the CPU oracle timed on this host's cores on a bounded sample of the 
 A "surrogate" object beside them reports the surrogate-accelerated parameter sweep: matrices/s of
 the network and of the plain simulation, the judged correlation and the network's MFMA rate.
 
+Fast mode runs one persistent launch per size tier at the tiers' default workgroup sizes (1024,
+512 and 256 lanes per cell). --autotune first times every size on a sample of each tier's cells
+(an eighth of them, and at least as many as the chip keeps resident) and keeps the fastest per
+tier: on the genome workload it keeps 1024 and 512 lanes for the two large tiers and finds 256
+and 512 lanes within noise for the small one, so the default run is the tuned one; the rocprof
+counters under profiles/ are keyed by the launch shape they were taken on. The timed steps run
+eagerly, so HIP events around each launch give the per-launch roofline in the timed region
+(mgpu_job_capture's graph replays carry no per-launch events).
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload genome|small|stress]
                     [--mode fast|det] [--cpu-seconds S] [--sweep-points P | --no-sweep]
+                    [--autotune]
 """
 from __future__ import annotations
 
@@ -179,6 +189,10 @@ def main():
     ap.add_argument("--no-sweep", action="store_true", help="skip the surrogate sweep leg")
     ap.add_argument("--threads", type=int, default=0,
                     help="fast mode: workgroup size per cell (256, 512 or 1024; 0 = default)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time every workgroup size per size tier first and keep the fastest")
+    ap.add_argument("--autotune-cells", type=int, default=0,
+                    help="cells the autotuner times each workgroup size on (0: an eighth)")
     ap.add_argument("--phases", action="store_true",
                     help="fast mode: after the timed steps, run one more step with the epoch-phase "
                          "timers on and print each phase's share of the time to stderr")
@@ -216,8 +230,18 @@ def main():
     log(f"workload {wl.name}: {len(chroms)} chromosomes, "
         f"{sum(len(p.targets) for p in plans)} cells, mode {args.mode}")
     job = S.Job(sim.ctx, sim.params, chroms, bars, plans, mode)
+    autotuned = None
     if args.threads:
         job.set_tuning(args.threads)
+    elif mode == L.MODE_FAST and args.autotune:
+        # the autotuner times every workgroup size on a sample of each size tier's cells and
+        # keeps the fastest per tier (untimed: before the warmup)
+        total = sum(len(p.targets) for p in plans)
+        sample = args.autotune_cells or max(1, total // 8)
+        sizes, tms = job.autotune(sample)
+        autotuned = {"sample_cells": sample, "threads": sizes,
+                     "ms_by_threads": {str(k): round(float(v), 1) for k, v in tms.items()}}
+        log(f"autotune on {sample} cells: {sizes} (ms by size {autotuned['ms_by_threads']})")
     tuning = job.tuning()
     launches = [{"threads": int(tuning.tier_threads[k]), "grid": int(tuning.tier_grid[k])}
                 for k in range(int(tuning.launches))]
@@ -307,7 +331,7 @@ def main():
                    "bin_size": int(sim.params.bin_size),
                    "diagonal_width": int(sim.params.diagonal_width),
                    "lefs_per_mbp": float(sim.params.lefs_per_mbp), "parallelism": "none",
-                   "launches": launches},
+                   "launches": launches, "autotuned": autotuned},
         "contacts_per_sec": contacts * args.steps / wall,
         "epochs_per_sec": epochs * args.steps / wall,
         "kernel_ms": float(np.mean(kernel_ms)),

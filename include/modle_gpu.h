@@ -286,7 +286,9 @@ typedef struct mgpu_tuning {
 int mgpu_job_set_tuning(mgpu_job* job, const mgpu_tuning* t);
 int mgpu_job_get_tuning(mgpu_job* job, mgpu_tuning* t);
 /* Times each fast-mode workgroup size (64 to 1024 lanes) on sample_cells cells spread over the
- * job (0: all), per size tier (or for the single launch a global_working_set job makes), keeps
+ * job (0: all; each tier gets its share, but at least 4 cells per CU or all of its cells, so
+ * every size runs at the occupancy of the full run), per size tier (or for the single launch a
+ * global_working_set job makes), keeps
  * the fastest for later runs and reports the tuning in *chosen (may be null); ms (5 entries,
  * may be null) receives the times summed over the tiers. Deterministic mode: no-op. */
 int mgpu_job_autotune(mgpu_job* job, uint64_t sample_cells, mgpu_tuning* chosen, float* ms);

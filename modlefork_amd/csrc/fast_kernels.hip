@@ -1837,7 +1837,11 @@ int fast_job_autotune(hipStream_t stream, FastJob* job, const WorkItem* d_items,
   std::vector<uint64_t> at, cnt;
   hipError_t e = hipMemcpy(all.data(), impl->d_order, all.size() * 4, hipMemcpyDeviceToHost);
   for (const Tier& t : runs) {
-    const uint64_t s = std::min<uint64_t>(t.count, std::max<uint64_t>(1, sample * t.count / impl->nitems));
+    // at least as many cells as a launch of the tier keeps resident on the chip (4 per CU at
+    // most), so every size is timed at the occupancy the full run has
+    const uint64_t fill = 4ull * impl->num_cus;
+    const uint64_t s = std::min<uint64_t>(
+        t.count, std::max<uint64_t>(fill, std::max<uint64_t>(1, sample * t.count / impl->nitems)));
     const uint64_t stride = t.count / s;
     at.push_back(pick.size());
     cnt.push_back(s);
