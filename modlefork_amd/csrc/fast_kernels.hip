@@ -367,6 +367,58 @@ __device__ inline void mark(const FastParams& P, Shared& S, int ph) {
   }
 }
 
+// merge_kept_new for both unit sides at once, their searches and walks interleaved in each lane
+// (independent dependency chains): kept orders kr (rev positions, new keys after kept ones at
+// ties) and kf (fwd positions, new keys first at ties), results in c.rr and c.fr.
+template <int NT>
+__device__ void merge_both(const uint32_t* kr, const uint32_t* kf, uint32_t nk,
+                           const uint64_t* keys, uint32_t k, Cell& c) {
+  if (nk == 0) {
+    for (uint32_t b = threadIdx.x; b < k; b += NT) c.rr[b] = c.fr[b] = (uint16_t)keys[b];
+    return;
+  }
+  auto before_r = [](uint64_t q, uint32_t p) { return (uint32_t)(q >> 32) < p; };
+  auto before_f = [](uint64_t q, uint32_t p) { return (uint32_t)(q >> 32) <= p; };
+  const uint32_t chunk = (nk + NT - 1) / NT;
+  const uint32_t ab = threadIdx.x * chunk, ae = min(nk, ab + chunk);
+  if (ab >= ae) return;
+  uint32_t lor = 0, lof = 0;
+  if (ab > 0) {
+    const uint32_t pr = c.rev[kr[ab - 1]], pf = c.fwd[kf[ab - 1]];
+    uint32_t hir = k, hif = k;
+    while (lor < hir || lof < hif) {
+      if (lor < hir) {
+        const uint32_t mid = (lor + hir) >> 1;
+        if (before_r(keys[mid], pr)) lor = mid + 1;
+        else hir = mid;
+      }
+      if (lof < hif) {
+        const uint32_t mid = (lof + hif) >> 1;
+        if (before_f(keys[mid], pf)) lof = mid + 1;
+        else hif = mid;
+      }
+    }
+  }
+  for (uint32_t a = ab; a < ae; ++a) {
+    const uint32_t lr = kr[a], lf = kf[a];
+    const uint32_t pr = c.rev[lr], pf = c.fwd[lf];
+    while (lor < k && before_r(keys[lor], pr)) {
+      c.rr[lor + a] = (uint16_t)keys[lor];
+      ++lor;
+    }
+    c.rr[a + lor] = (uint16_t)lr;
+    while (lof < k && before_f(keys[lof], pf)) {
+      c.fr[lof + a] = (uint16_t)keys[lof];
+      ++lof;
+    }
+    c.fr[a + lof] = (uint16_t)lf;
+  }
+  if (ae == nk) {
+    for (; lor < k; ++lor) c.rr[lor + nk] = (uint16_t)keys[lor];
+    for (; lof < k; ++lof) c.fr[lof + nk] = (uint16_t)keys[lof];
+  }
+}
+
 // Merges the LEFs flagged in is_new into one unit side's order: tmp_a[0, nkept) holds the kept
 // entries of last epoch's order (compacted stably); the result goes to ranks[0, nact). More
 // than kNewCap new LEFs are merged in batches through tmp_b (n-sized scratch). kNewFirst: new
@@ -498,10 +550,7 @@ __device__ void rerank_both(Cell& c, uint32_t nact, uint32_t nact_old, uint64_t*
     }
   } else {
     sort_new_keys<NT>(keys, k);
-    merge_kept_new<NT, false>(kr, nkept, keys, k, c.rev,
-                              [&](uint32_t i, uint32_t lef) { c.rr[i] = (uint16_t)lef; });
-    merge_kept_new<NT, true>(kf, nkept, keys, k, c.fwd,
-                             [&](uint32_t i, uint32_t lef) { c.fr[i] = (uint16_t)lef; });
+    merge_both<NT>(kr, kf, nkept, keys, k, c);
   }
   __syncthreads();
   mark(P, S, 2);
