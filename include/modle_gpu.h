@@ -144,6 +144,11 @@ typedef struct mgpu_stats {
   uint64_t missed;        /* out-of-band updates */
   double kernel_ms;       /* device time of the simulation launches (HIP events) */
   double total_ms;        /* wall time of the call */
+  /* Fast mode only: contacts (and 1D occupancy samples) abandoned after 65 536 rejected draws
+   * found no LEF inside the chromosome; sum(band) + missed + dropped == contacts. Always 0 in
+   * deterministic mode, which retries without bound as the reference does. */
+  uint64_t dropped;
+  uint64_t dropped_occupancy;
 } mgpu_stats;
 
 typedef struct mgpu_ctx mgpu_ctx;

@@ -157,6 +157,8 @@ class MgpuStats(C.Structure):
         ("missed", C.c_uint64),
         ("kernel_ms", C.c_double),
         ("total_ms", C.c_double),
+        ("dropped", C.c_uint64),
+        ("dropped_occupancy", C.c_uint64),
     ]
 
 

@@ -556,11 +556,13 @@ __device__ void rerank_both(Cell& c, uint32_t nact, uint32_t nact_old, uint64_t*
   mark(P, S, 2);
 }
 
-__device__ inline float gev_noise(const FastParams& P, uint32_t x) {
-  // mu + sigma * (1 - (-ln u)^xi) / xi; expm1 keeps the small-xi cancellation exact
-  const float y = logf(-logf(u01_open(x)));
-  if (P.gev_xi == 0.0f) return (P.gev_mu - P.gev_sigma) * y;
-  return P.gev_mu - P.gev_sigma * expm1f(P.gev_xi * y) / P.gev_xi;
+// Generalised extreme value noise (genextreme_value_distribution.hpp:87-105) in double
+// precision from a 53-bit uniform: mu + sigma * (1 - (-ln u)^xi) / xi, written with expm1 so the
+// small-xi cancellation stays exact; xi == 0 takes the reference's (mu - sigma) * ln(-ln u).
+__device__ inline double gev_noise(const FastParams& P, uint32_t a, uint32_t b) {
+  const double y = log(-log(u01d_open(a, b)));
+  if (P.gev_xi == 0.0) return (P.gev_mu - P.gev_sigma) * y;
+  return P.gev_mu - P.gev_sigma * expm1(P.gev_xi * y) / P.gev_xi;
 }
 
 __device__ inline void band_add(const FastChrom& ch, uint32_t row, uint32_t col) {
@@ -573,11 +575,13 @@ __device__ inline void band_add(const FastChrom& ch, uint32_t row, uint32_t col)
   atomicAdd(&ch.band[(uint64_t)j * ch.nrows + i], 1u);
 }
 
-__device__ inline uint32_t poisson(const Rng& R, double mean, uint32_t tag) {
-  const U4 d = R.draw(0, tag, kSPoisson);
+// Poisson(mean) LEF activations of burn-in epoch `epoch`; `sub` numbers the repeated draws of
+// an epoch that activated none (the first epoch repeats until one LEF is active).
+__device__ inline uint32_t poisson(const Rng& R, double mean, uint32_t epoch, uint32_t sub) {
+  const U4 d = R.draw(0, epoch, kSPoisson, sub);
   if (mean > 500.0) {  // normal approximation far from the inversion range
-    const float r = sqrtf(-2.0f * logf(u01_open(d.x)));
-    const double v = mean + sqrt(mean) * (double)(r * cospif(2.0f * u01(d.y)));
+    const double r = sqrt(-2.0 * log(u01d_open(d.x, d.y)));
+    const double v = mean + sqrt(mean) * r * cospi(2.0 * u01d(d.z, d.w));
     return v < 0 ? 0u : (uint32_t)(v + 0.5);
   }
   // inversion by sequential search
@@ -926,7 +930,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
           uint32_t sub = 0;
           do {
             ++S.burnin_epochs;
-            S.nact = min(S.nact + poisson(R, ch.bind_rate, epoch * 16 + sub++), N);
+            S.nact = min(S.nact + poisson(R, ch.bind_rate, epoch, sub++), N);
           } while (S.nact == 0);
         }
         __syncthreads();
@@ -1037,16 +1041,19 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
       uint64_t want = ch.cpe;
       if (P.target_contact_density > 0.0) want = min(want, target - contacts);
       const uint32_t sp = ch.start + 1, ep = ch.end - 1;
+      uint32_t dropped = 0, dropped_occ = 0;
       for (uint32_t s = threadIdx.x; s < want; s += NT) {
-        for (uint32_t attempt = 0; attempt < 1024; ++attempt) {
+        uint32_t attempt = 0;
+        for (; attempt < kFastMaxAttempts; ++attempt) {
           const U4 d = R.draw(s, epoch, kSContact, attempt);
           const uint32_t i = ubounded(d.x, n);
           const uint32_t r = c.rev[i], f = c.fwd[i];
           if (!(r > sp && r < ep && f > sp && f < ep)) continue;
           double p1 = (double)r, p2 = (double)f;
           if (P.noisify) {
-            p1 -= (double)gev_noise(P, d.y);
-            p2 += (double)gev_noise(P, d.z);
+            const U4 g = R.draw(s, epoch, kSContact, attempt | 0x40000000u);
+            p1 -= gev_noise(P, g.x, g.y);
+            p2 += gev_noise(P, g.z, g.w);
           }
           if (p2 < p1) {
             const double t = p1;
@@ -1065,18 +1072,21 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
           band_add(ch, (q1 - sp) / P.bin_size, (q2 - sp) / P.bin_size);
           break;
         }
+        dropped += attempt == kFastMaxAttempts;
         // 1D LEF occupancy (register_1d_lef_occupancy, register_contacts.cpp:200-232): one more
         // LEF per sampling event, both of its (noisified) units
         if (kOut && ch.occupancy) {
-          for (uint32_t attempt = 0; attempt < 1024; ++attempt) {
-            const U4 d = R.draw(s, epoch, kSOccupancy, attempt);
+          uint32_t a2 = 0;
+          for (; a2 < kFastMaxAttempts; ++a2) {
+            const U4 d = R.draw(s, epoch, kSOccupancy, a2);
             const uint32_t i = ubounded(d.x, n);
             const uint32_t r = c.rev[i], f = c.fwd[i];
             if (!(r > sp && r < ep && f > sp && f < ep)) continue;
             double p1 = (double)r, p2 = (double)f;
             if (P.noisify) {
-              p1 -= (double)gev_noise(P, d.y);
-              p2 += (double)gev_noise(P, d.z);
+              const U4 g = R.draw(s, epoch, kSOccupancy, a2 | 0x40000000u);
+              p1 -= gev_noise(P, g.x, g.y);
+              p2 += gev_noise(P, g.z, g.w);
             }
             if (p2 < p1) {
               const double t = p1;
@@ -1088,8 +1098,13 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
             atomicAdd(&ch.occupancy[((uint32_t)p2 - sp) / P.bin_size], 1ULL);
             break;
           }
+          dropped_occ += a2 == kFastMaxAttempts;
         }
       }
+      // a sample that found no LEF inside the chromosome in kFastMaxAttempts draws is counted,
+      // so that band + missed + dropped equals the contacts the cell was due to register
+      if (dropped) atomicAdd(&ch.dropped[0], (unsigned long long)dropped);
+      if (dropped_occ) atomicAdd(&ch.dropped[1], (unsigned long long)dropped_occ);
       // no barrier: the phases up to extrusion do not write the positions sampled here, and a
       // cell that ends here meets the barrier after it before the next cell is set up
       contacts = uniform64(contacts + want);
@@ -1098,17 +1113,23 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     }
 
     // ---- moves (generate_moves, simulation.cpp:265-323) ----
-    const float vr = bdone ? P.rev_speed : P.rev_speed_burnin;
-    const float vf = bdone ? P.fwd_speed : P.fwd_speed_burnin;
+    // round(max(0, N(v, sigma))) in double precision, both sides from one Box-Muller pair of
+    // 53-bit uniforms; moves are capped at kFastMaxMove (beyond any chromosome fast mode takes)
+    const double vr = bdone ? P.rev_speed : P.rev_speed_burnin;
+    const double vf = bdone ? P.fwd_speed : P.fwd_speed_burnin;
+    const bool noisy_moves = P.rev_std != 0.0 || P.fwd_std != 0.0;
     for (uint32_t i = threadIdx.x; i < n; i += NT) {
-      const U4 d = R.draw(i, epoch, kSMove);
-      const float rad = sqrtf(-2.0f * logf(u01_open(d.x)));
-      float sn, cs;
-      sincospif(2.0f * u01(d.y), &sn, &cs);
-      const float a = P.rev_std == 0.0f ? vr : vr + P.rev_std * rad * cs;
-      const float b = P.fwd_std == 0.0f ? vf : vf + P.fwd_std * rad * sn;
-      c.rm[i] = (uint32_t)roundf(a > 0.0f ? a : 0.0f);
-      c.fm[i] = (uint32_t)roundf(b > 0.0f ? b : 0.0f);
+      double a = vr, b = vf;
+      if (noisy_moves) {
+        const U4 d = R.draw(i, epoch, kSMove);
+        const double rad = sqrt(-2.0 * log(u01d_open(d.x, d.y)));
+        double sn, cs;
+        sincospi(2.0 * u01d(d.z, d.w), &sn, &cs);
+        if (P.rev_std != 0.0) a += P.rev_std * rad * cs;
+        if (P.fwd_std != 0.0) b += P.fwd_std * rad * sn;
+      }
+      c.rm[i] = (uint32_t)fmin(round(fmax(a, 0.0)), (double)kFastMaxMove);
+      c.fm[i] = (uint32_t)fmin(round(fmax(b, 0.0)), (double)kFastMaxMove);
     }
     __syncthreads();
     mark(P, S, 5);
@@ -1674,21 +1695,21 @@ int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& dchroms,
   auto* impl = new FastImpl();
   FastParams& P = impl->P;
   P.bin_size = (uint32_t)p.bin_size;
-  P.rev_speed = (float)p.rev_speed;
-  P.fwd_speed = (float)p.fwd_speed;
-  P.rev_speed_burnin = (float)p.rev_speed_burnin;
-  P.fwd_speed_burnin = (float)p.fwd_speed_burnin;
-  P.rev_std = (float)p.rev_speed_std;
-  P.fwd_std = (float)p.fwd_speed_std;
+  P.rev_speed = (double)p.rev_speed;
+  P.fwd_speed = (double)p.fwd_speed;
+  P.rev_speed_burnin = (double)p.rev_speed_burnin;
+  P.fwd_speed_burnin = (double)p.fwd_speed_burnin;
+  P.rev_std = p.rev_speed_std;
+  P.fwd_std = p.fwd_speed_std;
   P.t_bypass_keep = thresh(1.0 - p.p_bypass);
   P.t_pblock_major = thresh(p.pblock_major);
   P.t_pblock_minor = thresh(p.pblock_minor);
   P.inv_soft = 1.0 / p.soft_stall_mult;
   P.inv_hard = 1.0 / p.hard_stall_mult;
   P.t_loop = std::isinf(p.tad_to_loop_ratio) ? 0 : thresh(1.0 / (p.tad_to_loop_ratio + 1.0));
-  P.gev_mu = (float)p.gev_mu;
-  P.gev_sigma = (float)p.gev_sigma;
-  P.gev_xi = (float)p.gev_xi;
+  P.gev_mu = p.gev_mu;
+  P.gev_sigma = p.gev_sigma;
+  P.gev_xi = p.gev_xi;
   P.noisify = p.noisify;
   P.skip_burnin = p.skip_burnin;
   P.hist_len = (uint32_t)p.burnin_history_length;
@@ -1783,6 +1804,7 @@ int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& dchroms,
     f.bar_dir = d.bar_dir;
     f.band = d.band;
     f.missed = d.missed;
+    f.dropped = d.dropped;
     f.nrows = (uint32_t)d.nrows;
     f.cpe = (uint32_t)d.cpe;
     f.target_epochs = d.target_epochs;

@@ -34,6 +34,15 @@ constexpr int kFastPhases = 16;  // epoch phases timed when FastParams::timers i
 constexpr uint32_t kFastDefaultThreads = 512;  // lanes per cell of launches from device memory
 constexpr uint32_t kFreePos = 0xFFFFFFFFu;
 constexpr uint32_t kFastBurninCap = 100000;  // burn-in epochs after which fast mode stops it
+// Rejection sampling of a contact (or an occupancy sample) gives up after this many draws and
+// counts the sample as dropped; the reference retries without bound
+// (register_contacts.cpp:144-160,177-195).
+constexpr uint32_t kFastMaxAttempts = 1u << 16;
+// Positions and the per-epoch move arithmetic use 32-bit integers (signed where targets may lie
+// before the chromosome start): chromosome ends stay at or below kFastMaxEnd (2^30) and raw
+// moves are capped at 2^30, so every target, plus or minus a rank below 2^16, fits in int32.
+constexpr uint32_t kFastMaxMove = 1u << 30;
+static_assert(kFastMaxEnd == (1ull << 30), "fast-mode move arithmetic assumes ends <= 2^30");
 
 // collision word: kind in bits 29..31, index in bits 0..28
 constexpr uint32_t kCkNone = 0, kCkChrom = 1, kCkBar = 2, kCkPrimary = 3, kCkSecondary = 4,
@@ -84,9 +93,14 @@ struct Rng {
   }
 };
 
-// uniform in [0,1) with 24 random bits, and in (0,1] for logs
-__device__ inline float u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
-__device__ inline float u01_open(uint32_t x) { return ((float)(x >> 8) + 1.0f) * (1.0f / 16777216.0f); }
+// uniform doubles with 53 random bits taken from two words (27 from a, 26 from b), the
+// resolution of the reference's generate_canonical<double, 53>: in [0, 1), and in (0, 1] for logs
+__device__ inline double u01d(uint32_t a, uint32_t b) {
+  return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+__device__ inline double u01d_open(uint32_t a, uint32_t b) {
+  return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6) + 1.0) * (1.0 / 9007199254740992.0);
+}
 // integer in [0, range) by multiply-shift (Lemire); bias < range / 2^32, negligible here
 __device__ inline uint32_t ubounded(uint32_t x, uint32_t range) {
   return (uint32_t)(((uint64_t)x * range) >> 32);
@@ -102,13 +116,13 @@ __device__ __host__ inline uint64_t thresh(double p) {
 // ---- per-job constants ----
 struct FastParams {
   uint32_t bin_size;
-  float rev_speed, fwd_speed, rev_speed_burnin, fwd_speed_burnin;
-  float rev_std, fwd_std;
+  double rev_speed, fwd_speed, rev_speed_burnin, fwd_speed_burnin;
+  double rev_std, fwd_std;
   uint64_t t_bypass_keep;    // threshold of (1 - p_bypass): collision happens
   uint64_t t_pblock_major, t_pblock_minor;
   double inv_soft, inv_hard;
   uint64_t t_loop;           // threshold of 1 / (ratio + 1): a contact is a loop contact
-  float gev_mu, gev_sigma, gev_xi;
+  double gev_mu, gev_sigma, gev_xi;
   int32_t noisify, skip_burnin;
   uint32_t hist_len, window;
   uint64_t min_burnin_epochs, max_burnin_epochs;
@@ -128,6 +142,8 @@ struct FastChrom {
   const uint32_t* bar_t_occ;       // threshold of the stationary occupancy
   uint32_t* band;
   unsigned long long* missed;
+  // samples abandoned after kFastMaxAttempts rejected draws: [0] contacts, [1] 1D occupancy
+  unsigned long long* dropped;
   uint32_t nrows;
   uint32_t cpe;
   uint64_t target_epochs;

@@ -51,6 +51,9 @@ struct DetParams {
   StateLog log;
 };
 
+// Largest chromosome end (bp) fast mode accepts: its per-epoch move arithmetic is 32-bit.
+constexpr uint64_t kFastMaxEnd = 1ull << 30;
+
 // Device view of one chromosome: barrier table (sorted), band and its counters.
 struct DetChrom {
   uint64_t start, end;
@@ -61,6 +64,7 @@ struct DetChrom {
   const double* bar_stp_inactive;
   uint32_t* band;
   unsigned long long* missed;
+  unsigned long long* dropped;  // fast mode's abandoned samples: [0] contacts, [1] occupancy
   uint64_t nrows;
   uint64_t cpe;            // contacts per epoch
   uint64_t target_epochs;  // epochs criterion

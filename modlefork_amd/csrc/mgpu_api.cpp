@@ -414,6 +414,7 @@ struct mgpu_job {
   void* d_pool = nullptr;                // one allocation for all inputs and counters
   uint32_t* d_bands = nullptr;
   unsigned long long* d_missed = nullptr;
+  unsigned long long* d_dropped = nullptr;  // 2 per chromosome: contacts, occupancy samples
   unsigned long long* d_next = nullptr;
   DetChrom* d_chroms = nullptr;
   WorkItem* d_items = nullptr;
@@ -516,6 +517,13 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
       return fail(ctx, MGPU_ERANGE, "chromosome end %llu beyond the 32-bit position range",
                   (unsigned long long)chroms[c].end);
     }
+    if (mode == MGPU_MODE_FAST && chroms[c].end > kFastMaxEnd) {
+      delete job;
+      return fail(ctx, MGPU_ERANGE,
+                  "chromosome end %llu: fast mode takes chromosome ends up to %llu bp (its "
+                  "per-epoch move arithmetic is 32-bit); use deterministic mode",
+                  (unsigned long long)chroms[c].end, (unsigned long long)kFastMaxEnd);
+    }
     if (nlefs[c] == 0 || nlefs[c] > (1u << 24) || ncells[c] == 0) {
       delete job;
       return fail(ctx, MGPU_EINVAL, "chromosome %llu: invalid LEF or cell count",
@@ -568,6 +576,7 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
     job->band_total += ((nr * nc + 1) + 63) & ~63ULL;
   }
   const uint64_t off_missed = pool.add(nchroms * 8);
+  const uint64_t off_dropped = pool.add(nchroms * 16);
   const uint64_t off_next = pool.add(64);
   const uint64_t off_stats = pool.add(job->nitems * 4 * 8);
   const uint64_t off_bands = pool.add(job->band_total * 4);
@@ -603,6 +612,7 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
   }
   uint8_t* base = static_cast<uint8_t*>(job->d_pool);
   job->d_missed = reinterpret_cast<unsigned long long*>(base + off_missed);
+  job->d_dropped = reinterpret_cast<unsigned long long*>(base + off_dropped);
   job->d_next = reinterpret_cast<unsigned long long*>(base + off_next);
   job->d_chroms = reinterpret_cast<DetChrom*>(base + off_chroms);
   job->d_items = reinterpret_cast<WorkItem*>(base + off_items);
@@ -639,6 +649,7 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
     ch.bar_stp_inactive = reinterpret_cast<const double*>(base + off_si[c]);
     ch.band = job->d_bands + job->band_off[c];
     ch.missed = job->d_missed + c;
+    ch.dropped = job->d_dropped + 2 * c;
     ch.nrows = job->nrows[c];
     ch.cpe = cpe_of[c];
     ch.target_epochs = target_epochs[c];
@@ -731,6 +742,7 @@ namespace {
 hipError_t enqueue_clears(mgpu_job* job, hipStream_t s) {
   hipError_t e = launch_zero_words(s, job->d_bands, job->band_total * 4);
   if (e == hipSuccess) e = launch_zero_words(s, job->d_missed, job->nchroms * 8);
+  if (e == hipSuccess) e = launch_zero_words(s, job->d_dropped, job->nchroms * 16);
   if (e == hipSuccess) e = launch_zero_words(s, job->d_next, 64);
   if (e == hipSuccess && job->d_occ) e = launch_zero_words(s, job->d_occ, job->occ_total * 8);
   if (e == hipSuccess && job->log.rec) e = launch_zero_words(s, job->log.count, 8);
@@ -825,8 +837,10 @@ int mgpu_job_run(mgpu_job* job, mgpu_stats* stats) {
     HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     std::vector<uint64_t> cs(job->nitems * 4);
     HIPCHK(ctx, hipMemcpy(cs.data(), job->d_cell_stats, cs.size() * 8, hipMemcpyDeviceToHost));
-    std::vector<unsigned long long> missed(job->nchroms);
+    std::vector<unsigned long long> missed(job->nchroms), dropped(2 * job->nchroms);
     HIPCHK(ctx, hipMemcpy(missed.data(), job->d_missed, job->nchroms * 8, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(dropped.data(), job->d_dropped, job->nchroms * 16,
+                          hipMemcpyDeviceToHost));
     *stats = mgpu_stats{};
     stats->cells = job->nitems;
     for (uint64_t i = 0; i < job->nitems; ++i) {
@@ -835,6 +849,10 @@ int mgpu_job_run(mgpu_job* job, mgpu_stats* stats) {
       stats->contacts += cs[4 * i + 2];
     }
     for (auto m : missed) stats->missed += m;
+    for (uint64_t c = 0; c < job->nchroms; ++c) {
+      stats->dropped += dropped[2 * c];
+      stats->dropped_occupancy += dropped[2 * c + 1];
+    }
     stats->kernel_ms = ms;
     stats->total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   }

@@ -38,9 +38,10 @@ def fingerprint(ctx, wl):
 
 
 def main():
+    path = sys.argv[1] if len(sys.argv) > 1 else PATH
     with S.Context(0) as ctx:
         res = {name: fingerprint(ctx, wl) for name, wl in cases().items()}
-    with open(PATH, "w") as fh:
+    with open(path, "w") as fh:
         json.dump(res, fh, indent=1)
         fh.write("\n")
     print(json.dumps(res, indent=1))

@@ -84,8 +84,51 @@ def test_fast_mode_is_deterministic_and_conserves_contacts(ctx):
     band_b, missed_b, _ = b["chrS"]
     say(f"fast small/16: {sa.kernel_ms:.1f} ms, epochs {sa.epochs}, burn-in {sa.burnin_epochs}")
     assert np.array_equal(band_a, band_b) and missed_a == missed_b
+    assert sa.dropped == 0 and sa.dropped_occupancy == 0
     assert int(band_a.sum()) + missed_a == int(plan.targets.sum())
     assert sa.contacts == int(plan.targets.sum())
+
+
+def test_fast_mode_rejects_chromosomes_beyond_its_position_range(ctx):
+    """Fast mode's move arithmetic is 32-bit: chromosomes ending beyond 2^30 bp are refused with
+    MGPU_ERANGE at job creation (nothing runs), and deterministic mode (64-bit) accepts them."""
+    size = (1 << 30) + 5_000_000
+    recs = [W.BedRecord("chrH", size // 2, size // 2 + 1, 0.9, "+")]
+    wl = W.Workload("huge", [("chrH", size)], recs,
+                    dict(bin_size=5_000, diagonal_width=1_000_000, number_of_lefs_per_mbp=0.01,
+                         num_cells=1, target_contact_density=0.0001, seed=1))
+    sim = S.Simulation(wl.options, ctx)
+    names, chroms, bars, plans = sim.prepare(wl)
+    with pytest.raises(L.MgpuError, match="fast mode takes chromosome ends up to"):
+        S.Job(ctx, sim.params, chroms, bars, plans, L.MODE_FAST)
+    S.Job(ctx, sim.params, chroms, bars, plans, L.MODE_DETERMINISTIC).close()
+    # the same chromosome trimmed to end at 2^30 is accepted and conserves its contacts
+    size = 1 << 30
+    wl.chroms = [("chrH", size)]
+    wl.barriers = [W.BedRecord("chrH", size // 2, size // 2 + 1, 0.9, "+")]
+    out, st = run_device(ctx, wl, L.MODE_FAST)
+    band, missed, plan = out["chrH"]
+    say(f"2^30 bp chromosome: {st.cells} cell, {st.epochs} epochs, {int(band.sum())} contacts")
+    assert st.dropped == 0
+    assert int(band.sum()) + missed == int(plan.targets.sum())
+
+
+def test_stacked_lefs_at_the_ends_conserve_contacts(ctx):
+    """Many LEFs on a short chromosome with fast, processive extrusion stack at the two ends,
+    where contact sampling rejects most draws (a LEF with a unit at an end is outside the
+    sampling interval). Every contact is either in the band, missed (outside it) or counted as
+    dropped: band + missed + dropped == target."""
+    size = 2_000_000
+    recs = [W.BedRecord("chrT", 1_000_000, 1_000_001, 0.5, "+")]
+    wl = W.Workload("stack", [("chrT", size)], recs,
+                    dict(bin_size=5_000, diagonal_width=2_000_000, number_of_lefs_per_mbp=200.0,
+                         avg_lef_processivity=5_000_000, rev_extrusion_speed=50_000,
+                         fwd_extrusion_speed=50_000, num_cells=8, target_contact_density=2.0,
+                         seed=4))
+    out, st = run_device(ctx, wl, L.MODE_FAST)
+    band, missed, plan = out["chrT"]
+    say(f"stacked ends: {st.epochs} epochs, missed {missed}, dropped {st.dropped}")
+    assert int(band.sum()) + missed + st.dropped == int(plan.targets.sum()) == st.contacts
 
 
 @pytest.mark.parametrize("threads", [64, 128, 512, 1024])
@@ -307,6 +350,7 @@ def test_fast_mode_genome_slice_runs(ctx):
     wl = W.genome(ncells=8, nchroms=3)
     out, st = run_device(ctx, wl, L.MODE_FAST)
     say(f"fast genome slice: {st.cells} cells, {st.kernel_ms:.1f} ms, epochs {st.epochs}")
+    assert st.dropped == 0
     for name, (band, missed, plan) in out.items():
         assert int(band.sum()) + missed == int(plan.targets.sum())
 
