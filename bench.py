@@ -324,7 +324,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u64/f64" if mode == L.MODE_DETERMINISTIC else "u32/f32",
+        "dtype": "u64/f64" if mode == L.MODE_DETERMINISTIC else "u32/f64",
         "data": "synthetic",
         "config": {"workload": wl.name, "chromosomes": len(chroms), "cells": cells,
                    "mode": "deterministic" if mode == L.MODE_DETERMINISTIC else "fast",

@@ -323,6 +323,10 @@ int mgpu_debug_draws(mgpu_ctx* ctx, uint64_t seed, int kind, double a, double b,
  * (counter[4], key[2]) words, out receives n x 4 output words. No counterpart in the reference
  * (fast mode replaces its engine); exposed for known-answer tests. */
 int mgpu_debug_philox(mgpu_ctx* ctx, const uint32_t* in, uint64_t n, uint32_t* out);
+/* Fast mode's double-precision elementary functions on the device (no counterpart in the
+ * reference, whose draws call libm): fn 0 ln(x), 1 expm1(x), 2 sin(2 pi x) and cos(2 pi x) (out
+ * holds 2n values, the sines first). Exposed for accuracy tests. */
+int mgpu_debug_fast_math(mgpu_ctx* ctx, int fn, const double* x, uint64_t n, double* out);
 /* Fast mode's epoch-phase timers (no counterpart in the reference): enable != 0 makes later
  * job runs on this context accumulate, per phase, the 100 MHz wall-clock ticks of every cell;
  * out (n entries, may be null) receives the totals, which are then cleared. Phases: 0 stop test

@@ -265,6 +265,7 @@ SIGNATURES = [
     ("mgpu_debug_draws", C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.c_double, C.c_double,
                                    C.c_double, C.c_uint64, U64P]),
     ("mgpu_debug_philox", C.c_int, [C.c_void_p, U32P, C.c_uint64, U32P]),
+    ("mgpu_debug_fast_math", C.c_int, [C.c_void_p, C.c_int, DP, C.c_uint64, DP]),
     ("mgpu_debug_phase_times", C.c_int, [C.c_void_p, C.c_int, U64P, C.c_uint64]),
     ("mgpu_debug_fast_serial_fixups", C.c_int, [C.c_void_p, C.c_int]),
     ("mgpu_context_stream", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int)]),

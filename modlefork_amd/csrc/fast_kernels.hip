@@ -6,11 +6,21 @@
 #include <string>
 
 #include "fast_kernels.h"
+#include "fast_math.h"
 #include "fast_sim.h"
 
 namespace mgpu {
 
 namespace {
+// The lane's index in the workgroup, read through a volatile move: every index computation
+// derived from it stays inside the epoch phase that uses it. With threadIdx.x itself the compiler
+// hoists dozens of per-lane index and address values out of the epoch loop, where they occupy
+// VGPRs for the whole cell and spill.
+__device__ inline uint32_t tidx() {
+  uint32_t t;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(t) : "v"((uint32_t)threadIdx.x));
+  return t;
+}
 
 // ---------------------------------------------------------------------------------------------
 // block primitives (NT lanes, wave64)
@@ -34,7 +44,7 @@ struct OpMaxU {
 // Exclusive scan of one value per lane across the block; sh holds NT / 64 entries.
 template <int NT, class T, class Op>
 __device__ T block_exclusive(T v, Op op, T ident, T* sh) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = tidx() & 63, w = tidx() >> 6;
   T incl = v;
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
@@ -55,7 +65,7 @@ __device__ T block_exclusive(T v, Op op, T ident, T* sh) {
 // pair of barriers; sh holds 2 * NT / 64 entries.
 template <int NT, class T, class OpA, class OpB>
 __device__ void block_exclusive2(T& a, T& b, OpA opa, OpB opb, T ia, T ib, T* sh) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = tidx() & 63, w = tidx() >> 6;
   T xa = a, xb = b;
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
@@ -92,9 +102,9 @@ template <int NT>
 __device__ uint64_t block_sum_u64(uint64_t v, uint64_t* sh) {
   constexpr int NW = NT / 64;
   for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  if ((tidx() & 63) == 0) sh[tidx() >> 6] = v;
   __syncthreads();
-  uint64_t t = (threadIdx.x & 63) < NW ? sh[threadIdx.x & 63] : 0;
+  uint64_t t = (tidx() & 63) < NW ? sh[tidx() & 63] : 0;
   for (int d = NW / 2; d >= 1; d >>= 1) t += __shfl_xor(t, d, 64);
   __syncthreads();
   return t;
@@ -109,7 +119,7 @@ __device__ void block_sum3_u64(uint64_t& a, uint64_t& b, uint64_t& c, uint64_t* 
     b += __shfl_xor(b, d, 64);
     c += __shfl_xor(c, d, 64);
   }
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = tidx() & 63, w = tidx() >> 6;
   if (lane == 0) {
     sh[w] = a;
     sh[NW + w] = b;
@@ -246,7 +256,7 @@ template <int NT>
 __device__ void bitonic(uint64_t* keys, uint32_t m) {
   for (uint32_t k = 2; k <= m; k <<= 1) {
     for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t i = threadIdx.x; i < m; i += NT) {
+      for (uint32_t i = tidx(); i < m; i += NT) {
         const uint32_t l = i ^ j;
         if (l > i) {
           const uint64_t a = keys[i], b = keys[l];
@@ -268,12 +278,12 @@ __device__ void bitonic(uint64_t* keys, uint32_t m) {
 template <int NT>
 __device__ void odd_even_sort(uint32_t* a, uint32_t n, const uint32_t* pos) {
   int bad = 0;
-  for (uint32_t i = threadIdx.x; i + 1 < n; i += NT) bad |= pos[a[i + 1]] < pos[a[i]];
+  for (uint32_t i = tidx(); i + 1 < n; i += NT) bad |= pos[a[i + 1]] < pos[a[i]];
   if (!__syncthreads_or(bad)) return;
   int quiet = 0;
   for (uint32_t pass = 0; quiet < 2; ++pass) {
     int swapped = 0;
-    for (uint32_t i = 2 * threadIdx.x + (pass & 1); i + 1 < n; i += 2 * NT) {
+    for (uint32_t i = 2 * tidx() + (pass & 1); i + 1 < n; i += 2 * NT) {
       const uint32_t x = a[i], y = a[i + 1];
       if (pos[y] < pos[x]) {
         a[i] = y;
@@ -293,18 +303,18 @@ __device__ void sort_new_keys(uint64_t* keys, uint32_t k) {
   if (k <= NT && k <= 512) {
     uint64_t mine = 0;
     uint32_t r = 0;
-    if (threadIdx.x < k) {
-      mine = keys[threadIdx.x];
+    if (tidx() < k) {
+      mine = keys[tidx()];
       for (uint32_t j = 0; j < k; ++j) r += keys[j] < mine;
     }
     __syncthreads();
-    if (threadIdx.x < k) keys[r] = mine;
+    if (tidx() < k) keys[r] = mine;
     __syncthreads();
     return;
   }
   uint32_t m = 1;
   while (m < k) m <<= 1;
-  for (uint32_t i = k + threadIdx.x; i < m; i += NT) keys[i] = ~0ULL;
+  for (uint32_t i = k + tidx(); i < m; i += NT) keys[i] = ~0ULL;
   __syncthreads();
   bitonic<NT>(keys, m);
 }
@@ -319,7 +329,7 @@ template <int NT, bool kNewFirst, class Out>
 __device__ void merge_kept_new(const uint32_t* kept, uint32_t nk, const uint64_t* keys, uint32_t k,
                                const uint32_t* pos, Out out) {
   if (nk == 0) {
-    for (uint32_t b = threadIdx.x; b < k; b += NT) out(b, (uint32_t)keys[b]);
+    for (uint32_t b = tidx(); b < k; b += NT) out(b, (uint32_t)keys[b]);
     return;
   }
   // whether new key q goes before a kept LEF at position p (true on a prefix of the keys)
@@ -327,7 +337,7 @@ __device__ void merge_kept_new(const uint32_t* kept, uint32_t nk, const uint64_t
     return kNewFirst ? (uint32_t)(q >> 32) <= p : (uint32_t)(q >> 32) < p;
   };
   const uint32_t chunk = (nk + NT - 1) / NT;
-  const uint32_t ab = threadIdx.x * chunk, ae = min(nk, ab + chunk);
+  const uint32_t ab = tidx() * chunk, ae = min(nk, ab + chunk);
   if (ab >= ae) return;
   // lo: new keys that go before kept[ab - 1] (0 for the first run)
   uint32_t lo = 0;
@@ -360,7 +370,7 @@ __device__ unsigned long long g_fast_phase[kFastPhases];
 // Phase timers: called by every lane right after the barrier that ends a phase; lane 0 adds the
 // wall-clock ticks since the previous mark to the phase's total.
 __device__ inline void mark(const FastParams& P, Shared& S, int ph) {
-  if (P.timers && threadIdx.x == 0) {
+  if (P.timers && tidx() == 0) {
     const uint64_t t = wall_clock64();
     S.phase[ph] += t - S.tick;
     S.tick = t;
@@ -374,13 +384,13 @@ template <int NT>
 __device__ void merge_both(const uint32_t* kr, const uint32_t* kf, uint32_t nk,
                            const uint64_t* keys, uint32_t k, Cell& c) {
   if (nk == 0) {
-    for (uint32_t b = threadIdx.x; b < k; b += NT) c.rr[b] = c.fr[b] = (uint16_t)keys[b];
+    for (uint32_t b = tidx(); b < k; b += NT) c.rr[b] = c.fr[b] = (uint16_t)keys[b];
     return;
   }
   auto before_r = [](uint64_t q, uint32_t p) { return (uint32_t)(q >> 32) < p; };
   auto before_f = [](uint64_t q, uint32_t p) { return (uint32_t)(q >> 32) <= p; };
   const uint32_t chunk = (nk + NT - 1) / NT;
-  const uint32_t ab = threadIdx.x * chunk, ae = min(nk, ab + chunk);
+  const uint32_t ab = tidx() * chunk, ae = min(nk, ab + chunk);
   if (ab >= ae) return;
   uint32_t lor = 0, lof = 0;
   if (ab > 0) {
@@ -438,9 +448,9 @@ __device__ void merge_new(uint16_t* ranks, const uint32_t* pos, uint32_t nact, u
   for (uint32_t bt = 0; bt < batches; ++bt) {
     const uint32_t lo_id = batches == 1 ? 0 : bt * kNewCap;
     const uint32_t hi_id = batches == 1 ? nact : min(nact, lo_id + kNewCap);
-    if (threadIdx.x == 0) S.nnew = 0;
+    if (tidx() == 0) S.nnew = 0;
     __syncthreads();
-    for (uint32_t l = lo_id + threadIdx.x; l < hi_id; l += NT) {
+    for (uint32_t l = lo_id + tidx(); l < hi_id; l += NT) {
       if (is_new[l]) keys[atomicAdd(&S.nnew, 1u)] = pos_key(pos, l);
     }
     __syncthreads();
@@ -458,12 +468,12 @@ __device__ void merge_new(uint16_t* ranks, const uint32_t* pos, uint32_t nact, u
                                     [&](uint32_t i, uint32_t lef) { tmp_b[i] = lef; });
       __syncthreads();
       nkept += k;
-      for (uint32_t i = threadIdx.x; i < nkept; i += NT) tmp_a[i] = tmp_b[i];
+      for (uint32_t i = tidx(); i < nkept; i += NT) tmp_a[i] = tmp_b[i];
       __syncthreads();
     }
   }
   if (!written) {
-    for (uint32_t i = threadIdx.x; i < nact; i += NT) ranks[i] = (uint16_t)tmp_a[i];
+    for (uint32_t i = tidx(); i < nact; i += NT) ranks[i] = (uint16_t)tmp_a[i];
     __syncthreads();
   }
 }
@@ -487,7 +497,7 @@ __device__ void rerank_both(Cell& c, uint32_t nact, uint32_t nact_old, uint64_t*
   uint32_t* kf = c.fc;
   {
     const uint32_t ch = (nact_old + NT - 1) / NT;
-    const uint32_t b = threadIdx.x * ch, e = min(nact_old, b + ch);
+    const uint32_t b = tidx() * ch, e = min(nact_old, b + ch);
     // kept counts of this lane's chunk: rev side in the low 16 bits, fwd side in the high 16
     // (at most 65535 LEFs per cell, so the low half cannot carry into the high one)
     uint32_t cnt = 0;
@@ -500,8 +510,8 @@ __device__ void rerank_both(Cell& c, uint32_t nact, uint32_t nact_old, uint64_t*
       if (is_new[lr] == 0) kr[orv++] = lr;
       if (is_new[lf] == 0) kf[ofw++] = lf;
     }
-    if (threadIdx.x == NT - 1) S.nkept = orv;  // the same LEFs are kept on both sides
-    if (threadIdx.x == 0) S.nnew = 0;
+    if (tidx() == NT - 1) S.nkept = orv;  // the same LEFs are kept on both sides
+    if (tidx() == 0) S.nnew = 0;
     __syncthreads();
   }
   const uint32_t nkept = S.nkept;
@@ -514,9 +524,9 @@ __device__ void rerank_both(Cell& c, uint32_t nact, uint32_t nact_old, uint64_t*
   }
   // both kept orders must be sorted exactly for the merge; one check pass covers both
   int bad = 0;
-  for (uint32_t i = threadIdx.x; i + 1 < nkept; i += NT)
+  for (uint32_t i = tidx(); i + 1 < nkept; i += NT)
     bad |= (c.rev[kr[i + 1]] < c.rev[kr[i]]) | (c.fwd[kf[i + 1]] < c.fwd[kf[i]]);
-  for (uint32_t l = threadIdx.x; l < nact; l += NT) {
+  for (uint32_t l = tidx(); l < nact; l += NT) {
     if (is_new[l]) keys[atomicAdd(&S.nnew, 1u)] = pos_key(c.rev, l);
   }
   if (__syncthreads_or(bad)) {
@@ -525,7 +535,7 @@ __device__ void rerank_both(Cell& c, uint32_t nact, uint32_t nact_old, uint64_t*
     int quiet = 0;
     for (uint32_t pass = 0; quiet < 2; ++pass) {
       int swapped = 0;
-      for (uint32_t i = 2 * threadIdx.x + (pass & 1); i + 1 < nkept; i += 2 * NT) {
+      for (uint32_t i = 2 * tidx() + (pass & 1); i + 1 < nkept; i += 2 * NT) {
         const uint32_t x = kr[i], y = kr[i + 1];
         if (c.rev[y] < c.rev[x]) {
           kr[i] = y;
@@ -544,7 +554,7 @@ __device__ void rerank_both(Cell& c, uint32_t nact, uint32_t nact_old, uint64_t*
   }
   const uint32_t k = S.nnew;
   if (k == 0) {
-    for (uint32_t i = threadIdx.x; i < nact; i += NT) {
+    for (uint32_t i = tidx(); i < nact; i += NT) {
       c.rr[i] = (uint16_t)kr[i];
       c.fr[i] = (uint16_t)kf[i];
     }
@@ -560,9 +570,9 @@ __device__ void rerank_both(Cell& c, uint32_t nact, uint32_t nact_old, uint64_t*
 // precision from a 53-bit uniform: mu + sigma * (1 - (-ln u)^xi) / xi, written with expm1 so the
 // small-xi cancellation stays exact; xi == 0 takes the reference's (mu - sigma) * ln(-ln u).
 __device__ inline double gev_noise(const FastParams& P, uint32_t a, uint32_t b) {
-  const double y = log(-log(u01d_open(a, b)));
+  const double y = ln_d(-ln_d(u01d_open(a, b)));
   if (P.gev_xi == 0.0) return (P.gev_mu - P.gev_sigma) * y;
-  return P.gev_mu - P.gev_sigma * expm1(P.gev_xi * y) / P.gev_xi;
+  return P.gev_mu - P.gev_sigma * expm1_d(P.gev_xi * y) / P.gev_xi;
 }
 
 __device__ inline void band_add(const FastChrom& ch, uint32_t row, uint32_t col) {
@@ -577,16 +587,20 @@ __device__ inline void band_add(const FastChrom& ch, uint32_t row, uint32_t col)
 
 // Poisson(mean) LEF activations of burn-in epoch `epoch`; `sub` numbers the repeated draws of
 // an epoch that activated none (the first epoch repeats until one LEF is active).
-__device__ inline uint32_t poisson(const Rng& R, double mean, uint32_t epoch, uint32_t sub) {
+// exp_neg and sd are e^-mean and sqrt(mean), computed once per chromosome on the host.
+__device__ inline uint32_t poisson(const Rng& R, double mean, double exp_neg, double sd,
+                                   uint32_t epoch, uint32_t sub) {
   const U4 d = R.draw(0, epoch, kSPoisson, sub);
   if (mean > 500.0) {  // normal approximation far from the inversion range
-    const double r = sqrt(-2.0 * log(u01d_open(d.x, d.y)));
-    const double v = mean + sqrt(mean) * r * cospi(2.0 * u01d(d.z, d.w));
+    const double r = sqrt(-2.0 * ln_d(u01d_open(d.x, d.y)));
+    double sn, cs;
+    sincos_2pi_d(u01d(d.z, d.w), &sn, &cs);
+    const double v = mean + sd * r * cs;
     return v < 0 ? 0u : (uint32_t)(v + 0.5);
   }
   // inversion by sequential search
   const double u = ((double)d.x * 4294967296.0 + (double)d.y) * (1.0 / 18446744073709551616.0);
-  double p = exp(-mean), s = p;
+  double p = exp_neg, s = p;
   uint32_t k = 0;
   while (u > s && k < 4096) {
     ++k;
@@ -612,7 +626,7 @@ __device__ void secondary_both(const FastParams& P, const Rng& R, uint32_t epoch
                                Shared& S, uint32_t n, uint32_t lo_rev, uint32_t lo_fwd,
                                uint16_t* F, uint32_t* na) {
   const uint32_t chunk = (n + NT - 1) / NT;
-  const uint32_t jb = threadIdx.x * chunk, je = min(n, jb + chunk);
+  const uint32_t jb = tidx() * chunk, je = min(n, jb + chunk);
   auto unit = [&](int sd, uint32_t j) -> uint32_t { return sd == 0 ? c.rr[j] : c.fr[n - 1 - j]; };
   auto coll = [&](int sd) -> uint32_t* { return sd == 0 ? c.rc : c.fc; };
   auto target = [&](int sd, uint32_t l) -> int32_t {
@@ -732,7 +746,7 @@ __device__ void secondary_both(const FastParams& P, const Rng& R, uint32_t epoch
     for (uint32_t j = jb; j < je; ++j)
       if (ck_kind(c.fc[unit(1, j)]) == kCkAvoided) F[n + o1++] = (uint16_t)(n - 1 - j);
   }
-  if (threadIdx.x == NT - 1) {
+  if (tidx() == NT - 1) {
     S.navoid = o0;
     S.navoid_fwd = o1;
   }
@@ -798,7 +812,7 @@ template <int NT>
 __device__ void fix_avoided(Cell& c, const FastChrom& ch, uint32_t n, const uint16_t* list,
                             uint32_t cnt, bool rev_side, bool serial) {
   if (serial) {
-    if (threadIdx.x == 0) {
+    if (tidx() == 0) {
       if (rev_side) {
         for (uint32_t a = 0; a < cnt; ++a) fix_one_rev(c, ch, list[a]);
       } else {
@@ -807,7 +821,7 @@ __device__ void fix_avoided(Cell& c, const FastChrom& ch, uint32_t n, const uint
     }
     return;
   }
-  for (uint32_t a = threadIdx.x; a < cnt; a += NT) {
+  for (uint32_t a = tidx(); a < cnt; a += NT) {
     if (rev_side) {
       if (a != 0 && list[a - 1] + 1 == list[a]) continue;  // not the lowest rank of its run
       for (uint32_t b = a;; ++b) {
@@ -832,7 +846,7 @@ __device__ __attribute__((always_inline)) void log_state(const FastParams& P, co
                                                     Cell& c, Shared& S, uint32_t n,
                                                     uint32_t epoch, bool bdone, uint32_t item) {
   uint64_t st_rev = 0, st_fwd = 0, st_both = 0, n_bar = 0, n_pri = 0, n_sec = 0, loops = 0;
-  for (uint32_t i = threadIdx.x; i < n; i += NT) {
+  for (uint32_t i = tidx(); i < n; i += NT) {
     const uint32_t kr = ck_kind(c.rc[i]), kf = ck_kind(c.fc[i]);
     const bool sr = ck_stalled(c.rc[i]), sf = ck_stalled(c.fc[i]);
     st_rev += sr;
@@ -844,14 +858,14 @@ __device__ __attribute__((always_inline)) void log_state(const FastParams& P, co
     loops += (uint64_t)((c.fwd[i] + c.fm[i]) - (c.rev[i] - c.rm[i]));
   }
   uint64_t n_act = 0;
-  for (uint32_t b = threadIdx.x; b < ch.nbar; b += NT) n_act += c.bst[b];
+  for (uint32_t b = tidx(); b < ch.nbar; b += NT) n_act += c.bst[b];
   // counts < 2^17 and loop sums < 2^48: two counts share one 64-bit reduction
   const uint64_t a0 = block_sum_u64<NT>(st_rev | (st_fwd << 32), S.shx);
   const uint64_t a1 = block_sum_u64<NT>(st_both | (n_act << 32), S.shx);
   const uint64_t a2 = block_sum_u64<NT>(n_bar | (n_pri << 32), S.shx);
   const uint64_t a3 = block_sum_u64<NT>(n_sec, S.shx);
   const uint64_t a4 = block_sum_u64<NT>(loops, S.shx);
-  if (threadIdx.x == 0) {
+  if (tidx() == 0) {
     const unsigned long long slot = atomicAdd(P.log.count, 1ULL);
     if (slot < P.log.cap) {
       StateRecord r{};
@@ -884,15 +898,15 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
   const uint32_t N = ch.nlefs, NB = ch.nbar;
   const Rng R{(uint32_t)seed, (uint32_t)(seed >> 32)};
   target = uniform64(target);
-  for (uint32_t i = threadIdx.x; i < N; i += NT) {
+  for (uint32_t i = tidx(); i < N; i += NT) {
     c.rev[i] = c.fwd[i] = kFreePos;
     c.rr[i] = c.fr[i] = (uint16_t)i;
   }
-  for (uint32_t b = threadIdx.x; b < NB; b += NT) {
+  for (uint32_t b = tidx(); b < NB; b += NT) {
     const U4 d = R.draw(b, 0xFFFFFFFFu, kSBarInit);
     c.bst[b] = d.x < ch.bar_t_occ[b] ? 1 : 0;
   }
-  if (threadIdx.x == 0) {
+  if (tidx() == 0) {
     S.nact = P.skip_burnin ? N : 0;
     S.burnin_done = P.skip_burnin ? 1 : 0;
     S.hlen = 0;
@@ -926,11 +940,12 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
       const bool activating = S.nact != N;
       __syncthreads();  // every lane has read nact before lane 0 changes it
       if (activating) {
-        if (threadIdx.x == 0) {
+        if (tidx() == 0) {
           uint32_t sub = 0;
           do {
             ++S.burnin_epochs;
-            S.nact = min(S.nact + poisson(R, ch.bind_rate, epoch, sub++), N);
+            S.nact = min(S.nact + poisson(R, ch.bind_rate, ch.bind_exp_neg, ch.bind_sd, epoch,
+                                          sub++), N);
           } while (S.nact == 0);
         }
         __syncthreads();
@@ -940,7 +955,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
         // do not depend on the reduction order: x < 2^32, per-lane sums of x^2 are split into
         // 32-bit halves that cannot overflow across the block
         uint64_t sx = 0, sq_lo = 0, sq_hi = 0;
-        for (uint32_t i = threadIdx.x; i < N; i += NT) {
+        for (uint32_t i = tidx(); i < N; i += NT) {
           const uint64_t x = c.rev[i] == kFreePos ? 0 : (uint64_t)(c.fwd[i] - c.rev[i]);
           const uint64_t x2 = x * x;
           sx += x;
@@ -957,7 +972,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
         const uint32_t cap = P.hist_len, w = P.window;
         // the history is a ring: logical entry i (0 = oldest) sits at (hhead + i) % cap, so a
         // full history drops its oldest entry without moving the others
-        if (threadIdx.x == 0) {
+        if (tidx() == 0) {
           ++S.burnin_epochs;
           uint32_t slot;
           if (S.hlen == cap) {
@@ -976,7 +991,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
           // both counts (at most cap <= 4096) share one scan
           const uint32_t head = S.hhead;
           int32_t d2 = 0;
-          for (uint32_t k = threadIdx.x; k + w + 1 <= cap - 1; k += NT) {
+          for (uint32_t k = tidx(); k + w + 1 <= cap - 1; k += NT) {
             double a1 = 0, a2 = 0, b1 = 0, b2 = 0;
             uint32_t q = head + k >= cap ? head + k - cap : head + k;
             for (uint32_t j = 0; j < w; ++j) {
@@ -991,7 +1006,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
           }
           const int32_t tot = block_exclusive<NT>(d2, OpAddI{}, 0, S.shi) + d2;
           const int32_t nc = tot & 0xFFFF, na = tot >> 16;
-          if (threadIdx.x == NT - 1) {
+          if (tidx() == NT - 1) {
             const double r1 = (double)nc / (double)((int32_t)(cap - w) - nc);
             const double r2 = (double)na / (double)((int32_t)(cap - w) - na);
             S.stable = (r1 >= 0.95 && r1 <= 1.05 && r2 >= 0.95 && r2 <= 1.05) ? 1 : 0;
@@ -999,7 +1014,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
           __syncthreads();
           stable = S.stable;
         }
-        if (threadIdx.x == 0) {
+        if (tidx() == 0) {
           bool done = stable && epoch > P.min_burnin_epochs;
           // --max-burnin-epochs, and a hard stop so a cell that never stabilises cannot keep a
           // workgroup busy indefinitely
@@ -1019,7 +1034,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     // this pass only binds the LEFs activated this epoch (all of them in the first epoch); it is
     // skipped once burn-in has activated every LEF.
     if (n > nact_prev) {
-      for (uint32_t i = nact_prev + threadIdx.x; i < n; i += NT) {
+      for (uint32_t i = nact_prev + tidx(); i < n; i += NT) {
         uint32_t flag = 0;
         if (c.rev[i] == kFreePos) {
           const U4 d = R.draw(i, epoch, kSBind);
@@ -1042,7 +1057,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
       if (P.target_contact_density > 0.0) want = min(want, target - contacts);
       const uint32_t sp = ch.start + 1, ep = ch.end - 1;
       uint32_t dropped = 0, dropped_occ = 0;
-      for (uint32_t s = threadIdx.x; s < want; s += NT) {
+      for (uint32_t s = tidx(); s < want; s += NT) {
         uint32_t attempt = 0;
         for (; attempt < kFastMaxAttempts; ++attempt) {
           const U4 d = R.draw(s, epoch, kSContact, attempt);
@@ -1118,13 +1133,13 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     const double vr = bdone ? P.rev_speed : P.rev_speed_burnin;
     const double vf = bdone ? P.fwd_speed : P.fwd_speed_burnin;
     const bool noisy_moves = P.rev_std != 0.0 || P.fwd_std != 0.0;
-    for (uint32_t i = threadIdx.x; i < n; i += NT) {
+    for (uint32_t i = tidx(); i < n; i += NT) {
       double a = vr, b = vf;
       if (noisy_moves) {
         const U4 d = R.draw(i, epoch, kSMove);
-        const double rad = sqrt(-2.0 * log(u01d_open(d.x, d.y)));
+        const double rad = sqrt(-2.0 * ln_d(u01d_open(d.x, d.y)));
         double sn, cs;
-        sincospi(2.0 * u01d(d.z, d.w), &sn, &cs);
+        sincos_2pi_d(u01d(d.z, d.w), &sn, &cs);
         if (P.rev_std != 0.0) a += P.rev_std * rad * cs;
         if (P.fwd_std != 0.0) b += P.fwd_std * rad * sn;
       }
@@ -1141,14 +1156,14 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     {
       int32_t* T = reinterpret_cast<int32_t*>(c.scratch);
       int32_t* Tf = reinterpret_cast<int32_t*>(c.fc);  // collisions are cleared below
-      for (uint32_t k = threadIdx.x; k < n; k += NT) {
+      for (uint32_t k = tidx(); k < n; k += NT) {
         const uint32_t l = c.rr[k], l2 = c.fr[k];
         T[k] = (int32_t)c.rev[l] - (int32_t)c.rm[l] + (int32_t)k;
         Tf[k] = (int32_t)c.fwd[l2] + (int32_t)c.fm[l2] - (int32_t)k;
       }
       __syncthreads();
       const uint32_t chunk = (n + NT - 1) / NT;
-      const uint32_t ib = threadIdx.x * chunk, ie = min(n, ib + chunk);
+      const uint32_t ib = tidx() * chunk, ie = min(n, ib + chunk);
       int32_t ar = INT32_MAX, af = INT32_MIN;
       for (uint32_t i = ib; i < ie; ++i) {
         ar = min(ar, T[n - 1 - i]);
@@ -1177,7 +1192,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
 
     // ---- barrier states (extrusion_barriers.cpp:145-161) ----
     // both thresholds are loaded ahead of the draw, so their latency hides behind it
-    for (uint32_t b = threadIdx.x; b < NB; b += NT) {
+    for (uint32_t b = tidx(); b < NB; b += NT) {
       const uint32_t t_in = ch.bar_t_inactive[b], t_act = ch.bar_t_active[b];
       const U4 d = R.draw(b, epoch, kSBarrier);
       const uint8_t st = c.bst[b];
@@ -1185,7 +1200,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
       else if (st == 1 && d.x >= t_act) c.bst[b] = 0;
     }
     // ---- units at the chromosome ends (simulation_detect_collisions.cpp:30-125) ----
-    if (threadIdx.x == 0) {
+    if (tidx() == 0) {
       const uint32_t first_fwd = c.fwd[c.fr[0]];
       const uint32_t last_rev = c.rev[c.rr[n - 1]];
       uint32_t n5 = 0;
@@ -1224,7 +1239,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     uint32_t* sorted = c.scratch;
     const bool use_bk = bdone && n > 0;
     auto fill_sorted = [&](const uint32_t* pos, const uint16_t* ranks) {
-      for (uint32_t k = threadIdx.x; k < n; k += NT) {
+      for (uint32_t k = tidx(); k < n; k += NT) {
         const uint32_t pk = pos[ranks[k]];
         sorted[k] = pk;
         if (use_bk) {
@@ -1253,8 +1268,8 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
       const uint32_t rev_lo = n5 ? n5 - 1 : 0;
       const uint32_t fwd_top = n - (n3 ? n3 - 1 : 0);
       // the next barrier's position is loaded while this one is searched
-      uint32_t bp_next = threadIdx.x < NB ? ch.bar_pos[threadIdx.x] : 0u;
-      for (uint32_t b = threadIdx.x; b < NB; b += NT) {
+      uint32_t bp_next = tidx() < NB ? ch.bar_pos[tidx()] : 0u;
+      for (uint32_t b = tidx(); b < NB; b += NT) {
         const uint32_t bp = bp_next;
         if (b + NT < NB) bp_next = ch.bar_pos[b + NT];
         if (!c.bst[b]) continue;
@@ -1314,7 +1329,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
       // forward along the run (both orders are sorted by position)
       const uint32_t kmax = i2 > 0 ? i2 - 1 : 0;
       const uint32_t ckr = (kmax + NT - 1) / NT;
-      const uint32_t k_begin = threadIdx.x * ckr, k_end = min(kmax, k_begin + ckr);
+      const uint32_t k_begin = tidx() * ckr, k_end = min(kmax, k_begin + ckr);
       uint32_t lo = n5;
       for (uint32_t k = k_begin; k < k_end; ++k) {
         const uint32_t fl = c.fr[k];
@@ -1370,7 +1385,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     // as a position (fwd + corrected fwd move = barrier - 1, rev - corrected rev move =
     // barrier + 1), taken here from the barrier itself, so no pass waits for another. Primary
     // pairs are one-to-one (one fwd unit per rev unit), so every move is written by one lane.
-    for (uint32_t i = threadIdx.x; i < n; i += NT) {
+    for (uint32_t i = tidx(); i < n; i += NT) {
       const uint32_t rci = c.rc[i], fci = fc_final(c.fc[i]);
       c.fc[i] = fci;
       if (ck_kind(rci) == kCkBar) c.rm[i] = (c.rev[i] - ch.bar_pos[ck_idx(rci)]) - 1;
@@ -1425,7 +1440,7 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
       // the three release thresholds (no, one soft, two or one hard barrier stall), once
       const uint64_t t_rel[3] = {thresh(1.0 * base), thresh(P.inv_soft * base),
                                  thresh(P.inv_hard * base)};
-      for (uint32_t i = threadIdx.x; i < n; i += NT) {
+      for (uint32_t i = tidx(); i < n; i += NT) {
         c.rev[i] -= c.rm[i];
         c.fwd[i] += c.fm[i];
         const int h = (ck_kind(c.rc[i]) == kCkBar && ch.bar_dir[ck_idx(c.rc[i])] == 1) +
@@ -1445,13 +1460,13 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
     __syncthreads();
     mark(P, S, 14);
   }
-  if (threadIdx.x == 0 && stats) {
+  if (tidx() == 0 && stats) {
     stats[0] = epoch;
     stats[1] = S.burnin_epochs;
     stats[2] = contacts;
     stats[3] = 0;
   }
-  if (P.timers && threadIdx.x == 0)
+  if (P.timers && tidx() == 0)
     for (int k = 0; k < kFastPhases; ++k) atomicAdd(&g_fast_phase[k], (unsigned long long)S.phase[k]);
 }
 
@@ -1469,9 +1484,11 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
   extern __shared__ __align__(16) uint8_t lds[];
   __shared__ Shared S;
   for (;;) {
-    if (threadIdx.x == 0) S.item = atomicAdd(next, 1ULL);
+    if (tidx() == 0) S.item = atomicAdd(next, 1ULL);
     __syncthreads();
-    const uint64_t q = S.item;
+    // the same in every lane: in scalar registers, so the work item and the chromosome
+    // descriptor are scalar loads and stay in SGPRs through the cell
+    const uint64_t q = uniform64(S.item);
     __syncthreads();
     if (q >= nitems) return;
     const uint64_t it = order[q];
@@ -1495,13 +1512,30 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
 // Clears n 32-bit words. Runs and captured graphs clear bands and counters with this kernel
 // rather than with memset nodes.
 __global__ void zero_words_kernel(uint32_t* p, uint64_t n) {
-  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + tidx(); e < n;
        e += (uint64_t)gridDim.x * blockDim.x)
     p[e] = 0u;
 }
 
-__global__ void philox_kat_kernel(const uint32_t* in, uint32_t* out, int n) {
+// fast_math.h's functions on n inputs: fn 0 ln, 1 expm1, 2 sin and cos of 2 pi x (out holds 2n:
+// sines then cosines).
+__global__ void fast_math_kernel(int fn, const double* x, double* out, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (fn == 0) {
+    out[i] = ln_d(x[i]);
+  } else if (fn == 1) {
+    out[i] = expm1_d(x[i]);
+  } else {
+    double sn, cs;
+    sincos_2pi_d(x[i], &sn, &cs);
+    out[i] = sn;
+    out[n + i] = cs;
+  }
+}
+
+__global__ void philox_kat_kernel(const uint32_t* in, uint32_t* out, int n) {
+  const int i = blockIdx.x * blockDim.x + tidx();
   if (i >= n) return;
   const U4 r = philox(U4{in[6 * i], in[6 * i + 1], in[6 * i + 2], in[6 * i + 3]}, in[6 * i + 4],
                       in[6 * i + 5]);
@@ -1809,6 +1843,8 @@ int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& dchroms,
     f.cpe = (uint32_t)d.cpe;
     f.target_epochs = d.target_epochs;
     f.bind_rate = d.bind_rate;
+    f.bind_exp_neg = std::exp(-d.bind_rate);
+    f.bind_sd = std::sqrt(d.bind_rate);
     f.prob_release = release[2 * c];
     f.prob_release_burnin = release[2 * c + 1];
     f.lds_bytes = fast_cell_bytes(f.nlefs, f.nbar, P.hist_len);
@@ -2091,6 +2127,12 @@ hipError_t launch_zero_words(hipStream_t stream, void* p, uint64_t bytes) {
 
 hipError_t launch_philox_kat(hipStream_t stream, const uint32_t* d_in, uint32_t* d_out, int n) {
   hipLaunchKernelGGL(philox_kat_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, d_in, d_out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fast_math(hipStream_t stream, int fn, const double* d_in, double* d_out, int n) {
+  hipLaunchKernelGGL(fast_math_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, fn, d_in,
+                     d_out, n);
   return hipGetLastError();
 }
 

@@ -147,7 +147,8 @@ struct FastChrom {
   uint32_t nrows;
   uint32_t cpe;
   uint64_t target_epochs;
-  double bind_rate;
+  double bind_rate;                   // mean LEF activations per burn-in epoch
+  double bind_exp_neg, bind_sd;       // e^-bind_rate and sqrt(bind_rate)
   double prob_release, prob_release_burnin;  // per chromosome: LEF processivity may vary
   uint64_t lds_bytes;  // working-set bytes of one cell
   unsigned long long* occupancy;  // 1D LEF occupancy, ceil(size / bin) bins, or null

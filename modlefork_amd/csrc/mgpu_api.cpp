@@ -1274,4 +1274,22 @@ int mgpu_debug_philox(mgpu_ctx* ctx, const uint32_t* in, uint64_t n, uint32_t* o
   return MGPU_OK;
 }
 
+int mgpu_debug_fast_math(mgpu_ctx* ctx, int fn, const double* x, uint64_t n, double* out) {
+  if (!ctx || !x || !out) return fail(ctx, MGPU_EINVAL, "null argument");
+  if (fn < 0 || fn > 2) return fail(ctx, MGPU_EINVAL, "unknown function %d", fn);
+  if (n == 0) return MGPU_OK;
+  if (n > (1u << 24)) return fail(ctx, MGPU_EINVAL, "n out of range");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const uint64_t nout = fn == 2 ? 2 * n : n;
+  double* d = nullptr;
+  HIPCHK(ctx, hipMalloc(&d, (n + nout) * 8));
+  hipError_t e = hipMemcpy(d, x, n * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_fast_math(ctx->stream, fn, d, d + n, (int)n);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = hipMemcpy(out, d + n, nout * 8, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(ctx, MGPU_EDEVICE, "fast math failed: %s", hipGetErrorString(e));
+  return MGPU_OK;
+}
+
 }  // extern "C"

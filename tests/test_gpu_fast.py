@@ -49,6 +49,43 @@ def test_philox_known_answers(ctx):
         assert tuple(int(x) for x in row) == want
 
 
+def _device_math(ctx, fn, x):
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros(2 * len(x) if fn == 2 else len(x), np.float64)
+    ctx.check(L.lib().mgpu_debug_fast_math(ctx.ptr, fn, x.ctypes.data_as(C.POINTER(C.c_double)),
+                                           len(x), out.ctypes.data_as(C.POINTER(C.c_double))))
+    return out
+
+
+def test_fast_mode_double_precision_functions(ctx):
+    """The double-precision ln, expm1 and sin/cos of a full turn that fast mode's moves, GEV noise
+    and Poisson draws use, against numpy (ln, expm1) and 80-bit long double (sin, cos of 2 pi u):
+    within 4 ulp over the ranges the draws reach, including 53-bit uniforms in (0, 1], the
+    second logarithm of the GEV noise (arguments 1e-16 to 37) and its expm1 arguments."""
+    rng = np.random.default_rng(9)
+    u = (np.floor(rng.random(200_000) * 2.0**53) + 1.0) * 2.0**-53
+    big = np.exp(rng.uniform(np.log(1e-16), np.log(40.0), 100_000))
+    x = np.concatenate([u, big, [2.0**-53, 0.5, 1.0, np.sqrt(0.5), np.nextafter(np.sqrt(0.5), 0)]])
+    got, want = _device_math(ctx, 0, x), np.log(x)
+    err = np.abs(got - want) / np.spacing(np.maximum(np.abs(want), 1e-300))
+    say(f"ln: max error {err.max():.2f} ulp")
+    assert err.max() <= 4.0
+    z = np.concatenate([rng.uniform(-40, 5, 100_000), rng.uniform(-1e-3, 1e-3, 100_000),
+                        rng.uniform(-0.4, 0.4, 100_000), [0.0, 1e-300, -0.3465, 0.3466, 1.0, -1.0]])
+    got, want = _device_math(ctx, 1, z), np.expm1(z)
+    err = np.abs(got - want) / np.spacing(np.maximum(np.abs(want), 1e-300))
+    say(f"expm1: max error {err.max():.2f} ulp")
+    assert err.max() <= 4.0
+    t = np.concatenate([np.floor(rng.random(200_000) * 2.0**53) * 2.0**-53,
+                        [0.0, 0.125, 0.25, 0.5, 0.75, 1.0 - 2.0**-53]])
+    got = _device_math(ctx, 2, t)
+    ang = np.longdouble("6.283185307179586476925286766559005768") * t.astype(np.longdouble)
+    for name, dev, ref in (("sin", got[:len(t)], np.sin(ang)), ("cos", got[len(t):], np.cos(ang))):
+        err = np.abs(dev.astype(np.longdouble) - ref) / np.longdouble(2.0**-53)
+        say(f"{name}(2 pi u): max absolute error {float(err.max()):.2f} x 2^-53")
+        assert float(err.max()) <= 4.0
+
+
 def run_device(ctx, wl, mode, threads=0):
     sim = S.Simulation(wl.options, ctx)
     out, stats = sim.run_simulate(wl, mode, threads_per_cell=threads)
