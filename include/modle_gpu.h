@@ -323,6 +323,21 @@ int mgpu_debug_draws(mgpu_ctx* ctx, uint64_t seed, int kind, double a, double b,
  * (counter[4], key[2]) words, out receives n x 4 output words. No counterpart in the reference
  * (fast mode replaces its engine); exposed for known-answer tests. */
 int mgpu_debug_philox(mgpu_ctx* ctx, const uint32_t* in, uint64_t n, uint32_t* out);
+/* ContactMatrixDense<u32>::increment (contact_matrix_dense_safe_impl.hpp:53-87) as the device
+ * applies it in deterministic (mode 0, 64-bit coordinates) or fast mode (mode 1, 32-bit): the n
+ * increments at (rows[i], cols[i]) are added in order to band (nrows*ncols + 1 u32, the band
+ * layout of mgpu_simulate_chrom; in/out) and *missed (in/out) counts those whose diagonal
+ * offset |row - col| is nrows or more. Coordinates beyond the matrix are MGPU_ERANGE, as
+ * the reference's debug builds throw. Exposed for the reference's matrix known-answer tests. */
+int mgpu_debug_band_increments(mgpu_ctx* ctx, int mode, uint64_t nrows, uint64_t ncols,
+                               uint64_t n, const uint64_t* rows, const uint64_t* cols,
+                               uint32_t* band, uint64_t* missed);
+/* Deterministic mode's collision words (Collision<u32f>, collision_encoding_impl.hpp:73-240) on
+ * the device for n (index, event) pairs: the encoded word, its decoded index and event, and
+ * flags: bit 0 the COLLISION bit, bits 1-4 collision occurred with CHROM_BOUNDARY, LEF_BAR,
+ * LEF_LEF_PRIMARY, LEF_LEF_SECONDARY, bits 5-8 the same events avoided. */
+int mgpu_debug_collisions(mgpu_ctx* ctx, uint64_t n, const uint64_t* idx, const uint8_t* ev,
+                          uint64_t* words, uint64_t* index, uint32_t* event, uint32_t* flags);
 /* Fast mode's double-precision elementary functions on the device (no counterpart in the
  * reference, whose draws call libm): fn 0 ln(x), 1 expm1(x), 2 sin(2 pi x) and cos(2 pi x) (out
  * holds 2n values, the sines first). Exposed for accuracy tests. */

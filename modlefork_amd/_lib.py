@@ -266,6 +266,10 @@ SIGNATURES = [
                                    C.c_double, C.c_uint64, U64P]),
     ("mgpu_debug_philox", C.c_int, [C.c_void_p, U32P, C.c_uint64, U32P]),
     ("mgpu_debug_fast_math", C.c_int, [C.c_void_p, C.c_int, DP, C.c_uint64, DP]),
+    ("mgpu_debug_band_increments", C.c_int, [C.c_void_p, C.c_int, C.c_uint64, C.c_uint64,
+                                             C.c_uint64, U64P, U64P, U32P, U64P]),
+    ("mgpu_debug_collisions", C.c_int, [C.c_void_p, C.c_uint64, U64P, U8P, U64P, U64P, U32P,
+                                        U32P]),
     ("mgpu_debug_phase_times", C.c_int, [C.c_void_p, C.c_int, U64P, C.c_uint64]),
     ("mgpu_debug_fast_serial_fixups", C.c_int, [C.c_void_p, C.c_int]),
     ("mgpu_context_stream", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int)]),

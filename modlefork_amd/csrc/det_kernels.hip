@@ -166,6 +166,49 @@ hipError_t launch_det_draws(hipStream_t stream, uint64_t seed, int kind, double 
   return hipGetLastError();
 }
 
+// ContactMatrixDense<u32>::increment as deterministic mode applies it, for (rows[i], cols[i]) in
+// order, on one lane.
+__global__ void det_band_kernel(DetChrom ch, const uint64_t* rows, const uint64_t* cols,
+                                uint64_t n) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (uint64_t i = 0; i < n; ++i) band_add(ch, rows[i], cols[i]);
+}
+
+// Collision words of deterministic mode (the reference's 64-bit layout): encoded word, decoded
+// index and event, and flags: bit 0 COLLISION set, bits 1-4 collision occurred with CHROM,
+// LEF_BAR, PRIMARY, SECONDARY, bits 5-8 the same events avoided.
+__global__ void det_collision_kernel(const uint64_t* idx, const uint8_t* ev, uint64_t n,
+                                     uint64_t* words, uint64_t* index, uint32_t* event,
+                                     uint32_t* flags) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t w = cenc(idx[i], ev[i]);
+  const unsigned kinds[4] = {kEvChrom, kEvLefBar, kEvPrimary, kEvSecondary};
+  uint32_t f = c_any(w) ? 1u : 0u;
+  for (int k = 0; k < 4; ++k) {
+    f |= (c_is(w, kinds[k]) ? 1u : 0u) << (1 + k);
+    f |= (c_avoided(w, kinds[k]) ? 1u : 0u) << (5 + k);
+  }
+  words[i] = w;
+  index[i] = cidx(w);
+  event[i] = cev(w);
+  flags[i] = f;
+}
+
+hipError_t launch_det_band(hipStream_t stream, const DetChrom& ch, const uint64_t* d_rows,
+                           const uint64_t* d_cols, uint64_t n) {
+  hipLaunchKernelGGL(det_band_kernel, dim3(1), dim3(64), 0, stream, ch, d_rows, d_cols, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_det_collisions(hipStream_t stream, const uint64_t* d_idx, const uint8_t* d_ev,
+                                 uint64_t n, uint64_t* d_words, uint64_t* d_index,
+                                 uint32_t* d_event, uint32_t* d_flags) {
+  hipLaunchKernelGGL(det_collision_kernel, dim3((uint32_t)((n + 63) / 64)), dim3(64), 0, stream,
+                     d_idx, d_ev, n, d_words, d_index, d_event, d_flags);
+  return hipGetLastError();
+}
+
 uint64_t det_cell_bytes_host(uint64_t nlefs, uint64_t nbar, uint64_t hist) {
   uint64_t b = nlefs * (7 * 8 + 3 * 4) + nbar + 2 * (hist + 1) * 8;
   return (b + 255) & ~255ULL;

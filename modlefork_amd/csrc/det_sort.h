@@ -1,4 +1,5 @@
-// Deterministic-mode rank sorters (device, one lane).
+// Deterministic-mode rank sorters (device, one lane; the host orders barriers with the same
+// pdq_sort).
 //
 // rank_lefs (src/libmodle/cpu/simulation.cpp:402-487) orders LEF indices by unit position with
 // cpp-sort 1.13's pdq_sort (first epoch) or split_sort (later epochs), then re-sorts runs of
@@ -19,17 +20,17 @@ namespace mgpu {
 
 struct KeyLess {
   const uint64_t* key;
-  __device__ bool operator()(uint32_t a, uint32_t b) const { return key[a] < key[b]; }
+  __host__ __device__ bool operator()(uint32_t a, uint32_t b) const { return key[a] < key[b]; }
 };
 
-__device__ inline void swap_u32(uint32_t* a, uint32_t* b) {
+__host__ __device__ inline void swap_u32(uint32_t* a, uint32_t* b) {
   const uint32_t t = *a;
   *a = *b;
   *b = t;
 }
 
 template <class L>
-__device__ void ins_sort(uint32_t* begin, uint32_t* end, L less) {
+__host__ __device__ void ins_sort(uint32_t* begin, uint32_t* end, L less) {
   if (begin == end) return;
   for (uint32_t* cur = begin + 1; cur != end; ++cur) {
     uint32_t* sift = cur;
@@ -45,7 +46,7 @@ __device__ void ins_sort(uint32_t* begin, uint32_t* end, L less) {
 }
 
 template <class L>
-__device__ void unguarded_ins_sort(uint32_t* begin, uint32_t* end, L less) {
+__host__ __device__ void unguarded_ins_sort(uint32_t* begin, uint32_t* end, L less) {
   if (begin == end) return;
   for (uint32_t* cur = begin + 1; cur != end; ++cur) {
     uint32_t* sift = cur;
@@ -61,7 +62,7 @@ __device__ void unguarded_ins_sort(uint32_t* begin, uint32_t* end, L less) {
 }
 
 template <class L>
-__device__ bool partial_ins_sort(uint32_t* begin, uint32_t* end, L less) {
+__host__ __device__ bool partial_ins_sort(uint32_t* begin, uint32_t* end, L less) {
   if (begin == end) return true;
   uint64_t moved = 0;
   for (uint32_t* cur = begin + 1; cur != end; ++cur) {
@@ -81,14 +82,14 @@ __device__ bool partial_ins_sort(uint32_t* begin, uint32_t* end, L less) {
 }
 
 template <class L>
-__device__ inline void sort3(uint32_t* a, uint32_t* b, uint32_t* c, L less) {
+__host__ __device__ inline void sort3(uint32_t* a, uint32_t* b, uint32_t* c, L less) {
   if (less(*b, *a)) swap_u32(a, b);
   if (less(*c, *b)) swap_u32(b, c);
   if (less(*b, *a)) swap_u32(a, b);
 }
 
 template <class L>
-__device__ uint32_t* partition_right(uint32_t* begin, uint32_t* end, L less, bool& already) {
+__host__ __device__ uint32_t* partition_right(uint32_t* begin, uint32_t* end, L less, bool& already) {
   const uint32_t pivot = *begin;
   uint32_t* first = begin;
   uint32_t* last = end;
@@ -116,7 +117,7 @@ __device__ uint32_t* partition_right(uint32_t* begin, uint32_t* end, L less, boo
 }
 
 template <class L>
-__device__ uint32_t* partition_left(uint32_t* begin, uint32_t* end, L less) {
+__host__ __device__ uint32_t* partition_left(uint32_t* begin, uint32_t* end, L less) {
   const uint32_t pivot = *begin;
   uint32_t* first = begin;
   uint32_t* last = end;
@@ -144,7 +145,7 @@ __device__ uint32_t* partition_left(uint32_t* begin, uint32_t* end, L less) {
 
 // make_heap + sort_heap with the hole-sifting of libstdc++'s __adjust_heap/__push_heap.
 template <class L>
-__device__ void adjust_heap(uint32_t* first, int64_t hole, int64_t len, uint32_t value, L less) {
+__host__ __device__ void adjust_heap(uint32_t* first, int64_t hole, int64_t len, uint32_t value, L less) {
   const int64_t top = hole;
   int64_t child = hole;
   while (child < (len - 1) / 2) {
@@ -168,7 +169,7 @@ __device__ void adjust_heap(uint32_t* first, int64_t hole, int64_t len, uint32_t
 }
 
 template <class L>
-__device__ void heap_sort(uint32_t* first, uint32_t* last, L less) {
+__host__ __device__ void heap_sort(uint32_t* first, uint32_t* last, L less) {
   const int64_t len = last - first;
   if (len >= 2) {
     for (int64_t parent = (len - 2) / 2;; --parent) {
@@ -186,7 +187,7 @@ __device__ void heap_sort(uint32_t* first, uint32_t* last, L less) {
 
 // Iterative pdqsort: the left partition is sorted by an explicit stack instead of recursion.
 template <class L>
-__device__ void pdq_sort(uint32_t* base, uint32_t n, L less) {
+__host__ __device__ void pdq_sort(uint32_t* base, uint32_t n, L less) {
   if (n == 0) return;
   int lg = 0;
   for (uint32_t m = n; m >>= 1;) ++lg;
@@ -275,7 +276,7 @@ __device__ void pdq_sort(uint32_t* base, uint32_t n, L less) {
 }
 
 template <class L>
-__device__ void split_sort(uint32_t* a, uint32_t n, uint32_t* buf, L less) {
+__host__ __device__ void split_sort(uint32_t* a, uint32_t n, uint32_t* buf, L less) {
   if (n < 2) return;
   uint32_t middle = 0;
   for (uint32_t r = 1; r < n; ++r) {

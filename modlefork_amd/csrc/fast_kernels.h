@@ -73,6 +73,10 @@ void fast_job_set_log(FastJob* job, const StateLog& log);
 void fast_job_set_serial_fixups(FastJob* job, int serial);
 // Philox4x32-10 of n (counter[4], key[2]) tuples: in holds 6 words per tuple, out 4.
 hipError_t launch_philox_kat(hipStream_t stream, const uint32_t* d_in, uint32_t* d_out, int n);
+// Fast mode's band increments (32-bit coordinates) for (rows[i], cols[i]), in order.
+hipError_t launch_fast_band(hipStream_t stream, uint32_t* d_band, unsigned long long* d_missed,
+                            uint32_t nrows, const uint32_t* d_rows, const uint32_t* d_cols,
+                            uint64_t n);
 // fast mode's double-precision ln (fn 0), expm1 (fn 1) or sin and cos of 2 pi x (fn 2; d_out
 // holds 2n values, sines first) on n inputs.
 hipError_t launch_fast_math(hipStream_t stream, int fn, const double* d_in, double* d_out, int n);

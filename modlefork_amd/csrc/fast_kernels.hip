@@ -1534,6 +1534,13 @@ __global__ void fast_math_kernel(int fn, const double* x, double* out, int n) {
   }
 }
 
+// Fast mode's band increments for (rows[i], cols[i]), in order, on one lane.
+__global__ void fast_band_kernel(FastChrom ch, const uint32_t* rows, const uint32_t* cols,
+                                 uint64_t n) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (uint64_t i = 0; i < n; ++i) band_add(ch, rows[i], cols[i]);
+}
+
 __global__ void philox_kat_kernel(const uint32_t* in, uint32_t* out, int n) {
   const int i = blockIdx.x * blockDim.x + tidx();
   if (i >= n) return;
@@ -2127,6 +2134,17 @@ hipError_t launch_zero_words(hipStream_t stream, void* p, uint64_t bytes) {
 
 hipError_t launch_philox_kat(hipStream_t stream, const uint32_t* d_in, uint32_t* d_out, int n) {
   hipLaunchKernelGGL(philox_kat_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, d_in, d_out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fast_band(hipStream_t stream, uint32_t* d_band, unsigned long long* d_missed,
+                            uint32_t nrows, const uint32_t* d_rows, const uint32_t* d_cols,
+                            uint64_t n) {
+  FastChrom ch{};
+  ch.band = d_band;
+  ch.missed = d_missed;
+  ch.nrows = nrows;
+  hipLaunchKernelGGL(fast_band_kernel, dim3(1), dim3(64), 0, stream, ch, d_rows, d_cols, n);
   return hipGetLastError();
 }
 

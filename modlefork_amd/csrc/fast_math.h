@@ -60,7 +60,8 @@ __device__ inline double ln_d(double x) {
 
 // e^z - 1 for |z| < 700. z = k ln 2 + r with |r| <= ln(2) / 2 (Cody-Waite, two-part ln 2);
 // e^r - 1 = r + r^2/2! + ... + r^14/14! (the next term is below 2^-56 of the sum), and
-// e^z - 1 = 2^k (e^r - 1 + 1) - 1 when k != 0.
+// e^z - 1 = 2^k (e^r - 1) + (2^k - 1) when k != 0 (both terms exact scalings or exact for
+// |k| < 53, so the cancellation of 2^k e^r - 1 near z = -ln 2 loses nothing).
 __device__ inline double expm1_d(double z) {
   const double k = rint(z * MGPU_KD(1.4426950408889634));
   double r = fma(-k, MGPU_KD(MGPU_LN2_HI), z);
@@ -80,7 +81,7 @@ __device__ inline double expm1_d(double z) {
   p = fma(p, r, 0.5);                                             // 1/2!
   const double q = fma(p * r, r, r);                              // e^r - 1
   if (k == 0.0) return q;
-  return __builtin_amdgcn_ldexp(q + 1.0, (int)k) - 1.0;
+  return __builtin_amdgcn_ldexp(q, (int)k) + (__builtin_amdgcn_ldexp(1.0, (int)k) - 1.0);
 }
 
 // sin(2 pi u) and cos(2 pi u) for u in [0, 1). 4u = k + r with k = rint(4u) and |r| <= 1/2

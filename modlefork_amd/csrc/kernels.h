@@ -87,5 +87,13 @@ hipError_t launch_det_step(hipStream_t stream, const DetParams& P, const DetChro
 hipError_t launch_det_draws(hipStream_t stream, uint64_t seed, int kind, double a, double b,
                             double c, uint64_t n, uint64_t* d_out);
 uint64_t det_cell_bytes_host(uint64_t nlefs, uint64_t nbar, uint64_t hist);
+// Band increments of deterministic mode for (rows[i], cols[i]), in order (ch.band, ch.missed,
+// ch.nrows are read).
+hipError_t launch_det_band(hipStream_t stream, const DetChrom& ch, const uint64_t* d_rows,
+                           const uint64_t* d_cols, uint64_t n);
+// Collision words of deterministic mode for n (index, event) pairs.
+hipError_t launch_det_collisions(hipStream_t stream, const uint64_t* d_idx, const uint8_t* d_ev,
+                                 uint64_t n, uint64_t* d_words, uint64_t* d_index,
+                                 uint32_t* d_event, uint32_t* d_flags);
 
 }  // namespace mgpu

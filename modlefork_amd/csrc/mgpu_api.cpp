@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/modle_gpu.h"
+#include "det_sort.h"
 #include "fast_kernels.h"
 #include "kernels.h"
 #include "xxh3_host.h"
@@ -339,12 +340,24 @@ int mgpu_prepare_barriers(const mgpu_params* p, const mgpu_bed_record* recs, uin
     // motif '+' blocks extrusion in the rev direction, anything else in the fwd direction
     rows.push_back({bp, (uint8_t)(r.strand == '+' ? 1 : 2), sa, si});
   }
-  std::stable_sort(rows.begin(), rows.end(), [](const Row& a, const Row& b) { return a.pos < b.pos; });
+  // ExtrusionBarriers::sort (extrusion_barriers.cpp:239-259): cpp-sort's pdq_sort of the barrier
+  // indices by position (unstable: barriers at one position end up in pdq_sort's order, not the
+  // records'), then the permutation applied to every field
+  if (rows.size() > 0xFFFFFFFFull)
+    return fail(nullptr, MGPU_ERANGE, "more than 2^32 barriers on one chromosome");
+  std::vector<uint64_t> key(rows.size());
+  std::vector<uint32_t> idx(rows.size());
   for (uint64_t i = 0; i < rows.size(); ++i) {
-    pos[i] = rows[i].pos;
-    dir[i] = rows[i].dir;
-    stp_active[i] = rows[i].sa;
-    stp_inactive[i] = rows[i].si;
+    key[i] = rows[i].pos;
+    idx[i] = (uint32_t)i;
+  }
+  pdq_sort(idx.data(), (uint32_t)idx.size(), KeyLess{key.data()});
+  for (uint64_t i = 0; i < rows.size(); ++i) {
+    const Row& r = rows[idx[i]];
+    pos[i] = r.pos;
+    dir[i] = r.dir;
+    stp_active[i] = r.sa;
+    stp_inactive[i] = r.si;
   }
   *nout = rows.size();
   return MGPU_OK;
@@ -1271,6 +1284,97 @@ int mgpu_debug_philox(mgpu_ctx* ctx, const uint32_t* in, uint64_t n, uint32_t* o
   if (e == hipSuccess) e = hipMemcpy(out, d + 6 * n, n * 16, hipMemcpyDeviceToHost);
   (void)hipFree(d);
   if (e != hipSuccess) return fail(ctx, MGPU_EDEVICE, "philox failed: %s", hipGetErrorString(e));
+  return MGPU_OK;
+}
+
+int mgpu_debug_band_increments(mgpu_ctx* ctx, int mode, uint64_t nrows, uint64_t ncols,
+                               uint64_t n, const uint64_t* rows, const uint64_t* cols,
+                               uint32_t* band, uint64_t* missed) {
+  if (!ctx || !band || !missed || (n && (!rows || !cols)))
+    return fail(ctx, MGPU_EINVAL, "null argument");
+  if (mode != MGPU_MODE_DETERMINISTIC && mode != MGPU_MODE_FAST)
+    return fail(ctx, MGPU_EINVAL, "unknown mode %d", mode);
+  if (nrows == 0 || ncols == 0 || nrows * ncols > (1ull << 28) || n > (1ull << 24))
+    return fail(ctx, MGPU_EINVAL, "band or increment count out of range");
+  for (uint64_t i = 0; i < n; ++i) {
+    // the bounds ContactMatrixDense checks before an increment: the column within the matrix
+    // (larger coordinate), the offset below nrows or counted as missed
+    const uint64_t j = std::max(rows[i], cols[i]);
+    if (j >= ncols || (mode == MGPU_MODE_FAST && j > 0xFFFFFFFFull))
+      return fail(ctx, MGPU_ERANGE, "increment %llu at (%llu, %llu) outside the matrix",
+                  (unsigned long long)i, (unsigned long long)rows[i], (unsigned long long)cols[i]);
+  }
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const uint64_t belems = nrows * ncols + 1;
+  uint8_t* d = nullptr;
+  const uint64_t off_band = 0, off_missed = ((belems * 4 + 255) & ~255ull);
+  const uint64_t off_coords = off_missed + 256;
+  HIPCHK(ctx, hipMalloc(&d, off_coords + 16 * std::max<uint64_t>(n, 1)));
+  uint32_t* d_band = reinterpret_cast<uint32_t*>(d + off_band);
+  unsigned long long* d_missed = reinterpret_cast<unsigned long long*>(d + off_missed);
+  hipError_t e = hipMemcpy(d_band, band, belems * 4, hipMemcpyHostToDevice);
+  unsigned long long m0 = *missed;
+  if (e == hipSuccess) e = hipMemcpy(d_missed, &m0, 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess && n) {
+    if (mode == MGPU_MODE_DETERMINISTIC) {
+      uint64_t* d_rows = reinterpret_cast<uint64_t*>(d + off_coords);
+      uint64_t* d_cols = d_rows + n;
+      e = hipMemcpy(d_rows, rows, n * 8, hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipMemcpy(d_cols, cols, n * 8, hipMemcpyHostToDevice);
+      DetChrom ch{};
+      ch.band = d_band;
+      ch.missed = d_missed;
+      ch.nrows = nrows;
+      if (e == hipSuccess) e = launch_det_band(ctx->stream, ch, d_rows, d_cols, n);
+    } else {
+      std::vector<uint32_t> rc(2 * n);
+      for (uint64_t i = 0; i < n; ++i) {
+        rc[i] = (uint32_t)rows[i];
+        rc[n + i] = (uint32_t)cols[i];
+      }
+      uint32_t* d_rc = reinterpret_cast<uint32_t*>(d + off_coords);
+      e = hipMemcpy(d_rc, rc.data(), n * 8, hipMemcpyHostToDevice);
+      if (e == hipSuccess)
+        e = launch_fast_band(ctx->stream, d_band, d_missed, (uint32_t)nrows, d_rc, d_rc + n, n);
+    }
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = hipMemcpy(band, d_band, belems * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(&m0, d_missed, 8, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess)
+    return fail(ctx, MGPU_EDEVICE, "band increments failed: %s", hipGetErrorString(e));
+  *missed = m0;
+  return MGPU_OK;
+}
+
+int mgpu_debug_collisions(mgpu_ctx* ctx, uint64_t n, const uint64_t* idx, const uint8_t* ev,
+                          uint64_t* words, uint64_t* index, uint32_t* event, uint32_t* flags) {
+  if (!ctx || (n && (!idx || !ev || !words || !index || !event || !flags)))
+    return fail(ctx, MGPU_EINVAL, "null argument");
+  if (n == 0) return MGPU_OK;
+  if (n > (1u << 24)) return fail(ctx, MGPU_EINVAL, "n out of range");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  uint8_t* d = nullptr;
+  HIPCHK(ctx, hipMalloc(&d, n * 41 + 256));
+  uint64_t* d_idx = reinterpret_cast<uint64_t*>(d);
+  uint64_t* d_words = d_idx + n;
+  uint64_t* d_index = d_words + n;
+  uint32_t* d_event = reinterpret_cast<uint32_t*>(d_index + n);
+  uint32_t* d_flags = d_event + n;
+  uint8_t* d_ev = reinterpret_cast<uint8_t*>(d_flags + n);
+  hipError_t e = hipMemcpy(d_idx, idx, n * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_ev, ev, n, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = launch_det_collisions(ctx->stream, d_idx, d_ev, n, d_words, d_index, d_event, d_flags);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = hipMemcpy(words, d_words, n * 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(index, d_index, n * 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(event, d_event, n * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(flags, d_flags, n * 4, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess)
+    return fail(ctx, MGPU_EDEVICE, "collision words failed: %s", hipGetErrorString(e));
   return MGPU_OK;
 }
 

@@ -179,6 +179,9 @@ def prepare_barriers(records, chrom_start: int, chrom_end: int, c: dict) -> "R.B
             continue
         sa, si = barrier_stp(r.score, c)
         rows.append((pos, 1 if r.strand == "+" else 2, sa, si))
-    rows.sort(key=lambda t: t[0])
+    # cpp-sort pdq_sort of the indices by position (extrusion_barriers.cpp:243): equal positions
+    # keep pdq_sort's (unstable) order
+    order = R.pdq_order([t[0] for t in rows])
+    rows = [rows[i] for i in order]
     return R.Barriers([t[0] for t in rows], [t[1] for t in rows], [t[2] for t in rows],
                       [t[3] for t in rows])

@@ -1236,6 +1236,19 @@ void rc_process_collisions(const rc_config* cfg, const rc_chrom* chrom, uint64_t
 }
 
 /* extrusion_barriers_impl.hpp:106-128 */
+/* Collision words as the known-answer tests read them (collision_encoding_impl.hpp:150-240):
+ * ev 0 asks collision_occurred() (the COLLISION bit) or collision_avoided() (no COLLISION bit on
+ * a non-zero word); otherwise the event must equal ev | COLLISION, or ev without it. */
+uint64_t rc_coll_encode(uint64_t idx, unsigned ev) { return coll_enc(idx, ev); }
+uint64_t rc_coll_index(uint64_t c) { return coll_idx(c); }
+unsigned rc_coll_event(uint64_t c) { return coll_ev(c); }
+int rc_coll_occurred(uint64_t c, unsigned ev) {
+  return ev ? coll_occurred(c, ev) : coll_occurred_any(c);
+}
+int rc_coll_avoided(uint64_t c, unsigned ev) {
+  return ev ? coll_avoided(c, ev) : (!coll_occurred_any(c) && c != 0);
+}
+
 double rc_stp_active_from_occupancy(double stp_inactive, double occupancy) {
   if (occupancy == 0) return 0.0;
   const double tp_ia = 1.0 - stp_inactive;

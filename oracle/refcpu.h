@@ -133,6 +133,13 @@ void rc_bind_lefs(uint64_t start, uint64_t end, uint64_t* rev, uint64_t* fwd, ui
                   size_t n, size_t* rev_ranks, size_t* fwd_ranks, const uint8_t* mask,
                   rc_rng* g, uint64_t current_epoch);
 double rc_occupancy_from_stp(double stp_active, double stp_inactive);
+/* collision words (reference 64-bit layout): encode, decode and the occurred / avoided tests
+ * (ev 0: any event) */
+uint64_t rc_coll_encode(uint64_t idx, unsigned ev);
+uint64_t rc_coll_index(uint64_t c);
+unsigned rc_coll_event(uint64_t c);
+int rc_coll_occurred(uint64_t c, unsigned ev);
+int rc_coll_avoided(uint64_t c, unsigned ev);
 double rc_stp_active_from_occupancy(double stp_inactive, double occupancy);
 
 /* ---- contact matrix ---- */

@@ -177,6 +177,14 @@ def lib():
         L.rc_stp_active_from_occupancy.restype = C.c_double
         L.rc_stp_active_from_occupancy.argtypes = [C.c_double, C.c_double]
         L.rc_band_increment.argtypes = [C.POINTER(RcBand), C.c_uint64, C.c_uint64]
+        L.rc_coll_encode.restype = C.c_uint64
+        L.rc_coll_encode.argtypes = [C.c_uint64, C.c_uint]
+        L.rc_coll_index.restype = C.c_uint64
+        L.rc_coll_index.argtypes = [C.c_uint64]
+        L.rc_coll_event.restype = C.c_uint
+        L.rc_coll_event.argtypes = [C.c_uint64]
+        L.rc_coll_occurred.argtypes = [C.c_uint64, C.c_uint]
+        L.rc_coll_avoided.argtypes = [C.c_uint64, C.c_uint]
         L.rc_num_lefs.restype = C.c_uint64
         L.rc_num_lefs.argtypes = [C.c_double, C.c_uint64]
         L.rc_contacts_per_epoch.restype = C.c_uint64
@@ -241,6 +249,16 @@ def draw_sequence(seed: int, kind: int, n: int, a=0.0, b=0.0, c=0.0) -> np.ndarr
     out = np.zeros(n, dtype=np.uint64)
     lib().rc_draw_sequence(seed, kind, a, b, c, n, ptr(out, C.c_uint64))
     return out
+
+
+def pdq_order(keys) -> list:
+    """Indices 0..n-1 ordered by key with the oracle's restatement of cpp-sort's pdq_sort
+    (unstable: equal keys end up in pdq_sort's order)."""
+    key = np.ascontiguousarray(keys, dtype=np.uint64)
+    idx = np.arange(len(key), dtype=np.uintp)
+    if len(key):
+        lib().rc_pdq_sort_idx(ptr(idx, C.c_size_t), len(key), ptr(key, C.c_uint64))
+    return [int(i) for i in idx]
 
 
 def rng(seed: int) -> RcRng:

@@ -103,6 +103,33 @@ def test_prepare_barriers_matches_oracle(wl):
     assert np.all(np.diff(b.pos.astype(np.int64)) >= 0)
 
 
+def test_barriers_at_equal_midpoints_follow_pdq_sort():
+    """Barriers whose BED records share a midpoint: ExtrusionBarriers::sort orders the indices
+    with cpp-sort's pdq_sort, which is unstable (extrusion_barriers.cpp:239-259). The library's
+    order equals the oracle's pdq restatement field by field, on a set where that order differs
+    from the records' own (a stable sort's) order, and on one small enough for pdq_sort's
+    insertion-sort path (n < 24), where the two agree."""
+    rng = np.random.default_rng(12)
+    for n in (300, 20):
+        mids = rng.integers(0, 12, n) * 1000 + 500
+        recs = [W.BedRecord("c", int(m) - 1, int(m), float(rng.uniform(0.05, 0.95)),
+                            "+" if rng.random() < 0.5 else "-") for m in mids]
+        opts = dict(bin_size=1000, diagonal_width=10_000)
+        p = S.make_params(opts)
+        b = S.prepare_barriers(p, recs, 0, 20_000)
+        o = OC.prepare_barriers(recs, 0, 20_000, OC.transform(opts))
+        assert np.array_equal(b.pos, o.pos) and np.array_equal(b.dir, o.dir)
+        assert np.array_equal(b.stp_active, o.stp_active)
+        assert np.array_equal(b.stp_inactive, o.stp_inactive)
+        assert np.all(np.diff(b.pos.astype(np.int64)) >= 0)
+        stable = sorted(range(n), key=lambda i: (recs[i].start + recs[i].end + 1) // 2)
+        stable_sa = np.array([OC.barrier_stp(recs[i].score, OC.transform(opts))[0] for i in stable])
+        if n >= 24:
+            assert not np.array_equal(b.stp_active, stable_sa)
+        else:
+            assert np.array_equal(b.stp_active, stable_sa)
+
+
 def test_prepare_barriers_rejects_bad_records():
     p = S.make_params({})
     bad = [W.BedRecord("c", 10, 11, 1.5, "+")]

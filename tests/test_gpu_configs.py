@@ -1,0 +1,298 @@
+"""Every benchmark configuration at its real size on the GPU.
+
+  * genome (24 chromosomes of 40-250 Mbp, clustered barriers with Beta(8,2) occupancy, 512 cells
+    each): fast mode against the oracle statistically on the smallest and the largest
+    chromosome (128 of their cells, the 256-lane and the 1024-lane launch tiers), and every one
+    of the 12 288 cells in one run with exact conservation checks;
+  * stress (250 Mbp, 1 kb bins, 5 Mbp band, 10 000 LEFs, 12 000 barriers): fast mode against the
+    oracle statistically on 32 cells (the hybrid LDS / device-memory placement), and all 2 048
+    cells with exact conservation checks;
+  * surrogate at its shipped shape (600 input bins, 8 x 128-channel encoder with dilations up to
+    32, 100 x 400 head) with the trained weights on sweep inputs: forward and gradients against
+    the CPU oracle;
+  * the surrogate-accelerated sweep of 4 096 points with its 5 % judge.
+
+Statistical tolerances come from the oracle itself: its cells are simulated as two halves, and
+each statistic's split-half difference (at half the cells, so noisier than the comparison at
+full size) sets the allowed fast-vs-oracle difference, with the floors of the small
+configuration's test (test_gpu_fast.py) below which a difference is not looked at."""
+import sys
+
+import numpy as np
+import pytest
+
+import contact_stats as CS
+from modlefork_amd import _lib as L
+from modlefork_amd import simulation as S
+from modlefork_amd import workloads as W
+from oracle import config as OC
+from oracle import refcpu as R
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = S.Context(0)
+    yield c
+    c.close()
+
+
+def _fast_cells(ctx, wl, name, ncells):
+    """Fast mode on cells [0, ncells) of one chromosome of the workload, with the per-cell
+    targets of the full workload (so the cells are those of the full run)."""
+    sim = S.Simulation(wl.options, ctx)
+    names, chroms, bars, plans = sim.prepare(wl)
+    i = names.index(name)
+    p = plans[i]
+    sub = S.ChromPlan(p.nlefs, p.nrows, p.ncols, p.targets[:ncells].copy(), p.target_epochs)
+    job = S.Job(ctx, sim.params, [chroms[i]], [bars[i]], [sub], L.MODE_FAST)
+    st = job.run()
+    tun = job.tuning()
+    band, missed = job.band(0)
+    job.close()
+    return band, missed, st, p, tun
+
+
+def _oracle_halves(wl, name, ncells, nthreads=16):
+    c = OC.transform(wl.options)
+    rc = OC.to_rc(c)
+    size = dict(wl.chroms)[name]
+    bars = OC.prepare_barriers(wl.barriers_of(name), 0, size, c)
+    p = S.plan_chrom(S.make_params(wl.options), S.chrom_view(name, size),
+                     wl.options["num_cells"])
+    ch = R.make_chrom(name, size, 0, size)
+    h = ncells // 2
+    t = np.ascontiguousarray(p.targets, np.uint64)
+    a, ma, sa = R.simulate_cells(rc, ch, bars, p.nlefs, 0, h, t[:h], p.target_epochs, p.nrows,
+                                 p.ncols, nthreads)
+    b, mb, sb = R.simulate_cells(rc, ch, bars, p.nlefs, h, ncells, t[h:ncells], p.target_epochs,
+                                 p.nrows, p.ncols, nthreads)
+    return (a, ma, sa), (b, mb, sb), bars, p
+
+
+def _compare(fband, fst, a, b, nrows, ncols, bars, bin_size, label):
+    """Statistical comparison of a fast-mode band with the oracle's two halves (same cells,
+    same targets). Returns a list of failures."""
+    aband, bband = a[0], b[0]
+    oband = aband.astype(np.uint32) + bband
+    failures = []
+    # contact decay P(s): log-spaced diagonal groups with >= 5000 oracle contacts (the rest pooled
+    # into a tail group); allowed: max(10 %, 4 x the RMS split-half difference), tail 35 %
+    _, osums, _ = CS.decay_groups(oband, nrows, ncols)
+    keep = osums >= 5000
+    po, osums, groups = CS.decay_groups(oband, nrows, ncols, keep)
+    pf, _, _ = CS.decay_groups(fband, nrows, ncols, keep)
+    pa, _, _ = CS.decay_groups(aband, nrows, ncols, keep)
+    pb, _, _ = CS.decay_groups(bband, nrows, ncols, keep)
+    full = osums >= 5000
+    half = np.abs(pa - pb) / (0.5 * (pa + pb))
+    rms_half = float(np.sqrt(np.mean(half[full] ** 2))) if full.any() else 0.0
+    rel = np.abs(pf - po) / po
+    tol = np.where(full, max(0.10, 4 * rms_half), max(0.35, 4 * rms_half))
+    say(f"{label}: P(s) groups {[(int(x), int(y)) for x, y in groups]}")
+    say(f"{label}: P(s) relative differences {np.round(rel, 3).tolist()} "
+        f"(oracle split halves RMS {rms_half:.3f})")
+    if (rel > tol).any():
+        failures.append(f"P(s) {np.round(rel, 3).tolist()} over {np.round(tol, 3).tolist()}")
+    # coarse-grained matrices (50 kb blocks): the fast-vs-oracle correlation (full samples) at
+    # least the oracle's split-half correlation (half samples) less 0.01
+    w = max(1, 50_000 // bin_size)
+    r = CS.pearson(CS.block_sums(fband, nrows, ncols, w), CS.block_sums(oband, nrows, ncols, w))
+    r_half = CS.pearson(CS.block_sums(aband, nrows, ncols, w), CS.block_sums(bband, nrows, ncols, w))
+    say(f"{label}: block correlation {r:.4f} (oracle split halves {r_half:.4f})")
+    if r < r_half - 0.01:
+        failures.append(f"block correlation {r} vs split halves {r_half}")
+    # barrier stripes and convergent-pair dots
+    bins = (bars.pos // bin_size).astype(np.int64)
+    for what, fn, floor in (("stripe", CS.barrier_stripe_enrichment, 0.10),
+                            ("dot", CS.barrier_dot_enrichment, 0.30)):
+        eo, ef = fn(oband, nrows, ncols, bins, bars.dir), fn(fband, nrows, ncols, bins, bars.dir)
+        ea, eb = fn(aband, nrows, ncols, bins, bars.dir), fn(bband, nrows, ncols, bins, bars.dir)
+        allowed = max(floor, 3 * abs(ea - eb) / eo)
+        say(f"{label}: {what} enrichment oracle {eo:.3f} fast {ef:.3f} (halves {ea:.3f} / "
+            f"{eb:.3f}, allowed {allowed:.3f})")
+        if abs(ef - eo) / eo > allowed:
+            failures.append(f"{what} {ef} vs {eo}")
+    # epochs per cell: mean burn-in and total within max(5 %, 4 standard errors)
+    ost = np.concatenate([a[2], b[2]])
+    n = len(ost)
+    for lab, col, fval in (("burn-in", 1, fst.burnin_epochs / fst.cells),
+                           ("total", 0, fst.epochs / fst.cells)):
+        x = ost[:, col].astype(np.float64)
+        allowed = max(0.05 * x.mean(), 4.0 * x.std() * np.sqrt(2.0 / n))
+        say(f"{label}: {lab} epochs per cell oracle {x.mean():.1f} fast {fval:.1f} "
+            f"(allowed {allowed:.1f})")
+        if abs(fval - x.mean()) > allowed:
+            failures.append(f"{lab} epochs {fval} vs {x.mean()}")
+    return failures
+
+
+def _genome_chrom(which):
+    wl = W.genome()
+    by_len = sorted(wl.chroms, key=lambda c: c[1])
+    return wl, (by_len[0] if which == "smallest" else by_len[-1])[0]
+
+
+@pytest.mark.parametrize("which", ["smallest", "largest"])
+def test_genome_chromosome_statistics_match_oracle(ctx, which):
+    """128 cells of the genome workload's smallest (42 Mbp, 841 LEFs: four cells per CU) or
+    largest chromosome (241 Mbp, 4 827 LEFs: one 1024-lane cell per CU), fast mode against the
+    oracle, with the full workload's per-cell targets."""
+    wl, name = _genome_chrom(which)
+    ncells = 128
+    say(f"genome {which} ({name}): oracle, {ncells} cells in two halves")
+    a, b, bars, plan = _oracle_halves(wl, name, ncells)
+    say(f"genome {which} ({name}): fast mode")
+    fband, fmissed, fst, _, tun = _fast_cells(ctx, wl, name, ncells)
+    say(f"genome {which}: launch tiers {list(tun.tier_threads[:tun.launches])}, "
+        f"{fst.kernel_ms:.1f} ms")
+    tot = int(plan.targets[:ncells].sum())
+    assert fst.dropped == 0
+    assert int(fband.sum()) + fmissed == tot == int(a[0].sum()) + a[1] + int(b[0].sum()) + b[1]
+    c = OC.transform(wl.options)
+    failures = _compare(fband, fst, a, b, plan.nrows, plan.ncols, bars, c["bin_size"],
+                        f"genome {which}")
+    assert not failures, failures
+
+
+def _decay_is_decreasing(band, nrows, ncols, slack=1.05):
+    p, _ = CS.log_binned_decay(band, nrows, ncols)
+    return bool(np.all(p[1:] <= p[:-1] * slack)), p
+
+
+def test_genome_workload_full_size_conserves_contacts(ctx):
+    """All 24 x 512 cells of the genome workload in one run (the benchmark's step): every
+    chromosome's band plus its out-of-band count equals its planned contacts, no contact is
+    dropped, the padding element stays zero, and contact frequency falls with distance."""
+    wl = W.genome()
+    sim = S.Simulation(wl.options, ctx)
+    names, chroms, bars, plans = sim.prepare(wl)
+    job = S.Job(ctx, sim.params, chroms, bars, plans, L.MODE_FAST)
+    st = job.run()
+    say(f"genome full size: {st.cells} cells, {st.kernel_ms:.1f} ms, launches {job.launches()}")
+    assert st.cells == 24 * 512 and st.dropped == 0
+    total = 0
+    for i, name in enumerate(names):
+        band, missed = job.band(i)
+        p = plans[i]
+        want = int(p.targets.sum())
+        total += want
+        assert band[p.nrows * p.ncols] == 0, name
+        assert int(band.sum(dtype=np.uint64)) + missed == want, name
+        ok, dec = _decay_is_decreasing(band, p.nrows, p.ncols)
+        assert ok, (name, dec)
+    job.close()
+    assert st.contacts == total
+
+
+def test_stress_statistics_match_oracle(ctx):
+    """32 cells of the stress workload (250 Mbp, 1 kb bins, 5 000 diagonals, 10 000 LEFs, 12 000
+    barriers: working sets too large for LDS, run hybrid), fast mode against the oracle, with
+    the full workload's per-cell targets (610 352 contacts per cell)."""
+    wl = W.stress()
+    ncells = 32
+    say(f"stress: oracle, {ncells} cells in two halves")
+    a, b, bars, plan = _oracle_halves(wl, "chrL", ncells)
+    say("stress: fast mode")
+    fband, fmissed, fst, _, tun = _fast_cells(ctx, wl, "chrL", ncells)
+    say(f"stress: launch tiers {list(tun.tier_threads[:tun.launches])}, {fst.kernel_ms:.1f} ms")
+    tot = int(plan.targets[:ncells].sum())
+    assert fst.dropped == 0
+    assert int(fband.sum(dtype=np.uint64)) + fmissed == tot
+    assert int(a[0].sum(dtype=np.uint64)) + a[1] + int(b[0].sum(dtype=np.uint64)) + b[1] == tot
+    failures = _compare(fband, fst, a, b, plan.nrows, plan.ncols, bars, 1000, "stress")
+    assert not failures, failures
+
+
+def test_stress_workload_full_size_conserves_contacts(ctx):
+    """All 2 048 cells of the stress workload: the 5 GB band plus its out-of-band count equals
+    the planned 1.25 billion contacts, none dropped, and contact frequency falls with distance."""
+    wl = W.stress()
+    out, st = S.Simulation(wl.options, ctx).run_simulate(wl, L.MODE_FAST)
+    band, missed, plan = out["chrL"]
+    say(f"stress full size: {st.cells} cells, {st.kernel_ms:.1f} ms, {st.contacts} contacts")
+    assert st.cells == 2048 and st.dropped == 0
+    assert band[plan.nrows * plan.ncols] == 0
+    assert int(band.sum(dtype=np.uint64)) + missed == int(plan.targets.sum()) == st.contacts
+    ok, dec = _decay_is_decreasing(band, plan.nrows, plan.ncols)
+    assert ok, dec
+
+
+def test_surrogate_at_shipped_shape_matches_oracle(ctx):
+    """The shipped network (default shape, trained weights) on two windows of sweep inputs:
+    forward within 3 % of the output range (correlation > 0.999) and every parameter tensor's
+    gradient within cosine 0.99 and 12 % relative norm of the CPU oracle's, run on the same
+    bf16-rounded weights and inputs."""
+    import os
+
+    from modlefork_amd import surrogate as SG
+    from modlefork_amd import sweep as Sw
+    from oracle import surrogate_ref as SR
+
+    net = SG.Surrogate(ctx, SG.default_config(max_batch=2))
+    SG.load_params(net, os.path.join(os.path.dirname(SG.__file__), "data",
+                                     "surrogate_weights.npy"))
+    c = SR.SgConfig()
+    p = net.params()
+    layout = Sw.fixed_layout()
+    x = Sw.point_inputs(layout, Sw.sample_points(1, seed=42)[0])[5:7].astype(np.float32)
+    pr = SR.unflatten(c, SG.bf16_round(p).astype(np.float64))
+    xr = SG.bf16_round(x).astype(np.float64)
+    dev = net.forward(x)
+    ref, _ = SR.forward(c, pr, xr)
+    err = np.abs(dev - ref).max() / np.abs(ref).max()
+    corr = np.corrcoef(dev.ravel(), ref.ravel())[0, 1]
+    say(f"shipped surrogate forward: max error {err:.4f} of the output range, corr {corr:.6f}")
+    assert err < 0.03 and corr > 0.999
+    # targets: the oracle's prediction perturbed, so the loss and its gradients are not degenerate
+    rng = np.random.default_rng(8)
+    y = (ref + 0.3 * rng.standard_normal(ref.shape)).astype(np.float32)
+    loss, g = net.gradients(x, y)
+    rl, rg, _ = SR.loss_and_grads(c, pr, xr, y.astype(np.float64))
+    say(f"shipped surrogate loss device {loss:.6f} oracle {rl:.6f}")
+    assert abs(loss - rl) < 0.02 * rl
+    k = 0
+    for name, sh in SR.param_shapes(c):
+        n = int(np.prod(sh))
+        ga, gb = g[k:k + n].astype(np.float64), rg[k:k + n]
+        k += n
+        nb = np.linalg.norm(gb)
+        cos = float(ga @ gb / (np.linalg.norm(ga) * nb + 1e-30))
+        rel = float(np.linalg.norm(ga - gb) / (nb + 1e-30))
+        say(f"  {name:10s} cos {cos:.5f} rel {rel:.4f}")
+        assert cos > 0.99 and rel < 0.12, name
+    net.close()
+
+
+def test_full_sweep_judges_and_accepts_the_surrogate(ctx):
+    """The 4 096-point sweep with the shipped weights: 205 points (5 %) judged by simulation, the
+    surrogate accepted (at most a fifth of the judged points fail), every failing point replaced
+    by its simulation, the mean judged correlation at least 0.95 and the sweep faster than
+    simulating every point."""
+    import os
+
+    from modlefork_amd import surrogate as SG
+    from modlefork_amd import sweep as Sw
+
+    layout = Sw.fixed_layout()
+    # eight whole chromosomes per forward batch, as the benchmark runs it
+    net = SG.Surrogate(ctx, SG.default_config(
+        max_batch=8 * len(Sw.window_starts(layout.size // 5000))))
+    SG.load_params(net, os.path.join(os.path.dirname(SG.__file__), "data",
+                                     "surrogate_weights.npy"))
+    pts = Sw.sample_points(4096)
+    res, bands = Sw.run_sweep(ctx, net, layout, pts, log=say, rate_sample=32)
+    net.close()
+    say(f"sweep: {res.sweep_rate:.1f} matrices/s, plain simulation {res.sim_rate:.2f}")
+    assert len(res.judged) == 205 and not res.fell_back
+    assert len(res.replaced) <= 0.2 * len(res.judged)
+    assert sorted(bands) == sorted(res.replaced)
+    assert np.nanmean(res.correlations) >= 0.95
+    assert all(not c >= 0.95 for k, c in zip(res.judged, res.correlations) if k in bands)
+    assert res.sweep_rate > res.sim_rate

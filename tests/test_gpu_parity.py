@@ -182,3 +182,57 @@ def test_cell_subset_adds_up(ctx):
             acc.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(missed), None))
     full, _, _ = device_chrom(ctx, wl)
     assert np.array_equal(acc, full)
+
+
+UNIT_KAT = None
+
+
+def _unit_kat():
+    global UNIT_KAT
+    if UNIT_KAT is None:
+        import unit_kat_replay
+
+        UNIT_KAT = unit_kat_replay.load()
+    return UNIT_KAT
+
+
+@pytest.mark.parametrize("mode", [L.MODE_DETERMINISTIC, L.MODE_FAST])
+@pytest.mark.parametrize("case", ["CMatrix simple", "CMatrix in/decrement"])
+def test_device_band_increments_match_reference_kat(ctx, mode, case):
+    """contact_matrix_dense_test.cpp's increments applied by the device's band code (each mode's
+    own), including the out-of-band increment at (11, 0) of a 10-row band that only counts as
+    missed; a coordinate beyond the matrix is refused."""
+    import unit_kat_replay as K
+
+    def increment(band, missed, nrows, ncols, rows, cols):
+        m = C.c_uint64(missed)
+        r = np.asarray(rows, np.uint64)
+        c = np.asarray(cols, np.uint64)
+        ctx.check(L.lib().mgpu_debug_band_increments(
+            ctx.ptr, mode, nrows, ncols, len(r), r.ctypes.data_as(L.U64P),
+            c.ctypes.data_as(L.U64P), band.ctypes.data_as(L.U32P), C.byref(m)))
+        return m.value
+
+    say(f"band KAT {case}, mode {mode}")
+    K.replay_band(_unit_kat()["cmatrix"][case], increment)
+    band = np.zeros(10 * 20 + 1, np.uint32)
+    with pytest.raises(L.MgpuError, match="outside the matrix"):
+        increment(band, 0, 10, 20, [25], [25])
+
+
+def test_device_collision_words_match_reference_kat(ctx):
+    """collision_encoding_test.cpp's encodings and occurred / avoided checks on the device's
+    collision words (deterministic mode's reference layout)."""
+    import unit_kat_replay as K
+
+    def words(idx, ev):
+        i = np.array([idx], np.uint64)
+        e = np.array([ev], np.uint8)
+        w, ix = np.zeros(1, np.uint64), np.zeros(1, np.uint64)
+        evo, fl = np.zeros(1, np.uint32), np.zeros(1, np.uint32)
+        ctx.check(L.lib().mgpu_debug_collisions(
+            ctx.ptr, 1, i.ctypes.data_as(L.U64P), e.ctypes.data_as(L.U8P), w.ctypes.data_as(L.U64P),
+            ix.ctypes.data_as(L.U64P), evo.ctypes.data_as(L.U32P), fl.ctypes.data_as(L.U32P)))
+        return int(w[0]), int(ix[0]), int(evo[0]), int(fl[0])
+
+    assert K.replay_collisions(_unit_kat()["collisions"]["cases"], words) == 56
