@@ -191,3 +191,26 @@ def test_no_device_is_reported_as_error_not_fallback():
         pytest.skip("a device is present")
     with pytest.raises(L.MgpuError):
         S.Context(0)
+
+
+def _abi_caller():
+    import os
+    import subprocess
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    exe = os.path.join(here, "cpp", "abi_caller")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-s", "-C", os.path.join(here, "cpp")], check=True)
+    return exe
+
+
+def test_cpp_caller_fills_params_like_the_transform():
+    """A C++ program compiled against include/modle_gpu.h and include/modle_gpu_config.hpp (no
+    ctypes): fill_params_from_config on a modle::Config holding what Cli::transform_args leaves
+    for `-r 5kb -w 1Mbp --extrusion-barrier-occupancy 0.825` gives mgpu_config_transform's block
+    for the same options, and both hold the defaults' closed forms (4 kb per epoch, sigma 200 bp,
+    release 8000 / 300000, 187 activation epochs, occupied stp 1 - 0.3 * 0.175 / 0.825)."""
+    import subprocess
+
+    r = subprocess.run([_abi_caller(), "params"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout + r.stderr

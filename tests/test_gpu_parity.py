@@ -236,3 +236,19 @@ def test_device_collision_words_match_reference_kat(ctx):
         return int(w[0]), int(ix[0]), int(evo[0]), int(fl[0])
 
     assert K.replay_collisions(_unit_kat()["collisions"]["cases"], words) == 56
+
+
+def test_cpp_caller_simulates_through_the_c_abi():
+    """The C++ caller (tests/cpp/abi_caller.cpp, no ctypes) runs the integration's call sequence,
+    mgpu_init -> mgpu_prepare_barriers -> mgpu_plan_chrom -> mgpu_simulate_chrom, on a 2 Mbp
+    chromosome with 40 barriers and 4 cells: the deterministic band equals the oracle's bit for
+    bit, with its missed count, and fast mode conserves every contact."""
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "abi_caller")
+    assert os.path.exists(exe), "build the C++ caller first (__graft_entry__.build())"
+    say("C++ caller: run")
+    r = subprocess.run([exe, "run"], capture_output=True, text=True, timeout=240)
+    say(r.stdout.strip())
+    assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout + r.stderr
