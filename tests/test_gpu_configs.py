@@ -250,9 +250,12 @@ def test_surrogate_at_shipped_shape_matches_oracle(ctx):
     corr = np.corrcoef(dev.ravel(), ref.ravel())[0, 1]
     say(f"shipped surrogate forward: max error {err:.4f} of the output range, corr {corr:.6f}")
     assert err < 0.03 and corr > 0.999
-    # targets: the oracle's prediction perturbed, so the loss and its gradients are not degenerate
+    # targets 30 % below the prediction (plus noise): a residual with a systematic part, as in
+    # training. With targets equal to the prediction plus zero-mean noise, every gradient is a
+    # sum of cancelling terms, and bf16 activations alone move it by 25 % (measured on the
+    # oracle with bf16 rounding at each stored tensor); with this residual, by below 1 %.
     rng = np.random.default_rng(8)
-    y = (ref + 0.3 * rng.standard_normal(ref.shape)).astype(np.float32)
+    y = (0.7 * ref + 0.05 * rng.standard_normal(ref.shape)).astype(np.float32)
     loss, g = net.gradients(x, y)
     rl, rg, _ = SR.loss_and_grads(c, pr, xr, y.astype(np.float64))
     say(f"shipped surrogate loss device {loss:.6f} oracle {rl:.6f}")
