@@ -17,7 +17,8 @@ eagerly, so HIP events around each launch give the per-launch roofline in the ti
 (mgpu_job_capture's graph replays carry no per-launch events).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload genome|small|stress]
-                    [--mode fast|det] [--cpu-seconds S] [--sweep-points P | --no-sweep]
+                    [--mode fast|det] [--cpu-cells K] [--cpu-threads T] [--no-stress]
+                    [--sweep-points P | --no-sweep]
                     [--autotune]
 """
 from __future__ import annotations
@@ -63,9 +64,31 @@ def launch_bytes(ls) -> int:
     return 8 * int(ls.contacts) + 21 * int(ls.barriers) + 32 * int(ls.cells)
 
 
-def cpu_baseline(wl, options, budget_s, nthreads):
-    """The oracle on this host: for every chromosome one batch of `nthreads` cells runs in
-    parallel; the workload's wall time is extrapolated linearly in cells per chromosome."""
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo, as lscpu reports it)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(wl, options, cells_per_chrom, nthreads, single_cells=1):
+    """The oracle on this host's cores, on a stratified sample of the workload: cells
+    0 .. cells_per_chrom - 1 of every chromosome (the same fraction of each, so the sample has the
+    workload's mix of chromosome sizes), run as one task per cell from a shared queue on
+    `nthreads` worker threads, the way simulate_worker takes tasks (scheduler_simulate.cpp:
+    186-284); each chromosome's cells add into one shared band. The workload's wall time is
+    extrapolated from the summed per-cell times divided by the threads (a full run balances
+    thousands of cells over the threads, so its tail is negligible; the sample's own wall time,
+    which includes its tail, is reported beside it). A 1-thread run of `single_cells` cells per
+    chromosome gives the per-core rate."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+
     from oracle import config as OC
     from oracle import refcpu as R
 
@@ -74,12 +97,9 @@ def cpu_baseline(wl, options, budget_s, nthreads):
     rc = OC.to_rc(c)
     cfg = R.make_config(rc)
     ncells = options["num_cells"]
+    per_chrom = []
     total_cells = 0
-    est_wall = 0.0
-    sampled = 0
-    spent = 0.0
-    chroms = list(wl.chroms)
-    for name, size in chroms:
+    for name, size in wl.chroms:
         bars = OC.prepare_barriers(wl.barriers_of(name), 0, size, c)
         if len(bars.pos) == 0:
             continue
@@ -90,33 +110,73 @@ def cpu_baseline(wl, options, budget_s, nthreads):
         t = np.zeros(ncells, np.uint64)
         te = R.lib().rc_plan_targets(C.byref(cfg), ncells, nl, nrows * ncols,
                                      t.ctypes.data_as(C.POINTER(C.c_uint64)))
-        k = min(nthreads, ncells)
-        ch = R.make_chrom(name, size, 0, size)
+        per_chrom.append(dict(ch=R.make_chrom(name, size, 0, size), bars=bars, nl=nl, nrows=nrows,
+                              ncols=ncols, t=t, te=te, band=None, lock=threading.Lock()))
+
+    def run_cell(k, cell):
+        pc = per_chrom[k]
+        with pc["lock"]:
+            if pc["band"] is None:
+                pc["band"] = np.zeros(pc["nrows"] * pc["ncols"] + 1, np.uint32)
         t0 = time.perf_counter()
-        R.simulate_cells(rc, ch, bars, nl, 0, k, t[:k], te, nrows, ncols, nthreads)
-        dt = time.perf_counter() - t0
-        spent += dt
-        sampled += k
-        est_wall += dt * (ncells / k)
-        if spent > budget_s:
-            # remaining chromosomes: extrapolate by LEF count (cost per cell ~ LEFs x epochs)
-            done = chroms.index((name, size)) + 1
-            if done < len(chroms):
-                per_lef = est_wall / sum(R.lib().rc_num_lefs(c["number_of_lefs_per_mbp"], s)
-                                         for _, s in chroms[:done])
-                for _, s in chroms[done:]:
-                    est_wall += per_lef * R.lib().rc_num_lefs(c["number_of_lefs_per_mbp"], s)
-                    total_cells += ncells
-            break
+        R.simulate_cells(rc, pc["ch"], pc["bars"], pc["nl"], cell, cell + 1,
+                         pc["t"][cell:cell + 1], pc["te"], pc["nrows"], pc["ncols"], 1,
+                         band=pc["band"])
+        return time.perf_counter() - t0
+
+    # per-core rate: single_cells cells of every chromosome on this thread alone
+    single = [run_cell(k, cell) for k in range(len(per_chrom)) for cell in range(single_cells)]
+    tasks = [(k, cell) for k in range(len(per_chrom))
+             for cell in range(single_cells, single_cells + cells_per_chrom)]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=nthreads) as ex:
+        times = list(ex.map(lambda kc: run_cell(*kc), tasks))
+    wall = time.perf_counter() - t0
+    for pc in per_chrom:
+        pc["band"] = None
+    rate = len(tasks) * nthreads / sum(times)
     return {
-        "value": total_cells / est_wall,
+        "value": rate,
         "unit": "cells/s",
         "cores": nthreads,
         "kind": "port",
-        "sample": f"oracle (C restatement of the reference) on {sampled} cells: {min(nthreads, ncells)} "
-                  f"cells per chromosome run in parallel, workload wall time extrapolated "
-                  f"linearly in cells ({spent:.1f} s of CPU wall)",
+        "cpu_model": cpu_model(),
+        "per_core_cells_per_s": len(single) / sum(single),
+        "sample_wall_cells_per_s": len(tasks) / wall,
+        "sample": f"oracle (C restatement of the reference) on {len(tasks)} of the workload's "
+                  f"{total_cells} cells ({cells_per_chrom} of each chromosome), one task per "
+                  f"cell on {nthreads} threads; rate = cells x threads / summed per-cell time "
+                  f"({sum(times):.1f} thread-s; the sample's own wall {wall:.1f} s gives "
+                  f"{len(tasks) / wall:.1f} cells/s); per core from {len(single)} cells on one "
+                  f"thread ({sum(single):.1f} s)",
     }
+
+
+def stress_leg(ctx, mode, nthreads, no_cpu):
+    """The large-state stress workload (one 250 Mbp chromosome, 1 kb bins, 5 Mbp band, 10 000
+    LEFs, 12 000 barriers, 2048 cells; working sets run hybrid, LDS and device memory): one timed
+    run after a warm-up, and the oracle on 2 cells per thread of it."""
+    from modlefork_amd import simulation as S
+    from modlefork_amd import workloads as W
+
+    wl = W.stress()
+    sim = S.Simulation(wl.options, ctx)
+    names, chroms, bars, plans = sim.prepare(wl)
+    job = S.Job(ctx, sim.params, chroms, bars, plans, mode)
+    job.run()
+    t0 = time.perf_counter()
+    st = job.run()
+    wall = time.perf_counter() - t0
+    job.close()
+    out = {"workload": wl.name, "cells": int(st.cells), "cells_per_sec": st.cells / wall,
+           "contacts_per_sec": st.contacts / wall, "epochs_per_sec": st.epochs / wall,
+           "kernel_ms": float(st.kernel_ms), "dropped": int(st.dropped)}
+    log(f"stress: {st.cells} cells in {wall:.2f} s ({st.cells / wall:.1f} cells/s)")
+    if not no_cpu:
+        cpu = cpu_baseline(wl, wl.options, 2 * nthreads, nthreads)
+        out["cpu_baseline"] = cpu
+        out["speedup_vs_cpu"] = out["cells_per_sec"] / cpu["value"]
+    return out
 
 
 def surrogate_flops(cfg) -> float:
@@ -182,8 +242,15 @@ def main():
     ap.add_argument("--workload", default="genome")
     ap.add_argument("--mode", default=os.environ.get("MGPU_BENCH_MODE", "fast"))
     ap.add_argument("--ncells", type=int, default=0)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-cells", type=int, default=0,
+                    help="cells per chromosome of the CPU baseline's sample (0: 4 per thread, or "
+                         "2 per thread on the stress workload)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU baseline threads (0: this process's CPUs, at most 16: the host's "
+                         "share of one GPU)")
+    ap.add_argument("--no-stress", action="store_true",
+                    help="skip the stress leg (250 Mbp, 1 kb bins, 2048 cells: one GPU run and its "
+                         "CPU baseline)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep-points", type=int, default=4096)
     ap.add_argument("--no-sweep", action="store_true", help="skip the surrogate sweep leg")
@@ -308,10 +375,11 @@ def main():
                        "frac": dv["atomics_per_s"] / ATOMIC_PEAK_PER_S}
 
     cpu = None
+    nthreads = args.cpu_threads or min(len(os.sched_getaffinity(0)), 16)
     if not args.no_cpu:
-        nthreads = args.cpu_threads or min(os.cpu_count() or 1, 64)
-        log(f"cpu baseline: {nthreads} threads, ~{args.cpu_seconds:.0f} s")
-        cpu = cpu_baseline(wl, wl.options, args.cpu_seconds, nthreads)
+        k = args.cpu_cells or (2 if args.workload == "stress" else 4) * nthreads
+        log(f"cpu baseline: {k} cells per chromosome on {nthreads} threads")
+        cpu = cpu_baseline(wl, wl.options, k, nthreads)
 
     line = {
         "metric": "cells/sec + contacts/sec at 1 GPU",
@@ -352,6 +420,8 @@ def main():
     if cpu:
         line["speedup_vs_cpu"] = value / cpu["value"]
     job.close()
+    if not args.no_stress and args.workload == "genome":
+        line["stress"] = stress_leg(sim.ctx, mode, nthreads, args.no_cpu)
     if not args.no_sweep:
         log(f"surrogate sweep: {args.sweep_points} points")
         line["surrogate"] = surrogate_leg(sim.ctx, args.sweep_points)

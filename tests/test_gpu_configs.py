@@ -227,8 +227,8 @@ def test_stress_workload_full_size_conserves_contacts(ctx):
 def test_surrogate_at_shipped_shape_matches_oracle(ctx):
     """The shipped network (default shape, trained weights) on two windows of sweep inputs:
     forward within 3 % of the output range (correlation > 0.999) and every parameter tensor's
-    gradient within cosine 0.99 and 12 % relative norm of the CPU oracle's, run on the same
-    bf16-rounded weights and inputs."""
+    gradient within cosine 0.999 and 2 % relative norm of the CPU oracle's (fp64, run on the
+    same bf16-rounded weights and inputs), the first encoder layer's included."""
     import os
 
     from modlefork_amd import surrogate as SG
@@ -269,7 +269,7 @@ def test_surrogate_at_shipped_shape_matches_oracle(ctx):
         cos = float(ga @ gb / (np.linalg.norm(ga) * nb + 1e-30))
         rel = float(np.linalg.norm(ga - gb) / (nb + 1e-30))
         say(f"  {name:10s} cos {cos:.5f} rel {rel:.4f}")
-        assert cos > 0.99 and rel < 0.12, name
+        assert cos > 0.999 and rel < 0.02, name  # measured: rel <= 0.005 in every tensor
     net.close()
 
 
