@@ -193,6 +193,22 @@ int mgpu_simulate_chrom(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom* c
                         uint64_t ncells, const uint64_t* target_contacts, uint64_t target_epochs,
                         int mode, uint32_t* out_band, uint64_t* out_missed, mgpu_stats* stats);
 
+/* mgpu_simulate_chrom in fast mode without any device allocation. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). Inputs are host arrays as for
+ * mgpu_simulate_chrom; d_band is the caller's device memory for the chromosome's band
+ * (nrows*ncols + 1 u32), to which the cells' contacts are added; *out_missed receives the
+ * out-of-band count. Counters, barrier tables, work items and per-cell statistics live in a
+ * statically allocated 4 MiB scratch area per GPU (set up by mgpu_init), so the call takes at
+ * most 65 536 barriers and 16 384 cells, cells whose working set fits in LDS (about 5 000 LEFs
+ * with its barriers), no 1D occupancy or state log; larger inputs are MGPU_ERANGE / MGPU_EINVAL.
+ * Every call on one GPU shares that scratch area: calls that overlap on one GPU race. */
+int mgpu_simulate_chrom_noalloc(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom* chrom,
+                                const mgpu_barriers* barriers, uint64_t nlefs,
+                                uint64_t cell_begin, uint64_t ncells,
+                                const uint64_t* target_contacts, uint64_t target_epochs,
+                                uint32_t* d_band, uint64_t* out_missed, mgpu_stats* stats);
+
 /* Device-resident batch over many chromosomes: inputs are uploaded once, bands live in HBM until
  * read back. per_chrom arrays have nchroms entries; target_contacts[c] has ncells[c] entries. */
 int mgpu_job_create(mgpu_ctx* ctx, const mgpu_params* p, uint64_t nchroms,

@@ -265,6 +265,10 @@ SIGNATURES = [
     ("mgpu_debug_draws", C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.c_double, C.c_double,
                                    C.c_double, C.c_uint64, U64P]),
     ("mgpu_debug_philox", C.c_int, [C.c_void_p, U32P, C.c_uint64, U32P]),
+    ("mgpu_simulate_chrom_noalloc", C.c_int, [C.c_void_p, C.POINTER(MgpuParams),
+                                              C.POINTER(MgpuChrom), C.POINTER(MgpuBarriers),
+                                              C.c_uint64, C.c_uint64, C.c_uint64, U64P, C.c_uint64,
+                                              C.c_void_p, U64P, C.POINTER(MgpuStats)]),
     ("mgpu_debug_fast_math", C.c_int, [C.c_void_p, C.c_int, DP, C.c_uint64, DP]),
     ("mgpu_debug_band_increments", C.c_int, [C.c_void_p, C.c_int, C.c_uint64, C.c_uint64,
                                              C.c_uint64, U64P, U64P, U32P, U64P]),

@@ -73,6 +73,27 @@ void fast_job_set_log(FastJob* job, const StateLog& log);
 void fast_job_set_serial_fixups(FastJob* job, int serial);
 // Philox4x32-10 of n (counter[4], key[2]) tuples: in holds 6 words per tuple, out 4.
 hipError_t launch_philox_kat(hipStream_t stream, const uint32_t* d_in, uint32_t* d_out, int n);
+// ---- the allocation-free entry point ----
+// Scratch memory of fast_simulate_noalloc: one statically allocated copy per GPU (4 MiB), enough
+// for 65536 barriers and 16384 cells of one chromosome.
+constexpr uint64_t kNoAllocScratchBytes = 4ull << 20;
+constexpr uint64_t kNoAllocMaxBarriers = 65536;
+constexpr uint64_t kNoAllocMaxCells = 16384;
+// Resolves this device's copy of the scratch memory (call once per device with that device
+// current; later calls return at once).
+hipError_t fast_noalloc_init(int device);
+// Simulates the work items (all of chromosome `chrom`, whose band, nrows, cpe, target epochs and
+// bind rate are read from chrom; its barrier arrays are the host arrays pos/dir/sa/si) in fast mode
+// with no device allocation: inputs, counters and statistics go to this device's scratch memory,
+// contacts are added to chrom.band (the caller's device memory). Requires cells whose working set
+// fits in LDS. ev0 / ev1 bracket the launch; *missed, dropped[0..1] and sums (epochs, burn-in
+// epochs, contacts over the cells) are returned. 0 on success, -1 with fast_last_error() set.
+int fast_simulate_noalloc(hipStream_t stream, int device, uint32_t num_cus, const mgpu_params& p,
+                          const DetChrom& chrom, const uint64_t* pos, const uint8_t* dir,
+                          const double* sa, const double* si, double release,
+                          double release_burnin, const std::vector<WorkItem>& items,
+                          hipEvent_t ev0, hipEvent_t ev1, uint64_t* missed, uint64_t* dropped,
+                          uint64_t* sums);
 // Fast mode's band increments (32-bit coordinates) for (rows[i], cols[i]), in order.
 hipError_t launch_fast_band(hipStream_t stream, uint32_t* d_band, unsigned long long* d_missed,
                             uint32_t nrows, const uint32_t* d_rows, const uint32_t* d_cols,

@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <mutex>
 #include <string>
 
 #include "fast_kernels.h"
@@ -1730,11 +1732,12 @@ uint64_t fast_cell_bytes_host(uint32_t n, uint32_t nbar, uint32_t hist) {
   return fast_cell_bytes(n, nbar, hist);
 }
 
-int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& dchroms,
-                    const mgpu_barriers* host_bars, const std::vector<double>& release,
-                    const std::vector<WorkItem>& items, FastJob* job) {
-  auto* impl = new FastImpl();
-  FastParams& P = impl->P;
+namespace {
+
+// The job-wide kernel parameters of a parameter block; false (with g_fast_err set) when fast mode
+// cannot take them.
+bool fill_fast_params(const mgpu_params& p, FastParams& P) {
+  P = FastParams{};
   P.bin_size = (uint32_t)p.bin_size;
   P.rev_speed = (double)p.rev_speed;
   P.fwd_speed = (double)p.fwd_speed;
@@ -1760,6 +1763,63 @@ int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& dchroms,
   P.target_contact_density = p.target_contact_density;
   if (P.hist_len > 4096) {
     g_fast_err = "fast mode supports burn-in histories of up to 4096 epochs";
+    return false;
+  }
+  return true;
+}
+
+// The device view of one chromosome whose barrier tables (positions and the three thresholds,
+// 4 x nbar u32) start at bars; band, counters, direction table and occupancy from d.
+FastChrom make_fast_chrom(const FastParams& P, const DetChrom& d, uint32_t* bars,
+                          double release, double release_burnin) {
+  FastChrom f{};
+  f.start = (uint32_t)d.start;
+  f.end = (uint32_t)d.end;
+  f.nlefs = (uint32_t)d.nlefs;
+  f.nbar = (uint32_t)d.nbar;
+  f.bar_pos = bars;
+  f.bar_t_active = bars + d.nbar;
+  f.bar_t_inactive = bars + 2 * d.nbar;
+  f.bar_t_occ = bars + 3 * d.nbar;
+  f.bar_dir = d.bar_dir;
+  f.band = d.band;
+  f.missed = d.missed;
+  f.dropped = d.dropped;
+  f.nrows = (uint32_t)d.nrows;
+  f.cpe = (uint32_t)d.cpe;
+  f.target_epochs = d.target_epochs;
+  f.bind_rate = d.bind_rate;
+  f.bind_exp_neg = std::exp(-d.bind_rate);
+  f.bind_sd = std::sqrt(d.bind_rate);
+  f.prob_release = release;
+  f.prob_release_burnin = release_burnin;
+  f.lds_bytes = fast_cell_bytes(f.nlefs, f.nbar, P.hist_len);
+  f.occupancy = d.occupancy;
+  return f;
+}
+
+// The 32-bit barrier tables of one chromosome (positions, then the thresholds of the active and
+// inactive self-transitions and of the stationary occupancy) into hp[0, 4 * n).
+void fill_barrier_tables(uint32_t* hp, uint64_t n, const uint64_t* pos, const double* sa,
+                         const double* si) {
+  for (uint64_t i = 0; i < n; ++i) {
+    hp[i] = (uint32_t)pos[i];
+    hp[n + i] = t32(sa[i]);
+    hp[2 * n + i] = t32(si[i]);
+    const double occ =
+        (sa[i] + si[i] == 0) ? 0.0 : (1.0 - si[i]) / ((1.0 - si[i]) + (1.0 - sa[i]));
+    hp[3 * n + i] = t32(occ < 0.0 ? 0.0 : (occ > 1.0 ? 1.0 : occ));
+  }
+}
+
+}  // namespace
+
+int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& dchroms,
+                    const mgpu_barriers* host_bars, const std::vector<double>& release,
+                    const std::vector<WorkItem>& items, FastJob* job) {
+  auto* impl = new FastImpl();
+  FastParams& P = impl->P;
+  if (!fill_fast_params(p, P)) {
     delete impl;
     return -1;
   }
@@ -1821,41 +1881,9 @@ int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& dchroms,
   uint64_t off = 0;
   for (uint64_t c = 0; c < nch; ++c) {
     const DetChrom& d = dchroms[c];
-    const uint64_t* pos = host_bars[c].pos;
-    const double* sa = host_bars[c].stp_active;
-    const double* si = host_bars[c].stp_inactive;
-    FastChrom f{};
-    f.start = (uint32_t)d.start;
-    f.end = (uint32_t)d.end;
-    f.nlefs = (uint32_t)d.nlefs;
-    f.nbar = (uint32_t)d.nbar;
-    uint32_t* hp = hb.data() + off;
-    for (uint64_t i = 0; i < d.nbar; ++i) {
-      hp[i] = (uint32_t)pos[i];
-      hp[d.nbar + i] = t32(sa[i]);
-      hp[2 * d.nbar + i] = t32(si[i]);
-      const double occ =
-          (sa[i] + si[i] == 0) ? 0.0 : (1.0 - si[i]) / ((1.0 - si[i]) + (1.0 - sa[i]));
-      hp[3 * d.nbar + i] = t32(occ < 0.0 ? 0.0 : (occ > 1.0 ? 1.0 : occ));
-    }
-    f.bar_pos = bars + off;
-    f.bar_t_active = bars + off + d.nbar;
-    f.bar_t_inactive = bars + off + 2 * d.nbar;
-    f.bar_t_occ = bars + off + 3 * d.nbar;
-    f.bar_dir = d.bar_dir;
-    f.band = d.band;
-    f.missed = d.missed;
-    f.dropped = d.dropped;
-    f.nrows = (uint32_t)d.nrows;
-    f.cpe = (uint32_t)d.cpe;
-    f.target_epochs = d.target_epochs;
-    f.bind_rate = d.bind_rate;
-    f.bind_exp_neg = std::exp(-d.bind_rate);
-    f.bind_sd = std::sqrt(d.bind_rate);
-    f.prob_release = release[2 * c];
-    f.prob_release_burnin = release[2 * c + 1];
-    f.lds_bytes = fast_cell_bytes(f.nlefs, f.nbar, P.hist_len);
-    f.occupancy = d.occupancy;
+    fill_barrier_tables(hb.data() + off, d.nbar, host_bars[c].pos, host_bars[c].stp_active,
+                        host_bars[c].stp_inactive);
+    const FastChrom f = make_fast_chrom(P, d, bars + off, release[2 * c], release[2 * c + 1]);
     impl->occupancy |= d.occupancy != nullptr;
     impl->chroms.push_back(f);
     off += 4 * d.nbar;
@@ -2152,6 +2180,128 @@ hipError_t launch_fast_math(hipStream_t stream, int fn, const double* d_in, doub
   hipLaunchKernelGGL(fast_math_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, fn, d_in,
                      d_out, n);
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------
+// fast mode without device allocation (mgpu_simulate_chrom_noalloc)
+// ---------------------------------------------------------------------------------------------
+// Everything a call keeps in device memory besides the caller's band: counters, the chromosome
+// descriptor, the barrier tables, work items, work order and per-cell statistics. Each GPU holds
+// its own copy until the process ends; every call on a GPU reuses it, so calls that overlap on
+// one GPU race. In host code this name is a stand-in, not a device address: only the pointer
+// hipGetSymbolAddress returns for the current device is used.
+static __device__ __align__(256) uint8_t noalloc_scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mtx;
+void* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+
+uint64_t round256(uint64_t x) { return (x + 255) & ~255ULL; }
+
+}  // namespace
+
+hipError_t fast_noalloc_init(int device) {
+  if (device < 0 || device >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_noalloc_mtx);
+  if (g_noalloc_ptr[device]) return hipSuccess;
+  void* ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(noalloc_scratch));
+  if (e == hipSuccess && ptr == nullptr) e = hipErrorInvalidSymbol;
+  if (e == hipSuccess) e = set_lds_limits();  // the launches below change no kernel attribute
+  if (e == hipSuccess) g_noalloc_ptr[device] = ptr;
+  return e;
+}
+
+int fast_simulate_noalloc(hipStream_t stream, int device, uint32_t num_cus, const mgpu_params& p,
+                          const DetChrom& chrom, const uint64_t* pos, const uint8_t* dir,
+                          const double* sa, const double* si, double release,
+                          double release_burnin, const std::vector<WorkItem>& items,
+                          hipEvent_t ev0, hipEvent_t ev1, uint64_t* missed, uint64_t* dropped,
+                          uint64_t* sums) {
+  void* base_ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_noalloc_mtx);
+    if (device >= 0 && device < kNoAllocMaxDevices) base_ptr = g_noalloc_ptr[device];
+  }
+  if (!base_ptr) {
+    g_fast_err = "no scratch memory for this device (mgpu_init sets it up)";
+    return -1;
+  }
+  const uint64_t nbar = chrom.nbar, ncells = items.size();
+  if (nbar > kNoAllocMaxBarriers || ncells == 0 || ncells > kNoAllocMaxCells) {
+    g_fast_err = "the allocation-free entry point takes up to 65536 barriers and 1 to 16384 cells";
+    return -1;
+  }
+  FastImpl impl;
+  if (!fill_fast_params(p, impl.P)) return -1;
+  // scratch layout: counters (work queue, missed, dropped x 2), descriptor, barrier tables,
+  // directions, work items, order, per-cell statistics
+  const uint64_t off_desc = 256;
+  const uint64_t off_bars = off_desc + round256(sizeof(FastChrom));
+  const uint64_t off_dir = off_bars + round256(16 * nbar);
+  const uint64_t off_items = off_dir + round256(nbar);
+  const uint64_t off_order = off_items + round256(sizeof(WorkItem) * ncells);
+  const uint64_t off_stats = off_order + round256(4 * ncells);
+  const uint64_t total = off_stats + round256(32 * ncells);
+  if (total > kNoAllocScratchBytes) {
+    g_fast_err = "inputs exceed the allocation-free entry point's scratch memory";
+    return -1;
+  }
+  uint8_t* base = static_cast<uint8_t*>(base_ptr);
+  DetChrom d = chrom;
+  d.bar_dir = base + off_dir;
+  d.missed = reinterpret_cast<unsigned long long*>(base + 8);
+  d.dropped = reinterpret_cast<unsigned long long*>(base + 16);
+  d.occupancy = nullptr;
+  const FastChrom f = make_fast_chrom(impl.P, d, reinterpret_cast<uint32_t*>(base + off_bars),
+                                      release, release_burnin);
+  const uint32_t tier = tier_of(f.lds_bytes, hybrid_lds_bytes(f.nlefs, f.nbar, impl.P.hist_len));
+  if (tier < 1 || tier > 3) {
+    g_fast_err = "the allocation-free entry point takes cells whose working set fits in LDS";
+    return -1;
+  }
+  // host image of the scratch up to the statistics (the counters start at zero with it)
+  std::vector<uint8_t> stage(off_stats, 0);
+  std::memcpy(stage.data() + off_desc, &f, sizeof(FastChrom));
+  fill_barrier_tables(reinterpret_cast<uint32_t*>(stage.data() + off_bars), nbar, pos, sa, si);
+  if (nbar) std::memcpy(stage.data() + off_dir, dir, nbar);
+  std::memcpy(stage.data() + off_items, items.data(), sizeof(WorkItem) * ncells);
+  uint32_t* order = reinterpret_cast<uint32_t*>(stage.data() + off_order);
+  for (uint64_t i = 0; i < ncells; ++i) order[i] = (uint32_t)i;
+
+  impl.d_chroms = reinterpret_cast<FastChrom*>(base + off_desc);
+  impl.d_order = reinterpret_cast<uint32_t*>(base + off_order);
+  impl.d_next = reinterpret_cast<unsigned long long*>(base);
+  impl.num_cus = num_cus;
+  impl.nitems = ncells;
+  Tier t = tier_shape(tier);
+  t.shmem = f.lds_bytes;
+  t.begin = 0;
+  t.count = ncells;
+  const WorkItem* d_items = reinterpret_cast<const WorkItem*>(base + off_items);
+  uint64_t* d_stats = reinterpret_cast<uint64_t*>(base + off_stats);
+  std::vector<uint64_t> out(4 + 4 * ncells);
+  hipError_t e = hipMemcpyAsync(base, stage.data(), stage.size(), hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess) e = hipEventRecord(ev0, stream);
+  if (e == hipSuccess) e = launch_tier(stream, &impl, t, d_items, d_stats, impl.d_order, ncells);
+  if (e == hipSuccess) e = hipEventRecord(ev1, stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out.data(), base, 32, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out.data() + 4, d_stats, 32 * ncells, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) {
+    g_fast_err = std::string("allocation-free run failed: ") + hipGetErrorString(e);
+    return -1;
+  }
+  *missed = out[1];
+  dropped[0] = out[2];
+  dropped[1] = out[3];
+  sums[0] = sums[1] = sums[2] = 0;
+  for (uint64_t i = 0; i < ncells; ++i)
+    for (int k = 0; k < 3; ++k) sums[k] += out[4 + 4 * i + k];
+  return 0;
 }
 
 }  // namespace mgpu

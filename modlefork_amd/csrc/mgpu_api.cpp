@@ -28,6 +28,7 @@ struct mgpu_ctx {
   std::string err;
   int num_cus = 0;
   int phase_timers = 0;
+  std::string noalloc_err;  // why mgpu_simulate_chrom_noalloc cannot run on this device, if so
 };
 
 namespace {
@@ -137,6 +138,11 @@ int mgpu_init(int device, mgpu_ctx** out) {
   hipDeviceProp_t prop;
   HIPCHK(ctx, hipGetDeviceProperties(&prop, device));
   ctx->num_cus = prop.multiProcessorCount;
+  // this device's scratch memory for mgpu_simulate_chrom_noalloc (resolved once per device); a
+  // failure is kept and reported by that entry point, the others do not need it
+  const hipError_t ne = fast_noalloc_init(device);
+  if (ne != hipSuccess)
+    ctx->noalloc_err = std::string("scratch memory set-up failed: ") + hipGetErrorString(ne);
   *out = ctx;
   return MGPU_OK;
 }
@@ -1104,6 +1110,84 @@ int mgpu_simulate_chrom(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom* c
   }
   mgpu_job_destroy(job);
   return rc;
+}
+
+int mgpu_simulate_chrom_noalloc(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom* chrom,
+                                const mgpu_barriers* barriers, uint64_t nlefs,
+                                uint64_t cell_begin, uint64_t ncells,
+                                const uint64_t* target_contacts, uint64_t target_epochs,
+                                uint32_t* d_band, uint64_t* out_missed, mgpu_stats* stats) {
+  const auto t0 = std::chrono::steady_clock::now();
+  if (!ctx || !d_band || !out_missed || !barriers || !target_contacts)
+    return fail(ctx, MGPU_EINVAL, "null argument");
+  if (!ctx->noalloc_err.empty()) return fail(ctx, MGPU_EDEVICE, "%s", ctx->noalloc_err.c_str());
+  if (int rc = check_params(p)) return fail(ctx, rc, "%s", g_err.c_str());
+  if (int rc = check_chrom(chrom)) return fail(ctx, rc, "%s", g_err.c_str());
+  if (p->track_1d_lef_position || p->log_model_internal_state)
+    return fail(ctx, MGPU_EINVAL,
+                "the allocation-free entry point keeps no 1D occupancy or state log");
+  if (chrom->end > kFastMaxEnd)
+    return fail(ctx, MGPU_ERANGE, "chromosome end %llu: fast mode takes ends up to %llu bp",
+                (unsigned long long)chrom->end, (unsigned long long)kFastMaxEnd);
+  if (nlefs == 0 || nlefs > 65535) return fail(ctx, MGPU_EINVAL, "invalid LEF count");
+  if (ncells == 0 || ncells > kNoAllocMaxCells)
+    return fail(ctx, MGPU_ERANGE, "%llu cells: the allocation-free entry point takes 1 to %llu",
+                (unsigned long long)ncells, (unsigned long long)kNoAllocMaxCells);
+  const uint64_t nb = barriers->n;
+  if (nb > kNoAllocMaxBarriers)
+    return fail(ctx, MGPU_ERANGE, "%llu barriers: the allocation-free entry point takes up to %llu",
+                (unsigned long long)nb, (unsigned long long)kNoAllocMaxBarriers);
+  if (nb && (!barriers->pos || !barriers->dir || !barriers->stp_active || !barriers->stp_inactive))
+    return fail(ctx, MGPU_EINVAL, "null barrier arrays");
+  for (uint64_t i = 0; i < nb; ++i) {
+    if ((i && barriers->pos[i] < barriers->pos[i - 1]) || barriers->pos[i] < chrom->start ||
+        barriers->pos[i] >= chrom->end || (barriers->dir[i] != 1 && barriers->dir[i] != 2))
+      return fail(ctx, MGPU_EINVAL, "barrier %llu unsorted, outside the range or with an invalid "
+                  "direction", (unsigned long long)i);
+  }
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const uint64_t sim = chrom->end - chrom->start;
+  const uint64_t ncols = ceil_div(sim, p->bin_size);
+  DetChrom ch{};
+  ch.start = chrom->start;
+  ch.end = chrom->end;
+  ch.nlefs = nlefs;
+  ch.nbar = nb;
+  ch.band = d_band;
+  ch.nrows = std::min(ceil_div(p->diagonal_width, p->bin_size), ncols);
+  ch.cpe = contacts_per_epoch(p, nlefs);
+  ch.target_epochs = target_epochs;
+  ch.bind_rate = (double)nlefs / (double)p->burnin_target_epochs_for_lef_activation;
+  std::vector<WorkItem> items(ncells);
+  for (uint64_t k = 0; k < ncells; ++k) {
+    items[k].chrom = 0;
+    items[k].cell_id = cell_begin + k;
+    items[k].seed = mgpu_cell_seed(chrom, cell_begin + k, p->seed);
+    items[k].target = target_contacts[k];
+  }
+  uint64_t missed = 0, dropped[2] = {0, 0}, sums[3] = {0, 0, 0};
+  if (fast_simulate_noalloc(ctx->stream, ctx->device, (uint32_t)ctx->num_cus, *p, ch,
+                            barriers->pos, barriers->dir, barriers->stp_active,
+                            barriers->stp_inactive, p->prob_release, p->prob_release_burnin,
+                            items, ctx->ev0, ctx->ev1, &missed, dropped, sums) != 0)
+    return fail(ctx, MGPU_EDEVICE, "%s", fast_last_error());
+  *out_missed = missed;
+  if (stats) {
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    *stats = mgpu_stats{};
+    stats->cells = ncells;
+    stats->epochs = sums[0];
+    stats->burnin_epochs = sums[1];
+    stats->contacts = sums[2];
+    stats->missed = missed;
+    stats->dropped = dropped[0];
+    stats->dropped_occupancy = dropped[1];
+    stats->kernel_ms = ms;
+    stats->total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return MGPU_OK;
 }
 
 // ---- debug / known-answer entry points --------------------------------------------------------
