@@ -49,6 +49,31 @@ def _compile(unit):
     return obj
 
 
+HDF5_PREFIX = os.environ.get("MGPU_HDF5_PREFIX", "/opt/conda")
+COOL_OUT = os.path.join(HERE, "libmodle_cool.so")
+
+
+def build_cool(verbose: bool = False) -> str | None:
+    """libmodle_cool.so, the .cool writer (host C++ on the HDF5 C library; include/modle_cool.h).
+    Skipped, with a message, where the HDF5 headers are absent."""
+    inc = os.path.join(HDF5_PREFIX, "include")
+    lib = os.path.join(HDF5_PREFIX, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        print(f"HDF5 not found under {HDF5_PREFIX}: libmodle_cool.so not built", file=sys.stderr)
+        return None
+    tmp = COOL_OUT + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra", f"-I{inc}",
+           os.path.join(CSRC, "cool_writer.cpp"), "-o", tmp, f"-L{lib}", "-lhdf5",
+           f"-Wl,-rpath,{lib}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for cool_writer.cpp:\n{r.stderr}")
+    os.replace(tmp, COOL_OUT)
+    if verbose:
+        print(f"built {COOL_OUT}")
+    return COOL_OUT
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     subprocess.run([sys.executable, os.path.join(CSRC, "gen_rng_tables.py"),
@@ -63,6 +88,7 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, OUT)
     if verbose:
         print(f"built {OUT}")
+    build_cool(verbose)
     return OUT
 
 

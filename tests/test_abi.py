@@ -34,6 +34,25 @@ def test_library_exports_every_declared_symbol():
     assert sorted(n for n, _, _ in L.SIGNATURES) == declared
 
 
+def test_cool_library_exports_every_declared_symbol():
+    """libmodle_cool.so (the .cool writer, where HDF5 is available) exports every entry point of
+    include/modle_cool.h."""
+    import ctypes
+
+    from modlefork_amd import cooler as CL
+
+    if not os.path.exists(CL.LIB_PATH):
+        pytest.skip("HDF5 is not available: the .cool writer is not built")
+    text = open(os.path.join(ROOT, "include", "modle_cool.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    declared = sorted(set(re.findall(r"\b(mcool_\w+)\s*\(", text)))
+    assert declared == ["mcool_close", "mcool_last_error", "mcool_open", "mcool_write_chrom",
+                        "mcool_write_metadata"]
+    lib = ctypes.CDLL(CL.LIB_PATH)
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
 def test_version_string():
     assert b"modle" in L.lib().mgpu_version()
 

@@ -170,3 +170,31 @@ def test_launch_stats_add_up(ctx, mode):
     assert sum(x.barriers for x in ls) == sum(len(b) * len(p.targets) for b, p in zip(bars, plans))
     assert all(x.ms > 0 for x in ls) and sum(x.ms for x in ls) <= st.kernel_ms * 1.01 + 0.1
     job.close()
+
+
+def test_device_bands_written_as_cool(ctx, tmp_path):
+    """The simulated bands of a two-chromosome genome (deterministic mode, the bands bit-exact
+    with the oracle's) written as .cool through libmodle_cool.so and read back through the HDF5
+    C library: pixels rebuild both bands exactly, with the chromosome that has no barriers (not
+    simulated, scheduler_simulate.cpp:115-120) present as bins and index only."""
+    import os
+
+    from modlefork_amd import cooler as CL
+    from test_cool_writer import H5, H5READ, _rebuild
+
+    if not os.path.exists(CL.LIB_PATH) or not os.path.exists(H5READ):
+        pytest.skip("HDF5 is not available: the .cool writer is not built")
+    wl = tiny()
+    wl.chroms = [("chrT", 2_000_000), ("chrU", 1_000_000)]  # chrU has no barriers
+    wl.options["num_cells"] = 4
+    out, st = S.Simulation(wl.options, ctx).run_simulate(wl, L.MODE_DETERMINISTIC)
+    assert sorted(out) == ["chrT"]
+    path = str(tmp_path / "sim.cool")
+    CL.write_cool(path, wl.chroms, out, 5000, metadata={"num_cells": 4})
+    h = H5(path)
+    assert h.strings("chroms/name") == ["chrT", "chrU"]
+    band, missed, plan = out["chrT"]
+    p1, p2, cnt = h.i64("pixels/bin1_id"), h.i64("pixels/bin2_id"), h.i32("pixels/count")
+    assert np.array_equal(_rebuild(p1, p2, cnt, 0, plan.nrows, plan.ncols), band)
+    assert h.attr_i64("sum") == int(band.sum()) and h.attr_i64("nbins") == 400 + 200
+    say(f".cool: {h.attr_i64('nnz')} pixels, {h.attr_i64('sum')} contacts")
