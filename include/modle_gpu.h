@@ -239,8 +239,11 @@ int mgpu_job_band_device_ptr(mgpu_job* job, uint64_t c, void** dptr, uint64_t* n
 void mgpu_job_destroy(mgpu_job* job);
 
 /* Per-launch figures of the last run that was not replayed from a graph (no counterpart in the
- * reference: they time the device launches). Fast mode launches once per working-set size tier;
- * deterministic mode once. *n receives the number of launches; up to max are written. */
+ * reference: they time the device launches). Fast mode launches once per working-set size tier,
+ * each tier on a stream of its own, so the launches of a run overlap (a tier's workgroups start
+ * on the CUs the tier before it leaves); deterministic mode launches once. ms runs from the
+ * moment a launch's stream reaches it to its end. *n receives the number of launches; up to max
+ * are written. */
 typedef struct mgpu_launch_stats {
   uint32_t threads, grid; /* workgroup size and workgroups */
   uint64_t cells;

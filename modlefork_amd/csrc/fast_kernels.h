@@ -25,9 +25,10 @@ struct FastJob {
 int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& chroms,
                     const mgpu_barriers* host_bars, const std::vector<double>& release,
                     const std::vector<WorkItem>& items, FastJob* job);
-// timers != 0: accumulate per-phase wall-clock ticks (read with fast_phase_times). With nev
-// greater than the number of launches, ev[k] is recorded before launch k and ev[launches] after
-// the last one.
+// timers != 0: accumulate per-phase wall-clock ticks (read with fast_phase_times). With nev at
+// least twice the number of launches, ev[2k] and ev[2k + 1] are recorded before and after launch
+// k. A run of several size tiers launches each on a stream of its own, forked from and joined
+// back into `stream` (so the launches of one run may overlap).
 hipError_t fast_job_run(hipStream_t stream, FastJob* job, const WorkItem* d_items,
                         uint64_t* d_cell_stats, int timers, hipEvent_t* ev = nullptr,
                         uint32_t nev = 0);

@@ -43,22 +43,46 @@ struct OpMaxU {
   __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a > b ? a : b; }
 };
 
+// A 32-bit value of lane (lane - shift) through DPP (lanes with no source keep `ident`): the
+// data-parallel-primitive moves run on the VALU, not on the LDS pipe as ds_bpermute does.
+template <int kCtrl, int kRowMask, class T>
+__device__ __forceinline__ T dpp_move(T v, T ident) {
+  static_assert(sizeof(T) == 4, "32-bit values");
+  return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, ident),
+                                                           __builtin_bit_cast(int, v), kCtrl,
+                                                           kRowMask, 0xf, false));
+}
+
+// Inclusive scan across the 64 lanes of a wave: shifts by 1, 2, 4 and 8 within each row of 16
+// lanes (row_shr), then lane 15 of a row into the next row (row_bcast:15, rows 1 and 3) and lane
+// 31 into the upper half (row_bcast:31, rows 2 and 3).
+template <class T, class Op>
+__device__ __forceinline__ T wave_inclusive(T x, Op op, T ident) {
+  x = op(x, dpp_move<0x111, 0xf>(x, ident));
+  x = op(x, dpp_move<0x112, 0xf>(x, ident));
+  x = op(x, dpp_move<0x114, 0xf>(x, ident));
+  x = op(x, dpp_move<0x118, 0xf>(x, ident));
+  x = op(x, dpp_move<0x142, 0xa>(x, ident));
+  x = op(x, dpp_move<0x143, 0xc>(x, ident));
+  return x;
+}
+
+// The lane below's value (wave_shr:1), ident in lane 0.
+template <class T>
+__device__ __forceinline__ T wave_shift1(T x, T ident) {
+  return dpp_move<0x138, 0xf>(x, ident);
+}
+
 // Exclusive scan of one value per lane across the block; sh holds NT / 64 entries.
 template <int NT, class T, class Op>
 __device__ T block_exclusive(T v, Op op, T ident, T* sh) {
   const int lane = tidx() & 63, w = tidx() >> 6;
-  T incl = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const T y = __shfl_up(incl, d, 64);
-    if (lane >= d) incl = op(incl, y);
-  }
+  const T incl = wave_inclusive(v, op, ident);
   if (lane == 63) sh[w] = incl;
   __syncthreads();
   T pre = ident;
   for (int k = 0; k < w; ++k) pre = op(pre, sh[k]);
-  T ex = __shfl_up(incl, 1, 64);
-  if (lane == 0) ex = ident;
+  const T ex = wave_shift1(incl, ident);
   __syncthreads();
   return op(pre, ex);
 }
@@ -68,15 +92,7 @@ __device__ T block_exclusive(T v, Op op, T ident, T* sh) {
 template <int NT, class T, class OpA, class OpB>
 __device__ void block_exclusive2(T& a, T& b, OpA opa, OpB opb, T ia, T ib, T* sh) {
   const int lane = tidx() & 63, w = tidx() >> 6;
-  T xa = a, xb = b;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const T ya = __shfl_up(xa, d, 64), yb = __shfl_up(xb, d, 64);
-    if (lane >= d) {
-      xa = opa(xa, ya);
-      xb = opb(xb, yb);
-    }
-  }
+  const T xa = wave_inclusive(a, opa, ia), xb = wave_inclusive(b, opb, ib);
   if (lane == 63) {
     sh[w] = xa;
     sh[NT / 64 + w] = xb;
@@ -87,11 +103,7 @@ __device__ void block_exclusive2(T& a, T& b, OpA opa, OpB opb, T ia, T ib, T* sh
     pa = opa(pa, sh[k]);
     pb = opb(pb, sh[NT / 64 + k]);
   }
-  T ea = __shfl_up(xa, 1, 64), eb = __shfl_up(xb, 1, 64);
-  if (lane == 0) {
-    ea = ia;
-    eb = ib;
-  }
+  const T ea = wave_shift1(xa, ia), eb = wave_shift1(xb, ib);
   __syncthreads();
   a = opa(pa, ea);
   b = opb(pb, eb);
@@ -1578,7 +1590,11 @@ struct FastImpl {
   std::vector<FastChrom> chroms;
   FastChrom* d_chroms = nullptr;
   uint32_t* d_order = nullptr;
-  unsigned long long* d_next = nullptr;
+  unsigned long long* d_next = nullptr;  // work-queue counters, one per size tier
+  // the size tiers' launches run on streams of their own, forked from and joined back into the
+  // caller's stream, so a tier's cells start on CUs that the tier before has finished with
+  hipStream_t side[5] = {};
+  hipEvent_t fork = nullptr, join[5] = {};
   uint8_t* d_slab = nullptr;
   uint64_t slab_stride = 0;
   uint32_t num_cus = 1;
@@ -1626,47 +1642,54 @@ bool tiered(const FastImpl* impl) { return impl->threads == 0 && !impl->global; 
 template <int NT, bool kOut>
 hipError_t launch_fast_out(hipStream_t stream, FastImpl* impl, const WorkItem* d_items,
                            uint64_t* d_cell_stats, uint32_t grid, uint64_t shmem, int store,
-                           const uint32_t* order, uint64_t nitems) {
+                           const uint32_t* order, uint64_t nitems, unsigned long long* next,
+                           uint8_t* slab) {
   if (store == kStoreLds)
     hipLaunchKernelGGL((fast_cells_kernel<NT, kStoreLds, kOut>), dim3(grid), dim3(NT), shmem,
-                       stream, impl->P, impl->d_chroms, d_items, order, nitems, impl->d_next,
+                       stream, impl->P, impl->d_chroms, d_items, order, nitems, next,
                        nullptr, (uint64_t)0, d_cell_stats);
   else if (store == kStoreHybrid)
     hipLaunchKernelGGL((fast_cells_kernel<NT, kStoreHybrid, kOut>), dim3(grid), dim3(NT), shmem,
-                       stream, impl->P, impl->d_chroms, d_items, order, nitems, impl->d_next,
-                       impl->d_slab, impl->slab_stride, d_cell_stats);
+                       stream, impl->P, impl->d_chroms, d_items, order, nitems, next,
+                       slab, impl->slab_stride, d_cell_stats);
   else
     hipLaunchKernelGGL((fast_cells_kernel<NT, kStoreSlab, kOut>), dim3(grid), dim3(NT), 0, stream,
-                       impl->P, impl->d_chroms, d_items, order, nitems, impl->d_next,
-                       impl->d_slab, impl->slab_stride, d_cell_stats);
+                       impl->P, impl->d_chroms, d_items, order, nitems, next,
+                       slab, impl->slab_stride, d_cell_stats);
   return hipGetLastError();
 }
 
 template <int NT>
 hipError_t launch_fast(hipStream_t stream, FastImpl* impl, const WorkItem* d_items,
                        uint64_t* d_cell_stats, uint32_t grid, uint64_t shmem, int store,
-                       const uint32_t* order, uint64_t nitems) {
+                       const uint32_t* order, uint64_t nitems, unsigned long long* next,
+                       uint8_t* slab) {
   if (impl->occupancy || impl->P.log.rec)
     return launch_fast_out<NT, true>(stream, impl, d_items, d_cell_stats, grid, shmem, store,
-                                     order, nitems);
+                                     order, nitems, next, slab);
   return launch_fast_out<NT, false>(stream, impl, d_items, d_cell_stats, grid, shmem, store,
-                                    order, nitems);
+                                    order, nitems, next, slab);
 }
 
-// Launches the persistent kernel with tier t's shape over order[0, nitems) (the work counter is
-// cleared first, on the same stream, so launches in a row reuse it).
+// Launches the persistent kernel with tier t's shape over order[0, nitems) on work counter
+// `slot` (cleared first, on the same stream, so launches in a row reuse it; launches that run at
+// once use different slots) and, for working sets in device memory, the slab's workgroup slots
+// from slab_slot on (launches that run at once use disjoint slots).
 hipError_t launch_tier(hipStream_t stream, FastImpl* impl, const Tier& t, const WorkItem* d_items,
-                       uint64_t* d_cell_stats, const uint32_t* order, uint64_t nitems) {
+                       uint64_t* d_cell_stats, const uint32_t* order, uint64_t nitems,
+                       uint32_t slot = 0, uint64_t slab_slot = 0) {
   if (nitems == 0) return hipSuccess;
-  hipError_t e = launch_zero_words(stream, impl->d_next, 8);
+  unsigned long long* next = impl->d_next + slot;
+  uint8_t* slab = impl->d_slab ? impl->d_slab + slab_slot * impl->slab_stride : nullptr;
+  hipError_t e = launch_zero_words(stream, next, 8);
   if (e != hipSuccess) return e;
   const uint32_t grid = launch_grid(impl, t.threads, t.per_cu, nitems);
   switch (t.threads) {
-    case 64: return launch_fast<64>(stream, impl, d_items, d_cell_stats, grid, t.shmem, t.store, order, nitems);
-    case 128: return launch_fast<128>(stream, impl, d_items, d_cell_stats, grid, t.shmem, t.store, order, nitems);
-    case 256: return launch_fast<256>(stream, impl, d_items, d_cell_stats, grid, t.shmem, t.store, order, nitems);
-    case 512: return launch_fast<512>(stream, impl, d_items, d_cell_stats, grid, t.shmem, t.store, order, nitems);
-    case 1024: return launch_fast<1024>(stream, impl, d_items, d_cell_stats, grid, t.shmem, t.store, order, nitems);
+    case 64: return launch_fast<64>(stream, impl, d_items, d_cell_stats, grid, t.shmem, t.store, order, nitems, next, slab);
+    case 128: return launch_fast<128>(stream, impl, d_items, d_cell_stats, grid, t.shmem, t.store, order, nitems, next, slab);
+    case 256: return launch_fast<256>(stream, impl, d_items, d_cell_stats, grid, t.shmem, t.store, order, nitems, next, slab);
+    case 512: return launch_fast<512>(stream, impl, d_items, d_cell_stats, grid, t.shmem, t.store, order, nitems, next, slab);
+    case 1024: return launch_fast<1024>(stream, impl, d_items, d_cell_stats, grid, t.shmem, t.store, order, nitems, next, slab);
     default: return hipErrorInvalidValue;
   }
 }
@@ -1942,15 +1965,49 @@ hipError_t fast_job_run(hipStream_t stream, FastJob* job, const WorkItem* d_item
   impl->P.timers = timers ? 1u : 0u;
   const std::vector<Tier> runs =
       tiered(impl) ? impl->tiers : std::vector<Tier>{single_launch(impl)};
-  const bool mark = ev != nullptr && nev > runs.size();
-  for (size_t k = 0; k < runs.size(); ++k) {
-    const Tier& t = runs[k];
-    hipError_t e = mark ? hipEventRecord(ev[k], stream) : hipSuccess;
+  const bool mark = ev != nullptr && nev >= 2 * runs.size();
+  if (runs.size() == 1) {
+    const Tier& t = runs[0];
+    hipError_t e = mark ? hipEventRecord(ev[0], stream) : hipSuccess;
     if (e == hipSuccess)
       e = launch_tier(stream, impl, t, d_items, d_cell_stats, impl->d_order + t.begin, t.count);
+    if (e == hipSuccess && mark) e = hipEventRecord(ev[1], stream);
+    return e;
+  }
+  // several tiers: each on its own stream, in the tier order (the hardware dispatches their
+  // workgroups in launch order, so the largest cells still start first), each stream waiting for
+  // the work before the run and the caller's stream for all of them. Tiers whose working sets
+  // use the slab take disjoint workgroup slots of it.
+  std::vector<uint64_t> slab_slot(runs.size(), 0);
+  uint64_t slots = 0;
+  for (size_t k = 0; k < runs.size(); ++k) {
+    if (runs[k].store == kStoreLds) continue;
+    slab_slot[k] = slots;
+    slots += launch_grid(impl, runs[k].threads, runs[k].per_cu, runs[k].count);
+  }
+  if (slots > impl->max_grid) return hipErrorInvalidValue;  // sized for every tier at once
+  hipError_t e = hipSuccess;
+  if (!impl->fork) {
+    e = hipEventCreateWithFlags(&impl->fork, hipEventDisableTiming);
+    for (size_t k = 0; k < 5 && e == hipSuccess; ++k) {
+      e = hipStreamCreateWithFlags(&impl->side[k], hipStreamNonBlocking);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&impl->join[k], hipEventDisableTiming);
+    }
     if (e != hipSuccess) return e;
   }
-  return mark ? hipEventRecord(ev[runs.size()], stream) : hipSuccess;
+  e = hipEventRecord(impl->fork, stream);
+  for (size_t k = 0; k < runs.size() && e == hipSuccess; ++k) {
+    const Tier& t = runs[k];
+    e = hipStreamWaitEvent(impl->side[k], impl->fork, 0);
+    if (e == hipSuccess && mark) e = hipEventRecord(ev[2 * k], impl->side[k]);
+    if (e == hipSuccess)
+      e = launch_tier(impl->side[k], impl, t, d_items, d_cell_stats, impl->d_order + t.begin,
+                      t.count, (uint32_t)k, slab_slot[k]);
+    if (e == hipSuccess && mark) e = hipEventRecord(ev[2 * k + 1], impl->side[k]);
+    if (e == hipSuccess) e = hipEventRecord(impl->join[k], impl->side[k]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(stream, impl->join[k], 0);
+  }
+  return e;
 }
 
 namespace {
@@ -2143,6 +2200,14 @@ hipError_t fast_phase_times(uint64_t* out, int n) {
 void fast_job_destroy(FastJob* job) {
   auto* impl = static_cast<FastImpl*>(job->impl);
   if (!impl) return;
+  for (size_t k = 0; k < 5; ++k) {
+    if (impl->side[k]) {
+      (void)hipStreamSynchronize(impl->side[k]);
+      (void)hipStreamDestroy(impl->side[k]);
+    }
+    if (impl->join[k]) (void)hipEventDestroy(impl->join[k]);
+  }
+  if (impl->fork) (void)hipEventDestroy(impl->fork);
   if (impl->d_mem) (void)hipFree(impl->d_mem);
   if (impl->d_slab_own) (void)hipFree(impl->d_slab_own);
   delete impl;

@@ -24,7 +24,7 @@ struct mgpu_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipEvent_t lev[6] = {};  // around each launch of a run (fast mode launches by size tier)
+  hipEvent_t lev[10] = {};  // start and end of each launch of a run (fast mode: one per tier)
   std::string err;
   int num_cus = 0;
   int phase_timers = 0;
@@ -841,14 +841,14 @@ int mgpu_job_run(mgpu_job* job, mgpu_stats* stats) {
                   : 1;
     HIPCHK(ctx, enqueue_clears(job, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    HIPCHK(ctx, enqueue_simulation(job, ctx->stream, ctx->lev, 6));
+    HIPCHK(ctx, enqueue_simulation(job, ctx->stream, ctx->lev, 10));
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   const auto t1 = std::chrono::steady_clock::now();
   for (uint32_t k = 0; k < nlaunch; ++k) {
     float ms = 0.f;
-    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->lev[k], ctx->lev[k + 1]));
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->lev[2 * k], ctx->lev[2 * k + 1]));
     job->launch_ms.push_back(ms);
   }
   if (stats) {

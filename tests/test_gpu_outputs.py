@@ -168,7 +168,8 @@ def test_launch_stats_add_up(ctx, mode):
     assert sum(x.contacts for x in ls) == st.contacts
     assert sum(x.epochs for x in ls) == st.epochs
     assert sum(x.barriers for x in ls) == sum(len(b) * len(p.targets) for b, p in zip(bars, plans))
-    assert all(x.ms > 0 for x in ls) and sum(x.ms for x in ls) <= st.kernel_ms * 1.01 + 0.1
+    # the size tiers' launches run on streams of their own and overlap: each lies within the run
+    assert all(0 < x.ms <= st.kernel_ms * 1.01 + 0.1 for x in ls)
     job.close()
 
 
