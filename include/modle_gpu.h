@@ -175,6 +175,19 @@ double mgpu_occupancy_from_stp(double stp_active, double stp_inactive);
 int mgpu_prepare_barriers(const mgpu_params* p, const mgpu_bed_record* recs, uint64_t n,
                           uint64_t chrom_start, uint64_t chrom_end, uint64_t* pos, uint8_t* dir,
                           double* stp_active, double* stp_inactive, uint64_t* nout);
+/* mgpu_prepare_barriers for a BED file whose name field holds each barrier's not-bound
+ * self-transition probability (Genome::import_barriers with interpret_name_field_as_puu,
+ * genome.cpp:459-470; the simulate CLI's --interpret-extrusion-barrier-name-as-not-bound-stp,
+ * Config::interpret_bed_name_field_as_barrier_not_occupied_stp, simulation_config.hpp:93).
+ * name_puu[i] is record i's name field parsed as a number (the caller parses it, as
+ * utils::parse_numeric_or_throw does); it must lie in [0, 1] (MGPU_EINVAL otherwise, also for
+ * records outside [chrom_start, chrom_end), which the reference checks before dropping them) and
+ * takes the place of p->barrier_not_occupied_stp for that record. name_puu NULL is
+ * mgpu_prepare_barriers. */
+int mgpu_prepare_barriers_name_puu(const mgpu_params* p, const mgpu_bed_record* recs,
+                                   const double* name_puu, uint64_t n, uint64_t chrom_start,
+                                   uint64_t chrom_end, uint64_t* pos, uint8_t* dir,
+                                   double* stp_active, double* stp_inactive, uint64_t* nout);
 /* Per-chromosome plan of run_simulate (scheduler_simulate.cpp:128-153): number of LEFs, band shape
  * and per-cell contact targets (target_contacts has ncells entries) and target epochs. */
 int mgpu_plan_chrom(const mgpu_params* p, const mgpu_chrom* chrom, uint64_t ncells,

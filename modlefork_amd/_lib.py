@@ -230,6 +230,9 @@ SIGNATURES = [
     ("mgpu_occupancy_from_stp", C.c_double, [C.c_double, C.c_double]),
     ("mgpu_prepare_barriers", C.c_int, [C.POINTER(MgpuParams), C.POINTER(MgpuBedRecord), C.c_uint64,
                                         C.c_uint64, C.c_uint64, U64P, U8P, DP, DP, U64P]),
+    ("mgpu_prepare_barriers_name_puu", C.c_int, [C.POINTER(MgpuParams), C.POINTER(MgpuBedRecord),
+                                                 DP, C.c_uint64, C.c_uint64, C.c_uint64, U64P, U8P,
+                                                 DP, DP, U64P]),
     ("mgpu_plan_chrom", C.c_int, [C.POINTER(MgpuParams), C.POINTER(MgpuChrom), C.c_uint64, U64P,
                                   U64P, U64P, U64P, U64P]),
     ("mgpu_cell_seed", C.c_uint64, [C.POINTER(MgpuChrom), C.c_uint64, C.c_uint64]),

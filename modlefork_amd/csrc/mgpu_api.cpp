@@ -322,6 +322,14 @@ int mgpu_config_transform(const mgpu_config* in, uint32_t set_mask, mgpu_params*
 int mgpu_prepare_barriers(const mgpu_params* p, const mgpu_bed_record* recs, uint64_t n,
                           uint64_t chrom_start, uint64_t chrom_end, uint64_t* pos, uint8_t* dir,
                           double* stp_active, double* stp_inactive, uint64_t* nout) {
+  return mgpu_prepare_barriers_name_puu(p, recs, nullptr, n, chrom_start, chrom_end, pos, dir,
+                                        stp_active, stp_inactive, nout);
+}
+
+int mgpu_prepare_barriers_name_puu(const mgpu_params* p, const mgpu_bed_record* recs,
+                                   const double* name_puu, uint64_t n, uint64_t chrom_start,
+                                   uint64_t chrom_end, uint64_t* pos, uint8_t* dir,
+                                   double* stp_active, double* stp_inactive, uint64_t* nout) {
   if (!p || (n && !recs) || !nout) return fail(nullptr, MGPU_EINVAL, "null argument");
   struct Row {
     uint64_t pos;
@@ -337,10 +345,20 @@ int mgpu_prepare_barriers(const mgpu_params* p, const mgpu_bed_record* recs, uin
     if (r.score < 0.0 || r.score > 1.0)
       return fail(nullptr, MGPU_EINVAL, "record %llu: score must be in [0, 1]",
                   (unsigned long long)i);
+    // the name field as the record's not-bound stp (genome.cpp:459-468), range-checked for
+    // every record before the position test drops any
+    const double puu = name_puu ? name_puu[i] : p->barrier_not_occupied_stp;
+    if (name_puu && !(puu >= 0.0 && puu <= 1.0))
+      return fail(nullptr, MGPU_EINVAL, "record %llu: name field must be in [0, 1]",
+                  (unsigned long long)i);
     const uint64_t bp = (r.chrom_start + r.chrom_end + 1) / 2;
     if (bp < chrom_start || bp >= chrom_end) continue;
-    double sa = p->barrier_occupied_stp, si = p->barrier_not_occupied_stp;
-    if (!p->override_barrier_occupancy && r.score != 0.0) {
+    // compute_barrier_stp (genome.cpp:181-192), then the occupancy override of
+    // Simulation::Simulation (simulation.cpp:68-78), which restores the configured pair
+    double sa = p->barrier_occupied_stp, si = puu;
+    if (p->override_barrier_occupancy) {
+      si = p->barrier_not_occupied_stp;
+    } else if (r.score != 0.0) {
       sa = mgpu_stp_active_from_occupancy(si, r.score);
     }
     // motif '+' blocks extrusion in the rev direction, anything else in the fwd direction

@@ -80,7 +80,11 @@ def chrom_view(name: str, size: int, start: int = 0, end: int | None = None) -> 
     return ch
 
 
-def prepare_barriers(params: L.MgpuParams, records, start: int, end: int) -> Barriers:
+def prepare_barriers(params: L.MgpuParams, records, start: int, end: int,
+                     name_puu=None) -> Barriers:
+    """Barrier tables of one chromosome's BED records (mgpu_prepare_barriers). name_puu, when
+    given, holds each record's name field as a number: the record's not-bound stp, as the simulate
+    CLI's --interpret-extrusion-barrier-name-as-not-bound-stp reads it."""
     n = len(records)
     recs = (L.MgpuBedRecord * max(n, 1))()
     for i, r in enumerate(records):
@@ -90,9 +94,15 @@ def prepare_barriers(params: L.MgpuParams, records, start: int, end: int) -> Bar
     sa = np.zeros(max(n, 1), np.float64)
     si = np.zeros(max(n, 1), np.float64)
     nout = C.c_uint64(0)
-    L.check(L.lib().mgpu_prepare_barriers(C.byref(params), recs, n, start, end, _p(pos, C.c_uint64),
-                                          _p(dirs, C.c_uint8), _p(sa, C.c_double),
-                                          _p(si, C.c_double), C.byref(nout)))
+    puu = None
+    if name_puu is not None:
+        if len(name_puu) != n:
+            raise ValueError("name_puu needs one value per record")
+        puu = np.ascontiguousarray(name_puu, np.float64)
+    L.check(L.lib().mgpu_prepare_barriers_name_puu(
+        C.byref(params), recs, None if puu is None else _p(puu, C.c_double), n, start, end,
+        _p(pos, C.c_uint64), _p(dirs, C.c_uint8), _p(sa, C.c_double), _p(si, C.c_double),
+        C.byref(nout)))
     k = nout.value
     return Barriers(pos[:k], dirs[:k], sa[:k], si[:k])
 

@@ -159,25 +159,35 @@ def to_rc(c: dict) -> dict:
     )
 
 
-def barrier_stp(score: float, c: dict):
-    """compute_barrier_stp (genome.cpp:181-192) followed by the occupancy override of
+def barrier_stp(score: float, c: dict, puu=None):
+    """compute_barrier_stp (genome.cpp:181-192) with the default not-bound stp, or `puu` when the
+    BED name field gives it (genome.cpp:459-468), followed by the occupancy override of
     Simulation::Simulation (simulation.cpp:68-78). Returns (stp_active, stp_inactive)."""
     if c.get("override_extrusion_barrier_occupancy"):
         return c["barrier_occupied_stp"], c["barrier_not_occupied_stp"]
+    si = c["barrier_not_occupied_stp"] if puu is None else puu
     if score != 0.0:
-        return stp_active_from_occupancy(c["barrier_not_occupied_stp"], score), c["barrier_not_occupied_stp"]
-    return c["barrier_occupied_stp"], c["barrier_not_occupied_stp"]
+        return stp_active_from_occupancy(si, score), si
+    return c["barrier_occupied_stp"], si
 
 
-def prepare_barriers(records, chrom_start: int, chrom_end: int, c: dict) -> "R.Barriers":
-    """Chromosome::add_extrusion_barrier (genome.cpp:194-210) for each BED record, then the
-    position sort of ExtrusionBarriers (extrusion_barriers.cpp:234-259)."""
+def prepare_barriers(records, chrom_start: int, chrom_end: int, c: dict,
+                     name_puu=None) -> "R.Barriers":
+    """Genome::import_barriers' checks and Chromosome::add_extrusion_barrier (genome.cpp:194-210,
+    446-470) for each BED record, then the position sort of ExtrusionBarriers
+    (extrusion_barriers.cpp:234-259). name_puu, when given, holds each record's name field as a
+    number (interpret_name_field_as_puu)."""
     rows = []
-    for r in records:
+    for i, r in enumerate(records):
+        if r.score < 0 or r.score > 1:
+            raise ValueError(f"record {i}: score must be in [0, 1]")
+        puu = None if name_puu is None else float(name_puu[i])
+        if puu is not None and not 0.0 <= puu <= 1.0:
+            raise ValueError(f"record {i}: name field must be in [0, 1]")
         pos = (r.start + r.end + 1) // 2
         if pos < chrom_start or pos >= chrom_end:
             continue
-        sa, si = barrier_stp(r.score, c)
+        sa, si = barrier_stp(r.score, c, puu)
         rows.append((pos, 1 if r.strand == "+" else 2, sa, si))
     # cpp-sort pdq_sort of the indices by position (extrusion_barriers.cpp:243): equal positions
     # keep pdq_sort's (unstable) order

@@ -159,6 +159,57 @@ def test_prepare_barriers_rejects_bad_records():
         S.prepare_barriers(p, bad, 0, 100)
 
 
+def test_bed_name_field_as_not_bound_stp():
+    """--interpret-extrusion-barrier-name-as-not-bound-stp (genome.cpp:459-468): each record's
+    name field is its not-bound stp. With a score the active stp is the one whose stationary
+    occupancy is that score (the closed form of extrusion_barriers_impl.hpp:106-128, checked
+    here by inverting it); without one it is the configured default. The occupancy override
+    (simulation.cpp:68-78) restores the configured pair. A name outside [0, 1] is an error even
+    for a record outside the simulated range, as the reference checks before dropping it."""
+    rng = np.random.default_rng(5)
+    n = 200
+    recs = [W.BedRecord("c", int(m), int(m) + 1, float(rng.choice([0.0, rng.uniform(0.05, 0.95)])),
+                        "+" if rng.random() < 0.5 else "-")
+            for m in rng.choice(100_000, n, replace=False)]
+    puu = rng.uniform(0.05, 0.95, n)
+    opts = dict(bin_size=1000, diagonal_width=10_000)
+    c = OC.transform(opts)
+    p = S.make_params(opts)
+    b = S.prepare_barriers(p, recs, 0, 100_000, name_puu=puu)
+    o = OC.prepare_barriers(recs, 0, 100_000, c, name_puu=puu)
+    for f in ("pos", "dir", "stp_active", "stp_inactive"):
+        assert np.array_equal(getattr(b, f), getattr(o, f)), f
+    by_pos = {r.start + 1: (r, u) for r, u in zip(recs, puu)}  # midpoint of a 1-bp record
+    for pos, sa, si in zip(b.pos, b.stp_active, b.stp_inactive):
+        r, u = by_pos[int(pos)]
+        assert si == u
+        if r.score != 0.0:
+            # the closed form clamps stp_active to [0, 1]: an occupancy below the one an
+            # always-leaving active state gives, (1 - u) / (2 - u), maps to 0
+            if sa > 0.0:
+                assert abs(L.lib().mgpu_occupancy_from_stp(sa, si) - r.score) < 1e-12
+            else:
+                assert r.score <= (1 - u) / (2 - u) + 1e-12
+        else:
+            assert sa == p.barrier_occupied_stp
+    # without the name field the default not-bound stp applies
+    d = S.prepare_barriers(p, recs, 0, 100_000)
+    assert np.all(d.stp_inactive == p.barrier_not_occupied_stp)
+    # the override restores the configured pair
+    po = S.make_params(dict(opts, extrusion_barrier_occupancy=0.9))
+    assert po.override_barrier_occupancy == 1
+    ov = S.prepare_barriers(po, recs, 0, 100_000, name_puu=puu)
+    assert np.all(ov.stp_active == po.barrier_occupied_stp)
+    assert np.all(ov.stp_inactive == po.barrier_not_occupied_stp)
+    # an out-of-range name fails, also on a record outside the chromosome range
+    bad = [W.BedRecord("c", 10, 11, 0.5, "+"), W.BedRecord("c", 500, 501, 0.5, "+")]
+    with pytest.raises(L.MgpuError):
+        S.prepare_barriers(p, bad, 0, 100, name_puu=[0.5, 1.5])
+    with pytest.raises(ValueError):
+        OC.prepare_barriers(bad, 0, 100, c, name_puu=[0.5, 1.5])
+    assert len(S.prepare_barriers(p, bad, 0, 100, name_puu=[0.5, 0.25]).pos) == 1
+
+
 def test_prepare_barriers_drops_out_of_range_and_uses_midpoint():
     p = S.make_params({})
     recs = [W.BedRecord("c", 90, 100, 0.0, "+"), W.BedRecord("c", 10, 13, 0.0, "-"),
