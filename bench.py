@@ -341,38 +341,66 @@ def main():
     epochs = int(stats.epochs)
     value = cells * args.steps / wall
 
-    # roofline of the dominant launch (the size tier that takes longest), from the last timed
-    # run's per-launch HIP events and the algorithmic bytes of that launch's cells
-    dom = max(range(len(ls)), key=lambda k: ls[k].ms)
-    d = ls[dom]
-    dom_bytes = launch_bytes(d)
-    achieved = dom_bytes / (d.ms / 1000.0) / 1e9
+    # the rate a caller sees when the bands are handed over in host memory (as
+    # mgpu_simulate_chrom does): one more run, then every band copied to pageable host memory
+    t1 = time.perf_counter()
+    job.run()
+    t2 = time.perf_counter()
+    band_bytes = 0
+    for c in range(len(chroms)):
+        band, _ = job.band(c)
+        band_bytes += band.nbytes
+    t3 = time.perf_counter()
+    host_bands = {"cells_per_s": cells / (t3 - t1), "readback_ms": 1000.0 * (t3 - t2),
+                  "band_bytes": band_bytes,
+                  "readback_gb_per_s": band_bytes / (t3 - t2) / 1e9}
+    log(f"with the bands read back to host memory: {host_bands['cells_per_s']:.1f} cells/s "
+        f"({host_bands['readback_ms']:.1f} ms for {band_bytes / 1e9:.2f} GB)")
+
+    # roofline of the simulation kernel. Fast mode launches it once per working-set size tier,
+    # each launch on a stream of its own, so the launches of one run overlap on the chip (a
+    # tier's workgroups take the CUs the tier before it leaves); the kernel's figure is therefore
+    # all launches' algorithmic bytes over the run's device time (the span from the first
+    # launch's start to the last one's end, HIP events on the job's stream), with each launch's
+    # own bytes and duration beside it
     all_bytes = sum(launch_bytes(x) for x in ls)
+    span_ms = float(stats.kernel_ms)
+    achieved = all_bytes / (span_ms / 1000.0) / 1e9
+    per_launch = [{"threads": int(x.threads), "grid": int(x.grid), "cells": int(x.cells),
+                   "contacts": int(x.contacts), "ms": float(x.ms),
+                   "algorithmic_bytes": launch_bytes(x)} for x in ls]
+
+    # measured HBM bytes, LDS and atomic figures of each launch from the committed rocprof
+    # counter runs (taken for one launch shape; other shapes report none)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{args.mode}_{args.workload}.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
             prof = json.load(fh)
-        # counters were taken for one launch shape; other shapes report no traffic figure
-        if prof.get("launch_shape") == shape:
-            traffic = prof.get("hbm_bytes_per_launch", {}).get(str(d.threads))
-
-    # LDS and atomic figures of the dominant launch from the committed rocprof counter runs
-    # (taken for one launch shape; other shapes report none)
+        hb = prof.get("hbm_bytes_per_launch", {})
+        if prof.get("launch_shape") == shape and all(str(x.threads) in hb for x in ls):
+            traffic = float(sum(hb[str(x.threads)] for x in ls))
     lds = atomics = None
     cfile = os.path.join(ROOT, "profiles", f"counters_{args.mode}_{args.workload}.json")
     if os.path.exists(cfile):
         with open(cfile) as fh:
             cprof = json.load(fh)
-        dv = cprof.get("per_launch", {}).get(str(d.threads), {}).get("derived")
-        if cprof.get("launch_shape") == shape and dv:
-            lds = {"util": dv["lds_util"], "bank_conflict_ratio": dv["lds_bank_conflict_ratio"],
-                   "instructions": dv["lds_instructions"], "valu_issue_util": dv["valu_issue_util"],
-                   "wait_fraction": dv["wait_fraction"],
-                   "peak": "LDS busy on every cycle of all 256 CUs (SQ_LDS_IDX_ACTIVE)"}
-            atomics = {"per_s": dv["atomics_per_s"], "count": dv["atomics"],
-                       "peak_per_s": ATOMIC_PEAK_PER_S,
-                       "frac": dv["atomics_per_s"] / ATOMIC_PEAK_PER_S}
+        pl = cprof.get("per_launch", {})
+        if cprof.get("launch_shape") == shape and all(str(x.threads) in pl for x in ls):
+            lds = {"per_launch": {}, "peak": "LDS busy on every cycle of all 256 CUs "
+                                              "(SQ_LDS_IDX_ACTIVE)"}
+            count = 0.0
+            for x in ls:
+                dv = pl[str(x.threads)]["derived"]
+                lds["per_launch"][str(x.threads)] = {
+                    "util": dv["lds_util"], "bank_conflict_ratio": dv["lds_bank_conflict_ratio"],
+                    "instructions": dv["lds_instructions"],
+                    "valu_issue_util": dv["valu_issue_util"],
+                    "wait_fraction": dv["wait_fraction"]}
+                count += dv["atomics"]
+            per_s = count / (span_ms / 1000.0)
+            atomics = {"per_s": per_s, "count": count, "peak_per_s": ATOMIC_PEAK_PER_S,
+                       "frac": per_s / ATOMIC_PEAK_PER_S}
 
     cpu = None
     nthreads = args.cpu_threads or min(len(os.sched_getaffinity(0)), 16)
@@ -401,18 +429,14 @@ def main():
                    "lefs_per_mbp": float(sim.params.lefs_per_mbp), "parallelism": "none",
                    "launches": launches, "autotuned": autotuned},
         "contacts_per_sec": contacts * args.steps / wall,
+        "with_host_readback": host_bands,
         "epochs_per_sec": epochs * args.steps / wall,
         "kernel_ms": float(np.mean(kernel_ms)),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": f"fast_cells_kernel<{d.threads}>" if mode == L.MODE_FAST
-                               else "det_cells_kernel",
-                     "launch": {"threads": int(d.threads), "grid": int(d.grid),
-                                "cells": int(d.cells), "contacts": int(d.contacts),
-                                "ms": float(d.ms), "algorithmic_bytes": dom_bytes},
-                     "all_launches": {"algorithmic_bytes": all_bytes,
-                                      "ms": float(stats.kernel_ms),
-                                      "achieved": all_bytes / (stats.kernel_ms / 1000.0) / 1e9}},
+                     "kernel": "fast_cells_kernel" if mode == L.MODE_FAST else "det_cells_kernel",
+                     "ms": span_ms, "algorithmic_bytes": all_bytes,
+                     "launches": per_launch},
         "lds": lds,
         "atomics": atomics,
         "cpu_baseline": cpu,

@@ -918,7 +918,8 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
   }
   for (uint32_t b = tidx(); b < NB; b += NT) {
     const U4 d = R.draw(b, 0xFFFFFFFFu, kSBarInit);
-    c.bst[b] = d.x < ch.bar_t_occ[b] ? 1 : 0;
+    const uint32_t t = ch.bar_t_occ[b];
+    c.bst[b] = (d.x < t || t == kThresholdOne) ? 1 : 0;
   }
   if (tidx() == 0) {
     S.nact = P.skip_burnin ? N : 0;
@@ -1210,8 +1211,8 @@ __device__ __attribute__((always_inline)) void fast_cell(const FastParams& P, co
       const uint32_t t_in = ch.bar_t_inactive[b], t_act = ch.bar_t_active[b];
       const U4 d = R.draw(b, epoch, kSBarrier);
       const uint8_t st = c.bst[b];
-      if (st == 0 && d.x >= t_in) c.bst[b] = 1;
-      else if (st == 1 && d.x >= t_act) c.bst[b] = 0;
+      if (st == 0 && d.x >= t_in && t_in != kThresholdOne) c.bst[b] = 1;
+      else if (st == 1 && d.x >= t_act && t_act != kThresholdOne) c.bst[b] = 0;
     }
     // ---- units at the chromosome ends (simulation_detect_collisions.cpp:30-125) ----
     if (tidx() == 0) {
@@ -1568,6 +1569,10 @@ __global__ void philox_kat_kernel(const uint32_t* in, uint32_t* out, int n) {
 
 thread_local std::string g_fast_err;
 
+// 32-bit Bernoulli threshold of probability p: an event of probability p happens when a 32-bit
+// draw is below it. kThresholdOne stands for p = 1 (every p >= 1 - 2^-32), which no draw may
+// fail, so a barrier whose self-transition probability is 1 (a barrier a perturbation deletes:
+// stp_active 0, stp_inactive 1) never changes state, as in the reference.
 uint32_t t32(double p) {
   if (p <= 0.0) return 0;
   const double v = p * 4294967296.0;
@@ -1672,16 +1677,17 @@ hipError_t launch_fast(hipStream_t stream, FastImpl* impl, const WorkItem* d_ite
 }
 
 // Launches the persistent kernel with tier t's shape over order[0, nitems) on work counter
-// `slot` (cleared first, on the same stream, so launches in a row reuse it; launches that run at
-// once use different slots) and, for working sets in device memory, the slab's workgroup slots
-// from slab_slot on (launches that run at once use disjoint slots).
+// `slot` (cleared first, on the same stream, so launches in a row reuse it, unless `cleared` says
+// the caller has cleared it already; launches that run at once use different slots) and, for
+// working sets in device memory, the slab's workgroup slots from slab_slot on (launches that run
+// at once use disjoint slots).
 hipError_t launch_tier(hipStream_t stream, FastImpl* impl, const Tier& t, const WorkItem* d_items,
                        uint64_t* d_cell_stats, const uint32_t* order, uint64_t nitems,
-                       uint32_t slot = 0, uint64_t slab_slot = 0) {
+                       uint32_t slot = 0, uint64_t slab_slot = 0, bool cleared = false) {
   if (nitems == 0) return hipSuccess;
   unsigned long long* next = impl->d_next + slot;
   uint8_t* slab = impl->d_slab ? impl->d_slab + slab_slot * impl->slab_stride : nullptr;
-  hipError_t e = launch_zero_words(stream, next, 8);
+  hipError_t e = cleared ? hipSuccess : launch_zero_words(stream, next, 8);
   if (e != hipSuccess) return e;
   const uint32_t grid = launch_grid(impl, t.threads, t.per_cu, nitems);
   switch (t.threads) {
@@ -1942,6 +1948,22 @@ int fast_job_create(const mgpu_params& p, const std::vector<DetChrom>& dchroms,
     t.shmem = k == 0 ? 0 : bytes;
     if (t.count) impl->tiers.push_back(t);
   }
+  // The tiers' launches run on streams of their own, but the runtime spreads streams over only
+  // a few hardware queues (four by default), so a third concurrent launch shares a queue and
+  // waits for the launch before it there. With all three LDS tiers present, the four-per-CU
+  // cells therefore join the two-per-CU launch (they fit its LDS allocation and run at its
+  // width, 512 lanes, which times the same as 256 for them): two launches, each on its own queue.
+  const auto at = [&](int store, uint32_t per_cu) {
+    for (size_t i = 0; i < impl->tiers.size(); ++i)
+      if (impl->tiers[i].store == store && impl->tiers[i].per_cu == per_cu) return (int)i;
+    return -1;
+  };
+  const int t1 = at(kStoreLds, 1), t2 = at(kStoreLds, 2), t4 = at(kStoreLds, 4);
+  if (t1 >= 0 && t2 >= 0 && t4 == t2 + 1 &&
+      impl->tiers[t2].begin + impl->tiers[t2].count == impl->tiers[t4].begin) {
+    impl->tiers[t2].count += impl->tiers[t4].count;
+    impl->tiers.erase(impl->tiers.begin() + t4);
+  }
   if (e == hipSuccess)
     e = hipMemcpy(impl->d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice);
   impl->order_host = order;
@@ -1995,14 +2017,17 @@ hipError_t fast_job_run(hipStream_t stream, FastJob* job, const WorkItem* d_item
     }
     if (e != hipSuccess) return e;
   }
-  e = hipEventRecord(impl->fork, stream);
+  // every tier's work counter is cleared before the fork, so each side stream holds its launch
+  // alone and the events around it time the launch itself
+  e = launch_zero_words(stream, impl->d_next, 8 * runs.size());
+  if (e == hipSuccess) e = hipEventRecord(impl->fork, stream);
   for (size_t k = 0; k < runs.size() && e == hipSuccess; ++k) {
     const Tier& t = runs[k];
     e = hipStreamWaitEvent(impl->side[k], impl->fork, 0);
     if (e == hipSuccess && mark) e = hipEventRecord(ev[2 * k], impl->side[k]);
     if (e == hipSuccess)
       e = launch_tier(impl->side[k], impl, t, d_items, d_cell_stats, impl->d_order + t.begin,
-                      t.count, (uint32_t)k, slab_slot[k]);
+                      t.count, (uint32_t)k, slab_slot[k], true);
     if (e == hipSuccess && mark) e = hipEventRecord(ev[2 * k + 1], impl->side[k]);
     if (e == hipSuccess) e = hipEventRecord(impl->join[k], impl->side[k]);
     if (e == hipSuccess) e = hipStreamWaitEvent(stream, impl->join[k], 0);

@@ -42,6 +42,9 @@ constexpr uint32_t kFastMaxAttempts = 1u << 16;
 // before the chromosome start): chromosome ends stay at or below kFastMaxEnd (2^30) and raw
 // moves are capped at 2^30, so every target, plus or minus a rank below 2^16, fits in int32.
 constexpr uint32_t kFastMaxMove = 1u << 30;
+// Barrier probabilities are 32-bit thresholds (an event of probability p happens when a draw is
+// below p * 2^32); this value stands for p = 1, which no draw fails.
+constexpr uint32_t kThresholdOne = 0xFFFFFFFFu;
 static_assert(kFastMaxEnd == (1ull << 30), "fast-mode move arithmetic assumes ends <= 2^30");
 
 // collision word: kind in bits 29..31, index in bits 0..28

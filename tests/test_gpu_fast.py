@@ -270,17 +270,17 @@ def tier_workload(ncells=4):
 
 
 def test_size_tiers_give_the_single_launch_band(ctx):
-    """The default run makes one launch per size tier (device memory, then 1, 2 and 4 cells per
-    CU, then the hybrid of LDS and device memory); one launch of every cell from device memory
-    gives the same bands and statistics, and so do replays of the tiered run captured into a
-    graph."""
+    """The default run makes one launch per size tier (device memory, then 1 and 2 cells per CU,
+    the cells that fit 4 per CU joining the 2-per-CU launch since a 1-per-CU launch exists, then
+    the hybrid of LDS and device memory); one launch of every cell from device memory gives the
+    same bands and statistics, and so do replays of the tiered run captured into a graph."""
     wl = tier_workload()
     sim = S.Simulation(wl.options, ctx)
     names, chroms, bars, plans = sim.prepare(wl)
     job = S.Job(ctx, sim.params, chroms, bars, plans, L.MODE_FAST)
     launches = job.launches()
     say(f"tier launches (threads, grid): {launches}")
-    assert [t for t, _ in launches] == [512, 1024, 512, 256, 1024]
+    assert [t for t, _ in launches] == [512, 1024, 512, 1024]
     st = job.run()
     tiered = [job.band(i) for i in range(len(names))]
     job.set_tuning(512)
