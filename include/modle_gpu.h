@@ -169,9 +169,32 @@ int mgpu_config_transform(const mgpu_config* c, uint32_t set_mask, mgpu_params* 
  * extrusion_barriers_impl.hpp:106-128. */
 double mgpu_stp_active_from_occupancy(double stp_inactive, double occupancy);
 double mgpu_occupancy_from_stp(double stp_active, double stp_inactive);
-/* Chromosome::add_extrusion_barrier for each record (genome.cpp:181-210), the occupancy override
- * of Simulation::Simulation (simulation.cpp:68-78) and the position sort of ExtrusionBarriers
- * (extrusion_barriers.cpp:234-259). Outputs hold n entries; *nout receives the number kept. */
+/* A chromosome's barriers as Genome::import_barriers leaves them (genome.cpp:433-480, with
+ * Chromosome::add_extrusion_barrier, genome.cpp:181-210, and the occupancy override of
+ * Simulation::Simulation, simulation.cpp:68-78): recs are the chromosome's BED records in file
+ * order; those whose midpoint (start + end + 1) / 2 lies in [chrom_start, chrom_end) are kept, in
+ * the order of the chromosome's interval tree (the records ordered by start twice by pdq_sort,
+ * once by the parser's tree and once by the chromosome's, interval_tree_impl.hpp:58-68).
+ * bed_start / bed_end (may be NULL) receive each kept record's interval, which perturbate's
+ * window queries search (map_barriers_to_window, scheduler_common.cpp:113-127). name_puu as in
+ * mgpu_prepare_barriers_name_puu (NULL: the default not-bound stp). Outputs hold n entries;
+ * *nout receives the number kept. */
+int mgpu_import_barriers(const mgpu_params* p, const mgpu_bed_record* recs,
+                         const double* name_puu, uint64_t n, uint64_t chrom_start,
+                         uint64_t chrom_end, uint64_t* bed_start, uint64_t* bed_end,
+                         uint64_t* pos, uint8_t* dir, double* stp_active, double* stp_inactive,
+                         uint64_t* nout);
+/* The order cpp-sort's pdq_sort (1.13, the published pdqsort) gives indices [0, n) sorted by
+ * keys[i] (unstable): the order of the interval trees the reference builds of BED files
+ * (IITree::make_BST, interval_tree_impl.hpp:58-68), e.g. perturbate's deletions and feature
+ * tracks. Host only. */
+int mgpu_pdq_order(const uint64_t* keys, uint64_t n, uint32_t* order);
+/* The position sort of the ExtrusionBarriers constructor (extrusion_barriers.cpp:234-259, a
+ * pdq_sort of the indices by position), in place: the order the simulation reads. */
+int mgpu_sort_barriers(uint64_t n, uint64_t* pos, uint8_t* dir, double* stp_active,
+                       double* stp_inactive);
+/* mgpu_import_barriers then mgpu_sort_barriers: a chromosome's barrier tables from its BED
+ * records. Outputs hold n entries; *nout receives the number kept. */
 int mgpu_prepare_barriers(const mgpu_params* p, const mgpu_bed_record* recs, uint64_t n,
                           uint64_t chrom_start, uint64_t chrom_end, uint64_t* pos, uint8_t* dir,
                           double* stp_active, double* stp_inactive, uint64_t* nout);
@@ -238,6 +261,53 @@ int mgpu_job_create_multi(mgpu_ctx* ctx, const mgpu_params* params, uint64_t nch
                           const uint64_t* nlefs, const uint64_t* ncells,
                           const uint64_t* const* target_contacts, const uint64_t* target_epochs,
                           int mode, mgpu_job** out);
+/* ---- perturbate / replay windows ----
+ * One window task of modle perturbate and modle replay (Simulation::TaskPW,
+ * simulation.hpp:78-95, simulated by Simulation::simulate_window through simulate_one_cell,
+ * scheduler_perturbate.cpp:438-474 and 372-436, scheduler_replay.cpp:103-154): chrom.name and
+ * chrom.size are the whole chromosome's (the cell's seed is Chromosome::hash(seed, cell_id),
+ * simulation.cpp:1065), chrom.start and chrom.end the outer window [window_start, window_end);
+ * barriers are the window's (TaskPW::barriers, map_barriers_to_window,
+ * scheduler_common.cpp:113-127), sorted, with the task's deletion applied (mgpu_apply_deletion);
+ * nlefs, cell_id, target_contacts and target_epochs are the task's num_lefs, cell_id,
+ * num_target_contacts and num_target_epochs (scheduler_perturbate.cpp:286-313). */
+typedef struct mgpu_window_task {
+  mgpu_chrom chrom;
+  mgpu_barriers barriers;
+  uint64_t nlefs;
+  uint64_t cell_id;
+  uint64_t target_contacts;
+  uint64_t target_epochs;
+} mgpu_window_task;
+/* A job of window tasks: task t is the job's chromosome t, with one cell (id tasks[t].cell_id)
+ * and its own band over the window (ContactMatrixDense resized to the window,
+ * scheduler_perturbate.cpp:418-419); mgpu_job_run simulates every task in one persistent
+ * launch, mgpu_job_read_band(job, t, ...) and mgpu_job_block_sums read the results. */
+int mgpu_job_create_windows(mgpu_ctx* ctx, const mgpu_params* p, uint64_t ntasks,
+                            const mgpu_window_task* tasks, int mode, mgpu_job** out);
+/* The barrier configuration of simulate_window (scheduler_perturbate.cpp:463-472): barriers
+ * whose position lies in [deletion_begin, deletion_begin + deletion_size) get stp_active 0 and
+ * stp_inactive 1 (occupancy 0: inactive from the start and for good). *num_active receives the
+ * number of barriers left (the BEDPE's num_active_barriers). Host only. */
+int mgpu_apply_deletion(const uint64_t* pos, uint64_t n, uint64_t deletion_begin,
+                        uint64_t deletion_size, double* stp_active, double* stp_inactive,
+                        uint64_t* num_active);
+/* ContactMatrixDense<u32>::unsafe_get_block (contact_matrix_dense_unsafe_impl.hpp:89-113) on the
+ * device for n queries: out[i] is the sum, in u32, of the block_size x block_size pixels of
+ * chromosome (task) chrom[i]'s band centred on (row[i], col[i]), rows and columns clamped to
+ * the band's bins (edge extension), pixels beyond the band counting 0; block_size 1 reads the
+ * pixel (unsafe_get). block_size must be odd; the reference also expects it below nrows.
+ * Bins are relative to the chromosome's (window's) start. */
+int mgpu_job_block_sums(mgpu_job* job, uint64_t n, const uint64_t* chrom, const uint64_t* row,
+                        const uint64_t* col, uint64_t block_size, uint32_t* out);
+/* The same on a band in host memory (nrows*ncols + 1 u32, the band layout above), such as the
+ * reference matrix a perturbate run reads from a .cool file (read_reference_contacts,
+ * scheduler_perturbate.cpp:81-91), whose blocks simulate_window compares with the simulated
+ * ones (scheduler_perturbate.cpp:503-506). */
+int mgpu_band_block_sums(mgpu_ctx* ctx, const uint32_t* band, uint64_t nrows, uint64_t ncols,
+                         uint64_t n, const uint64_t* row, const uint64_t* col,
+                         uint64_t block_size, uint32_t* out);
+
 /* Zeroes the bands and simulates every cell of the job. */
 int mgpu_job_run(mgpu_job* job, mgpu_stats* stats);
 /* Records one run (the clears of bands and counters, then the persistent simulation launch, in

@@ -148,6 +148,17 @@ class MgpuBedRecord(C.Structure):
     ]
 
 
+class MgpuWindowTask(C.Structure):
+    _fields_ = [
+        ("chrom", MgpuChrom),
+        ("barriers", MgpuBarriers),
+        ("nlefs", C.c_uint64),
+        ("cell_id", C.c_uint64),
+        ("target_contacts", C.c_uint64),
+        ("target_epochs", C.c_uint64),
+    ]
+
+
 class MgpuStats(C.Structure):
     _fields_ = [
         ("cells", C.c_uint64),
@@ -230,6 +241,12 @@ SIGNATURES = [
     ("mgpu_occupancy_from_stp", C.c_double, [C.c_double, C.c_double]),
     ("mgpu_prepare_barriers", C.c_int, [C.POINTER(MgpuParams), C.POINTER(MgpuBedRecord), C.c_uint64,
                                         C.c_uint64, C.c_uint64, U64P, U8P, DP, DP, U64P]),
+    ("mgpu_import_barriers", C.c_int, [C.POINTER(MgpuParams), C.POINTER(MgpuBedRecord), DP,
+                                       C.c_uint64, C.c_uint64, C.c_uint64, U64P, U64P, U64P, U8P,
+                                       DP, DP, U64P]),
+    ("mgpu_sort_barriers", C.c_int, [C.c_uint64, U64P, U8P, DP, DP]),
+    ("mgpu_pdq_order", C.c_int, [U64P, C.c_uint64, U32P]),
+    ("mgpu_apply_deletion", C.c_int, [U64P, C.c_uint64, C.c_uint64, C.c_uint64, DP, DP, U64P]),
     ("mgpu_prepare_barriers_name_puu", C.c_int, [C.POINTER(MgpuParams), C.POINTER(MgpuBedRecord),
                                                  DP, C.c_uint64, C.c_uint64, C.c_uint64, U64P, U8P,
                                                  DP, DP, U64P]),
@@ -246,6 +263,13 @@ SIGNATURES = [
     ("mgpu_job_create_multi", C.c_int, [C.c_void_p, C.POINTER(MgpuParams), C.c_uint64,
                                         C.POINTER(MgpuChrom), C.POINTER(MgpuBarriers), U64P, U64P,
                                         C.POINTER(U64P), U64P, C.c_int, C.POINTER(C.c_void_p)]),
+    ("mgpu_job_create_windows", C.c_int, [C.c_void_p, C.POINTER(MgpuParams), C.c_uint64,
+                                          C.POINTER(MgpuWindowTask), C.c_int,
+                                          C.POINTER(C.c_void_p)]),
+    ("mgpu_job_block_sums", C.c_int, [C.c_void_p, C.c_uint64, U64P, U64P, U64P, C.c_uint64,
+                                      U32P]),
+    ("mgpu_band_block_sums", C.c_int, [C.c_void_p, U32P, C.c_uint64, C.c_uint64, C.c_uint64,
+                                       U64P, U64P, C.c_uint64, U32P]),
     ("mgpu_job_run", C.c_int, [C.c_void_p, C.POINTER(MgpuStats)]),
     ("mgpu_job_capture", C.c_int, [C.c_void_p]),
     ("mgpu_job_release_graph", C.c_int, [C.c_void_p]),

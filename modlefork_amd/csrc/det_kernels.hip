@@ -195,6 +195,46 @@ __global__ void det_collision_kernel(const uint64_t* idx, const uint8_t* ev, uin
   flags[i] = f;
 }
 
+// ContactMatrixDense<u32>::unsafe_get_block (contact_matrix_dense_unsafe_impl.hpp:89-113) for
+// query i on band q.chrom[i]: the block_size x block_size pixels centred on (row, col), each
+// coordinate clamped to [0, ncols - 1], each pixel read as unsafe_get does (transposed to the
+// upper triangle, 0 beyond the band), summed in u32. One lane per query.
+__global__ void block_sums_kernel(const BlockBand* bands, const uint64_t* chrom,
+                                  const uint64_t* row, const uint64_t* col, uint64_t n,
+                                  uint32_t block_size, uint32_t* out) {
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const BlockBand b = bands[chrom[q]];
+  if (block_size == 1) {  // unsafe_get(row, col) itself
+    const uint64_t r = row[q], c = col[q];
+    const uint64_t i = r > c ? r - c : c - r, j = r > c ? r : c;
+    out[q] = i < b.nrows ? b.band[j * b.nrows + i] : 0u;
+    return;
+  }
+  const int64_t half = (int64_t)(block_size / 2);
+  const int64_t r0 = (int64_t)row[q] - half, c0 = (int64_t)col[q] - half;
+  const int64_t last = (int64_t)b.ncols - 1;
+  uint32_t sum = 0;
+  for (int64_t r = r0; r < r0 + (int64_t)block_size; ++r) {
+    const uint64_t rr = (uint64_t)(r < 0 ? 0 : (r > last ? last : r));
+    for (int64_t c = c0; c < c0 + (int64_t)block_size; ++c) {
+      const uint64_t cc = (uint64_t)(c < 0 ? 0 : (c > last ? last : c));
+      const uint64_t i = rr > cc ? rr - cc : cc - rr, j = rr > cc ? rr : cc;
+      if (i < b.nrows) sum += b.band[j * b.nrows + i];
+    }
+  }
+  out[q] = sum;
+}
+
+hipError_t launch_block_sums(hipStream_t stream, const BlockBand* d_bands, const uint64_t* d_chrom,
+                             const uint64_t* d_row, const uint64_t* d_col, uint64_t n,
+                             uint32_t block_size, uint32_t* d_out) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(block_sums_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, stream,
+                     d_bands, d_chrom, d_row, d_col, n, block_size, d_out);
+  return hipGetLastError();
+}
+
 hipError_t launch_det_band(hipStream_t stream, const DetChrom& ch, const uint64_t* d_rows,
                            const uint64_t* d_cols, uint64_t n) {
   hipLaunchKernelGGL(det_band_kernel, dim3(1), dim3(64), 0, stream, ch, d_rows, d_cols, n);

@@ -91,6 +91,15 @@ uint64_t det_cell_bytes_host(uint64_t nlefs, uint64_t nbar, uint64_t hist);
 // ch.nrows are read).
 hipError_t launch_det_band(hipStream_t stream, const DetChrom& ch, const uint64_t* d_rows,
                            const uint64_t* d_cols, uint64_t n);
+// One band as the block sums read it: nrows x ncols pixels, data[col * nrows + (col - row)].
+struct BlockBand {
+  const uint32_t* band;
+  uint64_t nrows, ncols;
+};
+// unsafe_get_block of n queries (band chrom[i], centre (row[i], col[i])), block_size odd.
+hipError_t launch_block_sums(hipStream_t stream, const BlockBand* d_bands, const uint64_t* d_chrom,
+                             const uint64_t* d_row, const uint64_t* d_col, uint64_t n,
+                             uint32_t block_size, uint32_t* d_out);
 // Collision words of deterministic mode for n (index, event) pairs.
 hipError_t launch_det_collisions(hipStream_t stream, const uint64_t* d_idx, const uint8_t* d_ev,
                                  uint64_t n, uint64_t* d_words, uint64_t* d_index,

@@ -319,6 +319,105 @@ int mgpu_config_transform(const mgpu_config* in, uint32_t set_mask, mgpu_params*
   return MGPU_OK;
 }
 
+namespace {
+// The permutation cpp-sort's pdq_sort makes of indices [0, n) ordered by key (unstable: equal
+// keys end up in pdq_sort's order).
+std::vector<uint32_t> pdq_order(const std::vector<uint64_t>& key) {
+  std::vector<uint32_t> idx(key.size());
+  for (uint32_t i = 0; i < idx.size(); ++i) idx[i] = i;
+  pdq_sort(idx.data(), (uint32_t)idx.size(), KeyLess{key.data()});
+  return idx;
+}
+}  // namespace
+
+int mgpu_import_barriers(const mgpu_params* p, const mgpu_bed_record* recs,
+                         const double* name_puu, uint64_t n, uint64_t chrom_start,
+                         uint64_t chrom_end, uint64_t* bed_start, uint64_t* bed_end,
+                         uint64_t* pos, uint8_t* dir, double* stp_active, double* stp_inactive,
+                         uint64_t* nout) {
+  if (!p || (n && !recs) || !nout) return fail(nullptr, MGPU_EINVAL, "null argument");
+  if (n > 0xFFFFFFFFull) return fail(nullptr, MGPU_ERANGE, "more than 2^32 records");
+  for (uint64_t i = 0; i < n; ++i) {
+    const mgpu_bed_record& r = recs[i];
+    if (r.strand != '+' && r.strand != '-' && r.strand != '.')
+      return fail(nullptr, MGPU_EINVAL, "record %llu: invalid strand", (unsigned long long)i);
+    if (r.score < 0.0 || r.score > 1.0)
+      return fail(nullptr, MGPU_EINVAL, "record %llu: score must be in [0, 1]",
+                  (unsigned long long)i);
+    // the name field as the record's not-bound stp (genome.cpp:459-468), range-checked for
+    // every record before the position test drops any
+    if (name_puu && !(name_puu[i] >= 0.0 && name_puu[i] <= 1.0))
+      return fail(nullptr, MGPU_EINVAL, "record %llu: name field must be in [0, 1]",
+                  (unsigned long long)i);
+  }
+  // The parser's per-chromosome interval tree orders the records by start (IITree::make_BST,
+  // interval_tree_impl.hpp:58-68, a pdq_sort of the indices), Genome::import_barriers walks
+  // them in that order and the chromosome's own tree keeps those in range
+  // (Chromosome::add_extrusion_barrier, genome.cpp:194-210), ordered by start once more when
+  // import_barriers indexes it (genome.cpp:474).
+  std::vector<uint64_t> key(n);
+  for (uint64_t i = 0; i < n; ++i) key[i] = recs[i].chrom_start;
+  std::vector<uint32_t> kept;
+  kept.reserve(n);
+  for (const uint32_t i : pdq_order(key)) {
+    const uint64_t bp = (recs[i].chrom_start + recs[i].chrom_end + 1) / 2;
+    if (bp >= chrom_start && bp < chrom_end) kept.push_back(i);
+  }
+  std::vector<uint64_t> key2(kept.size());
+  for (uint64_t k = 0; k < kept.size(); ++k) key2[k] = recs[kept[k]].chrom_start;
+  const std::vector<uint32_t> order = pdq_order(key2);
+  for (uint64_t k = 0; k < kept.size(); ++k) {
+    const uint32_t i = kept[order[k]];
+    const mgpu_bed_record& r = recs[i];
+    // compute_barrier_stp (genome.cpp:181-192) with the default or the record's not-bound stp,
+    // then the occupancy override of Simulation::Simulation (simulation.cpp:68-78), which
+    // restores the configured pair
+    double sa = p->barrier_occupied_stp, si = name_puu ? name_puu[i] : p->barrier_not_occupied_stp;
+    if (p->override_barrier_occupancy) {
+      si = p->barrier_not_occupied_stp;
+    } else if (r.score != 0.0) {
+      sa = mgpu_stp_active_from_occupancy(si, r.score);
+    }
+    if (bed_start) bed_start[k] = r.chrom_start;
+    if (bed_end) bed_end[k] = r.chrom_end;
+    pos[k] = (r.chrom_start + r.chrom_end + 1) / 2;
+    // motif '+' blocks extrusion in the rev direction, anything else in the fwd direction
+    dir[k] = (uint8_t)(r.strand == '+' ? 1 : 2);
+    stp_active[k] = sa;
+    stp_inactive[k] = si;
+  }
+  *nout = kept.size();
+  return MGPU_OK;
+}
+
+int mgpu_pdq_order(const uint64_t* keys, uint64_t n, uint32_t* order) {
+  if (n && (!keys || !order)) return fail(nullptr, MGPU_EINVAL, "null argument");
+  if (n > 0xFFFFFFFFull) return fail(nullptr, MGPU_ERANGE, "more than 2^32 keys");
+  const std::vector<uint32_t> idx = pdq_order(std::vector<uint64_t>(keys, keys + n));
+  std::copy(idx.begin(), idx.end(), order);
+  return MGPU_OK;
+}
+
+int mgpu_sort_barriers(uint64_t n, uint64_t* pos, uint8_t* dir, double* stp_active,
+                       double* stp_inactive) {
+  if (n && (!pos || !dir || !stp_active || !stp_inactive))
+    return fail(nullptr, MGPU_EINVAL, "null argument");
+  if (n > 0xFFFFFFFFull) return fail(nullptr, MGPU_ERANGE, "more than 2^32 barriers");
+  // ExtrusionBarriers::sort (extrusion_barriers.cpp:239-259): pdq_sort of the indices by
+  // position, then the permutation applied to every field
+  const std::vector<uint64_t> key(pos, pos + n);
+  const std::vector<uint32_t> idx = pdq_order(key);
+  const std::vector<uint8_t> d0(dir, dir + n);
+  const std::vector<double> a0(stp_active, stp_active + n), i0(stp_inactive, stp_inactive + n);
+  for (uint64_t k = 0; k < n; ++k) {
+    pos[k] = key[idx[k]];
+    dir[k] = d0[idx[k]];
+    stp_active[k] = a0[idx[k]];
+    stp_inactive[k] = i0[idx[k]];
+  }
+  return MGPU_OK;
+}
+
 int mgpu_prepare_barriers(const mgpu_params* p, const mgpu_bed_record* recs, uint64_t n,
                           uint64_t chrom_start, uint64_t chrom_end, uint64_t* pos, uint8_t* dir,
                           double* stp_active, double* stp_inactive, uint64_t* nout) {
@@ -330,61 +429,10 @@ int mgpu_prepare_barriers_name_puu(const mgpu_params* p, const mgpu_bed_record* 
                                    const double* name_puu, uint64_t n, uint64_t chrom_start,
                                    uint64_t chrom_end, uint64_t* pos, uint8_t* dir,
                                    double* stp_active, double* stp_inactive, uint64_t* nout) {
-  if (!p || (n && !recs) || !nout) return fail(nullptr, MGPU_EINVAL, "null argument");
-  struct Row {
-    uint64_t pos;
-    uint8_t dir;
-    double sa, si;
-  };
-  std::vector<Row> rows;
-  rows.reserve(n);
-  for (uint64_t i = 0; i < n; ++i) {
-    const mgpu_bed_record& r = recs[i];
-    if (r.strand != '+' && r.strand != '-' && r.strand != '.')
-      return fail(nullptr, MGPU_EINVAL, "record %llu: invalid strand", (unsigned long long)i);
-    if (r.score < 0.0 || r.score > 1.0)
-      return fail(nullptr, MGPU_EINVAL, "record %llu: score must be in [0, 1]",
-                  (unsigned long long)i);
-    // the name field as the record's not-bound stp (genome.cpp:459-468), range-checked for
-    // every record before the position test drops any
-    const double puu = name_puu ? name_puu[i] : p->barrier_not_occupied_stp;
-    if (name_puu && !(puu >= 0.0 && puu <= 1.0))
-      return fail(nullptr, MGPU_EINVAL, "record %llu: name field must be in [0, 1]",
-                  (unsigned long long)i);
-    const uint64_t bp = (r.chrom_start + r.chrom_end + 1) / 2;
-    if (bp < chrom_start || bp >= chrom_end) continue;
-    // compute_barrier_stp (genome.cpp:181-192), then the occupancy override of
-    // Simulation::Simulation (simulation.cpp:68-78), which restores the configured pair
-    double sa = p->barrier_occupied_stp, si = puu;
-    if (p->override_barrier_occupancy) {
-      si = p->barrier_not_occupied_stp;
-    } else if (r.score != 0.0) {
-      sa = mgpu_stp_active_from_occupancy(si, r.score);
-    }
-    // motif '+' blocks extrusion in the rev direction, anything else in the fwd direction
-    rows.push_back({bp, (uint8_t)(r.strand == '+' ? 1 : 2), sa, si});
-  }
-  // ExtrusionBarriers::sort (extrusion_barriers.cpp:239-259): cpp-sort's pdq_sort of the barrier
-  // indices by position (unstable: barriers at one position end up in pdq_sort's order, not the
-  // records'), then the permutation applied to every field
-  if (rows.size() > 0xFFFFFFFFull)
-    return fail(nullptr, MGPU_ERANGE, "more than 2^32 barriers on one chromosome");
-  std::vector<uint64_t> key(rows.size());
-  std::vector<uint32_t> idx(rows.size());
-  for (uint64_t i = 0; i < rows.size(); ++i) {
-    key[i] = rows[i].pos;
-    idx[i] = (uint32_t)i;
-  }
-  pdq_sort(idx.data(), (uint32_t)idx.size(), KeyLess{key.data()});
-  for (uint64_t i = 0; i < rows.size(); ++i) {
-    const Row& r = rows[idx[i]];
-    pos[i] = r.pos;
-    dir[i] = r.dir;
-    stp_active[i] = r.sa;
-    stp_inactive[i] = r.si;
-  }
-  *nout = rows.size();
-  return MGPU_OK;
+  int rc = mgpu_import_barriers(p, recs, name_puu, n, chrom_start, chrom_end, nullptr, nullptr,
+                                pos, dir, stp_active, stp_inactive, nout);
+  if (rc) return rc;
+  return mgpu_sort_barriers(*nout, pos, dir, stp_active, stp_inactive);
 }
 
 int mgpu_plan_chrom(const mgpu_params* p, const mgpu_chrom* chrom, uint64_t ncells,
@@ -503,11 +551,13 @@ bool same_job_wide(const mgpu_params& a, const mgpu_params& b) {
   return std::memcmp(&x, &y, sizeof(mgpu_params)) == 0;
 }
 
-// params has nchroms entries when per_chrom, else one.
+// params has nchroms entries when per_chrom, else one. Chromosome c's cells are cell ids
+// [cell_begin[c], cell_begin[c] + ncells[c]) (from 0 when cell_begin is null).
 int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_t nchroms,
                const mgpu_chrom* chroms, const mgpu_barriers* barriers, const uint64_t* nlefs,
                const uint64_t* ncells, const uint64_t* const* target_contacts,
-               const uint64_t* target_epochs, int mode, mgpu_job** out) {
+               const uint64_t* target_epochs, int mode, mgpu_job** out,
+               const uint64_t* cell_begin = nullptr) {
   if (!ctx || !out || !params) return fail(ctx, MGPU_EINVAL, "null argument");
   if (nchroms == 0 || !chroms || !barriers || !nlefs || !ncells || !target_contacts ||
       !target_epochs)
@@ -696,11 +746,12 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
     job->chroms.push_back(ch);
     release[2 * c] = pc.prob_release;
     release[2 * c + 1] = pc.prob_release_burnin;
+    const uint64_t cb = cell_begin ? cell_begin[c] : 0;
     for (uint64_t k = 0; k < ncells[c]; ++k) {
       WorkItem w{};
       w.chrom = (uint32_t)c;
-      w.cell_id = k;
-      w.seed = mgpu_cell_seed(&chroms[c], k, p->seed);
+      w.cell_id = cb + k;
+      w.seed = mgpu_cell_seed(&chroms[c], cb + k, p->seed);
       w.target = target_contacts[c][k];
       items.push_back(w);
       job->item_chrom.push_back((uint32_t)c);
@@ -759,6 +810,132 @@ int mgpu_job_create(mgpu_ctx* ctx, const mgpu_params* p, uint64_t nchroms, const
                     mgpu_job** out) {
   return job_create(ctx, p, false, nchroms, chroms, barriers, nlefs, ncells, target_contacts,
                     target_epochs, mode, out);
+}
+
+int mgpu_job_create_windows(mgpu_ctx* ctx, const mgpu_params* p, uint64_t ntasks,
+                            const mgpu_window_task* tasks, int mode, mgpu_job** out) {
+  if (!ctx || !p || !out || (ntasks && !tasks)) return fail(ctx, MGPU_EINVAL, "null argument");
+  if (ntasks == 0) return fail(ctx, MGPU_EINVAL, "no window tasks");
+  std::vector<mgpu_chrom> chroms(ntasks);
+  std::vector<mgpu_barriers> bars(ntasks);
+  std::vector<uint64_t> nlefs(ntasks), ncells(ntasks, 1), cell(ntasks), epochs(ntasks);
+  std::vector<const uint64_t*> targets(ntasks);
+  for (uint64_t t = 0; t < ntasks; ++t) {
+    chroms[t] = tasks[t].chrom;
+    bars[t] = tasks[t].barriers;
+    nlefs[t] = tasks[t].nlefs;
+    cell[t] = tasks[t].cell_id;
+    epochs[t] = tasks[t].target_epochs;
+    targets[t] = &tasks[t].target_contacts;
+  }
+  return job_create(ctx, p, false, ntasks, chroms.data(), bars.data(), nlefs.data(),
+                    ncells.data(), targets.data(), epochs.data(), mode, out, cell.data());
+}
+
+int mgpu_apply_deletion(const uint64_t* pos, uint64_t n, uint64_t deletion_begin,
+                        uint64_t deletion_size, double* stp_active, double* stp_inactive,
+                        uint64_t* num_active) {
+  if ((n && (!pos || !stp_active || !stp_inactive)) || !num_active)
+    return fail(nullptr, MGPU_EINVAL, "null argument");
+  uint64_t active = n;
+  for (uint64_t i = 0; i < n; ++i) {
+    if (pos[i] >= deletion_begin && pos[i] < deletion_begin + deletion_size) {
+      stp_active[i] = 0.0;
+      stp_inactive[i] = 1.0;
+      --active;
+    }
+  }
+  *num_active = active;
+  return MGPU_OK;
+}
+
+namespace {
+// unsafe_get_block of n queries on the bands of tab (device pointers): queries are checked on
+// the host, run on the context's stream and copied back.
+int run_block_sums(mgpu_ctx* ctx, const std::vector<BlockBand>& tab, uint64_t n,
+                   const uint64_t* chrom, const uint64_t* row, const uint64_t* col,
+                   uint64_t block_size, uint32_t* out) {
+  if (block_size == 0 || block_size % 2 == 0 || block_size > 0xFFFFFFFFull)
+    return fail(ctx, MGPU_EINVAL, "block size must be odd");
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint64_t c = chrom ? chrom[i] : 0;
+    if (c >= tab.size())
+      return fail(ctx, MGPU_EINVAL, "query %llu: no band %llu", (unsigned long long)i,
+                  (unsigned long long)c);
+    const uint64_t nc = tab[c].ncols;
+    if (row[i] >= nc || col[i] >= nc)
+      return fail(ctx, MGPU_EINVAL, "query %llu: pixel (%llu, %llu) outside the %llu bins",
+                  (unsigned long long)i, (unsigned long long)row[i], (unsigned long long)col[i],
+                  (unsigned long long)nc);
+  }
+  if (n == 0) return MGPU_OK;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const uint64_t tab_bytes = (tab.size() * sizeof(BlockBand) + 255) & ~255ULL;
+  const uint64_t q_bytes = (n * 8 + 255) & ~255ULL;
+  uint8_t* d = nullptr;
+  hipError_t e = hipMalloc(&d, tab_bytes + 3 * q_bytes + n * 4);
+  if (e != hipSuccess)
+    return fail(ctx, MGPU_ENOMEM, "device allocation failed: %s", hipGetErrorString(e));
+  auto* d_tab = reinterpret_cast<BlockBand*>(d);
+  auto* d_chrom = reinterpret_cast<uint64_t*>(d + tab_bytes);
+  auto* d_row = reinterpret_cast<uint64_t*>(d + tab_bytes + q_bytes);
+  auto* d_col = reinterpret_cast<uint64_t*>(d + tab_bytes + 2 * q_bytes);
+  auto* d_out = reinterpret_cast<uint32_t*>(d + tab_bytes + 3 * q_bytes);
+  std::vector<uint64_t> zeros;
+  if (!chrom) zeros.assign(n, 0);
+  e = hipMemcpyAsync(d_tab, tab.data(), tab.size() * sizeof(BlockBand), hipMemcpyHostToDevice,
+                     ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_chrom, chrom ? chrom : zeros.data(), n * 8, hipMemcpyHostToDevice,
+                       ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_row, row, n * 8, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_col, col, n * 8, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = launch_block_sums(ctx->stream, d_tab, d_chrom, d_row, d_col, n, (uint32_t)block_size,
+                          d_out);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(ctx, MGPU_EDEVICE, "block sums failed: %s", hipGetErrorString(e));
+  return MGPU_OK;
+}
+}  // namespace
+
+int mgpu_job_block_sums(mgpu_job* job, uint64_t n, const uint64_t* chrom, const uint64_t* row,
+                        const uint64_t* col, uint64_t block_size, uint32_t* out) {
+  if (!job || (n && (!chrom || !row || !col || !out)))
+    return fail(job ? job->ctx : nullptr, MGPU_EINVAL, "null argument");
+  std::vector<BlockBand> tab(job->nchroms);
+  for (uint64_t c = 0; c < job->nchroms; ++c) {
+    tab[c].band = job->d_bands + job->band_off[c];
+    tab[c].nrows = job->nrows[c];
+    tab[c].ncols = (job->band_len[c] - 1) / job->nrows[c];
+  }
+  return run_block_sums(job->ctx, tab, n, chrom, row, col, block_size, out);
+}
+
+int mgpu_band_block_sums(mgpu_ctx* ctx, const uint32_t* band, uint64_t nrows, uint64_t ncols,
+                         uint64_t n, const uint64_t* row, const uint64_t* col,
+                         uint64_t block_size, uint32_t* out) {
+  if (!ctx || !band || (n && (!row || !col || !out)))
+    return fail(ctx, MGPU_EINVAL, "null argument");
+  if (nrows == 0 || ncols == 0 || nrows > ncols)
+    return fail(ctx, MGPU_EINVAL, "band shape %llu x %llu", (unsigned long long)nrows,
+                (unsigned long long)ncols);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const uint64_t belems = nrows * ncols + 1;
+  uint32_t* d_band = nullptr;
+  HIPCHK(ctx, hipMalloc(&d_band, belems * 4));
+  hipError_t e = hipMemcpy(d_band, band, belems * 4, hipMemcpyHostToDevice);
+  int rc = MGPU_OK;
+  if (e != hipSuccess) {
+    rc = fail(ctx, MGPU_EDEVICE, "band upload failed: %s", hipGetErrorString(e));
+  } else {
+    const std::vector<BlockBand> tab{BlockBand{d_band, nrows, ncols}};
+    rc = run_block_sums(ctx, tab, n, nullptr, row, col, block_size, out);
+  }
+  (void)hipFree(d_band);
+  return rc;
 }
 
 int mgpu_job_create_multi(mgpu_ctx* ctx, const mgpu_params* params, uint64_t nchroms,
@@ -1099,25 +1276,9 @@ int mgpu_simulate_chrom(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom* c
                         int mode, uint32_t* out_band, uint64_t* out_missed, mgpu_stats* stats) {
   if (!ctx || !out_band) return fail(ctx, MGPU_EINVAL, "null argument");
   mgpu_job* job = nullptr;
-  // cell ids start at cell_begin: shift the ids after creating the items
-  int rc = mgpu_job_create(ctx, p, 1, chrom, barriers, &nlefs, &ncells, &target_contacts,
-                           &target_epochs, mode, &job);
+  int rc = job_create(ctx, p, false, 1, chrom, barriers, &nlefs, &ncells, &target_contacts,
+                      &target_epochs, mode, &job, &cell_begin);
   if (rc) return rc;
-  if (cell_begin != 0) {
-    std::vector<WorkItem> items(ncells);
-    for (uint64_t k = 0; k < ncells; ++k) {
-      items[k].chrom = 0;
-      items[k].cell_id = cell_begin + k;
-      items[k].seed = mgpu_cell_seed(chrom, cell_begin + k, p->seed);
-      items[k].target = target_contacts[k];
-    }
-    hipError_t e = hipMemcpy(job->d_items, items.data(), ncells * sizeof(WorkItem),
-                             hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-      mgpu_job_destroy(job);
-      return fail(ctx, MGPU_EDEVICE, "item upload failed: %s", hipGetErrorString(e));
-    }
-  }
   rc = mgpu_job_run(job, stats);
   if (!rc) {
     std::vector<uint32_t> tmp(job->band_len[0]);

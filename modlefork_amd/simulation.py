@@ -80,6 +80,60 @@ def chrom_view(name: str, size: int, start: int = 0, end: int | None = None) -> 
     return ch
 
 
+@dataclass
+class ImportedBarriers:
+    """A chromosome's barriers in the order of its interval tree (mgpu_import_barriers), with
+    the BED interval of each."""
+    bed_start: np.ndarray
+    bed_end: np.ndarray
+    pos: np.ndarray
+    dir: np.ndarray
+    stp_active: np.ndarray
+    stp_inactive: np.ndarray
+
+    def __len__(self):
+        return len(self.pos)
+
+    def subset(self, i0: int, i1: int) -> "ImportedBarriers":
+        return ImportedBarriers(self.bed_start[i0:i1].copy(), self.bed_end[i0:i1].copy(),
+                                self.pos[i0:i1].copy(), self.dir[i0:i1].copy(),
+                                self.stp_active[i0:i1].copy(), self.stp_inactive[i0:i1].copy())
+
+
+def import_barriers(params: L.MgpuParams, records, start: int, end: int,
+                    name_puu=None) -> ImportedBarriers:
+    """One chromosome's BED records (file order) as Genome::import_barriers keeps them
+    (mgpu_import_barriers)."""
+    n = len(records)
+    recs = (L.MgpuBedRecord * max(n, 1))()
+    for i, r in enumerate(records):
+        recs[i] = L.MgpuBedRecord(r.start, r.end, r.score, r.strand.encode())
+    m = max(n, 1)
+    bs, be, pos = np.zeros(m, np.uint64), np.zeros(m, np.uint64), np.zeros(m, np.uint64)
+    dirs, sa, si = np.zeros(m, np.uint8), np.zeros(m, np.float64), np.zeros(m, np.float64)
+    puu = None
+    if name_puu is not None:
+        if len(name_puu) != n:
+            raise ValueError("name_puu needs one value per record")
+        puu = np.ascontiguousarray(name_puu, np.float64)
+    nout = C.c_uint64(0)
+    L.check(L.lib().mgpu_import_barriers(
+        C.byref(params), recs, None if puu is None else _p(puu, C.c_double), n, start, end,
+        _p(bs, C.c_uint64), _p(be, C.c_uint64), _p(pos, C.c_uint64), _p(dirs, C.c_uint8),
+        _p(sa, C.c_double), _p(si, C.c_double), C.byref(nout)))
+    k = nout.value
+    return ImportedBarriers(bs[:k], be[:k], pos[:k], dirs[:k], sa[:k], si[:k])
+
+
+def sort_barriers(imp: ImportedBarriers) -> Barriers:
+    """The ExtrusionBarriers constructor's position sort (mgpu_sort_barriers) of a copy."""
+    b = Barriers(imp.pos.copy(), imp.dir.copy(), imp.stp_active.copy(), imp.stp_inactive.copy())
+    L.check(L.lib().mgpu_sort_barriers(len(b), _p(b.pos, C.c_uint64), _p(b.dir, C.c_uint8),
+                                       _p(b.stp_active, C.c_double),
+                                       _p(b.stp_inactive, C.c_double)))
+    return b
+
+
 def prepare_barriers(params: L.MgpuParams, records, start: int, end: int,
                      name_puu=None) -> Barriers:
     """Barrier tables of one chromosome's BED records (mgpu_prepare_barriers). name_puu, when
@@ -186,6 +240,20 @@ class Job:
         st = L.MgpuStats()
         self.ctx.check(L.lib().mgpu_job_run(self.ptr, C.byref(st)))
         return st
+
+    def block_sums(self, chrom, row, col, block_size: int) -> np.ndarray:
+        """unsafe_get_block of the bands on the device (mgpu_job_block_sums): one u32 sum per
+        (chromosome or task, row bin, column bin)."""
+        c = np.ascontiguousarray(chrom, np.uint64)
+        r = np.ascontiguousarray(row, np.uint64)
+        q = np.ascontiguousarray(col, np.uint64)
+        if not len(c) == len(r) == len(q):
+            raise ValueError("chrom, row and col differ in length")
+        out = np.zeros(max(len(c), 1), np.uint32)
+        self.ctx.check(L.lib().mgpu_job_block_sums(self.ptr, len(c), _p(c, C.c_uint64),
+                                                   _p(r, C.c_uint64), _p(q, C.c_uint64),
+                                                   block_size, _p(out, C.c_uint32)))
+        return out[:len(c)]
 
     def set_tuning(self, threads_per_cell: int = 0, global_working_set: bool | None = None,
                    tier_threads=None):
@@ -390,3 +458,62 @@ def occupancy_track(counts: np.ndarray) -> np.ndarray:
     (simulation.cpp:244-252): each bin's count over the chromosome's largest count, as float."""
     m = float(counts.max()) if len(counts) else 0.0
     return (counts.astype(np.float64) / m).astype(np.float32) if m else counts.astype(np.float32)
+
+
+@dataclass
+class WindowSpec:
+    """One window task for WindowJob (mgpu_window_task): the window [start, end) of chromosome
+    `name` of `size` bp, its sorted barriers (deletion applied), LEF count, cell id and targets."""
+    name: str
+    size: int
+    start: int
+    end: int
+    barriers: Barriers
+    nlefs: int
+    cell_id: int
+    target_contacts: int
+    target_epochs: int
+
+
+class WindowJob(Job):
+    """Device-resident batch of perturbate / replay window tasks (mgpu_job_create_windows): task
+    t is the job's chromosome t, with one cell and its own band over the window."""
+
+    def __init__(self, ctx: Context, params, tasks, mode):
+        self.ctx = ctx
+        n = len(tasks)
+        if n == 0:
+            raise ValueError("no window tasks")
+        bins, width = int(params.bin_size), int(params.diagonal_width)
+        self.plans = []
+        self._views = []
+        arr = (L.MgpuWindowTask * n)()
+        for k, t in enumerate(tasks):
+            ncols = -(-(t.end - t.start) // bins)
+            nrows = min(-(-width // bins), ncols)
+            self.plans.append(ChromPlan(t.nlefs, nrows, ncols,
+                                        np.array([t.target_contacts], np.uint64), t.target_epochs))
+            ch = chrom_view(t.name, t.size, t.start, t.end)
+            self._views.append(ch)
+            arr[k] = L.MgpuWindowTask(ch, t.barriers.view, t.nlefs, t.cell_id, t.target_contacts,
+                                      t.target_epochs)
+        self._tasks = arr
+        self._keep = tasks
+        self.ptr = C.c_void_p()
+        ctx.check(L.lib().mgpu_job_create_windows(ctx.ptr, C.byref(params), n, arr, mode,
+                                                  C.byref(self.ptr)))
+
+
+def band_block_sums(ctx: Context, band: np.ndarray, nrows: int, ncols: int, row, col,
+                    block_size: int) -> np.ndarray:
+    """unsafe_get_block on a band in host memory, on the device (mgpu_band_block_sums)."""
+    b = np.ascontiguousarray(band, np.uint32)
+    if len(b) != nrows * ncols + 1:
+        raise ValueError("band length is not nrows * ncols + 1")
+    r = np.ascontiguousarray(row, np.uint64)
+    q = np.ascontiguousarray(col, np.uint64)
+    out = np.zeros(max(len(r), 1), np.uint32)
+    ctx.check(L.lib().mgpu_band_block_sums(ctx.ptr, _p(b, C.c_uint32), nrows, ncols, len(r),
+                                           _p(r, C.c_uint64), _p(q, C.c_uint64), block_size,
+                                           _p(out, C.c_uint32)))
+    return out[:len(r)]

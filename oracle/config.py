@@ -171,27 +171,41 @@ def barrier_stp(score: float, c: dict, puu=None):
     return c["barrier_occupied_stp"], si
 
 
-def prepare_barriers(records, chrom_start: int, chrom_end: int, c: dict,
-                     name_puu=None) -> "R.Barriers":
-    """Genome::import_barriers' checks and Chromosome::add_extrusion_barrier (genome.cpp:194-210,
-    446-470) for each BED record, then the position sort of ExtrusionBarriers
-    (extrusion_barriers.cpp:234-259). name_puu, when given, holds each record's name field as a
-    number (interpret_name_field_as_puu)."""
-    rows = []
+def import_barriers(records, chrom_start: int, chrom_end: int, c: dict, name_puu=None):
+    """Genome::import_barriers (genome.cpp:433-480) for one chromosome's BED records in file
+    order: the checks of every record, then the records ordered by start (the parser's interval
+    tree, IITree::make_BST's pdq_sort, interval_tree_impl.hpp:58-68), those whose midpoint lies in
+    [chrom_start, chrom_end) kept (Chromosome::add_extrusion_barrier, genome.cpp:194-210) and
+    ordered by start again (the chromosome's own tree). Returns rows (bed_start, bed_end, pos,
+    dir, stp_active, stp_inactive) in that order. name_puu, when given, holds each record's name
+    field as a number (interpret_name_field_as_puu)."""
     for i, r in enumerate(records):
         if r.score < 0 or r.score > 1:
             raise ValueError(f"record {i}: score must be in [0, 1]")
-        puu = None if name_puu is None else float(name_puu[i])
-        if puu is not None and not 0.0 <= puu <= 1.0:
+        if name_puu is not None and not 0.0 <= float(name_puu[i]) <= 1.0:
             raise ValueError(f"record {i}: name field must be in [0, 1]")
-        pos = (r.start + r.end + 1) // 2
-        if pos < chrom_start or pos >= chrom_end:
-            continue
-        sa, si = barrier_stp(r.score, c, puu)
-        rows.append((pos, 1 if r.strand == "+" else 2, sa, si))
-    # cpp-sort pdq_sort of the indices by position (extrusion_barriers.cpp:243): equal positions
-    # keep pdq_sort's (unstable) order
-    order = R.pdq_order([t[0] for t in rows])
-    rows = [rows[i] for i in order]
-    return R.Barriers([t[0] for t in rows], [t[1] for t in rows], [t[2] for t in rows],
-                      [t[3] for t in rows])
+    kept = [i for i in R.pdq_order([r.start for r in records])
+            if chrom_start <= (records[i].start + records[i].end + 1) // 2 < chrom_end]
+    rows = []
+    for k in R.pdq_order([records[i].start for i in kept]):
+        i = kept[k]
+        r = records[i]
+        sa, si = barrier_stp(r.score, c, None if name_puu is None else float(name_puu[i]))
+        rows.append((r.start, r.end, (r.start + r.end + 1) // 2, 1 if r.strand == "+" else 2,
+                     sa, si))
+    return rows
+
+
+def sort_barriers(rows) -> "R.Barriers":
+    """The ExtrusionBarriers constructor's position sort (extrusion_barriers.cpp:234-259): cpp-sort
+    pdq_sort of the indices by position; equal positions keep pdq_sort's (unstable) order.
+    rows as import_barriers returns them."""
+    rows = [rows[i] for i in R.pdq_order([t[2] for t in rows])]
+    return R.Barriers([t[2] for t in rows], [t[3] for t in rows], [t[4] for t in rows],
+                      [t[5] for t in rows])
+
+
+def prepare_barriers(records, chrom_start: int, chrom_end: int, c: dict,
+                     name_puu=None) -> "R.Barriers":
+    """A chromosome's barrier tables from its BED records: import_barriers, then sort_barriers."""
+    return sort_barriers(import_barriers(records, chrom_start, chrom_end, c, name_puu))

@@ -1282,6 +1282,33 @@ void rc_band_increment(rc_band* m, uint64_t row, uint64_t col) {
   __atomic_fetch_add(&m->data[j * m->nrows + i], 1u, __ATOMIC_RELAXED);
 }
 
+/* ContactMatrixDense::unsafe_get, contact_matrix_dense_unsafe_impl.hpp:33-43: the pixel of
+ * (row, col) transposed to the upper triangle, 0 beyond the band. */
+uint32_t rc_band_get(const rc_band* m, uint64_t row, uint64_t col) {
+  const uint64_t i = row > col ? row - col : col - row;
+  const uint64_t j = row > col ? row : col;
+  return i < m->nrows ? m->data[j * m->nrows + i] : 0u;
+}
+
+/* ContactMatrixDense::unsafe_get_block, contact_matrix_dense_unsafe_impl.hpp:89-113: the sum (in
+ * the matrix's u32) of the block_size x block_size pixels around (row, col), with every
+ * coordinate clamped to [0, ncols - 1]. */
+uint32_t rc_band_get_block(const rc_band* m, uint64_t row, uint64_t col, uint64_t block_size) {
+  if (block_size == 1) return rc_band_get(m, row, col);
+  const int64_t bs = (int64_t)block_size;
+  const int64_t first_row = (int64_t)row - bs / 2, first_col = (int64_t)col - bs / 2;
+  const int64_t hi = (int64_t)m->ncols - 1;
+  uint32_t n = 0;
+  for (int64_t i = first_row; i < first_row + bs; ++i) {
+    for (int64_t j = first_col; j < first_col + bs; ++j) {
+      const uint64_t ii = (uint64_t)(i < 0 ? 0 : (i > hi ? hi : i));
+      const uint64_t jj = (uint64_t)(j < 0 ? 0 : (j > hi ? hi : j));
+      n += rc_band_get(m, ii, jj);
+    }
+  }
+  return n;
+}
+
 /* ========================================================================================== */
 /* Whole cell                                                                                  */
 /* ========================================================================================== */

@@ -149,6 +149,8 @@ typedef struct rc_band {
   uint64_t missed;
 } rc_band;
 void rc_band_increment(rc_band* m, uint64_t row, uint64_t col);
+uint32_t rc_band_get(const rc_band* m, uint64_t row, uint64_t col);
+uint32_t rc_band_get_block(const rc_band* m, uint64_t row, uint64_t col, uint64_t block_size);
 uint64_t rc_band_nrows(uint64_t diagonal_width, uint64_t bin_size, uint64_t length);
 uint64_t rc_band_ncols(uint64_t bin_size, uint64_t length);
 

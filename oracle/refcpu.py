@@ -177,6 +177,10 @@ def lib():
         L.rc_stp_active_from_occupancy.restype = C.c_double
         L.rc_stp_active_from_occupancy.argtypes = [C.c_double, C.c_double]
         L.rc_band_increment.argtypes = [C.POINTER(RcBand), C.c_uint64, C.c_uint64]
+        L.rc_band_get.argtypes = [C.POINTER(RcBand), C.c_uint64, C.c_uint64]
+        L.rc_band_get.restype = C.c_uint32
+        L.rc_band_get_block.argtypes = [C.POINTER(RcBand), C.c_uint64, C.c_uint64, C.c_uint64]
+        L.rc_band_get_block.restype = C.c_uint32
         L.rc_coll_encode.restype = C.c_uint64
         L.rc_coll_encode.argtypes = [C.c_uint64, C.c_uint]
         L.rc_coll_index.restype = C.c_uint64
@@ -338,3 +342,10 @@ def simulate_cell_trace(cfg: dict, chrom, barriers: Barriers, nlefs: int, cell: 
     if rcode != 0:
         raise RuntimeError("oracle simulation failed")
     return band, int(rb.missed), (st.epochs, st.burnin_epochs, st.contacts, st.draws), trace
+
+
+def band_get_block(band, nrows: int, ncols: int, row: int, col: int, block_size: int) -> int:
+    """unsafe_get_block of one pixel block (rc_band_get_block) on a u32 band array."""
+    b = np.ascontiguousarray(band, np.uint32)
+    rb = RcBand(nrows, ncols, ptr(b, C.c_uint32), 0)
+    return int(lib().rc_band_get_block(C.byref(rb), row, col, block_size))

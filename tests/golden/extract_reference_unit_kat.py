@@ -10,7 +10,8 @@ It reads the Catch2 sources as text and keeps the values they hold and the answe
     "compute_occupancy" and the positions of "sort";
   * test/units/contact_matrix/contact_matrix_dense_test.cpp: the matrix shape and the sequence of
     increments, decrements and checks of "CMatrix simple" and "CMatrix in/decrement" (the checks
-    under utils::ndebug_not_defined(), which expect exceptions of debug builds, are left out);
+    under utils::ndebug_not_defined(), which expect exceptions of debug builds, are left out), and
+    the fills and unsafe_get_block checks of "CMatrix get w/ block";
   * test/units/simulation_cpu/collision_encoding_test.cpp: every (index, event) the cases encode,
     each collision_occurred / collision_avoided / decode_* check with its expected answer, in
     order, as a list of operations on one collision word (the checks of invalid events under
@@ -117,7 +118,29 @@ def cmatrix(text):
             elif stmt and not stmt.startswith("ContactMatrixDense"):
                 raise ValueError(f"{name}: unhandled statement {stmt!r}")
         out[name] = {"nrows": nrows, "ncols": ncols, "ops": ops}
+    out["CMatrix get w/ block"] = cmatrix_block(cs["CMatrix get w/ block"])
     return out
+
+
+def cmatrix_block(body):
+    """The fills (nested loops over i and j with the value set at (i, j), kept as the loop bounds
+    and the value's arithmetic expression in i and j) and the unsafe_get_block checks."""
+    m = re.search(r"ContactMatrixDense<\w*>\s+(\w+)\((\d+),\s*(\d+)\)", body)
+    var, nrows, ncols = m.group(1), int(m.group(2)), int(m.group(3))
+    fills = []
+    loop = (r"for \(u32 i = (\d+); i < (\d+); \+\+i\)\s*\{\s*"
+            r"for \(u32 j = (\w+); j < (\d+); \+\+j\)\s*\{\s*"
+            rf"{var}\.set\(i, j, ([^;]+)\);")
+    for m in re.finditer(loop, body):
+        j0 = m.group(3)
+        fills.append({"i": [int(m.group(1)), int(m.group(2))],
+                      "j": [j0 if j0 == "i" else int(j0), int(m.group(4))],
+                      "value": " ".join(m.group(5).split())})
+    checks = [[int(a), int(b), int(c), int(d)] for a, b, c, d in re.findall(
+        rf"CHECK\({var}\.unsafe_get_block\((\d+), (\d+), (\d+)\) == (\d+)\)", body)]
+    if not fills or not checks:
+        raise ValueError("CMatrix get w/ block: nothing extracted")
+    return {"nrows": nrows, "ncols": ncols, "fills": fills, "block_checks": checks}
 
 
 def collisions(text):

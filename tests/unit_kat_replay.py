@@ -99,3 +99,23 @@ def replay_collisions(cases, words):
                 raise ValueError(op)
             nchecks += 1
     return nchecks
+
+
+def _value(expr: str, i: int, j: int) -> int:
+    """An extracted fill expression: integers, i, j, +, - and parentheses only."""
+    if not set(expr) <= set("0123456789ij+-() "):
+        raise ValueError(f"unexpected fill expression {expr!r}")
+    return int(eval(expr, {"__builtins__": {}}, {"i": i, "j": j}))
+
+
+def block_band(case):
+    """The band of "CMatrix get w/ block": each fill sets pixel (i, j) over its loop ranges (j
+    from i when the loop says so) to its expression."""
+    nrows, ncols = case["nrows"], case["ncols"]
+    band = np.zeros(nrows * ncols + 1, np.uint32)
+    for f in case["fills"]:
+        for i in range(f["i"][0], f["i"][1]):
+            j0 = i if f["j"][0] == "i" else f["j"][0]
+            for j in range(j0, f["j"][1]):
+                band[pixel_index(nrows, i, j)] = _value(f["value"], i, j)
+    return band
