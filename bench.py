@@ -179,6 +179,107 @@ def stress_leg(ctx, mode, nthreads, no_cpu):
     return out
 
 
+def perturbate_inputs(wl, params):
+    """ChromInput list of a perturbate workload (barriers in interval-tree order, features)."""
+    from modlefork_amd import perturbate as P
+    from modlefork_amd import simulation as S
+
+    chroms = []
+    for name, size in wl.chroms:
+        imp = S.import_barriers(params, wl.barriers_of(name), 0, size)
+        tracks = wl.features.get(name, [[], []])
+        f = [[P.Feature(name, a, b, n, 0.0, st) for a, b, n, st in tr] for tr in tracks]
+        chroms.append(P.ChromInput(name, size, 0, size, imp, f[0], f[1]))
+    return chroms
+
+
+def perturbate_cpu_baseline(wl, tasks, chroms, ntasks, nthreads):
+    """The oracle on a stratified sample of the perturbate tasks (every k-th), one task per
+    thread from a shared queue: per-task times give tasks x threads / summed time."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import config as OC
+    from oracle import refcpu as R
+
+    R.build()
+    c = OC.transform(wl.options)
+    rc = OC.to_rc(c)
+    by_name = {ch.name: ch for ch in chroms}
+    pick = tasks[::max(1, len(tasks) // ntasks)][:ntasks]
+
+    def run_task(t):
+        imp = by_name[t.chrom].barriers
+        i0, i1 = t.barrier_range
+        rows = [(int(imp.bed_start[i]), int(imp.bed_end[i]), int(imp.pos[i]), int(imp.dir[i]),
+                 float(imp.stp_active[i]), float(imp.stp_inactive[i])) for i in range(i0, i1)]
+        b = OC.sort_barriers(rows)
+        keep = (b.pos >= t.window_start) & (b.pos < t.window_end)
+        sa, si = b.stp_active[keep].copy(), b.stp_inactive[keep].copy()
+        gone = (b.pos[keep] >= t.deletion_begin) & (b.pos[keep] < t.deletion_begin + t.deletion_size)
+        sa[gone], si[gone] = 0.0, 1.0
+        bars = R.Barriers(b.pos[keep], b.dir[keep], sa, si)
+        ncols = -(-(t.window_end - t.window_start) // c["bin_size"])
+        nrows = min(-(-c["diagonal_width"] // c["bin_size"]), ncols)
+        t0 = time.perf_counter()
+        R.simulate_cells(rc, R.make_chrom(t.chrom, t.chrom_size, t.window_start, t.window_end),
+                         bars, t.num_lefs, t.cell_id, t.cell_id + 1,
+                         np.array([t.num_target_contacts], np.uint64), t.num_target_epochs,
+                         nrows, ncols, 1)
+        return time.perf_counter() - t0
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=nthreads) as ex:
+        times = list(ex.map(run_task, pick))
+    wall = time.perf_counter() - t0
+    return {"value": len(pick) * nthreads / sum(times), "unit": "tasks/s", "cores": nthreads,
+            "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"oracle on {len(pick)} of the {len(tasks)} tasks (every "
+                      f"{max(1, len(tasks) // ntasks)}-th), one task per thread on {nthreads} "
+                      f"threads; rate = tasks x threads / summed per-task time "
+                      f"({sum(times):.1f} thread-s; the sample's wall {wall:.1f} s)"}
+
+
+def perturbate_leg(ctx, nthreads, no_cpu):
+    """modle perturbate's simulations and block readout on the perturb workload (one 30 Mbp
+    chromosome, 3 Mbp band, one 10 kb deletion per barrier, windows of 12 Mbp): every
+    (window, deletion) task in one fast-mode job, timed after a warm-up run, then the 9 x 9 block
+    sums of every feature pair of every task on the device."""
+    from modlefork_amd import _lib as L
+    from modlefork_amd import perturbate as P
+    from modlefork_amd import simulation as S
+    from modlefork_amd import workloads as W
+
+    wl = W.perturb()
+    params = S.make_params(wl.options)
+    chroms = perturbate_inputs(wl, params)
+    tasks = P.plan(params, chroms)
+    specs, _ = P.window_specs(chroms, tasks)
+    job = S.WindowJob(ctx, params, specs, L.MODE_FAST)
+    job.run()
+    t0 = time.perf_counter()
+    st = job.run()
+    wall = time.perf_counter() - t0
+    qs = P.pair_queries(chroms, tasks, int(params.bin_size), int(params.diagonal_width))
+    ti = np.array([q.task for q in qs], np.uint64)
+    r1 = np.array([q.rel_bin1 for q in qs], np.uint64)
+    r2 = np.array([q.rel_bin2 for q in qs], np.uint64)
+    t1 = time.perf_counter()
+    job.block_sums(ti, r1, r2, 9)
+    tq = time.perf_counter() - t1
+    job.close()
+    out = {"workload": wl.name, "tasks": len(tasks), "tasks_per_sec": len(tasks) / wall,
+           "contacts_per_sec": st.contacts / wall, "epochs_per_sec": st.epochs / wall,
+           "kernel_ms": float(st.kernel_ms), "dropped": int(st.dropped),
+           "pair_queries": len(qs), "block_sums_ms": 1000.0 * tq}
+    log(f"perturbate: {len(tasks)} tasks in {wall:.2f} s ({len(tasks) / wall:.1f} tasks/s), "
+        f"{len(qs)} block sums in {1000 * tq:.1f} ms")
+    if not no_cpu:
+        cpu = perturbate_cpu_baseline(wl, tasks, chroms, 2 * nthreads, nthreads)
+        out["cpu_baseline"] = cpu
+        out["speedup_vs_cpu"] = out["tasks_per_sec"] / cpu["value"]
+    return out
+
+
 def surrogate_flops(cfg) -> float:
     """Algorithmic FLOPs of one surrogate forward pass over one window (multiply-adds of the real
     channels counted as 2 FLOPs; padding and the pairwise sum not counted)."""
@@ -248,6 +349,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="CPU baseline threads (0: this process's CPUs, at most 16: the host's "
                          "share of one GPU)")
+    ap.add_argument("--no-perturbate", action="store_true",
+                    help="skip the perturbate leg (windows with deletions and block readout)")
     ap.add_argument("--no-stress", action="store_true",
                     help="skip the stress leg (250 Mbp, 1 kb bins, 2048 cells: one GPU run and its "
                          "CPU baseline)")
@@ -446,6 +549,8 @@ def main():
     job.close()
     if not args.no_stress and args.workload == "genome":
         line["stress"] = stress_leg(sim.ctx, mode, nthreads, args.no_cpu)
+    if not args.no_perturbate and args.workload == "genome":
+        line["perturbate"] = perturbate_leg(sim.ctx, nthreads, args.no_cpu)
     if not args.no_sweep:
         log(f"surrogate sweep: {args.sweep_points} points")
         line["surrogate"] = surrogate_leg(sim.ctx, args.sweep_points)
