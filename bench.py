@@ -320,6 +320,8 @@ def surrogate_leg(ctx, npoints: int):
         "corr_threshold": 0.95,
         "corr_mean": float(np.nanmean(corr)), "corr_min": float(np.nanmin(corr)),
         "raw_corr_mean": float(np.nanmean(res.raw_correlations)),
+        "raw_corr_5kb_all_diagonals_mean": float(np.nanmean(res.raw_5kb)),
+        "judge_contact_density": 30.0,  # also the plain simulation's, whose rate is the baseline
         "judge_reliability": float(np.nanmean(res.reliabilities)),
         "passed": int(np.sum(corr >= 0.95)), "replaced": len(res.replaced),
         "surrogate_rejected": res.fell_back,

@@ -299,3 +299,46 @@ def test_full_sweep_judges_and_accepts_the_surrogate(ctx):
     assert np.nanmean(res.correlations) >= 0.95
     assert all(not c >= 0.95 for k, c in zip(res.judged, res.correlations) if k in bands)
     assert res.sweep_rate > res.sim_rate
+    say(f"raw 5 kb correlation over all 99 off-diagonals: {np.nanmean(res.raw_5kb):.4f}")
+
+
+def test_attenuation_correction_is_calibrated(ctx):
+    """The judge's correction for the simulation's own noise, checked with a prediction whose
+    agreement with the expected band is known: a simulation of the same point with 16 times the
+    judge's cells and contacts (other seeds). Spearman-Brown gives its reliability
+    q16 = 16 q / (1 + 15 q) from the judge's reliability q, so on each judged diagonal its raw
+    correlation with the judge's simulation should be sqrt(q * q16) and the corrected one
+    sqrt(q16), close to 1. Over three sweep points: the mean corrected correlation (unclipped)
+    lies within 0.03 of the expected mean of sqrt(q16), the raw one within 0.02 of the mean of
+    sqrt(q * q16); a prediction of the wrong point (another density and processivity) fails the
+    0.95 bar."""
+    from modlefork_amd import sweep as Sw
+
+    layout = Sw.fixed_layout()
+    pts = [Sw.Point(12.0, 200e3, 0.9), Sw.Point(25.0, 400e3, 0.7), Sw.Point(35.0, 550e3, 1.0)]
+    wrong = Sw.Point(6.0, 120e3, 0.6)
+    for k, p in enumerate(pts):
+        halves, _ = Sw.simulate_points(ctx, layout, [p, p], 256, 15.0, seed=3,
+                                       stored_diagonals=100, name_ids=[10 * k, 10 * k + 1])
+        big, _ = Sw.simulate_points(ctx, layout, [p, wrong], 8192, 480.0, seed=4,
+                                    stored_diagonals=100, name_ids=[10 * k + 2, 10 * k + 3])
+        a, b = halves
+        full = a + b
+        ds = Sw.well_sampled(full, Sw.MIN_COUNT)
+        h = Sw.diag_correlations(a, b, diagonals=ds)
+        q = 2 * h / (1 + h)
+        keep = q >= Sw.MIN_RELIABILITY
+        ds = [D for D, kk in zip(ds, keep) if kk]
+        q = q[keep]
+        r = Sw.diag_correlations(big[0], full, diagonals=ds)
+        q16 = 16 * q / (1 + 15 * q)
+        corrected = float(np.mean(r / np.sqrt(q)))
+        say(f"point {k}: {len(ds)} judged diagonals, reliability {q.mean():.3f}, raw "
+            f"{r.mean():.4f} (expected {np.mean(np.sqrt(q * q16)):.4f}), corrected {corrected:.4f} "
+            f"(expected {np.mean(np.sqrt(q16)):.4f})")
+        assert len(ds) >= 5
+        assert abs(corrected - np.mean(np.sqrt(q16))) < 0.03
+        assert abs(r.mean() - np.mean(np.sqrt(q * q16))) < 0.02
+        c_wrong, _, _ = Sw.judge(big[1], a, b)
+        say(f"point {k}: the wrong point's simulation as prediction scores {c_wrong:.4f}")
+        assert not c_wrong >= 0.95
