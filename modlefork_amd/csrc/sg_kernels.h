@@ -47,8 +47,10 @@ hipError_t launch_pair_fwd(hipStream_t s, const uint16_t* P, const uint16_t* Q, 
                            uint16_t* Z, int B, int ND, int L, int C);
 hipError_t launch_pair_bwd(hipStream_t s, const uint16_t* GZ, const uint16_t* Z, uint16_t* dP,
                            uint16_t* dQ, float* dE, int B, int ND, int L, int C);
+// scratch: 9 * B * D * I floats (the tap projections) for the two-pass C == 64 path; null
+// takes the one-pass kernel.
 hipError_t launch_out_fwd(hipStream_t s, const uint16_t* Z, const float* V, const float* c0,
-                          float* out, int B, int D, int I, int C);
+                          float* out, int B, int D, int I, int C, float* scratch = nullptr);
 hipError_t launch_loss(hipStream_t s, const float* out, const float* y, float* g, float scale,
                        float* loss, int B, int D, int I);
 hipError_t launch_out_bwd(hipStream_t s, const uint16_t* Z, const float* g, const float* V,

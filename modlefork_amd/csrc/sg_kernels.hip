@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "sg_kernels.h"
 
 namespace mgpu {
@@ -83,29 +85,51 @@ __global__ void __launch_bounds__(256)
     pd[r] = (int)((mm / a.I) % a.D);
     prow[r] = mm - (int64_t)pd[r] * a.I - pi[r];  // first position of this lane's (b) image
   }
+  // the epilogue's inputs, read before the MFMA loop so their latency hides behind it: the
+  // lane's two output channels' biases and its residual chunks (32 positions x 8 channels per
+  // round r, four chunks per lane)
+  float bias[2];
+  for (int c = 0; c < 2; ++c) bias[c] = a.bias ? a.bias[n0 + 32 * c + (lane & 31)] : 0.0f;
+  uint4 resv[2][4];
+  for (int r = 0; r < 2; ++r)
+    for (int u = 0; u < 4; ++u) {
+      const int e = lane + 64 * u, pos = e >> 3, q = e & 7;
+      const int64_t m = m0 + 32 * r + pos;
+      resv[r][u] = a.res && a.out2 && m < npos
+                       ? *reinterpret_cast<const uint4*>(a.res + m * a.Cout + n0 + 8 * q)
+                       : make_uint4(0, 0, 0, 0);
+    }
   f32x16 acc[2][2];
   for (int r = 0; r < 2; ++r)
     for (int c = 0; c < 2; ++c) acc[r][c] = f32x16{};
   const int kofs = 8 * (lane >> 5);
-  for (int t = 0; t < T; ++t) {
+  // k-steps (tap t, 16 input channels from ci): the operands of step s + 1 are loaded before
+  // the MFMAs of step s, so each global load overlaps a step's MFMAs
+  const int ksteps = a.Cin / 16, nsteps = T * ksteps;
+  auto load_step = [&](int st, Pack8* A, Pack8* Bv) {
+    const int t = st / ksteps, ci = (st - t * ksteps) * 16;
     const int dd = t / a.KI - kh, di = (t % a.KI - kw) * dil;
-    const uint16_t* xrow[2];
     for (int r = 0; r < 2; ++r) {
       const int d = pd[r] + dd, i = pi[r] + di;
       const bool ok = pvalid[r] && d >= 0 && d < a.D && i >= 0 && i < a.I;
-      xrow[r] = ok ? a.X + (prow[r] + (int64_t)d * a.I + i) * a.Cin : nullptr;
+      A[r].u = ok ? *reinterpret_cast<const uint4*>(a.X + (prow[r] + (int64_t)d * a.I + i) * a.Cin +
+                                                    ci + kofs)
+                  : make_uint4(0, 0, 0, 0);
     }
-    const uint16_t* wrow[2];
     for (int c = 0; c < 2; ++c)
-      wrow[c] = a.W + ((int64_t)(n0 + 32 * c + (lane & 31)) * T + t) * a.Cin;
-    for (int ci = 0; ci < a.Cin; ci += 16) {
-      Pack8 A[2], Bv[2];
-      for (int r = 0; r < 2; ++r)
-        A[r].u = xrow[r] ? *reinterpret_cast<const uint4*>(xrow[r] + ci + kofs) : make_uint4(0, 0, 0, 0);
-      for (int c = 0; c < 2; ++c) Bv[c].u = *reinterpret_cast<const uint4*>(wrow[c] + ci + kofs);
-      for (int r = 0; r < 2; ++r)
-        for (int c = 0; c < 2; ++c)
-          acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[r].v, Bv[c].v, acc[r][c], 0, 0, 0);
+      Bv[c].u = *reinterpret_cast<const uint4*>(
+          a.W + ((int64_t)(n0 + 32 * c + (lane & 31)) * T + t) * a.Cin + ci + kofs);
+  };
+  Pack8 A[2], Bv[2], An[2], Bn[2];
+  load_step(0, A, Bv);
+  for (int st = 0; st < nsteps; ++st) {
+    if (st + 1 < nsteps) load_step(st + 1, An, Bn);
+    for (int r = 0; r < 2; ++r)
+      for (int c = 0; c < 2; ++c)
+        acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[r].v, Bv[c].v, acc[r][c], 0, 0, 0);
+    for (int k = 0; k < 2; ++k) {
+      A[k] = An[k];
+      Bv[k] = Bn[k];
     }
   }
   // epilogue: v = acc + bias; relu; out1 = v; out2 = res + v (all bf16), staged through LDS 32
@@ -116,22 +140,22 @@ __global__ void __launch_bounds__(256)
   for (int r = 0; r < 2; ++r) {
     for (int c = 0; c < 2; ++c) {
       const int cl = 32 * c + (lane & 31);
-      const float bias = a.bias ? a.bias[n0 + cl] : 0.0f;
       for (int k = 0; k < 16; ++k) {
-        float v = acc[r][c][k] + bias;
+        float v = acc[r][c][k] + bias[c];
         if (a.relu) v = v > 0.0f ? v : 0.0f;
         stage[acc_row(k, lane) * kEpStride + cl] = v;
       }
     }
     __syncthreads();
-    for (int e = lane; e < 32 * 8; e += 64) {
-      const int pos = e >> 3, q = e & 7;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = lane + 64 * u, pos = e >> 3, q = e & 7;
       const int64_t m = m0 + 32 * r + pos;
       if (m < npos) {
         const int64_t o = m * a.Cout + n0 + 8 * q;
         const float* sv = stage + pos * kEpStride + 8 * q;
         Pack8 res;
-        res.u = a.res && a.out2 ? *reinterpret_cast<const uint4*>(a.res + o) : make_uint4(0, 0, 0, 0);
+        res.u = resv[r][u];
         Pack8 o1, o2;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -149,122 +173,164 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------------------------
 // 3 x 3 convolution, 64 -> 64 channels, on a D x I grid: the head's layer shape
 // ---------------------------------------------------------------------------------------------
-// One workgroup computes a tile of 4 rows (d) x 64 columns (i) of one image, all 64 output
-// channels; wave w owns row w (64 positions, 2 x 2 MFMA tiles). The input patch the tile reads
-// (6 x 66 positions, zero outside the image) is staged once in LDS, so each input position is
-// fetched from memory once per tile instead of once per tap and wave. Positions are padded to
-// 144 bytes: 16 consecutive positions then start 36 banks apart and a 16-byte read per lane is
-// free of bank conflicts.
+// A persistent workgroup per CU walks tiles of 4 rows (d) x 64 columns (i) of the images, all
+// 64 output channels; wave w owns row w of a tile (64 positions, 2 x 2 MFMA tiles). The layer's
+// weights (9 taps x 64 x 64) sit in LDS for the workgroup's lifetime, so the MFMA loop reads
+// both operands from LDS and the only global loads are the input patches: the patch a tile
+// reads (6 x 66 positions, zero outside the image) is staged in LDS once per tile, and the next
+// tile's patch is loaded into registers while this tile computes, then stored after its
+// epilogue. Patch positions and weight rows are padded to 144 bytes: 16 consecutive rows then
+// start 36 banks apart and a 16-byte read per lane is free of bank conflicts.
 constexpr int kT3Rows = 4, kT3Cols = 64, kT3C = 64;
 constexpr int kT3PRows = kT3Rows + 2, kT3PCols = kT3Cols + 2;
-constexpr int kT3Stride = kT3C + 8;  // bf16 per patch position
+constexpr int kT3Stride = kT3C + 8;  // bf16 per patch position and per weight row
+constexpr int kT3PatchElems = kT3PRows * kT3PCols * kT3Stride;
+constexpr int kT3WeightElems = 9 * kT3C * kT3Stride;  // [tap][co][ci]
+constexpr int kT3Lds = 2 * (kT3PatchElems + kT3WeightElems);
+constexpr int kT3Chunks = kT3PRows * kT3PCols * 8;  // 16-byte chunks of a patch
+constexpr int kT3PerLane = (kT3Chunks + 255) / 256;
 
-__global__ void __launch_bounds__(256) conv3x3_c64_kernel(ConvArgs a) {
-  __shared__ __align__(16) uint16_t patch[kT3PRows * kT3PCols * kT3Stride];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int i0 = blockIdx.x * kT3Cols, d0 = blockIdx.y * kT3Rows, b = blockIdx.z;
-  const uint16_t* X = a.X + (int64_t)b * a.D * a.I * kT3C;
-  // stage the patch: 16-byte chunks, eight per position
-  for (int e = threadIdx.x; e < kT3PRows * kT3PCols * 8; e += 256) {
+struct T3Tile {
+  int b, d0, i0;
+};
+
+__device__ inline T3Tile t3_tile(int tile, int ntx, int nty) {
+  return T3Tile{tile / (ntx * nty), (tile / ntx) % nty * kT3Rows, tile % ntx * kT3Cols};
+}
+
+// this lane's chunks of tile t's patch, zero outside the image
+__device__ inline void t3_load(const ConvArgs& a, T3Tile t, uint4* stg) {
+  const uint16_t* X = a.X + (int64_t)t.b * a.D * a.I * kT3C;
+#pragma unroll
+  for (int m = 0; m < kT3PerLane; ++m) {
+    const int e = threadIdx.x + 256 * m;
     const int q = e & 7, pos = e >> 3;
-    const int pr = pos / kT3PCols, pc = pos % kT3PCols;
-    const int d = d0 + pr - 1, i = i0 + pc - 1;
-    uint4 v = make_uint4(0, 0, 0, 0);
-    if (d >= 0 && d < a.D && i >= 0 && i < a.I)
-      v = *reinterpret_cast<const uint4*>(X + ((int64_t)d * a.I + i) * kT3C + 8 * q);
-    *reinterpret_cast<uint4*>(patch + pos * kT3Stride + 8 * q) = v;
+    const int d = t.d0 + pos / kT3PCols - 1, i = t.i0 + pos % kT3PCols - 1;
+    stg[m] = make_uint4(0, 0, 0, 0);
+    if (e < kT3Chunks && d >= 0 && d < a.D && i >= 0 && i < a.I)
+      stg[m] = *reinterpret_cast<const uint4*>(X + ((int64_t)d * a.I + i) * kT3C + 8 * q);
   }
+}
+
+__device__ inline void t3_store(uint16_t* patch, const uint4* stg) {
+#pragma unroll
+  for (int m = 0; m < kT3PerLane; ++m) {
+    const int e = threadIdx.x + 256 * m;
+    if (e < kT3Chunks)
+      *reinterpret_cast<uint4*>(patch + (e >> 3) * kT3Stride + 8 * (e & 7)) = stg[m];
+  }
+}
+
+__global__ void __launch_bounds__(256) conv3x3_c64_kernel(ConvArgs a, int ntx, int nty, int ntiles) {
+  extern __shared__ __align__(16) uint16_t lds3[];
+  uint16_t* const patch = lds3;
+  uint16_t* const wl = lds3 + kT3PatchElems;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // the weights, W[co][t][ci] in memory, as wl[t][co][ci]
+  for (int e = threadIdx.x; e < 9 * kT3C * 8; e += 256) {
+    const int q = e & 7, row = e >> 3;  // row = co * 9 + t
+    const int co = row / 9, t = row % 9;
+    *reinterpret_cast<uint4*>(wl + (t * kT3C + co) * kT3Stride + 8 * q) =
+        *reinterpret_cast<const uint4*>(a.W + (int64_t)row * kT3C + 8 * q);
+  }
+  int tile = blockIdx.x;
+  if (tile >= ntiles) return;
+  uint4 stg[kT3PerLane];
+  t3_load(a, t3_tile(tile, ntx, nty), stg);
+  t3_store(patch, stg);
   __syncthreads();
-  f32x16 acc[2][2];
-  for (int r = 0; r < 2; ++r)
-    for (int c = 0; c < 2; ++c) acc[r][c] = f32x16{};
   const int kofs = 8 * (lane >> 5);
-  // weight fragments of one tap (4 k-steps x 2 column tiles), loaded a tap ahead of their use
-  // so the global-load latency overlaps the previous tap's MFMAs
-  const uint16_t* wbase[2];
-  for (int c = 0; c < 2; ++c) wbase[c] = a.W + (int64_t)(32 * c + (lane & 31)) * 9 * kT3C + kofs;
-  uint4 wn[4][2];
-#pragma unroll
-  for (int j = 0; j < 4; ++j)
-    for (int c = 0; c < 2; ++c) wn[j][c] = *reinterpret_cast<const uint4*>(wbase[c] + 16 * j);
-#pragma unroll 1
-  for (int t = 0; t < 9; ++t) {
-    uint4 wc[4][2];
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-      for (int c = 0; c < 2; ++c) wc[j][c] = wn[j][c];
-    if (t + 1 < 9) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        for (int c = 0; c < 2; ++c)
-          wn[j][c] = *reinterpret_cast<const uint4*>(wbase[c] + (t + 1) * kT3C + 16 * j);
-    }
-    const int dd = t / 3, di = t % 3;  // patch offsets of tap t (the patch starts at -1, -1)
-    const uint16_t* arow[2];
+  // this lane's output channels' biases (32 c + lane % 32), read once
+  float bias[2];
+  for (int c = 0; c < 2; ++c) bias[c] = a.bias ? a.bias[32 * c + (lane & 31)] : 0.0f;
+  for (; tile < ntiles; tile += gridDim.x) {
+    const T3Tile tt = t3_tile(tile, ntx, nty);
+    const int next = tile + gridDim.x;
+    // the next tile's patch, in flight while this one computes
+    if (next < ntiles) t3_load(a, t3_tile(next, ntx, nty), stg);
+    f32x16 acc[2][2];
     for (int r = 0; r < 2; ++r)
-      arow[r] = patch + ((wave + dd) * kT3PCols + 32 * r + (lane & 31) + di) * kT3Stride + kofs;
+      for (int c = 0; c < 2; ++c) acc[r][c] = f32x16{};
+    // 36 k-steps (9 taps x 4 steps of 16 channels), the operands of step s + 1 read from LDS
+    // while the MFMAs of step s run (one wave per SIMD: nothing else hides the LDS latency)
+    const uint16_t* abase = patch + (wave * kT3PCols + (lane & 31)) * kT3Stride + kofs;
+    const uint16_t* bbase = wl + (lane & 31) * kT3Stride + kofs;
+    auto load_step = [&](int st, Pack8* A, Pack8* Bv) {
+      const int t = st >> 2, j = st & 3;
+      const int dd = t / 3, di = t % 3;  // patch offsets of tap t (the patch starts at -1, -1)
+      for (int r = 0; r < 2; ++r)
+        A[r].u = *reinterpret_cast<const uint4*>(abase + (dd * kT3PCols + 32 * r + di) * kT3Stride +
+                                                 16 * j);
+      for (int c = 0; c < 2; ++c)
+        Bv[c].u = *reinterpret_cast<const uint4*>(bbase + (t * kT3C + 32 * c) * kT3Stride + 16 * j);
+    };
+    Pack8 A[2][2], Bv[2][2];
+    load_step(0, A[0], Bv[0]);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      Pack8 A[2], Bv[2];
-      for (int r = 0; r < 2; ++r) A[r].u = *reinterpret_cast<const uint4*>(arow[r] + 16 * j);
-      for (int c = 0; c < 2; ++c) Bv[c].u = wc[j][c];
+    for (int st = 0; st < 36; ++st) {
+      const int cur = st & 1;
+      if (st + 1 < 36) load_step(st + 1, A[cur ^ 1], Bv[cur ^ 1]);
       for (int r = 0; r < 2; ++r)
         for (int c = 0; c < 2; ++c)
-          acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[r].v, Bv[c].v, acc[r][c], 0, 0, 0);
+          acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[cur][r].v, Bv[cur][c].v,
+                                                               acc[r][c], 0, 0, 0);
     }
-  }
-  // epilogue as in conv_fwd_kernel (v = acc + bias; relu; out1 = v; out2 = res + v), through
-  // LDS: each wave stages 32 positions x 64 channels of fp32 at a time, then reads them back a
-  // channel octet per lane, so the residual reads and the output writes are 16-byte vectors
-  // covering whole positions. A residual that is the layer's own input is read from the patch
-  // (its centre rows) before the staging overwrites it.
-  const bool res_in_patch = a.res == a.X;
-  uint4 rp[2][4];
-  if (res_in_patch) {
-    for (int r = 0; r < 2; ++r)
+    const int i0 = tt.i0, d0 = tt.d0, b = tt.b;
+    // epilogue as in conv_fwd_kernel (v = acc + bias; relu; out1 = v; out2 = res + v), through
+    // LDS: each wave stages 32 positions x 64 channels of fp32 at a time, then reads them back a
+    // channel octet per lane, so the residual reads and the output writes are 16-byte vectors
+    // covering whole positions. A residual that is the layer's own input is read from the patch
+    // (its centre rows) before the staging overwrites it.
+    const bool res_in_patch = a.res == a.X;
+    uint4 rp[2][4];
+    if (res_in_patch) {
+      for (int r = 0; r < 2; ++r)
+        for (int u = 0; u < 4; ++u) {
+          const int e = lane + 64 * u, pos = e >> 3, q = e & 7;
+          rp[r][u] = *reinterpret_cast<const uint4*>(
+              patch + ((wave + 1) * kT3PCols + 32 * r + pos + 1) * kT3Stride + 8 * q);
+        }
+    }
+    __syncthreads();  // every wave is done with the patch
+    constexpr int kEpStride = kT3C + 4;  // floats per staged position
+    float* stage = reinterpret_cast<float*>(patch) + wave * 32 * kEpStride;
+    const int d = d0 + wave;
+    const int64_t row = ((int64_t)b * a.D + (d < a.D ? d : 0)) * a.I;
+    for (int r = 0; r < 2; ++r) {
+      for (int c = 0; c < 2; ++c) {
+        const int co = 32 * c + (lane & 31);
+        for (int k = 0; k < 16; ++k) {
+          float v = acc[r][c][k] + bias[c];
+          if (a.relu) v = v > 0.0f ? v : 0.0f;
+          stage[acc_row(k, lane) * kEpStride + co] = v;
+        }
+      }
+      __syncthreads();
+#pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int e = lane + 64 * u, pos = e >> 3, q = e & 7;
-        rp[r][u] = *reinterpret_cast<const uint4*>(
-            patch + ((wave + 1) * kT3PCols + 32 * r + pos + 1) * kT3Stride + 8 * q);
-      }
-  }
-  __syncthreads();  // every wave is done with the patch
-  constexpr int kEpStride = kT3C + 4;  // floats per staged position
-  float* stage = reinterpret_cast<float*>(patch) + wave * 32 * kEpStride;
-  const int d = d0 + wave;
-  const int64_t row = ((int64_t)b * a.D + (d < a.D ? d : 0)) * a.I;
-  for (int r = 0; r < 2; ++r) {
-    for (int c = 0; c < 2; ++c) {
-      const int co = 32 * c + (lane & 31);
-      const float bias = a.bias ? a.bias[co] : 0.0f;
-      for (int k = 0; k < 16; ++k) {
-        float v = acc[r][c][k] + bias;
-        if (a.relu) v = v > 0.0f ? v : 0.0f;
-        stage[acc_row(k, lane) * kEpStride + co] = v;
-      }
-    }
-    __syncthreads();
+        const int i = i0 + 32 * r + pos;
+        if (d < a.D && i < a.I) {
+          const int64_t o = (row + i) * kT3C + 8 * q;
+          const float* sv = stage + pos * kEpStride + 8 * q;
+          Pack8 res;
+          if (!a.res || !a.out2) res.u = make_uint4(0, 0, 0, 0);
+          else if (res_in_patch) res.u = rp[r][u];
+          else res.u = *reinterpret_cast<const uint4*>(a.res + o);
+          Pack8 o1, o2;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int e = lane + 64 * u, pos = e >> 3, q = e & 7;
-      const int i = i0 + 32 * r + pos;
-      if (d < a.D && i < a.I) {
-        const int64_t o = (row + i) * kT3C + 8 * q;
-        const float* sv = stage + pos * kEpStride + 8 * q;
-        Pack8 res;
-        if (!a.res || !a.out2) res.u = make_uint4(0, 0, 0, 0);
-        else if (res_in_patch) res.u = rp[r][u];
-        else res.u = *reinterpret_cast<const uint4*>(a.res + o);
-        Pack8 o1, o2;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          o1.h[k] = f2bf(sv[k]);
-          o2.h[k] = f2bf(bf2f(res.h[k]) + sv[k]);
+          for (int k = 0; k < 8; ++k) {
+            o1.h[k] = f2bf(sv[k]);
+            o2.h[k] = f2bf(bf2f(res.h[k]) + sv[k]);
+          }
+          if (a.out1) *reinterpret_cast<uint4*>(a.out1 + o) = o1.u;
+          if (a.out2) *reinterpret_cast<uint4*>(a.out2 + o) = o2.u;
         }
-        if (a.out1) *reinterpret_cast<uint4*>(a.out1 + o) = o1.u;
-        if (a.out2) *reinterpret_cast<uint4*>(a.out2 + o) = o2.u;
       }
+      __syncthreads();
     }
+    // the next tile's patch replaces this one's (the epilogue's last barrier ended every read)
+    if (next < ntiles) t3_store(patch, stg);
     __syncthreads();
   }
 }
@@ -527,6 +593,92 @@ __global__ void __launch_bounds__(256) out_fwd8_kernel(const uint16_t* Z, const 
   if (p < npos && q == 0) out[p] = s + c0[0];
 }
 
+// The output layer in two passes over memory: out[p] = c0 + sum_t Y_t[p + off_t], with
+// Y_t[q] = sum_c V[t][c] Z[q][c] the projection of each position onto the nine taps' weights.
+// Each position's 64 channels are read once (out_fwd8_kernel reads every position nine times);
+// the planes Y_t, 36 bytes per position, are summed with coalesced shifted reads.
+// out_proj_kernel: eight lanes per position, lane q holding channel octet q (a wave reads eight
+// neighbouring positions contiguously); the nine partial sums meet in DPP sums over the eight
+// lanes, and lane q writes plane q (lane 0 also plane 8). Each lane group handles kProjRounds
+// positions 32 apart, their loads issued together so that several are in flight per lane.
+// C == 64.
+constexpr int kProjRounds = 4;
+
+// A lane's value moved along its row by a DPP pattern (a VALU move, no LDS).
+template <int kCtrl>
+__device__ inline float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xF, 0xF, false));
+}
+
+// The sum over each aligned group of eight lanes, in every lane of the group: quad_perm
+// [1,0,3,2] and [2,3,0,1] sum each quad, row_half_mirror (lane i of eight from lane 7 - i) adds
+// the other quad's sum.
+__device__ inline float sum8_dpp(float v) {
+  v += dpp_f<0xB1>(v);
+  v += dpp_f<0x4E>(v);
+  v += dpp_f<0x141>(v);
+  return v;
+}
+
+__global__ void __launch_bounds__(256) out_proj_kernel(const uint16_t* Z, const float* V,
+                                                       float* Y, int64_t npos) {
+  constexpr int C = 64;
+  const int q = threadIdx.x & 7;
+  const int64_t p0 = (int64_t)blockIdx.x * (kProjRounds * 32) + (threadIdx.x >> 3);
+  // the lane's weights, V[t][8q .. 8q + 7] for the nine taps, held in registers
+  float w[9][8];
+#pragma unroll
+  for (int t = 0; t < 9; ++t)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) w[t][k] = V[t * C + 8 * q + k];
+  // every load of the lane is issued before the first is used
+  Pack8 v[kProjRounds];
+#pragma unroll
+  for (int r = 0; r < kProjRounds; ++r) {
+    const int64_t p = p0 + 32 * r;
+    v[r].u = p < npos ? *reinterpret_cast<const uint4*>(Z + p * C + 8 * q) : make_uint4(0, 0, 0, 0);
+  }
+#pragma unroll
+  for (int r = 0; r < kProjRounds; ++r) {
+    const int64_t p = p0 + 32 * r;
+    float y[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) y[t] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float z = bf2f(v[r].h[k]);
+#pragma unroll
+      for (int t = 0; t < 9; ++t) y[t] += w[t][k] * z;
+    }
+#pragma unroll
+    for (int t = 0; t < 9; ++t) y[t] = sum8_dpp(y[t]);
+    if (p < npos) {
+      float mine = y[0];
+#pragma unroll
+      for (int t = 1; t < 8; ++t) mine = q == t ? y[t] : mine;
+      Y[(int64_t)q * npos + p] = mine;
+      if (q == 0) Y[8 * npos + p] = y[8];
+    }
+  }
+}
+
+// out[p] = c0 + sum over the taps whose neighbour lies inside the [D][I] grid of Y_t[p + off_t]
+__global__ void __launch_bounds__(256) out_sum_kernel(const float* Y, const float* c0, float* out,
+                                                      int B, int D, int I) {
+  const int64_t npos = (int64_t)B * D * I;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npos) return;
+  const int i = (int)(p % I), d = (int)((p / I) % D);
+  float s = c0[0];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) {
+    const int dd = t / 3 - 1, di = t % 3 - 1;
+    if (d + dd < 0 || d + dd >= D || i + di < 0 || i + di >= I) continue;
+    s += Y[(int64_t)t * npos + p + (int64_t)dd * I + di];
+  }
+  out[p] = s;
+}
+
 // loss: masked squared error; g[p] = scale * (out - y) on valid pixels (i + d < I), 0 elsewhere;
 // sums of (out - y)^2 over valid pixels added to loss[0]
 __global__ void loss_kernel(const float* out, const float* y, float* g, float scale, float* loss,
@@ -691,11 +843,26 @@ hipError_t launch_mfma_probe(hipStream_t s, const uint16_t* A, const uint16_t* B
 
 hipError_t launch_conv_fwd(hipStream_t s, const ConvArgs& a) {
   if (a.Cin % 16 || a.Cout % 64 || a.KD < 1 || a.KI < 1 || !a.X || !a.W) return hipErrorInvalidValue;
-  if (a.KD == 3 && a.KI == 3 && a.dil <= 1 && a.Cin == kT3C && a.Cout == kT3C && a.B <= 65535 &&
-      a.D <= 65535 * kT3Rows) {
-    const dim3 grid((unsigned)((a.I + kT3Cols - 1) / kT3Cols), (unsigned)((a.D + kT3Rows - 1) / kT3Rows),
-                    (unsigned)a.B);
-    hipLaunchKernelGGL(conv3x3_c64_kernel, grid, dim3(256), 0, s, a);
+  if (a.KD == 3 && a.KI == 3 && a.dil <= 1 && a.Cin == kT3C && a.Cout == kT3C) {
+    const int64_t ntx = (a.I + kT3Cols - 1) / kT3Cols, nty = (a.D + kT3Rows - 1) / kT3Rows;
+    const int64_t ntiles = ntx * nty * a.B;
+    if (ntiles > INT32_MAX) return hipErrorInvalidValue;
+    // one persistent workgroup per CU (the weights and a patch take most of its LDS)
+    static int cus[64] = {}, lds_set[64] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= 64) return e != hipSuccess ? e : hipErrorInvalidDevice;
+    if (!lds_set[dev]) {
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(conv3x3_c64_kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, kT3Lds);
+      if (e == hipSuccess)
+        e = hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev);
+      if (e != hipSuccess) return e;
+      lds_set[dev] = 1;
+    }
+    const int grid = (int)std::min<int64_t>(ntiles, cus[dev] > 0 ? cus[dev] : 256);
+    hipLaunchKernelGGL(conv3x3_c64_kernel, dim3(grid), dim3(256), kT3Lds, s, a, (int)ntx, (int)nty,
+                       (int)ntiles);
     return hipGetLastError();
   }
   const int64_t npos = (int64_t)a.B * a.D * a.I;
@@ -760,8 +927,17 @@ hipError_t launch_pair_bwd(hipStream_t s, const uint16_t* GZ, const uint16_t* Z,
 }
 
 hipError_t launch_out_fwd(hipStream_t s, const uint16_t* Z, const float* V, const float* c0,
-                          float* out, int B, int D, int I, int C) {
+                          float* out, int B, int D, int I, int C, float* scratch) {
   if (C % 8) return hipErrorInvalidValue;
+  if (C == 64 && scratch) {
+    const int64_t npos = (int64_t)B * D * I;
+    hipLaunchKernelGGL(out_proj_kernel,
+                       dim3((unsigned)((npos + kProjRounds * 32 - 1) / (kProjRounds * 32))),
+                       dim3(256), 0, s, Z, V, scratch, npos);
+    hipLaunchKernelGGL(out_sum_kernel, dim3((unsigned)((npos + 255) / 256)), dim3(256), 0, s,
+                       scratch, c0, out, B, D, I);
+    return hipGetLastError();
+  }
   if (C == 64) {
     const int64_t npos = (int64_t)B * D * I;
     hipLaunchKernelGGL(out_fwd8_kernel, dim3((unsigned)((npos + 31) / 32)), dim3(256), 0, s, Z, V,

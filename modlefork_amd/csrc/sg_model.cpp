@@ -242,8 +242,10 @@ int forward(mgpu_sg* s, int B, bool keep_act = true) {
     hz.out2 = s->z[j + 1];
     SGCHK(s, launch_conv_fwd(st, hz));
   }
+  // the tap projections of the output layer go to the gradient buffer, which only backward
+  // (after this layer) writes: 36 bytes per position of its 2 * H
   SGCHK(s, launch_out_fwd(st, s->z[s->Lh - 1], s->p32 + s->vo.off, s->p32 + s->co.off, s->out, B,
-                          s->ND, s->Lout, s->H));
+                          s->ND, s->Lout, s->H, reinterpret_cast<float*>(s->gz)));
   return MGPU_OK;
 }
 
