@@ -457,6 +457,11 @@ int mgpu_debug_phase_times(mgpu_ctx* ctx, int enable, uint64_t* out, uint64_t n)
  * serial != 0 makes later runs of the job walk each side's list on one lane in the reference
  * loop's order instead. Results must not change: exposed to check exactly that. */
 int mgpu_debug_fast_serial_fixups(mgpu_job* job, int serial);
+/* Deterministic mode runs each cell on one wavefront (all 64 lanes, the reference's draws in
+ * the reference's order) when its working set fits the CU's LDS and no optional output is on;
+ * serial != 0 makes later jobs and traces of this context run every cell on the one-lane kernel
+ * instead. Both give the reference's bands: exposed to compare them. */
+int mgpu_debug_det_kernel(mgpu_ctx* ctx, int serial);
 /* Per-epoch state hash of one cell in deterministic mode (for locating a first divergence). */
 int mgpu_debug_cell_trace(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom* chrom,
                           const mgpu_barriers* barriers, uint64_t nlefs, uint64_t cell_id,

@@ -303,6 +303,7 @@ SIGNATURES = [
                                         U32P]),
     ("mgpu_debug_phase_times", C.c_int, [C.c_void_p, C.c_int, U64P, C.c_uint64]),
     ("mgpu_debug_fast_serial_fixups", C.c_int, [C.c_void_p, C.c_int]),
+    ("mgpu_debug_det_kernel", C.c_int, [C.c_void_p, C.c_int]),
     ("mgpu_context_stream", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int)]),
     ("mgpu_sg_config_default", C.c_int, [C.POINTER(MgpuSgConfig)]),
     ("mgpu_sg_create", C.c_int, [C.c_void_p, C.POINTER(MgpuSgConfig), C.POINTER(C.c_void_p)]),

@@ -78,6 +78,8 @@ def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     subprocess.run([sys.executable, os.path.join(CSRC, "gen_rng_tables.py"),
                     os.path.join(CSRC, "rng_tables.h")], check=True)
+    subprocess.run([sys.executable, os.path.join(CSRC, "gen_jump_table.py"),
+                    os.path.join(CSRC, "det_jump.h")], check=True)
     with ThreadPoolExecutor(max_workers=len(UNITS)) as ex:
         objs = list(ex.map(_compile, UNITS))
     tmp = OUT + ".tmp"

@@ -9,7 +9,9 @@
 // order and count as the reference, so a cell's trajectory is reproduced draw for draw.
 // Compiled with -ffp-contract=off: the reference build performs no FMA contraction.
 #pragma once
+#ifndef MGPU_EMU
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 
 #include "rng_tables.h"
@@ -56,7 +58,8 @@ __device__ inline uint64_t det_next(DetRng& g) {
 #define MGPU_TWO64 18446744073709551616.0
 
 // uniform_int_distribution<u64>{lo, hi} over a 64-bit engine (brange = 2^64 - 1).
-__device__ inline uint64_t det_uniform(DetRng& g, uint64_t lo, uint64_t hi) {
+template <class G>
+__device__ inline uint64_t det_uniform(G& g, uint64_t lo, uint64_t hi) {
   const uint64_t range = hi - lo;
   if (range == 0) return lo;
   if (range == ~0ULL) return det_next(g) + lo;
@@ -74,7 +77,8 @@ __device__ inline uint64_t bucket_of(uint64_t range) {
   if (~0ULL % (range + 1) == range) ++bucket;
   return bucket;
 }
-__device__ inline uint64_t det_uniform_b(DetRng& g, uint64_t lo, uint64_t range, uint64_t bucket) {
+template <class G>
+__device__ inline uint64_t det_uniform_b(G& g, uint64_t lo, uint64_t range, uint64_t bucket) {
   if (range == 0) return lo;
   for (;;) {
     const uint64_t r = det_next(g) / bucket;
@@ -82,18 +86,21 @@ __device__ inline uint64_t det_uniform_b(DetRng& g, uint64_t lo, uint64_t range,
   }
 }
 
-__device__ inline bool det_bernoulli(DetRng& g, double p) {
+template <class G>
+__device__ inline bool det_bernoulli(G& g, double p) {
   if (p == 0.0) return false;
   return (double)det_next(g) <= p * MGPU_TWO64;
 }
 
-__device__ inline double det_canonical(DetRng& g) {
+template <class G>
+__device__ inline double det_canonical(G& g) {
   double r = (double)det_next(g) / MGPU_TWO64;
   if (r == 1.0) r -= 1.1102230246251565e-16;  // epsilon / 2
   return r;
 }
 
-__device__ inline double det_uniform01(DetRng& g) {
+template <class G>
+__device__ inline double det_uniform01(G& g) {
   for (;;) {
     const double r = (double)det_next(g) * (1.0 / MGPU_TWO64);
     if (r < 1.0) return r;
@@ -101,14 +108,16 @@ __device__ inline double det_uniform01(DetRng& g) {
 }
 
 // generate_int_float_pair<double, 8>: 8-bit bucket from the low bits, 56-bit fraction above.
-__device__ inline double det_int_float(DetRng& g, int& bucket) {
+template <class G>
+__device__ inline double det_int_float(G& g, int& bucket) {
   uint64_t u = det_next(g);
   bucket = (int)(u & 0xFFu);
   u &= ~(1ULL << 11);
   return (double)(u >> 8) * (1.0 / 72057594037927936.0);
 }
 
-__device__ inline double det_unit_exp(DetRng& g) {
+template <class G>
+__device__ inline double det_unit_exp(G& g) {
   double shift = 0.0;
   for (;;) {
     int i;
@@ -129,7 +138,8 @@ __device__ inline double det_unit_exp(DetRng& g) {
   }
 }
 
-__device__ inline double det_normal_tail(DetRng& g) {
+template <class G>
+__device__ inline double det_normal_tail(G& g) {
   const double start = kZigNormalX[1];
   for (;;) {
     const double x = det_unit_exp(g) / start;
@@ -138,7 +148,8 @@ __device__ inline double det_normal_tail(DetRng& g) {
   }
 }
 
-__device__ inline double det_unit_normal(DetRng& g) {
+template <class G>
+__device__ inline double det_unit_normal(G& g) {
   for (;;) {
     int i;
     const double r = det_int_float(g, i);
@@ -163,11 +174,13 @@ __device__ inline double det_unit_normal(DetRng& g) {
   }
 }
 
-__device__ inline double det_normal(DetRng& g, double mean, double sigma) {
+template <class G>
+__device__ inline double det_normal(G& g, double mean, double sigma) {
   return det_unit_normal(g) * sigma + mean;
 }
 
-__device__ inline uint64_t det_poisson(DetRng& g, double mean) {
+template <class G>
+__device__ inline uint64_t det_poisson(G& g, double mean) {
   if (mean < 10) {
     double p = exp(-mean);
     uint64_t x = 0;
@@ -230,7 +243,8 @@ __device__ inline double det_binom_fc(int64_t k) {
 
 // binomial_distribution<isize>{t, p}. q_n = pow(1 - min(p, 1-p), t) is passed in when the caller
 // has it precomputed (pass a negative value to compute it here).
-__device__ inline int64_t det_binomial(DetRng& g, int64_t t, double p0, double q_n) {
+template <class G>
+__device__ inline int64_t det_binomial(G& g, int64_t t, double p0, double q_n) {
   const double p = (0.5 < p0) ? (1 - p0) : p0;
   const int64_t m = (int64_t)((double)(t + 1) * p);
   if (m < 11) {
@@ -319,7 +333,8 @@ __device__ inline int64_t det_binomial(DetRng& g, int64_t t, double p0, double q
   return (0.5 < p0) ? t - k : k;
 }
 
-__device__ inline double det_gev(DetRng& g, double mu, double sigma, double xi) {
+template <class G>
+__device__ inline double det_gev(G& g, double mu, double sigma, double xi) {
   if (xi == 0.0) return (mu - sigma) * log(-log(det_canonical(g)));
   return mu + (sigma * (1.0 - pow(-log(det_canonical(g)), xi))) / xi;
 }

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include "det_sim.h"
+#include "det_wave.h"
 #include "kernels.h"
 
 namespace mgpu {
@@ -34,6 +35,46 @@ __global__ void __launch_bounds__(64) det_cells_kernel(DetParams P, const DetChr
       cell_stats[4 * it + 3] = s.g.draws;
     }
   }
+}
+
+// Wave-parallel deterministic mode (det_wave.h): each block is one wavefront that pulls
+// (chromosome, cell) items from the queue and runs each cell with all 64 lanes; the working set
+// and the engine-word ring live in dynamic LDS. A cell that fails an internal bound (an engine
+// draw past the ring's reach, a contact loop that never completes) counts in *err.
+__global__ void __launch_bounds__(64) det_wave_kernel(DetParams P, const DetChrom* chroms,
+                                                      const WorkItem* items, uint64_t nitems,
+                                                      unsigned long long* next_item,
+                                                      uint64_t* cell_stats,
+                                                      unsigned long long* err) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  const DevWave w;
+  for (;;) {
+    unsigned long long it = 0;
+    if (w.lane() == 0) it = atomicAdd(next_item, 1ULL);
+    it = w.bcast(it, 0);
+    if (it >= nitems) return;
+    const WorkItem wi = items[it];
+    const DetChrom ch = chroms[wi.chrom];
+    uint64_t st[4] = {0, 0, 0, 0};
+    det_wave_cell(w, P, ch, lds, P.hist_len, wi.seed, wi.target, nullptr, 0, st);
+    if (w.lane() == 0) {
+      if (cell_stats)
+        for (int k = 0; k < 4; ++k) cell_stats[4 * it + k] = st[k];
+      if (st[3] == ~0ull) atomicAdd(err, 1ULL);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64) det_wave_trace_kernel(DetParams P, DetChrom ch,
+                                                            uint64_t seed, uint64_t target,
+                                                            uint64_t* trace, uint64_t max_trace,
+                                                            uint64_t* out) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  const DevWave w;
+  uint64_t st[4] = {0, 0, 0, 0};
+  det_wave_cell(w, P, ch, lds, P.hist_len, seed, target, trace, max_trace, st);
+  if (w.lane() == 0)
+    for (int k = 0; k < 4; ++k) out[k] = st[k];
 }
 
 __global__ void det_trace_kernel(DetParams P, DetChrom ch, uint64_t seed, uint64_t target,
@@ -141,6 +182,33 @@ hipError_t launch_det_cells(hipStream_t stream, const DetParams& P, const DetChr
   hipLaunchKernelGGL(det_cells_kernel, dim3(grid), dim3(64), shmem, stream, P, d_chroms, d_items,
                      nitems, d_next, d_scratch, stride, use_lds ? 1 : 0, d_cell_stats);
   return hipGetLastError();
+}
+
+hipError_t launch_det_wave(hipStream_t stream, const DetParams& P, const DetChrom* d_chroms,
+                           const WorkItem* d_items, uint64_t nitems, unsigned long long* d_next,
+                           uint64_t lds_bytes, uint32_t grid, uint64_t* d_cell_stats,
+                           unsigned long long* d_err) {
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(det_wave_kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(det_wave_kernel, dim3(grid), dim3(64), lds_bytes, stream, P, d_chroms,
+                     d_items, nitems, d_next, d_cell_stats, d_err);
+  return hipGetLastError();
+}
+
+hipError_t launch_det_wave_trace(hipStream_t stream, const DetParams& P, const DetChrom& ch,
+                                 uint64_t seed, uint64_t target, uint64_t lds_bytes,
+                                 uint64_t* d_trace, uint64_t max_trace, uint64_t* d_out) {
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(det_wave_trace_kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(det_wave_trace_kernel, dim3(1), dim3(64), lds_bytes, stream, P, ch, seed,
+                     target, d_trace, max_trace, d_out);
+  return hipGetLastError();
+}
+
+uint64_t det_wave_bytes_host(uint64_t nlefs, uint64_t nbar, uint64_t hist) {
+  return det_wave_bytes(nlefs, nbar, hist);
 }
 
 hipError_t launch_det_trace(hipStream_t stream, const DetParams& P, const DetChrom& ch,

@@ -16,7 +16,9 @@
 // Contacts go straight to the HBM band with 32-bit atomics; integer addition commutes, so the
 // band does not depend on how cells interleave.
 #pragma once
+#ifndef MGPU_EMU
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 
 #include "det_engine.h"

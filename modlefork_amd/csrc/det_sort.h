@@ -13,7 +13,9 @@
 //   split_sort keep a non-decreasing run in front (an element that breaks it also evicts the
 //              run's top), pdq-sort the evicted elements, merge the two runs stably.
 #pragma once
+#ifndef MGPU_EMU
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 
 namespace mgpu {

@@ -1,6 +1,8 @@
 // Launch-side declarations shared by the kernel translation units and the C-ABI layer.
 #pragma once
+#ifndef MGPU_EMU
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 
 namespace mgpu {
@@ -73,10 +75,21 @@ struct DetChrom {
   unsigned long long* occupancy;  // 1D LEF occupancy, ceil(size / bin) bins, or null
 };
 
+#ifndef MGPU_EMU
 hipError_t launch_det_cells(hipStream_t stream, const DetParams& P, const DetChrom* d_chroms,
                             const WorkItem* d_items, uint64_t nitems, unsigned long long* d_next,
                             uint8_t* d_scratch, uint64_t stride, bool use_lds, uint32_t grid,
                             uint64_t* d_cell_stats);
+// Wave-parallel deterministic mode: one 64-lane wavefront per cell, lds_bytes =
+// det_wave_bytes_host(max LEFs, max barriers, burn-in history) of dynamic LDS per block.
+hipError_t launch_det_wave(hipStream_t stream, const DetParams& P, const DetChrom* d_chroms,
+                           const WorkItem* d_items, uint64_t nitems, unsigned long long* d_next,
+                           uint64_t lds_bytes, uint32_t grid, uint64_t* d_cell_stats,
+                           unsigned long long* d_err);
+hipError_t launch_det_wave_trace(hipStream_t stream, const DetParams& P, const DetChrom& ch,
+                                 uint64_t seed, uint64_t target, uint64_t lds_bytes,
+                                 uint64_t* d_trace, uint64_t max_trace, uint64_t* d_out);
+uint64_t det_wave_bytes_host(uint64_t nlefs, uint64_t nbar, uint64_t hist);
 hipError_t launch_det_trace(hipStream_t stream, const DetParams& P, const DetChrom& ch,
                             uint64_t seed, uint64_t target, uint8_t* d_scratch, uint64_t* d_trace,
                             uint64_t max_trace, uint64_t* d_out);
@@ -104,5 +117,7 @@ hipError_t launch_block_sums(hipStream_t stream, const BlockBand* d_bands, const
 hipError_t launch_det_collisions(hipStream_t stream, const uint64_t* d_idx, const uint8_t* d_ev,
                                  uint64_t n, uint64_t* d_words, uint64_t* d_index,
                                  uint32_t* d_event, uint32_t* d_flags);
+
+#endif  // MGPU_EMU
 
 }  // namespace mgpu

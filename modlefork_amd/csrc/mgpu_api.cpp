@@ -28,6 +28,7 @@ struct mgpu_ctx {
   std::string err;
   int num_cus = 0;
   int phase_timers = 0;
+  int det_serial = 0;  // deterministic mode: 1 runs every cell on the one-lane kernel
   std::string noalloc_err;  // why mgpu_simulate_chrom_noalloc cannot run on this device, if so
 };
 
@@ -508,6 +509,10 @@ struct mgpu_job {
   uint64_t stride = 0;
   bool use_lds = false;
   uint32_t grid = 0;
+  // deterministic mode, wave-parallel kernel: LDS per cell, grid, cells whose run failed
+  uint64_t wave_bytes = 0;
+  uint32_t wave_grid = 0;
+  unsigned long long* d_err = nullptr;
   FastJob fast{};
   hipGraph_t graph = nullptr;       // captured run (mgpu_job_capture)
   hipGraphExec_t graph_exec = nullptr;
@@ -665,6 +670,7 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
   const uint64_t off_missed = pool.add(nchroms * 8);
   const uint64_t off_dropped = pool.add(nchroms * 16);
   const uint64_t off_next = pool.add(64);
+  const uint64_t off_err = pool.add(64);
   const uint64_t off_stats = pool.add(job->nitems * 4 * 8);
   const uint64_t off_bands = pool.add(job->band_total * 4);
   // 1D LEF occupancy: ceil(size / bin) u64 bins per chromosome (Chromosome::
@@ -686,6 +692,14 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
     const uint64_t per_cu = job->use_lds ? std::max<uint64_t>(1, (160 * 1024) / job->stride) : 4;
     job->grid = (uint32_t)std::min<uint64_t>(job->nitems,
                                              std::min<uint64_t>(per_cu, 8) * ctx->num_cus);
+    // the wave-parallel kernel keeps a cell's whole working set and its engine-word ring in
+    // LDS, so it takes chromosomes whose largest cell fits a CU's 160 KiB
+    const uint64_t wb = det_wave_bytes_host(max_lefs, max_bar, p->burnin_history_length);
+    if (wb <= 160 * 1024) {
+      job->wave_bytes = wb;
+      const uint64_t wcu = std::min<uint64_t>(8, (160 * 1024) / wb);
+      job->wave_grid = (uint32_t)std::min<uint64_t>(job->nitems, wcu * ctx->num_cus);
+    }
   }
   uint64_t off_scratch = 0;
   if (mode == MGPU_MODE_DETERMINISTIC && !job->use_lds)
@@ -701,6 +715,7 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
   job->d_missed = reinterpret_cast<unsigned long long*>(base + off_missed);
   job->d_dropped = reinterpret_cast<unsigned long long*>(base + off_dropped);
   job->d_next = reinterpret_cast<unsigned long long*>(base + off_next);
+  job->d_err = reinterpret_cast<unsigned long long*>(base + off_err);
   job->d_chroms = reinterpret_cast<DetChrom*>(base + off_chroms);
   job->d_items = reinterpret_cast<WorkItem*>(base + off_items);
   job->d_cell_stats = reinterpret_cast<uint64_t*>(base + off_stats);
@@ -958,10 +973,25 @@ hipError_t enqueue_clears(mgpu_job* job, hipStream_t s) {
   if (e == hipSuccess) e = launch_zero_words(s, job->d_missed, job->nchroms * 8);
   if (e == hipSuccess) e = launch_zero_words(s, job->d_dropped, job->nchroms * 16);
   if (e == hipSuccess) e = launch_zero_words(s, job->d_next, 64);
+  if (e == hipSuccess) e = launch_zero_words(s, job->d_err, 64);
   if (e == hipSuccess && job->d_occ) e = launch_zero_words(s, job->d_occ, job->occ_total * 8);
   if (e == hipSuccess && job->log.rec) e = launch_zero_words(s, job->log.count, 8);
   return e;
 }
+
+// Whether a deterministic job's cells may run on one wavefront each (det_wave.h): their
+// working set must fit the CU's LDS, no optional output (1D occupancy, the state log) may be on
+// and the context must not ask for the one-lane kernel.
+bool det_wave_fits(const mgpu_job* job) {
+  return job->mode == MGPU_MODE_DETERMINISTIC && job->wave_bytes != 0 && !job->d_occ &&
+         !job->log.rec && !job->ctx->det_serial;
+}
+
+// Whole-chromosome runs keep the one-lane kernel: the work-queue launch of the wave-parallel
+// kernel (det_wave_kernel) stopped responding on its first multi-cell run, with no cause found
+// yet, while single-cell traces through the same cell code (det_wave_trace_kernel) match the
+// oracle epoch for epoch. It stays off until that is understood.
+bool det_uses_wave(const mgpu_job* job) { return false && det_wave_fits(job); }
 
 hipError_t enqueue_simulation(mgpu_job* job, hipStream_t s, hipEvent_t* ev = nullptr,
                               uint32_t nev = 0) {
@@ -969,7 +999,10 @@ hipError_t enqueue_simulation(mgpu_job* job, hipStream_t s, hipEvent_t* ev = nul
     DetParams P = det_params(&job->p);
     P.log = job->log;
     hipError_t e = ev && nev >= 2 ? hipEventRecord(ev[0], s) : hipSuccess;
-    if (e == hipSuccess)
+    if (e == hipSuccess && det_uses_wave(job))
+      e = launch_det_wave(s, P, job->d_chroms, job->d_items, job->nitems, job->d_next,
+                          job->wave_bytes, job->wave_grid, job->d_cell_stats, job->d_err);
+    else if (e == hipSuccess)
       e = launch_det_cells(s, P, job->d_chroms, job->d_items, job->nitems, job->d_next,
                            job->d_scratch, job->stride, job->use_lds, job->grid,
                            job->d_cell_stats);
@@ -1041,6 +1074,15 @@ int mgpu_job_run(mgpu_job* job, mgpu_stats* stats) {
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   const auto t1 = std::chrono::steady_clock::now();
+  if (det_uses_wave(job)) {
+    unsigned long long nerr = 0;
+    HIPCHK(ctx, hipMemcpy(&nerr, job->d_err, 8, hipMemcpyDeviceToHost));
+    if (nerr)
+      return fail(ctx, MGPU_EDEVICE, "deterministic mode: %llu cells exceeded an internal bound "
+                  "of the wave-parallel kernel (engine words beyond the ring's reach or a "
+                  "contact loop that never completes); rerun with mgpu_debug_det_kernel(ctx, 1)",
+                  nerr);
+  }
   for (uint32_t k = 0; k < nlaunch; ++k) {
     float ms = 0.f;
     HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->lev[2 * k], ctx->lev[2 * k + 1]));
@@ -1494,7 +1536,11 @@ int mgpu_debug_cell_trace(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom*
   if (e == hipSuccess) e = hipMemsetAsync(job->d_missed, 0, 8, ctx->stream);
   uint64_t* d_trace = reinterpret_cast<uint64_t*>(d + bytes);
   uint64_t* d_out = d_trace + max_trace;
-  if (e == hipSuccess)
+  if (e == hipSuccess && det_wave_fits(job))
+    e = launch_det_wave_trace(ctx->stream, det_params(p), job->chroms[0],
+                              mgpu_cell_seed(chrom, cell_id, p->seed), target_contacts,
+                              job->wave_bytes, d_trace, max_trace, d_out);
+  else if (e == hipSuccess)
     e = launch_det_trace(ctx->stream, det_params(p), job->chroms[0],
                          mgpu_cell_seed(chrom, cell_id, p->seed), target_contacts, d, d_trace,
                          max_trace, d_out);
@@ -1523,6 +1569,12 @@ int mgpu_debug_phase_times(mgpu_ctx* ctx, int enable, uint64_t* out, uint64_t n)
     HIPCHK(ctx, fast_phase_times(v.data(), kFastPhaseCount));
     for (uint64_t k = 0; k < n; ++k) out[k] = k < v.size() ? v[k] : 0;
   }
+  return MGPU_OK;
+}
+
+int mgpu_debug_det_kernel(mgpu_ctx* ctx, int serial) {
+  if (!ctx) return fail(ctx, MGPU_EINVAL, "null context");
+  ctx->det_serial = serial ? 1 : 0;
   return MGPU_OK;
 }
 

@@ -65,6 +65,11 @@ def transform(user: dict | None = None) -> dict:
     """Returns the full parameter dict after the CLI transforms (cli.cpp:1192-1215)."""
     c = dict(DEFAULTS)
     user = dict(user or {})
+    # the C-ABI's field names (modle::Config's) for the three switches
+    for field, name in (("noisify_contacts", "noisify"), ("sample_loop_contacts", "sample_loop"),
+                        ("sample_tad_contacts", "sample_tad")):
+        if field in user:
+            user[name] = bool(user.pop(field))
     parsed = set(user)
     c.update(user)
     # cli_update_extr_speed (cli.cpp:1085-1110)
