@@ -37,31 +37,30 @@ __global__ void __launch_bounds__(64) det_cells_kernel(DetParams P, const DetChr
   }
 }
 
-// Wave-parallel deterministic mode (det_wave.h): each block is one wavefront that pulls
-// (chromosome, cell) items from the queue and runs each cell with all 64 lanes; the working set
-// and the engine-word ring live in dynamic LDS. A cell that fails an internal bound (an engine
-// draw past the ring's reach, a contact loop that never completes) counts in *err.
+// Wave-parallel deterministic mode (det_wave.h): block b is one wavefront that simulates work
+// item b with all 64 lanes; the working set and the engine-word ring live in dynamic LDS, and the
+// hardware dispatcher starts the next block wherever one finishes. There is deliberately no
+// persistent loop over a work queue: with one, the optimizer merged the lane-0 region that
+// stores a finished cell's statistics with the next iteration's lane-0 queue fetch, and the
+// compiled loop sent lanes 1-63 back into the cell without lane 0 (the item index they then
+// broadcast from lane 0 was stale). A cell that fails an internal bound (an engine draw past
+// the ring's reach, a contact loop that never completes) counts in *err.
 __global__ void __launch_bounds__(64) det_wave_kernel(DetParams P, const DetChrom* chroms,
                                                       const WorkItem* items, uint64_t nitems,
-                                                      unsigned long long* next_item,
                                                       uint64_t* cell_stats,
                                                       unsigned long long* err) {
   extern __shared__ __align__(16) uint8_t lds[];
   const DevWave w;
-  for (;;) {
-    unsigned long long it = 0;
-    if (w.lane() == 0) it = atomicAdd(next_item, 1ULL);
-    it = w.bcast(it, 0);
-    if (it >= nitems) return;
-    const WorkItem wi = items[it];
-    const DetChrom ch = chroms[wi.chrom];
-    uint64_t st[4] = {0, 0, 0, 0};
-    det_wave_cell(w, P, ch, lds, P.hist_len, wi.seed, wi.target, nullptr, 0, st);
-    if (w.lane() == 0) {
-      if (cell_stats)
-        for (int k = 0; k < 4; ++k) cell_stats[4 * it + k] = st[k];
-      if (st[3] == ~0ull) atomicAdd(err, 1ULL);
-    }
+  const uint64_t it = blockIdx.x;
+  if (it >= nitems) return;
+  const WorkItem wi = items[it];
+  const DetChrom ch = chroms[wi.chrom];
+  uint64_t st[4] = {0, 0, 0, 0};
+  det_wave_cell(w, P, ch, lds, P.hist_len, wi.seed, wi.target, nullptr, 0, st);
+  if (w.lane() == 0) {
+    if (cell_stats)
+      for (int k = 0; k < 4; ++k) cell_stats[4 * it + k] = st[k];
+    if (st[3] == ~0ull) atomicAdd(err, 1ULL);
   }
 }
 
@@ -185,14 +184,15 @@ hipError_t launch_det_cells(hipStream_t stream, const DetParams& P, const DetChr
 }
 
 hipError_t launch_det_wave(hipStream_t stream, const DetParams& P, const DetChrom* d_chroms,
-                           const WorkItem* d_items, uint64_t nitems, unsigned long long* d_next,
-                           uint64_t lds_bytes, uint32_t grid, uint64_t* d_cell_stats,
-                           unsigned long long* d_err) {
+                           const WorkItem* d_items, uint64_t nitems, uint64_t lds_bytes,
+                           uint64_t* d_cell_stats, unsigned long long* d_err) {
+  if (nitems == 0) return hipSuccess;
+  if (nitems > 0x7FFFFFFFull) return hipErrorInvalidValue;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(det_wave_kernel),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(det_wave_kernel, dim3(grid), dim3(64), lds_bytes, stream, P, d_chroms,
-                     d_items, nitems, d_next, d_cell_stats, d_err);
+  hipLaunchKernelGGL(det_wave_kernel, dim3((uint32_t)nitems), dim3(64), lds_bytes, stream, P,
+                     d_chroms, d_items, nitems, d_cell_stats, d_err);
   return hipGetLastError();
 }
 

@@ -80,12 +80,11 @@ hipError_t launch_det_cells(hipStream_t stream, const DetParams& P, const DetChr
                             const WorkItem* d_items, uint64_t nitems, unsigned long long* d_next,
                             uint8_t* d_scratch, uint64_t stride, bool use_lds, uint32_t grid,
                             uint64_t* d_cell_stats);
-// Wave-parallel deterministic mode: one 64-lane wavefront per cell, lds_bytes =
+// Wave-parallel deterministic mode: one 64-lane block per work item, lds_bytes =
 // det_wave_bytes_host(max LEFs, max barriers, burn-in history) of dynamic LDS per block.
 hipError_t launch_det_wave(hipStream_t stream, const DetParams& P, const DetChrom* d_chroms,
-                           const WorkItem* d_items, uint64_t nitems, unsigned long long* d_next,
-                           uint64_t lds_bytes, uint32_t grid, uint64_t* d_cell_stats,
-                           unsigned long long* d_err);
+                           const WorkItem* d_items, uint64_t nitems, uint64_t lds_bytes,
+                           uint64_t* d_cell_stats, unsigned long long* d_err);
 hipError_t launch_det_wave_trace(hipStream_t stream, const DetParams& P, const DetChrom& ch,
                                  uint64_t seed, uint64_t target, uint64_t lds_bytes,
                                  uint64_t* d_trace, uint64_t max_trace, uint64_t* d_out);

@@ -509,9 +509,8 @@ struct mgpu_job {
   uint64_t stride = 0;
   bool use_lds = false;
   uint32_t grid = 0;
-  // deterministic mode, wave-parallel kernel: LDS per cell, grid, cells whose run failed
+  // deterministic mode, wave-parallel kernel: LDS per cell, cells whose run failed
   uint64_t wave_bytes = 0;
-  uint32_t wave_grid = 0;
   unsigned long long* d_err = nullptr;
   FastJob fast{};
   hipGraph_t graph = nullptr;       // captured run (mgpu_job_capture)
@@ -695,11 +694,7 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
     // the wave-parallel kernel keeps a cell's whole working set and its engine-word ring in
     // LDS, so it takes chromosomes whose largest cell fits a CU's 160 KiB
     const uint64_t wb = det_wave_bytes_host(max_lefs, max_bar, p->burnin_history_length);
-    if (wb <= 160 * 1024) {
-      job->wave_bytes = wb;
-      const uint64_t wcu = std::min<uint64_t>(8, (160 * 1024) / wb);
-      job->wave_grid = (uint32_t)std::min<uint64_t>(job->nitems, wcu * ctx->num_cus);
-    }
+    if (wb <= 160 * 1024) job->wave_bytes = wb;
   }
   uint64_t off_scratch = 0;
   if (mode == MGPU_MODE_DETERMINISTIC && !job->use_lds)
@@ -987,11 +982,7 @@ bool det_wave_fits(const mgpu_job* job) {
          !job->log.rec && !job->ctx->det_serial;
 }
 
-// Whole-chromosome runs keep the one-lane kernel: the work-queue launch of the wave-parallel
-// kernel (det_wave_kernel) stopped responding on its first multi-cell run, with no cause found
-// yet, while single-cell traces through the same cell code (det_wave_trace_kernel) match the
-// oracle epoch for epoch. It stays off until that is understood.
-bool det_uses_wave(const mgpu_job* job) { return false && det_wave_fits(job); }
+bool det_uses_wave(const mgpu_job* job) { return det_wave_fits(job); }
 
 hipError_t enqueue_simulation(mgpu_job* job, hipStream_t s, hipEvent_t* ev = nullptr,
                               uint32_t nev = 0) {
@@ -1000,8 +991,8 @@ hipError_t enqueue_simulation(mgpu_job* job, hipStream_t s, hipEvent_t* ev = nul
     P.log = job->log;
     hipError_t e = ev && nev >= 2 ? hipEventRecord(ev[0], s) : hipSuccess;
     if (e == hipSuccess && det_uses_wave(job))
-      e = launch_det_wave(s, P, job->d_chroms, job->d_items, job->nitems, job->d_next,
-                          job->wave_bytes, job->wave_grid, job->d_cell_stats, job->d_err);
+      e = launch_det_wave(s, P, job->d_chroms, job->d_items, job->nitems, job->wave_bytes,
+                          job->d_cell_stats, job->d_err);
     else if (e == hipSuccess)
       e = launch_det_cells(s, P, job->d_chroms, job->d_items, job->nitems, job->d_next,
                            job->d_scratch, job->stride, job->use_lds, job->grid,

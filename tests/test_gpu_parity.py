@@ -150,9 +150,6 @@ def test_tiny_chromosome_cell_traces_match(ctx, det_kernel):
 
 
 def test_tiny_chromosome_band_bit_exact(ctx, det_kernel):
-    if det_kernel == "wave":
-        pytest.skip("whole-chromosome runs of the wave-parallel kernel are switched off: its "
-                    "work-queue launch stopped responding on the GPU, cause not yet found")
     wl = small_setup()
     band, missed, st, _ = oracle_chrom(wl)
     say(f"tiny: device ({det_kernel} kernel)")
@@ -176,8 +173,6 @@ VARIANTS = {
 }
 
 
-@pytest.mark.skip(reason="runs the wave-parallel kernel's work-queue launch, switched off "
-                         "after it stopped responding on the GPU (cause not yet found)")
 @pytest.mark.parametrize("variant", list(VARIANTS))
 def test_parameter_variants_band_bit_exact(ctx, variant):
     """The wave-parallel kernel's rare paths (redrawn samples, serial rank orders and move walks,
@@ -195,10 +190,10 @@ def test_parameter_variants_band_bit_exact(ctx, variant):
     assert np.array_equal(dband, band), int(np.count_nonzero(dband != band))
 
 
-def test_small_config_band_bit_exact(ctx):
+def test_small_config_band_bit_exact(ctx, det_kernel):
     """20 Mbp, 5 kb bins, 3 Mbp band, 400 LEFs, 200 barriers, 64 cells, seed 0."""
     wl = W.small()
-    say("small: oracle (64 cells)")
+    say(f"small: oracle (64 cells, {det_kernel} kernel)")
     band, missed, st, _ = oracle_chrom(wl)
     say("small: device")
     dband, dmissed, stats = device_chrom(ctx, wl)
