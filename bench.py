@@ -366,8 +366,9 @@ def main():
     ap.add_argument("--autotune-cells", type=int, default=0,
                     help="cells the autotuner times each workgroup size on (0: an eighth)")
     ap.add_argument("--phases", action="store_true",
-                    help="fast mode: after the timed steps, run one more step with the epoch-phase "
-                         "timers on and print each phase's share of the time to stderr")
+                    help="after the timed steps, run one more step with the epoch-phase timers on "
+                         "and print each phase's share of the time to stderr (fast mode, or the "
+                         "wave-parallel deterministic kernel)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (reference for HBM counters)")
     args = ap.parse_args()
@@ -432,14 +433,17 @@ def main():
         log(f"step {i}: kernel {stats.kernel_ms:.1f} ms, {stats.epochs} epochs")
     wall = time.perf_counter() - t0
     ls = job.launch_stats()  # per-launch times of the last timed run
-    if args.phases and mode == L.MODE_FAST:
+    if args.phases:
+        det = mode == L.MODE_DETERMINISTIC
         sim.ctx.phase_times(True)
         job.run()
-        ph = sim.ctx.phase_times(False).astype(np.float64)
-        log(f"epoch phases (share of cell time, {shape} threads per cell):")
+        ph = sim.ctx.phase_times(False, deterministic=det).astype(np.float64)
+        names = L.DET_PHASE_NAMES if det else L.PHASE_NAMES
+        log(f"epoch phases (share of cell time, "
+            f"{'one wavefront' if det else f'{shape} threads'} per cell):")
         for k, v in enumerate(ph):
             if v:
-                log(f"  {L.PHASE_NAMES[k]:28s} {100.0 * v / ph.sum():5.1f} %")
+                log(f"  {names[k]:28s} {100.0 * v / ph.sum():5.1f} %  ({v / 1e8:.3f} s summed)")
     ms_per_step = 1000.0 * wall / args.steps
     cells = int(stats.cells)
     contacts = int(stats.contacts)
@@ -539,7 +543,7 @@ def main():
         "kernel_ms": float(np.mean(kernel_ms)),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "fast_cells_kernel" if mode == L.MODE_FAST else "det_cells_kernel",
+                     "kernel": job.kernel(),
                      "ms": span_ms, "algorithmic_bytes": all_bytes,
                      "launches": per_launch},
         "lds": lds,

@@ -336,6 +336,11 @@ typedef struct mgpu_launch_stats {
   double ms;         /* device time between HIP events around the launch */
 } mgpu_launch_stats;
 int mgpu_job_launch_stats(mgpu_job* job, mgpu_launch_stats* out, uint32_t max, uint32_t* n);
+/* Name of the simulation kernel the job's runs launch (as rocprof lists it, without template
+ * arguments): "fast_cells_kernel", "det_wave_kernel" (deterministic mode, one wavefront per
+ * cell) or "det_cells_kernel" (deterministic mode on one lane per cell: cells too large for
+ * the LDS, optional outputs on, or mgpu_debug_det_kernel). No counterpart in the reference. */
+int mgpu_job_kernel(mgpu_job* job, char* name, uint64_t cap);
 
 /* ---- optional outputs ----
  * 1D LEF occupancy (Chromosome::lef_1d_occupancy, genome.cpp:236-245, filled by
@@ -450,7 +455,11 @@ int mgpu_debug_fast_math(mgpu_ctx* ctx, int fn, const double* x, uint64_t n, dou
  * and burn-in, 1 bind, 2 rank (both sides; the rev side alone in epochs that bind more than
  * 1024 LEFs), 3 rank fwd in those epochs, 4 contacts, 5 moves, 6 adjust, 7 barrier states
  * and chromosome ends, 8 LEF-BAR, 9 primary LEF-LEF, 10 move corrections, 11 secondary scans
- * of both sides, 13 bypass fix-ups of both sides, 14 extrude and release (12 and 15 unused). */
+ * of both sides, 13 bypass fix-ups of both sides, 14 extrude and release (12 and 15 unused).
+ * Entries 16-27 (when n > 16) are deterministic mode's wave-parallel kernel: 16 burn-in, 17 bind,
+ * 18 rank, 19 contacts, 20 moves and adjust, 21 barrier states, 22 chromosome ends, 23 LEF-BAR,
+ * 24 primary LEF-LEF and move corrections, 25 secondary, 26 bypass fix-ups, 27 extrude and
+ * release. */
 int mgpu_debug_phase_times(mgpu_ctx* ctx, int enable, uint64_t* out, uint64_t n);
 /* Fast mode's bypass fix-ups (fix_secondary_lef_lef_collisions,
  * simulation_detect_collisions.cpp:520-647) run in parallel over runs of consecutive ranks;

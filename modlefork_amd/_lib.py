@@ -228,6 +228,10 @@ PHASE_NAMES = ["stop test and burn-in", "bind", "rank", "rank fwd (batched)", "c
                "adjust", "barrier states and ends", "LEF-BAR", "primary LEF-LEF",
                "move corrections", "secondary (both sides)", "unused", "bypass fix-ups",
                "extrude and release", "unused"]
+# the wave-parallel deterministic kernel's phases (mgpu_debug_phase_times entries 16-27)
+DET_PHASE_NAMES = ["burn-in", "bind", "rank", "contacts", "moves and adjust", "barrier states",
+                   "chromosome ends", "LEF-BAR", "primary and corrections", "secondary",
+                   "bypass fix-ups", "extrude and release"]
 
 # (name, restype, argtypes) for every entry point of include/modle_gpu.h
 SIGNATURES = [
@@ -304,6 +308,7 @@ SIGNATURES = [
     ("mgpu_debug_phase_times", C.c_int, [C.c_void_p, C.c_int, U64P, C.c_uint64]),
     ("mgpu_debug_fast_serial_fixups", C.c_int, [C.c_void_p, C.c_int]),
     ("mgpu_debug_det_kernel", C.c_int, [C.c_void_p, C.c_int]),
+    ("mgpu_job_kernel", C.c_int, [C.c_void_p, C.c_char_p, C.c_uint64]),
     ("mgpu_context_stream", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int)]),
     ("mgpu_sg_config_default", C.c_int, [C.POINTER(MgpuSgConfig)]),
     ("mgpu_sg_create", C.c_int, [C.c_void_p, C.POINTER(MgpuSgConfig), C.POINTER(C.c_void_p)]),

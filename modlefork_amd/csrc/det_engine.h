@@ -57,6 +57,17 @@ __device__ inline uint64_t det_next(DetRng& g) {
 
 #define MGPU_TWO64 18446744073709551616.0
 
+// Normal ziggurat layer tables as a sampler reads them; a word source may supply its own copy
+// (an overload for its type), e.g. one staged in LDS.
+template <class G>
+__device__ inline double zig_nx(const G&, int i) {
+  return kZigNormalX[i];
+}
+template <class G>
+__device__ inline double zig_ny(const G&, int i) {
+  return kZigNormalY[i];
+}
+
 // uniform_int_distribution<u64>{lo, hi} over a 64-bit engine (brange = 2^64 - 1).
 template <class G>
 __device__ inline uint64_t det_uniform(G& g, uint64_t lo, uint64_t hi) {
@@ -140,7 +151,7 @@ __device__ inline double det_unit_exp(G& g) {
 
 template <class G>
 __device__ inline double det_normal_tail(G& g) {
-  const double start = kZigNormalX[1];
+  const double start = zig_nx(g, 1);
   for (;;) {
     const double x = det_unit_exp(g) / start;
     const double y = det_unit_exp(g) / 1.0;
@@ -155,11 +166,11 @@ __device__ inline double det_unit_normal(G& g) {
     const double r = det_int_float(g, i);
     const double sign = (double)((i & 1) * 2 - 1);
     i >>= 1;
-    const double xi = kZigNormalX[i], xi1 = kZigNormalX[i + 1];
+    const double xi = zig_nx(g, i), xi1 = zig_nx(g, i + 1);
     const double x = r * xi;
     if (x < xi1) return x * sign;
     if (i == 0) return det_normal_tail(g) * sign;
-    const double yi = kZigNormalY[i], yi1 = kZigNormalY[i + 1];
+    const double yi = zig_ny(g, i), yi1 = zig_ny(g, i + 1);
     const double y01 = det_uniform01(g);
     const double y = yi + y01 * (yi1 - yi);
     double above_ub, above_lb;

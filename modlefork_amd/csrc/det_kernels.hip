@@ -211,6 +211,24 @@ uint64_t det_wave_bytes_host(uint64_t nlefs, uint64_t nbar, uint64_t hist) {
   return det_wave_bytes(nlefs, nbar, hist);
 }
 
+__device__ unsigned long long g_det_wave_phase[kDetWavePhases];
+
+unsigned long long* det_wave_phase_ptr() {
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(g_det_wave_phase)) != hipSuccess) return nullptr;
+  return static_cast<unsigned long long*>(p);
+}
+
+hipError_t det_wave_phase_times(uint64_t* out, int n) {
+  unsigned long long h[kDetWavePhases] = {};
+  hipError_t e = hipMemcpyFromSymbol(h, HIP_SYMBOL(g_det_wave_phase), sizeof(h), 0,
+                                     hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return e;
+  for (int k = 0; k < n && k < kDetWavePhases; ++k) out[k] = h[k];
+  const unsigned long long z[kDetWavePhases] = {};
+  return hipMemcpyToSymbol(HIP_SYMBOL(g_det_wave_phase), z, sizeof(z), 0, hipMemcpyHostToDevice);
+}
+
 hipError_t launch_det_trace(hipStream_t stream, const DetParams& P, const DetChrom& ch,
                             uint64_t seed, uint64_t target, uint8_t* d_scratch, uint64_t* d_trace,
                             uint64_t max_trace, uint64_t* d_out) {

@@ -59,6 +59,12 @@ struct DevWave {
     const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)src);
     return ((uint64_t)hi << 32) | lo;
   }
+  // the value of lane src (per lane; src in [0, 64))
+  __device__ uint64_t shfl(uint64_t v, uint32_t src) const {
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, (int)src);
+    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), (int)src);
+    return ((uint64_t)hi << 32) | lo;
+  }
 };
 #endif
 
@@ -102,22 +108,27 @@ __device__ inline void det_jump_batch(uint64_t& s0, uint64_t& s1, uint64_t& s2, 
   s3 = y3;
 }
 
-// Wave-uniform ring bookkeeping plus this lane's engine state for its next chunk.
+// Wave-uniform ring bookkeeping plus this lane's engine state for its next chunk; zn: the normal
+// ziggurat tables (x[129] then y[129]) staged in LDS.
 struct WRing {
   uint64_t* ring;
   uint64_t cursor, hi;
   uint64_t g0, g1, g2, g3;
+  const double* zn;
 };
 
 // A sampler's view of the ring from a given word offset.
 struct WSrc {
   const uint64_t* ring;
   uint64_t pos, hi;
+  const double* zn;
 };
 __device__ inline uint64_t det_next(WSrc& s) {
   const uint64_t p = s.pos++;
   return p < s.hi ? s.ring[p & (kWRing - 1)] : 0;
 }
+__device__ inline double zig_nx(const WSrc& g, int i) { return g.zn[i]; }
+__device__ inline double zig_ny(const WSrc& g, int i) { return g.zn[129 + i]; }
 
 template <class W>
 __device__ inline void wr_init(const W& w, WRing& r, uint64_t seed, uint64_t* states) {
@@ -189,7 +200,7 @@ __device__ inline void w_draws(const W& w, WRing& r, uint64_t count, bool revers
       const uint64_t off = r.cursor + (uint64_t)__builtin_popcountll(pending & lt) * F;
       uint64_t used = F;
       if (mine) {
-        WSrc src{r.ring, off, r.hi};
+        WSrc src{r.ring, off, r.hi, r.zn};
         val = eval(e, src);
         used = src.pos - off;
       }
@@ -217,7 +228,7 @@ __device__ inline uint64_t w_single(const W& w, WRing& r, uint32_t& err, Eval ev
   wr_refill(w, r);
   uint64_t val = 0, used = 0;
   if (w.lane() == 0) {
-    WSrc src{r.ring, r.cursor, r.hi};
+    WSrc src{r.ring, r.cursor, r.hi, r.zn};
     val = eval(src);
     used = src.pos - r.cursor;
   }
@@ -242,11 +253,29 @@ struct WCell {
   bool burnin_done;
   uint32_t err;
   WRing r;
+  // phase timers (DetParams::phase): ticks per phase, kept alike on every lane
+  bool timed;
+  uint64_t tlast;
+  uint64_t tacc[12];
 };
+
+#ifdef MGPU_EMU
+inline uint64_t wclock() { return 0; }
+#else
+__device__ inline uint64_t wclock() { return wall_clock64(); }
+#endif
+// Closes phase k at a wave-uniform point: every lane reads the clock (no divergent branch).
+__device__ inline void wmark(WCell& s, int k) {
+  if (s.timed) {
+    const uint64_t t = wclock();
+    s.tacc[k] += t - s.tlast;
+    s.tlast = t;
+  }
+}
 
 __device__ __host__ inline uint64_t det_wave_bytes(uint64_t nlefs, uint64_t nbar, uint64_t hist) {
   uint64_t b = nlefs * (10 * 8 + 5 * 4) + ((nbar * 5 + 7) & ~7ull) + 2 * (hist + 1) * 8;
-  b += kWRing * 8 + kWL * 4 * 8;
+  b += kWRing * 8 + kWL * 4 * 8 + 2 * 129 * 8;
   return (b + 255) & ~255ull;
 }
 
@@ -257,6 +286,8 @@ __device__ inline void det_wave_carve(WCell& s, uint8_t* base, uint64_t nlefs, u
   p += kWRing;
   s.xs = p;
   p += kWL * 4;
+  s.r.zn = reinterpret_cast<const double*>(p);
+  p += 2 * 129;
   s.rev = p;
   s.fwd = p + nlefs;
   s.ep = p + 2 * nlefs;
@@ -317,15 +348,11 @@ __device__ inline bool w_any(const W& w, uint64_t n, P pred) {
   return any;
 }
 
-// Sum over lanes (every lane gets it).
+// Sum over lanes (every lane gets it), by a butterfly of lane exchanges.
 template <class W>
-__device__ inline uint64_t w_sum(const W& w, uint64_t* xs, uint64_t v) {
-  xs[w.lane()] = v;
-  w.sync();
-  uint64_t t = 0;
-  for (uint32_t l = 0; l < kWL; ++l) t += xs[l];
-  w.sync();
-  return t;
+__device__ inline uint64_t w_sum(const W& w, uint64_t*, uint64_t v) {
+  for (uint32_t d = 1; d < kWL; d <<= 1) v += w.shfl(v, w.lane() ^ d);
+  return v;
 }
 
 // Smallest k in [lo, hi) with pred(k), else hi (every lane gets it; pred monotone not needed).
@@ -547,24 +574,30 @@ __device__ inline bool w_adjust_side(const W& w, const DetChrom& ch, WCell& s, u
       }
     }
   }
-  s.xs[3 * lane] = (uint64_t)val;
-  s.xs[3 * lane + 1] = dep ? 1 : 0;
-  s.xs[3 * lane + 2] = lo < hi ? 1 : 0;
-  w.sync();
-  int64_t carry = 0;
-  if (rev) {
-    for (uint32_t l = kWL - 1; l > lane; --l) {
-      if (s.xs[3 * l + 2] == 0) continue;
-      const int64_t v = (int64_t)s.xs[3 * l];
-      carry = s.xs[3 * l + 1] != 0 ? comb(v, carry) : v;
-    }
-  } else {
-    for (uint32_t l = 0; l < lane; ++l) {
-      if (s.xs[3 * l + 2] == 0) continue;
-      const int64_t v = (int64_t)s.xs[3 * l];
-      carry = s.xs[3 * l + 1] != 0 ? comb(v, carry) : v;
+  // carry into each block: a scan, in walk order, of the blocks' summaries composed as
+  // (dep_a, v_a) o (dep_b, v_b) = (dep_a && dep_b, dep_a ? comb(v_a, v_b) : v_a); an empty block
+  // is the identity (dep, the identity of comb)
+  const int64_t ident = rev ? INT64_MAX : INT64_MIN;
+  if (lo >= hi) {
+    dep = true;
+    val = ident;
+  }
+  for (uint32_t d = 1; d < kWL; d <<= 1) {
+    // the partner lane precedes this one in walk order: higher lanes for rev, lower for fwd
+    const bool has = rev ? lane + d < kWL : lane >= d;
+    const uint32_t src = has ? (rev ? lane + d : lane - d) : lane;
+    const int64_t ov = (int64_t)w.shfl((uint64_t)val, src);
+    const bool od = w.shfl(dep ? 1 : 0, src) != 0;
+    if (has) {
+      if (dep) val = comb(val, ov);
+      dep = dep && od;
     }
   }
+  // exclusive: the composed summary of the blocks before this lane's
+  const bool first = rev ? lane + 1 >= kWL : lane == 0;
+  const uint32_t prev = first ? lane : (rev ? lane + 1 : lane - 1);
+  const int64_t pv = (int64_t)w.shfl((uint64_t)val, prev);
+  const int64_t carry = first ? ident : pv;
   if (lo < hi) {
     if (rev) {
       int64_t v = carry;
@@ -645,13 +678,12 @@ __device__ inline void w_generate_moves(const W& w, const DetParams& P, const De
 // collisions (simulation_detect_collisions.cpp, simulation_correct_moves.cpp)
 // ---------------------------------------------------------------------------------------------
 template <class W>
-__device__ inline uint64_t w_max(const W& w, uint64_t* xs, uint64_t v) {
-  xs[w.lane()] = v;
-  w.sync();
-  uint64_t t = 0;
-  for (uint32_t l = 0; l < kWL; ++l) t = xs[l] > t ? xs[l] : t;
-  w.sync();
-  return t;
+__device__ inline uint64_t w_max(const W& w, uint64_t*, uint64_t v) {
+  for (uint32_t d = 1; d < kWL; d <<= 1) {
+    const uint64_t o = w.shfl(v, w.lane() ^ d);
+    v = o > v ? o : v;
+  }
+  return v;
 }
 
 // Chromosome-end collisions (det_boundaries' two walks): the rev walk stops at the first rank
@@ -969,7 +1001,7 @@ __device__ inline void w_secondary_side(const W& w, const DetParams& P, WCell& s
       const uint64_t c = idx + lane;
       bool ok = false;
       if (c < nc) {
-        WSrc g{s.r.ring, s.r.cursor + lane, s.r.hi};
+        WSrc g{s.r.ring, s.r.cursor + lane, s.r.hi, s.r.zn};
         ok = det_bernoulli(g, pkeep);
       }
       const uint64_t fails = w.ballot(c < nc && !ok);
@@ -1068,12 +1100,17 @@ __device__ inline void w_process_collisions(const W& w, const DetParams& P, cons
   w.sync();
   uint64_t n5, n3;
   w_boundaries(w, ch, s, n, n5, n3);
+  wmark(s, 6);
   w_lef_bar(w, P, ch, s, n, n5, n3);
+  wmark(s, 7);
   w_primary(w, P, ch, s, n, n5, n3);
   w_correct(w, ch, s, n);
+  wmark(s, 8);
   w_secondary_side(w, P, s, n, n5, n3, true);
   w_secondary_side(w, P, s, n, n5, n3, false);
+  wmark(s, 9);
   w_fix_secondary(w, ch, s, n, n5, n3);
+  wmark(s, 10);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1166,7 +1203,7 @@ __device__ inline void w_register(const W& w, const DetParams& P, const DetChrom
   while (nreg != 0) {
     wr_refill(w, s.r);
     const uint64_t off = s.r.cursor + (uint64_t)lane * F;
-    WSrc g{s.r.ring, off, s.r.hi};
+    WSrc g{s.r.ring, off, s.r.hi, s.r.zn};
     bool ok = false;
     uint64_t q1 = 0, q2 = 0;
     do {
@@ -1283,9 +1320,19 @@ __device__ inline void det_wave_cell(const W& w, const DetParams& P, const DetCh
   s.contacts = 0;
   s.burnin_done = false;
   s.err = 0;
+  s.timed = P.phase != nullptr;
+  s.tlast = s.timed ? wclock() : 0;
+  for (int k = 0; k < 12; ++k) s.tacc[k] = 0;
   for (uint64_t i = lane; i < N; i += kWL) {
     s.rev[i] = s.fwd[i] = s.ep[i] = kUnbound;
     s.rr[i] = s.fr[i] = (uint32_t)i;
+  }
+  {
+    double* zn = const_cast<double*>(s.r.zn);
+    for (uint32_t i = lane; i < 129; i += kWL) {
+      zn[i] = kZigNormalX[i];
+      zn[129 + i] = kZigNormalY[i];
+    }
   }
   wr_init(w, s.r, seed, s.xs);
   // initial barrier states: Bernoulli(occupancy), no draw where the occupancy is 0
@@ -1316,6 +1363,7 @@ __device__ inline void det_wave_cell(const W& w, const DetParams& P, const DetCh
                                                   : s.epoch - s.burnin_epochs < ch.target_epochs;
     if (!go) break;
     if (!s.burnin_done) w_burnin(w, P, ch, s);
+    wmark(s, 0);
     const uint64_t n = s.nact;
     // select + bind: every unbound LEF of the active span draws a position in [start, end-1]
     if (bind_range == 0) {
@@ -1336,12 +1384,16 @@ __device__ inline void det_wave_cell(const W& w, const DetParams& P, const DetCh
             s.ep[i] = s.epoch;
           });
     }
+    wmark(s, 1);
     w_rank(w, s, n, s.epoch != 0);
+    wmark(s, 2);
     if (s.burnin_done) {
       w_sample_contacts(w, P, ch, s, target);
+      wmark(s, 3);
       if (target != 0 && s.contacts >= target) break;
     }
     w_generate_moves(w, P, ch, s, n);
+    wmark(s, 4);
     w_draws(
         w, s.r, ch.nbar, false, 1, s.err, s.dbg, [&](uint64_t) { return true; },
         [&](uint64_t, WSrc& g) -> uint64_t {
@@ -1356,6 +1408,7 @@ __device__ inline void det_wave_cell(const W& w, const DetParams& P, const DetCh
           if (s.bst[i] == 0 && u > ch.bar_stp_inactive[i]) s.bst[i] = 1;
           else if (s.bst[i] == 1 && u > ch.bar_stp_active[i]) s.bst[i] = 0;
         });
+    wmark(s, 5);
     for (uint64_t i = lane; i < n; i += kWL) s.rc[i] = s.fc[i] = 0;
     w.sync();
     w_process_collisions(w, P, ch, s, n);
@@ -1377,8 +1430,11 @@ __device__ inline void det_wave_cell(const W& w, const DetParams& P, const DetCh
       }
       w.sync();
     }
+    wmark(s, 11);
     if (s.err) break;
   }
+  if (lane == 0 && s.timed)
+    for (int k = 0; k < 12; ++k) atomicAdd(&P.phase[k], (unsigned long long)s.tacc[k]);
   if (lane == 0 && stats) {
     stats[0] = s.epoch;
     stats[1] = s.burnin_epochs;

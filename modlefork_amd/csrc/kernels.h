@@ -51,6 +51,8 @@ struct DetParams {
   uint64_t min_burnin_epochs, max_burnin_epochs;
   double target_contact_density;
   StateLog log;
+  // wave-parallel kernel only: per-phase 100 MHz wall-clock ticks are added here when set
+  unsigned long long* phase;
 };
 
 // Largest chromosome end (bp) fast mode accepts: its per-epoch move arithmetic is 32-bit.
@@ -89,6 +91,11 @@ hipError_t launch_det_wave_trace(hipStream_t stream, const DetParams& P, const D
                                  uint64_t seed, uint64_t target, uint64_t lds_bytes,
                                  uint64_t* d_trace, uint64_t max_trace, uint64_t* d_out);
 uint64_t det_wave_bytes_host(uint64_t nlefs, uint64_t nbar, uint64_t hist);
+// The wave-parallel kernel's phase-tick accumulators (kDetWavePhases of them, a device array),
+// and a read that clears them.
+constexpr int kDetWavePhases = 12;
+unsigned long long* det_wave_phase_ptr();
+hipError_t det_wave_phase_times(uint64_t* out, int n);
 hipError_t launch_det_trace(hipStream_t stream, const DetParams& P, const DetChrom& ch,
                             uint64_t seed, uint64_t target, uint8_t* d_scratch, uint64_t* d_trace,
                             uint64_t max_trace, uint64_t* d_out);

@@ -989,6 +989,7 @@ hipError_t enqueue_simulation(mgpu_job* job, hipStream_t s, hipEvent_t* ev = nul
   if (job->mode == MGPU_MODE_DETERMINISTIC) {
     DetParams P = det_params(&job->p);
     P.log = job->log;
+    if (job->ctx->phase_timers && det_uses_wave(job)) P.phase = det_wave_phase_ptr();
     hipError_t e = ev && nev >= 2 ? hipEventRecord(ev[0], s) : hipSuccess;
     if (e == hipSuccess && det_uses_wave(job))
       e = launch_det_wave(s, P, job->d_chroms, job->d_items, job->nitems, job->wave_bytes,
@@ -1171,7 +1172,8 @@ int mgpu_job_launch_stats(mgpu_job* job, mgpu_launch_stats* out, uint32_t max, u
   if (!job || !n) return fail(job ? job->ctx : nullptr, MGPU_EINVAL, "null argument");
   mgpu_ctx* ctx = job->ctx;
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  uint32_t threads[5] = {64, 0, 0, 0, 0}, grid[5] = {job->grid, 0, 0, 0, 0};
+  uint32_t threads[5] = {64, 0, 0, 0, 0};
+  uint32_t grid[5] = {det_uses_wave(job) ? (uint32_t)job->nitems : job->grid, 0, 0, 0, 0};
   uint64_t cells[5] = {job->nitems, 0, 0, 0, 0}, begin[5] = {0, 0, 0, 0, 0};
   uint32_t k = 1;
   const uint32_t* order = nullptr;
@@ -1556,10 +1558,20 @@ int mgpu_debug_phase_times(mgpu_ctx* ctx, int enable, uint64_t* out, uint64_t n)
   HIPCHK(ctx, hipSetDevice(ctx->device));
   ctx->phase_timers = enable ? 1 : 0;
   if (out && n) {
-    std::vector<uint64_t> v(kFastPhaseCount, 0);
+    std::vector<uint64_t> v(kFastPhaseCount + kDetWavePhases, 0);
     HIPCHK(ctx, fast_phase_times(v.data(), kFastPhaseCount));
+    HIPCHK(ctx, det_wave_phase_times(v.data() + kFastPhaseCount, kDetWavePhases));
     for (uint64_t k = 0; k < n; ++k) out[k] = k < v.size() ? v[k] : 0;
   }
+  return MGPU_OK;
+}
+
+int mgpu_job_kernel(mgpu_job* job, char* name, uint64_t cap) {
+  if (!job || !name || cap == 0) return fail(job ? job->ctx : nullptr, MGPU_EINVAL, "null argument");
+  const char* k = job->mode == MGPU_MODE_FAST ? "fast_cells_kernel"
+                  : det_uses_wave(job)         ? "det_wave_kernel"
+                                               : "det_cells_kernel";
+  snprintf(name, cap, "%s", k);
   return MGPU_OK;
 }
 

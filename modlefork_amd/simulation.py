@@ -196,12 +196,13 @@ class Context:
     def __exit__(self, *exc):
         self.close()
 
-    def phase_times(self, enable: bool = True):
-        """Reads and clears fast mode's per-phase tick totals (100 MHz) and sets the timers."""
-        out = np.zeros(16, np.uint64)
+    def phase_times(self, enable: bool = True, deterministic: bool = False):
+        """Reads and clears the per-phase tick totals (100 MHz) and sets the timers: fast mode's
+        16 phases, or with `deterministic` the wave-parallel kernel's 12 (L.DET_PHASE_NAMES)."""
+        out = np.zeros(28, np.uint64)
         self.check(L.lib().mgpu_debug_phase_times(self.ptr, 1 if enable else 0,
-                                                  _p(out, C.c_uint64), 16))
-        return out
+                                                  _p(out, C.c_uint64), 28))
+        return out[16:] if deterministic else out[:16]
 
     def check(self, rc):
         L.check(rc, self.ptr)
@@ -327,6 +328,12 @@ class Job:
             return np.frombuffer(buf, dtype=STATE_DTYPE, count=n.value).copy(), n.value
         self.ctx.check(rc)
         return np.zeros(0, STATE_DTYPE), 0
+
+    def kernel(self) -> str:
+        """Name of the simulation kernel this job launches (as rocprof lists it)."""
+        buf = C.create_string_buffer(64)
+        self.ctx.check(L.lib().mgpu_job_kernel(self.ptr, buf, 64))
+        return buf.value.decode()
 
     def band(self, c: int):
         p = self.plans[c]

@@ -107,6 +107,7 @@ struct EmuWave {
     sync();
     return r;
   }
+  uint64_t shfl(uint64_t v, uint32_t src) const { return bcast(v, src); }
 };
 
 struct Job {
