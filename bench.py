@@ -179,6 +179,46 @@ def stress_leg(ctx, mode, nthreads, no_cpu):
     return out
 
 
+def deterministic_leg(ctx, nthreads, no_cpu):
+    """Deterministic mode (the reference's draws, bands bit-exact against the oracle) on the
+    small workload: 20 Mbp, 400 LEFs, 200 barriers, 64 cells, one wavefront per cell. Three
+    timed runs after a warm-up, the same layout with 512 cells (the chip keeps about that many
+    wave-parallel cells resident, so this is the mode's throughput), and the oracle on the 64
+    cells on this host's threads."""
+    from modlefork_amd import _lib as L
+    from modlefork_amd import simulation as S
+    from modlefork_amd import workloads as W
+
+    out = {}
+    for ncells in (64, 512):
+        wl = W.small(ncells=ncells)
+        sim = S.Simulation(wl.options, ctx)
+        names, chroms, bars, plans = sim.prepare(wl)
+        job = S.Job(ctx, sim.params, chroms, bars, plans, L.MODE_DETERMINISTIC)
+        kernel = job.kernel()
+        job.run()
+        runs = 3 if ncells == 64 else 1
+        t0 = time.perf_counter()
+        for _ in range(runs):
+            st = job.run()
+        wall = (time.perf_counter() - t0) / runs
+        job.close()
+        rec = {"cells": int(st.cells), "cells_per_sec": st.cells / wall,
+               "contacts_per_sec": st.contacts / wall, "epochs_per_sec": st.epochs / wall,
+               "kernel_ms": float(st.kernel_ms), "kernel": kernel}
+        log(f"deterministic {wl.name}: {st.cells} cells in {wall * 1e3:.1f} ms "
+            f"({st.cells / wall:.1f} cells/s, {kernel})")
+        if ncells == 64:
+            out.update({"workload": wl.name, **rec})
+            if not no_cpu:
+                cpu = cpu_baseline(wl, wl.options, 64 - 4, nthreads, single_cells=4)
+                out["cpu_baseline"] = cpu
+                out["speedup_vs_cpu"] = out["cells_per_sec"] / cpu["value"]
+        else:
+            out["batch_512_cells"] = rec
+    return out
+
+
 def perturbate_inputs(wl, params):
     """ChromInput list of a perturbate workload (barriers in interval-tree order, features)."""
     from modlefork_amd import perturbate as P
@@ -357,6 +397,8 @@ def main():
                     help="skip the stress leg (250 Mbp, 1 kb bins, 2048 cells: one GPU run and its "
                          "CPU baseline)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-deterministic", action="store_true",
+                    help="skip the deterministic-mode leg (small workload, bit-exact mode)")
     ap.add_argument("--sweep-points", type=int, default=4096)
     ap.add_argument("--no-sweep", action="store_true", help="skip the surrogate sweep leg")
     ap.add_argument("--threads", type=int, default=0,
@@ -557,6 +599,8 @@ def main():
         line["stress"] = stress_leg(sim.ctx, mode, nthreads, args.no_cpu)
     if not args.no_perturbate and args.workload == "genome":
         line["perturbate"] = perturbate_leg(sim.ctx, nthreads, args.no_cpu)
+    if not args.no_deterministic and args.workload == "genome":
+        line["deterministic"] = deterministic_leg(sim.ctx, nthreads, args.no_cpu)
     if not args.no_sweep:
         log(f"surrogate sweep: {args.sweep_points} points")
         line["surrogate"] = surrogate_leg(sim.ctx, args.sweep_points)
