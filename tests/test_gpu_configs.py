@@ -161,6 +161,39 @@ def test_genome_chromosome_statistics_match_oracle(ctx, which):
     assert not failures, failures
 
 
+def test_genome_chromosome_deterministic_bit_exact(ctx):
+    """Deterministic mode at genome scale: 16 cells of the genome workload's smallest chromosome
+    (42 Mbp, 841 LEFs, clustered barriers with Beta(8,2) occupancy) with the full workload's
+    per-cell targets, one wavefront per cell (its 127 KB working set in LDS), against the oracle
+    bit for bit: band, missed updates and epoch counts."""
+    wl, name = _genome_chrom("smallest")
+    ncells = 16
+    c = OC.transform(wl.options)
+    rc = OC.to_rc(c)
+    size = dict(wl.chroms)[name]
+    bars = OC.prepare_barriers(wl.barriers_of(name), 0, size, c)
+    sim = S.Simulation(wl.options, ctx)
+    names, chroms, dbars, plans = sim.prepare(wl)
+    i = names.index(name)
+    p = plans[i]
+    t = np.ascontiguousarray(p.targets[:ncells], np.uint64)
+    say(f"genome deterministic ({name}, {p.nlefs} LEFs): oracle, {ncells} cells")
+    ch = R.make_chrom(name, size, 0, size)
+    oband, omissed, ost = R.simulate_cells(rc, ch, bars, p.nlefs, 0, ncells, t, p.target_epochs,
+                                           p.nrows, p.ncols, 16)
+    say(f"genome deterministic ({name}): device")
+    sub = S.ChromPlan(p.nlefs, p.nrows, p.ncols, t.copy(), p.target_epochs)
+    job = S.Job(ctx, sim.params, [chroms[i]], [dbars[i]], [sub], L.MODE_DETERMINISTIC)
+    assert job.kernel() == "det_wave_kernel"
+    st = job.run()
+    band, missed = job.band(0)
+    job.close()
+    say(f"genome deterministic: {st.kernel_ms:.1f} ms, {st.epochs} epochs")
+    assert int(st.epochs) == int(ost[:, 0].sum())
+    assert missed == omissed
+    assert np.array_equal(band, oband), int(np.count_nonzero(band != oband))
+
+
 def _decay_is_decreasing(band, nrows, ncols, slack=1.05):
     p, _ = CS.log_binned_decay(band, nrows, ncols)
     return bool(np.all(p[1:] <= p[:-1] * slack)), p
