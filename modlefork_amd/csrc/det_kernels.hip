@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(64) det_wave_kernel(DetParams P, const DetChro
   const WorkItem wi = items[it];
   const DetChrom ch = chroms[wi.chrom];
   uint64_t st[4] = {0, 0, 0, 0};
-  det_wave_cell(w, P, ch, lds, P.hist_len, wi.seed, wi.target, nullptr, 0, st);
+  det_wave_cell(w, P, ch, lds, nullptr, P.hist_len, wi.seed, wi.target, nullptr, 0, st);
   if (w.lane() == 0) {
     if (cell_stats)
       for (int k = 0; k < 4; ++k) cell_stats[4 * it + k] = st[k];
@@ -64,14 +64,40 @@ __global__ void __launch_bounds__(64) det_wave_kernel(DetParams P, const DetChro
   }
 }
 
+// The same for cells whose arrays do not fit a CU's LDS: block b keeps its engine-word ring,
+// wave scratch and ziggurat tables in LDS and the cell's arrays in slab + b * stride (device
+// memory, one slot per work item).
+__global__ void __launch_bounds__(64) det_wave_slab_kernel(DetParams P, const DetChrom* chroms,
+                                                           const WorkItem* items, uint64_t nitems,
+                                                           uint8_t* slab, uint64_t stride,
+                                                           uint64_t* cell_stats,
+                                                           unsigned long long* err) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  const DevWave w;
+  const uint64_t it = blockIdx.x;
+  if (it >= nitems) return;
+  const WorkItem wi = items[it];
+  const DetChrom ch = chroms[wi.chrom];
+  uint64_t st[4] = {0, 0, 0, 0};
+  det_wave_cell(w, P, ch, lds, slab + it * stride, P.hist_len, wi.seed, wi.target, nullptr, 0,
+                st);
+  if (w.lane() == 0) {
+    if (cell_stats)
+      for (int k = 0; k < 4; ++k) cell_stats[4 * it + k] = st[k];
+    if (st[3] == ~0ull) atomicAdd(err, 1ULL);
+  }
+}
+
+// One cell with its per-epoch state hashes; cell_mem: the cell's arrays in device memory, or null
+// for all of it in LDS.
 __global__ void __launch_bounds__(64) det_wave_trace_kernel(DetParams P, DetChrom ch,
                                                             uint64_t seed, uint64_t target,
                                                             uint64_t* trace, uint64_t max_trace,
-                                                            uint64_t* out) {
+                                                            uint64_t* out, uint8_t* cell_mem) {
   extern __shared__ __align__(16) uint8_t lds[];
   const DevWave w;
   uint64_t st[4] = {0, 0, 0, 0};
-  det_wave_cell(w, P, ch, lds, P.hist_len, seed, target, trace, max_trace, st);
+  det_wave_cell(w, P, ch, lds, cell_mem, P.hist_len, seed, target, trace, max_trace, st);
   if (w.lane() == 0)
     for (int k = 0; k < 4; ++k) out[k] = st[k];
 }
@@ -196,19 +222,39 @@ hipError_t launch_det_wave(hipStream_t stream, const DetParams& P, const DetChro
   return hipGetLastError();
 }
 
+hipError_t launch_det_wave_slab(hipStream_t stream, const DetParams& P, const DetChrom* d_chroms,
+                                const WorkItem* d_items, uint64_t nitems, uint8_t* d_slab,
+                                uint64_t stride, uint64_t* d_cell_stats,
+                                unsigned long long* d_err) {
+  if (nitems == 0) return hipSuccess;
+  if (nitems > 0x7FFFFFFFull) return hipErrorInvalidValue;
+  const uint64_t lds_bytes = det_wave_fixed_bytes();
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(det_wave_slab_kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(det_wave_slab_kernel, dim3((uint32_t)nitems), dim3(64), lds_bytes, stream,
+                     P, d_chroms, d_items, nitems, d_slab, stride, d_cell_stats, d_err);
+  return hipGetLastError();
+}
+
 hipError_t launch_det_wave_trace(hipStream_t stream, const DetParams& P, const DetChrom& ch,
                                  uint64_t seed, uint64_t target, uint64_t lds_bytes,
-                                 uint64_t* d_trace, uint64_t max_trace, uint64_t* d_out) {
+                                 uint8_t* d_cell, uint64_t* d_trace, uint64_t max_trace,
+                                 uint64_t* d_out) {
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(det_wave_trace_kernel),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(det_wave_trace_kernel, dim3(1), dim3(64), lds_bytes, stream, P, ch, seed,
-                     target, d_trace, max_trace, d_out);
+                     target, d_trace, max_trace, d_out, d_cell);
   return hipGetLastError();
 }
 
 uint64_t det_wave_bytes_host(uint64_t nlefs, uint64_t nbar, uint64_t hist) {
   return det_wave_bytes(nlefs, nbar, hist);
+}
+uint64_t det_wave_fixed_bytes_host() { return det_wave_fixed_bytes(); }
+uint64_t det_wave_cell_bytes_host(uint64_t nlefs, uint64_t nbar, uint64_t hist) {
+  return det_wave_cell_bytes(nlefs, nbar, hist);
 }
 
 __device__ unsigned long long g_det_wave_phase[kDetWavePhases];

@@ -273,21 +273,33 @@ __device__ inline void wmark(WCell& s, int k) {
   }
 }
 
-__device__ __host__ inline uint64_t det_wave_bytes(uint64_t nlefs, uint64_t nbar, uint64_t hist) {
-  uint64_t b = nlefs * (10 * 8 + 5 * 4) + ((nbar * 5 + 7) & ~7ull) + 2 * (hist + 1) * 8;
-  b += kWRing * 8 + kWL * 4 * 8 + 2 * 129 * 8;
+// LDS the wave always keeps: the engine-word ring, wave scratch and the normal ziggurat tables.
+__device__ __host__ inline uint64_t det_wave_fixed_bytes() {
+  return ((kWRing * 8 + kWL * 4 * 8 + 2 * 129 * 8) + 255) & ~255ull;
+}
+// The cell's own arrays: 10 u64 and 5 u32 per LEF, the burn-in histories, per barrier a u32 and
+// a state byte.
+__device__ __host__ inline uint64_t det_wave_cell_bytes(uint64_t nlefs, uint64_t nbar,
+                                                        uint64_t hist) {
+  const uint64_t b = nlefs * (10 * 8 + 5 * 4) + ((nbar * 5 + 7) & ~7ull) + 2 * (hist + 1) * 8;
   return (b + 255) & ~255ull;
 }
+// Everything in LDS.
+__device__ __host__ inline uint64_t det_wave_bytes(uint64_t nlefs, uint64_t nbar, uint64_t hist) {
+  return det_wave_fixed_bytes() + det_wave_cell_bytes(nlefs, nbar, hist);
+}
 
-__device__ inline void det_wave_carve(WCell& s, uint8_t* base, uint64_t nlefs, uint64_t nbar,
-                                      uint64_t hist) {
-  uint64_t* p = reinterpret_cast<uint64_t*>(base);
-  s.r.ring = p;
-  p += kWRing;
-  s.xs = p;
-  p += kWL * 4;
-  s.r.zn = reinterpret_cast<const double*>(p);
-  p += 2 * 129;
+// lds: the wave's LDS; cell: the cell's arrays (det_wave_cell_bytes), or null to place them in
+// LDS after the fixed part.
+__device__ inline void det_wave_carve(WCell& s, uint8_t* lds, uint8_t* cell, uint64_t nlefs,
+                                      uint64_t nbar, uint64_t hist) {
+  uint64_t* f = reinterpret_cast<uint64_t*>(lds);
+  s.r.ring = f;
+  f += kWRing;
+  s.xs = f;
+  f += kWL * 4;
+  s.r.zn = reinterpret_cast<const double*>(f);
+  uint64_t* p = reinterpret_cast<uint64_t*>(cell ? cell : lds + det_wave_fixed_bytes());
   s.rev = p;
   s.fwd = p + nlefs;
   s.ep = p + 2 * nlefs;
@@ -1301,15 +1313,18 @@ __device__ inline void w_release(const W& w, const DetParams& P, const DetChrom&
       });
 }
 
-// simulate_one_cell (simulation.cpp:1056-1152) on one wavefront. base: det_wave_bytes() of LDS.
+// simulate_one_cell (simulation.cpp:1056-1152) on one wavefront. base: LDS of det_wave_bytes(),
+// or of det_wave_fixed_bytes() when cell_mem (det_wave_cell_bytes() of device memory) holds the
+// cell's arrays.
 template <class W>
 __device__ inline void det_wave_cell(const W& w, const DetParams& P, const DetChrom& ch,
-                                     uint8_t* base, uint64_t hist, uint64_t seed, uint64_t target,
+                                     uint8_t* base, uint8_t* cell_mem, uint64_t hist,
+                                     uint64_t seed, uint64_t target,
                                      uint64_t* trace, uint64_t max_trace, uint64_t* stats,
                                      uint64_t* dbg = nullptr) {
   const uint32_t lane = w.lane();
   WCell s;
-  det_wave_carve(s, base, ch.nlefs, ch.nbar, hist);
+  det_wave_carve(s, base, cell_mem, ch.nlefs, ch.nbar, hist);
   s.dbg = dbg;
   const uint64_t N = ch.nlefs;
   s.n = N;

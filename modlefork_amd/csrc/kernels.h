@@ -58,6 +58,10 @@ struct DetParams {
 // Largest chromosome end (bp) fast mode accepts: its per-epoch move arithmetic is 32-bit.
 constexpr uint64_t kFastMaxEnd = 1ull << 30;
 
+// Largest device-memory slab the wave-parallel deterministic kernel takes for cells too large for
+// LDS; a job with more cells than fit runs as several launches over the same slab.
+constexpr uint64_t kWaveSlabBytes = 8ull << 30;
+
 // Device view of one chromosome: barrier table (sorted), band and its counters.
 struct DetChrom {
   uint64_t start, end;
@@ -87,10 +91,21 @@ hipError_t launch_det_cells(hipStream_t stream, const DetParams& P, const DetChr
 hipError_t launch_det_wave(hipStream_t stream, const DetParams& P, const DetChrom* d_chroms,
                            const WorkItem* d_items, uint64_t nitems, uint64_t lds_bytes,
                            uint64_t* d_cell_stats, unsigned long long* d_err);
+// The same for cells whose arrays live in device memory: d_slab holds nitems slots of `stride`
+// bytes (det_wave_cell_bytes_host), LDS holds det_wave_fixed_bytes_host() per block.
+hipError_t launch_det_wave_slab(hipStream_t stream, const DetParams& P, const DetChrom* d_chroms,
+                                const WorkItem* d_items, uint64_t nitems, uint8_t* d_slab,
+                                uint64_t stride, uint64_t* d_cell_stats,
+                                unsigned long long* d_err);
+// One cell with its per-epoch hashes; d_cell: the cell's arrays in device memory, or null when
+// lds_bytes holds everything.
 hipError_t launch_det_wave_trace(hipStream_t stream, const DetParams& P, const DetChrom& ch,
                                  uint64_t seed, uint64_t target, uint64_t lds_bytes,
-                                 uint64_t* d_trace, uint64_t max_trace, uint64_t* d_out);
+                                 uint8_t* d_cell, uint64_t* d_trace, uint64_t max_trace,
+                                 uint64_t* d_out);
 uint64_t det_wave_bytes_host(uint64_t nlefs, uint64_t nbar, uint64_t hist);
+uint64_t det_wave_fixed_bytes_host();
+uint64_t det_wave_cell_bytes_host(uint64_t nlefs, uint64_t nbar, uint64_t hist);
 // The wave-parallel kernel's phase-tick accumulators (kDetWavePhases of them, a device array),
 // and a read that clears them.
 constexpr int kDetWavePhases = 12;

@@ -509,8 +509,13 @@ struct mgpu_job {
   uint64_t stride = 0;
   bool use_lds = false;
   uint32_t grid = 0;
-  // deterministic mode, wave-parallel kernel: LDS per cell, cells whose run failed
+  // deterministic mode, wave-parallel kernel: LDS per block, cells whose run failed; for cells
+  // too large for LDS, the bytes of one cell's arrays in device memory and the number of slots
+  // (cells resident at once; launches take the work items in chunks of that many)
   uint64_t wave_bytes = 0;
+  uint64_t wave_stride = 0;
+  uint64_t wave_slots = 0;
+  uint8_t* d_wave_slab = nullptr;
   unsigned long long* d_err = nullptr;
   FastJob fast{};
   hipGraph_t graph = nullptr;       // captured run (mgpu_job_capture)
@@ -692,13 +697,26 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
     job->grid = (uint32_t)std::min<uint64_t>(job->nitems,
                                              std::min<uint64_t>(per_cu, 8) * ctx->num_cus);
     // the wave-parallel kernel keeps a cell's whole working set and its engine-word ring in
-    // LDS, so it takes chromosomes whose largest cell fits a CU's 160 KiB
+    // LDS when the largest cell fits a CU's 160 KiB; otherwise the ring stays in LDS and each
+    // cell's arrays take a slot of a device-memory slab (at most kWaveSlabBytes, so a job with
+    // more cells than slots runs as several launches)
     const uint64_t wb = det_wave_bytes_host(max_lefs, max_bar, p->burnin_history_length);
-    if (wb <= 160 * 1024) job->wave_bytes = wb;
+    if (wb <= 160 * 1024) {
+      job->wave_bytes = wb;
+    } else {
+      job->wave_bytes = det_wave_fixed_bytes_host();
+      job->wave_stride = det_wave_cell_bytes_host(max_lefs, max_bar, p->burnin_history_length);
+      job->wave_slots = std::max<uint64_t>(1, std::min<uint64_t>(job->nitems,
+                                                               kWaveSlabBytes / job->wave_stride));
+    }
   }
+  // the one-lane kernel's device-memory working sets and the wave kernel's slab are used by
+  // different runs of the job, never by one, so they share a span
   uint64_t off_scratch = 0;
-  if (mode == MGPU_MODE_DETERMINISTIC && !job->use_lds)
-    off_scratch = pool.add(job->stride * job->grid);
+  const uint64_t lane_scratch =
+      mode == MGPU_MODE_DETERMINISTIC && !job->use_lds ? job->stride * job->grid : 0;
+  const uint64_t slab_bytes = job->wave_stride * job->wave_slots;
+  if (lane_scratch || slab_bytes) off_scratch = pool.add(std::max(lane_scratch, slab_bytes));
 
   hipError_t e = hipMalloc(&job->d_pool, pool.total);
   if (e != hipSuccess) {
@@ -716,7 +734,8 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
   job->d_cell_stats = reinterpret_cast<uint64_t*>(base + off_stats);
   job->d_bands = reinterpret_cast<uint32_t*>(base + off_bands);
   if (p->track_1d_lef_position) job->d_occ = reinterpret_cast<unsigned long long*>(base + off_occ);
-  if (mode == MGPU_MODE_DETERMINISTIC && !job->use_lds) job->d_scratch = base + off_scratch;
+  if (lane_scratch) job->d_scratch = base + off_scratch;
+  if (slab_bytes) job->d_wave_slab = base + off_scratch;
 
   std::vector<uint8_t> stage(inputs_end, 0);
   std::vector<WorkItem> items;
@@ -974,9 +993,8 @@ hipError_t enqueue_clears(mgpu_job* job, hipStream_t s) {
   return e;
 }
 
-// Whether a deterministic job's cells may run on one wavefront each (det_wave.h): their
-// working set must fit the CU's LDS, no optional output (1D occupancy, the state log) may be on
-// and the context must not ask for the one-lane kernel.
+// Whether a deterministic job's cells run on one wavefront each (det_wave.h): no optional output
+// (1D occupancy, the state log) may be on and the context must not ask for the one-lane kernel.
 bool det_wave_fits(const mgpu_job* job) {
   return job->mode == MGPU_MODE_DETERMINISTIC && job->wave_bytes != 0 && !job->d_occ &&
          !job->log.rec && !job->ctx->det_serial;
@@ -991,10 +1009,15 @@ hipError_t enqueue_simulation(mgpu_job* job, hipStream_t s, hipEvent_t* ev = nul
     P.log = job->log;
     if (job->ctx->phase_timers && det_uses_wave(job)) P.phase = det_wave_phase_ptr();
     hipError_t e = ev && nev >= 2 ? hipEventRecord(ev[0], s) : hipSuccess;
-    if (e == hipSuccess && det_uses_wave(job))
+    if (e == hipSuccess && det_uses_wave(job) && job->wave_stride) {
+      for (uint64_t first = 0; e == hipSuccess && first < job->nitems; first += job->wave_slots)
+        e = launch_det_wave_slab(s, P, job->d_chroms, job->d_items + first,
+                                 std::min(job->wave_slots, job->nitems - first), job->d_wave_slab,
+                                 job->wave_stride, job->d_cell_stats + 4 * first, job->d_err);
+    } else if (e == hipSuccess && det_uses_wave(job)) {
       e = launch_det_wave(s, P, job->d_chroms, job->d_items, job->nitems, job->wave_bytes,
                           job->d_cell_stats, job->d_err);
-    else if (e == hipSuccess)
+    } else if (e == hipSuccess)
       e = launch_det_cells(s, P, job->d_chroms, job->d_items, job->nitems, job->d_next,
                            job->d_scratch, job->stride, job->use_lds, job->grid,
                            job->d_cell_stats);
@@ -1173,7 +1196,10 @@ int mgpu_job_launch_stats(mgpu_job* job, mgpu_launch_stats* out, uint32_t max, u
   mgpu_ctx* ctx = job->ctx;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   uint32_t threads[5] = {64, 0, 0, 0, 0};
-  uint32_t grid[5] = {det_uses_wave(job) ? (uint32_t)job->nitems : job->grid, 0, 0, 0, 0};
+  uint32_t grid[5] = {!det_uses_wave(job) ? job->grid
+                      : job->wave_stride ? (uint32_t)job->wave_slots
+                                         : (uint32_t)job->nitems,
+                      0, 0, 0, 0};
   uint64_t cells[5] = {job->nitems, 0, 0, 0, 0}, begin[5] = {0, 0, 0, 0, 0};
   uint32_t k = 1;
   const uint32_t* order = nullptr;
@@ -1522,7 +1548,9 @@ int mgpu_debug_cell_trace(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom*
   int rc = mgpu_job_create(ctx, p, 1, chrom, barriers, &nlefs, &one, &tc, &target_epochs,
                            MGPU_MODE_DETERMINISTIC, &job);
   if (rc) return rc;
-  const uint64_t bytes = det_cell_bytes_host(nlefs, barriers->n, p->burnin_history_length);
+  const uint64_t bytes =
+      std::max(det_cell_bytes_host(nlefs, barriers->n, p->burnin_history_length),
+               det_wave_cell_bytes_host(nlefs, barriers->n, p->burnin_history_length));
   uint8_t* d = nullptr;
   hipError_t e = hipMalloc(&d, bytes + max_trace * 8 + 64);
   if (e == hipSuccess) e = hipMemsetAsync(job->d_bands, 0, job->band_total * 4, ctx->stream);
@@ -1532,7 +1560,8 @@ int mgpu_debug_cell_trace(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom*
   if (e == hipSuccess && det_wave_fits(job))
     e = launch_det_wave_trace(ctx->stream, det_params(p), job->chroms[0],
                               mgpu_cell_seed(chrom, cell_id, p->seed), target_contacts,
-                              job->wave_bytes, d_trace, max_trace, d_out);
+                              job->wave_bytes, job->wave_stride ? d : nullptr, d_trace, max_trace,
+                              d_out);
   else if (e == hipSuccess)
     e = launch_det_trace(ctx->stream, det_params(p), job->chroms[0],
                          mgpu_cell_seed(chrom, cell_id, p->seed), target_contacts, d, d_trace,
@@ -1569,6 +1598,7 @@ int mgpu_debug_phase_times(mgpu_ctx* ctx, int enable, uint64_t* out, uint64_t n)
 int mgpu_job_kernel(mgpu_job* job, char* name, uint64_t cap) {
   if (!job || !name || cap == 0) return fail(job ? job->ctx : nullptr, MGPU_EINVAL, "null argument");
   const char* k = job->mode == MGPU_MODE_FAST ? "fast_cells_kernel"
+                  : det_uses_wave(job) && job->wave_stride ? "det_wave_slab_kernel"
                   : det_uses_wave(job)         ? "det_wave_kernel"
                                                : "det_cells_kernel";
   snprintf(name, cap, "%s", k);

@@ -85,6 +85,7 @@ struct Sched {
 
 Sched* g_sched = nullptr;
 int g_lane_order = 0;
+int g_separate_cell = 0;
 uint64_t g_last_syncs = 0;
 
 struct EmuWave {
@@ -114,6 +115,7 @@ struct Job {
   mgpu::DetParams P;
   mgpu::DetChrom ch;
   uint8_t* lds;
+  uint8_t* cell;  // the cell's arrays outside the LDS image, or null
   uint64_t seed, target;
   uint64_t* trace;
   uint64_t max_trace;
@@ -125,7 +127,8 @@ Job* g_job = nullptr;
 void lane_entry(int ln) {
   if (_setjmp(g_sched->buf[ln]) == 0) swapcontext(&g_sched->ctx[ln], &g_sched->init_ctx);
   const EmuWave w{(uint32_t)ln};
-  mgpu::det_wave_cell(w, g_job->P, g_job->ch, g_job->lds, g_job->P.hist_len, g_job->seed,
+  mgpu::det_wave_cell(w, g_job->P, g_job->ch, g_job->lds, g_job->cell, g_job->P.hist_len,
+                      g_job->seed,
                       g_job->target, g_job->trace, g_job->max_trace,
                       ln == 0 ? g_job->stats : nullptr, g_job->dbg);
   Sched& s = *g_sched;
@@ -144,6 +147,10 @@ extern "C" {
 // Order in which the emulated lanes run between two synchronisations: 0 ascending, 1
 // descending. Results must not depend on it (a missing synchronisation would show).
 void emu_set_lane_order(int order) { g_lane_order = order; }
+
+// 1: the cell's arrays in an allocation of their own (the layout of the device-memory
+// placement), 0: after the fixed part of the LDS image.
+void emu_set_separate_cell(int on) { g_separate_cell = on; }
 
 // Wave synchronisations lane 0 met in the last emulated cell (ballots count two).
 uint64_t emu_last_syncs() { return g_last_syncs; }
@@ -217,8 +224,18 @@ int emu_det_wave_cell(const rc_config* c, const rc_chrom* chrom, const rc_barrie
   ch.qn = qn.data();
   ch.occupancy = nullptr;
 
+  // the cell's arrays in a separate allocation when the caller asks (as the device-memory
+  // placement of large cells does), else after the fixed part
   std::vector<uint64_t> lds(mgpu::det_wave_bytes(nlefs, b->n, P.hist_len) / 8 + 1, 0);
+  std::vector<uint64_t> cell;
   job.lds = reinterpret_cast<uint8_t*>(lds.data());
+  job.cell = nullptr;
+  if (g_separate_cell) {
+    lds.assign(mgpu::det_wave_fixed_bytes() / 8 + 1, 0);
+    cell.assign(mgpu::det_wave_cell_bytes(nlefs, b->n, P.hist_len) / 8 + 1, 0);
+    job.lds = reinterpret_cast<uint8_t*>(lds.data());
+    job.cell = reinterpret_cast<uint8_t*>(cell.data());
+  }
   job.seed = rc_cell_seed(chrom->name, chrom->name_len, chrom->size, cell_id, c->seed);
   job.target = target;
   job.trace = trace;

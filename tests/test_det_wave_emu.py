@@ -38,6 +38,8 @@ def emu():
             C.POINTER(C.c_uint64)]
         _lib.emu_det_wave_bytes.restype = C.c_uint64
         _lib.emu_det_wave_bytes.argtypes = [C.c_uint64] * 3
+        _lib.emu_set_separate_cell.restype = None
+        _lib.emu_set_separate_cell.argtypes = [C.c_int]
     return _lib
 
 
@@ -148,6 +150,16 @@ def test_small_configuration_cells_match_the_oracle():
                                  t.ctypes.data_as(C.POINTER(C.c_uint64)))
     ch = R.make_chrom(name, size, 0, size)
     compare_cells((rc, ch, bars, nl, t, te, nrows, ncols), (0, 37))
+
+
+def test_cell_arrays_in_separate_memory_match_the_oracle():
+    """The placement large cells use: ring, wave scratch and ziggurat tables in the block's LDS,
+    the cell's arrays in a slot of their own (device memory on the GPU)."""
+    emu().emu_set_separate_cell(1)
+    try:
+        compare_cells(setup_case(2_000_000, 40, 13, 4), range(2))
+    finally:
+        emu().emu_set_separate_cell(0)
 
 
 def test_every_exact_fallback_path_was_taken():

@@ -161,13 +161,16 @@ def test_genome_chromosome_statistics_match_oracle(ctx, which):
     assert not failures, failures
 
 
-def test_genome_chromosome_deterministic_bit_exact(ctx):
-    """Deterministic mode at genome scale: 16 cells of the genome workload's smallest chromosome
-    (42 Mbp, 841 LEFs, clustered barriers with Beta(8,2) occupancy) with the full workload's
-    per-cell targets, one wavefront per cell (its 127 KB working set in LDS), against the oracle
-    bit for bit: band, missed updates and epoch counts."""
-    wl, name = _genome_chrom("smallest")
-    ncells = 16
+@pytest.mark.parametrize("which,ncells,kernel", [("smallest", 16, "det_wave_kernel"),
+                                                 ("largest", 8, "det_wave_slab_kernel")])
+def test_genome_chromosome_deterministic_bit_exact(ctx, which, ncells, kernel):
+    """Deterministic mode at genome scale, against the oracle bit for bit (band, missed updates
+    and epoch counts), with the full workload's per-cell targets: 16 cells of the genome
+    workload's smallest chromosome (42 Mbp, 841 LEFs, clustered barriers with Beta(8,2)
+    occupancy; one wavefront per cell, its 127 KB working set in LDS) and 8 cells of its largest
+    (241 Mbp, 4 827 LEFs; the cell's arrays, about 0.5 MB, in a device-memory slot, its
+    engine-word ring in LDS)."""
+    wl, name = _genome_chrom(which)
     c = OC.transform(wl.options)
     rc = OC.to_rc(c)
     size = dict(wl.chroms)[name]
@@ -184,11 +187,11 @@ def test_genome_chromosome_deterministic_bit_exact(ctx):
     say(f"genome deterministic ({name}): device")
     sub = S.ChromPlan(p.nlefs, p.nrows, p.ncols, t.copy(), p.target_epochs)
     job = S.Job(ctx, sim.params, [chroms[i]], [dbars[i]], [sub], L.MODE_DETERMINISTIC)
-    assert job.kernel() == "det_wave_kernel"
+    assert job.kernel() == kernel
     st = job.run()
     band, missed = job.band(0)
     job.close()
-    say(f"genome deterministic: {st.kernel_ms:.1f} ms, {st.epochs} epochs")
+    say(f"genome deterministic ({name}): {st.kernel_ms:.1f} ms, {st.epochs} epochs")
     assert int(st.epochs) == int(ost[:, 0].sum())
     assert missed == omissed
     assert np.array_equal(band, oband), int(np.count_nonzero(band != oband))
