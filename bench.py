@@ -179,12 +179,17 @@ def stress_leg(ctx, mode, nthreads, no_cpu):
     return out
 
 
-def deterministic_leg(ctx, nthreads, no_cpu):
+def deterministic_leg(ctx, nthreads, no_cpu, genome_cpu=None, phases=False):
     """Deterministic mode (the reference's draws, bands bit-exact against the oracle) on the
     small workload: 20 Mbp, 400 LEFs, 200 barriers, 64 cells, one wavefront per cell. Three
     timed runs after a warm-up, the same layout with 512 cells (the chip keeps about that many
     wave-parallel cells resident, so this is the mode's throughput), and the oracle on the 64
-    cells on this host's threads."""
+    cells on this host's threads. Then the genome workload's first 64 cells of each of its 24
+    chromosomes with their full-workload targets (1 536 cells, 841-4 827 LEFs: the cells' arrays
+    in device-memory slots, engine-word rings in LDS), one timed run after a warm-up, against
+    the CPU rate the genome workload's own baseline measured (genome_cpu). With `phases`, one
+    more genome-sample run with the per-phase timers gives the summed wavefront time, and so how
+    many wavefronts were busy on average over the kernel."""
     from modlefork_amd import _lib as L
     from modlefork_amd import simulation as S
     from modlefork_amd import workloads as W
@@ -216,6 +221,48 @@ def deterministic_leg(ctx, nthreads, no_cpu):
                 out["speedup_vs_cpu"] = out["cells_per_sec"] / cpu["value"]
         else:
             out["batch_512_cells"] = rec
+
+    wl = W.genome()
+    sim = S.Simulation(wl.options, ctx)
+    names, chroms, bars, plans = sim.prepare(wl)
+    sub = [S.ChromPlan(p.nlefs, p.nrows, p.ncols, np.ascontiguousarray(p.targets[:64]),
+                       p.target_epochs) for p in plans]
+    job = S.Job(ctx, sim.params, chroms, bars, sub, L.MODE_DETERMINISTIC)
+    kernel = job.kernel()
+    job.run()
+    t0 = time.perf_counter()
+    st = job.run()
+    wall = time.perf_counter() - t0
+    busy = None
+    if phases:
+        ctx.phase_times(True)
+        pst = job.run()
+        ph = ctx.phase_times(False, deterministic=True).astype(np.float64)
+        wave_s = float(ph.sum()) / 1e8
+        busy = {"wave_seconds": wave_s, "kernel_ms": float(pst.kernel_ms),
+                "mean_busy_wavefronts": wave_s / (float(pst.kernel_ms) / 1e3),
+                "us_per_cell_epoch": wave_s / float(pst.epochs) * 1e6,
+                "phase_shares": {L.DET_PHASE_NAMES[k]: float(v / ph.sum())
+                                 for k, v in enumerate(ph) if v}}
+        log(f"deterministic genome sample phases: {wave_s:.1f} s of wavefront time in "
+            f"{pst.kernel_ms:.1f} ms ({busy['mean_busy_wavefronts']:.0f} wavefronts busy on "
+            f"average, {busy['us_per_cell_epoch']:.0f} us per cell-epoch)")
+        for k, v in enumerate(ph):
+            if v:
+                log(f"  {L.DET_PHASE_NAMES[k]:28s} {100.0 * v / ph.sum():5.1f} %")
+    job.close()
+    rec = {"workload": "genome-24chroms-5kb, first 64 cells per chromosome",
+           "cells": int(st.cells), "cells_per_sec": st.cells / wall,
+           "contacts_per_sec": st.contacts / wall, "epochs_per_sec": st.epochs / wall,
+           "kernel_ms": float(st.kernel_ms), "kernel": kernel}
+    log(f"deterministic genome sample: {st.cells} cells in {wall * 1e3:.1f} ms "
+        f"({st.cells / wall:.1f} cells/s, {kernel})")
+    if genome_cpu:
+        rec["cpu_cells_per_sec"] = genome_cpu["value"]
+        rec["speedup_vs_cpu"] = rec["cells_per_sec"] / genome_cpu["value"]
+    if busy:
+        rec["phases"] = busy
+    out["genome_sample"] = rec
     return out
 
 
@@ -600,7 +647,8 @@ def main():
     if not args.no_perturbate and args.workload == "genome":
         line["perturbate"] = perturbate_leg(sim.ctx, nthreads, args.no_cpu)
     if not args.no_deterministic and args.workload == "genome":
-        line["deterministic"] = deterministic_leg(sim.ctx, nthreads, args.no_cpu)
+        line["deterministic"] = deterministic_leg(sim.ctx, nthreads, args.no_cpu, cpu,
+                                                  phases=args.phases)
     if not args.no_sweep:
         log(f"surrogate sweep: {args.sweep_points} points")
         line["surrogate"] = surrogate_leg(sim.ctx, args.sweep_points)

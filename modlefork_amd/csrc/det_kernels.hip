@@ -38,25 +38,27 @@ __global__ void __launch_bounds__(64) det_cells_kernel(DetParams P, const DetChr
 }
 
 // Wave-parallel deterministic mode (det_wave.h): block b is one wavefront that simulates work
-// item b with all 64 lanes; the working set and the engine-word ring live in dynamic LDS, and the
-// hardware dispatcher starts the next block wherever one finishes. There is deliberately no
+// item order[b] with all 64 lanes; the working set and the engine-word ring live in dynamic LDS,
+// and the hardware dispatcher starts the next block wherever one finishes (the host orders the
+// items by expected cost, largest first, so the longest cells do not start last). There is deliberately no
 // persistent loop over a work queue: with one, the optimizer merged the lane-0 region that
 // stores a finished cell's statistics with the next iteration's lane-0 queue fetch, and the
 // compiled loop sent lanes 1-63 back into the cell without lane 0 (the item index they then
 // broadcast from lane 0 was stale). A cell that fails an internal bound (an engine draw past
 // the ring's reach, a contact loop that never completes) counts in *err.
 __global__ void __launch_bounds__(64) det_wave_kernel(DetParams P, const DetChrom* chroms,
-                                                      const WorkItem* items, uint64_t nitems,
-                                                      uint64_t* cell_stats,
+                                                      const WorkItem* items, const uint32_t* order,
+                                                      uint64_t nitems, uint64_t* cell_stats,
                                                       unsigned long long* err) {
   extern __shared__ __align__(16) uint8_t lds[];
   const DevWave w;
-  const uint64_t it = blockIdx.x;
-  if (it >= nitems) return;
+  if (blockIdx.x >= nitems) return;
+  const uint64_t it = order[blockIdx.x];
   const WorkItem wi = items[it];
   const DetChrom ch = chroms[wi.chrom];
   uint64_t st[4] = {0, 0, 0, 0};
-  det_wave_cell(w, P, ch, lds, nullptr, P.hist_len, wi.seed, wi.target, nullptr, 0, st);
+  det_wave_cell<DevWave, kPlaceLds>(w, P, ch, lds, nullptr, P.hist_len, wi.seed, wi.target,
+                                    nullptr, 0, st);
   if (w.lane() == 0) {
     if (cell_stats)
       for (int k = 0; k < 4; ++k) cell_stats[4 * it + k] = st[k];
@@ -65,22 +67,23 @@ __global__ void __launch_bounds__(64) det_wave_kernel(DetParams P, const DetChro
 }
 
 // The same for cells whose arrays do not fit a CU's LDS: block b keeps its engine-word ring,
-// wave scratch and ziggurat tables in LDS and the cell's arrays in slab + b * stride (device
-// memory, one slot per work item).
+// wave scratch and ziggurat tables in LDS and the arrays of work item order[b] in
+// slab + b * stride (device memory, one slot per block of the launch).
 __global__ void __launch_bounds__(64) det_wave_slab_kernel(DetParams P, const DetChrom* chroms,
-                                                           const WorkItem* items, uint64_t nitems,
+                                                           const WorkItem* items,
+                                                           const uint32_t* order, uint64_t nitems,
                                                            uint8_t* slab, uint64_t stride,
                                                            uint64_t* cell_stats,
                                                            unsigned long long* err) {
   extern __shared__ __align__(16) uint8_t lds[];
   const DevWave w;
-  const uint64_t it = blockIdx.x;
-  if (it >= nitems) return;
+  if (blockIdx.x >= nitems) return;
+  const uint64_t it = order[blockIdx.x];
   const WorkItem wi = items[it];
   const DetChrom ch = chroms[wi.chrom];
   uint64_t st[4] = {0, 0, 0, 0};
-  det_wave_cell(w, P, ch, lds, slab + it * stride, P.hist_len, wi.seed, wi.target, nullptr, 0,
-                st);
+  det_wave_cell<DevWave, kPlaceMem>(w, P, ch, lds, slab + (uint64_t)blockIdx.x * stride, P.hist_len, wi.seed,
+                wi.target, nullptr, 0, st);
   if (w.lane() == 0) {
     if (cell_stats)
       for (int k = 0; k < 4; ++k) cell_stats[4 * it + k] = st[k];
@@ -210,21 +213,21 @@ hipError_t launch_det_cells(hipStream_t stream, const DetParams& P, const DetChr
 }
 
 hipError_t launch_det_wave(hipStream_t stream, const DetParams& P, const DetChrom* d_chroms,
-                           const WorkItem* d_items, uint64_t nitems, uint64_t lds_bytes,
-                           uint64_t* d_cell_stats, unsigned long long* d_err) {
+                           const WorkItem* d_items, const uint32_t* d_order, uint64_t nitems,
+                           uint64_t lds_bytes, uint64_t* d_cell_stats, unsigned long long* d_err) {
   if (nitems == 0) return hipSuccess;
   if (nitems > 0x7FFFFFFFull) return hipErrorInvalidValue;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(det_wave_kernel),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(det_wave_kernel, dim3((uint32_t)nitems), dim3(64), lds_bytes, stream, P,
-                     d_chroms, d_items, nitems, d_cell_stats, d_err);
+                     d_chroms, d_items, d_order, nitems, d_cell_stats, d_err);
   return hipGetLastError();
 }
 
 hipError_t launch_det_wave_slab(hipStream_t stream, const DetParams& P, const DetChrom* d_chroms,
-                                const WorkItem* d_items, uint64_t nitems, uint8_t* d_slab,
-                                uint64_t stride, uint64_t* d_cell_stats,
+                                const WorkItem* d_items, const uint32_t* d_order, uint64_t nitems,
+                                uint8_t* d_slab, uint64_t stride, uint64_t* d_cell_stats,
                                 unsigned long long* d_err) {
   if (nitems == 0) return hipSuccess;
   if (nitems > 0x7FFFFFFFull) return hipErrorInvalidValue;
@@ -233,7 +236,7 @@ hipError_t launch_det_wave_slab(hipStream_t stream, const DetParams& P, const De
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(det_wave_slab_kernel, dim3((uint32_t)nitems), dim3(64), lds_bytes, stream,
-                     P, d_chroms, d_items, nitems, d_slab, stride, d_cell_stats, d_err);
+                     P, d_chroms, d_items, d_order, nitems, d_slab, stride, d_cell_stats, d_err);
   return hipGetLastError();
 }
 

@@ -44,6 +44,7 @@ constexpr uint32_t kWL = 64;                  // lanes per cell
 constexpr uint64_t kWChunk = 32;              // words one lane generates per batch
 constexpr uint64_t kWBatch = kWL * kWChunk;   // 2048: the jump kDetJump advances
 constexpr uint64_t kWRing = 2 * kWBatch;      // ring of engine words (32 KB)
+static_assert(kWBatch == kDetJumpSteps, "det_jump.h must advance the engine by one batch");
 constexpr uint64_t kWMaxNewCount = 1024;      // larger batches of new LEFs sort on lane 0
 constexpr uint32_t kWMaxRepair = 64;          // odd-even passes before the serial sort
 constexpr uint64_t kWMaxAttempts = 1ull << 26;  // contact attempts before giving up (error)
@@ -289,8 +290,14 @@ __device__ __host__ inline uint64_t det_wave_bytes(uint64_t nlefs, uint64_t nbar
   return det_wave_fixed_bytes() + det_wave_cell_bytes(nlefs, nbar, hist);
 }
 
-// lds: the wave's LDS; cell: the cell's arrays (det_wave_cell_bytes), or null to place them in
-// LDS after the fixed part.
+// Where a cell's arrays live: in LDS after the fixed part, in device memory at cell_mem, or
+// either, chosen when the cell starts (cell_mem null: LDS). A fixed placement lets the compiler
+// address the arrays with LDS or global instructions instead of flat ones.
+enum : int { kPlaceLds = 0, kPlaceMem = 1, kPlaceAny = 2 };
+
+// lds: the wave's LDS; cell: the cell's arrays (det_wave_cell_bytes) for kPlaceMem, or for
+// kPlaceAny when not null.
+template <int kPlace>
 __device__ inline void det_wave_carve(WCell& s, uint8_t* lds, uint8_t* cell, uint64_t nlefs,
                                       uint64_t nbar, uint64_t hist) {
   uint64_t* f = reinterpret_cast<uint64_t*>(lds);
@@ -299,7 +306,10 @@ __device__ inline void det_wave_carve(WCell& s, uint8_t* lds, uint8_t* cell, uin
   s.xs = f;
   f += kWL * 4;
   s.r.zn = reinterpret_cast<const double*>(f);
-  uint64_t* p = reinterpret_cast<uint64_t*>(cell ? cell : lds + det_wave_fixed_bytes());
+  uint8_t* const in_lds = lds + det_wave_fixed_bytes();
+  uint64_t* p = reinterpret_cast<uint64_t*>(kPlace == kPlaceLds   ? in_lds
+                                            : kPlace == kPlaceMem ? cell
+                                                                  : (cell ? cell : in_lds));
   s.rev = p;
   s.fwd = p + nlefs;
   s.ep = p + 2 * nlefs;
@@ -1315,8 +1325,8 @@ __device__ inline void w_release(const W& w, const DetParams& P, const DetChrom&
 
 // simulate_one_cell (simulation.cpp:1056-1152) on one wavefront. base: LDS of det_wave_bytes(),
 // or of det_wave_fixed_bytes() when cell_mem (det_wave_cell_bytes() of device memory) holds the
-// cell's arrays.
-template <class W>
+// cell's arrays (kPlaceMem, or kPlaceAny with cell_mem not null).
+template <class W, int kPlace = kPlaceAny>
 __device__ inline void det_wave_cell(const W& w, const DetParams& P, const DetChrom& ch,
                                      uint8_t* base, uint8_t* cell_mem, uint64_t hist,
                                      uint64_t seed, uint64_t target,
@@ -1324,7 +1334,7 @@ __device__ inline void det_wave_cell(const W& w, const DetParams& P, const DetCh
                                      uint64_t* dbg = nullptr) {
   const uint32_t lane = w.lane();
   WCell s;
-  det_wave_carve(s, base, cell_mem, ch.nlefs, ch.nbar, hist);
+  det_wave_carve<kPlace>(s, base, cell_mem, ch.nlefs, ch.nbar, hist);
   s.dbg = dbg;
   const uint64_t N = ch.nlefs;
   s.n = N;

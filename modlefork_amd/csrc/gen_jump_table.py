@@ -1,5 +1,5 @@
-"""Writes det_jump.h: the Xoshiro256++ state transition raised to the power kWaveBatch (2048),
-as 256 columns of four 64-bit words.
+"""Writes det_jump.h: the Xoshiro256++ state transition raised to the power JUMP (2048, the
+words one batch of the wave-parallel engine ring holds), as 256 columns of four 64-bit words.
 
 The engine's state update (s1 << 17, the xor cascade and rotl(s3, 45)) is linear over GF(2), so
 the state after J steps is M^J applied to the state. Column b of M^J is the state reached after
@@ -40,6 +40,7 @@ def main(out):
     with open(out, "w") as fh:
         fh.write("// Generated by gen_jump_table.py: Xoshiro256++ transition to the power "
                  f"{JUMP}, column b = image of state bit b.\n#pragma once\n#include <stdint.h>\n"
+                 f"constexpr uint64_t kDetJumpSteps = {JUMP};\n"
                  f"__device__ __constant__ uint64_t kDetJump[256][4] = {{\n    {{"
                  + "},\n    {".join(rows) + "}};\n")
 

@@ -86,16 +86,17 @@ hipError_t launch_det_cells(hipStream_t stream, const DetParams& P, const DetChr
                             const WorkItem* d_items, uint64_t nitems, unsigned long long* d_next,
                             uint8_t* d_scratch, uint64_t stride, bool use_lds, uint32_t grid,
                             uint64_t* d_cell_stats);
-// Wave-parallel deterministic mode: one 64-lane block per work item, lds_bytes =
-// det_wave_bytes_host(max LEFs, max barriers, burn-in history) of dynamic LDS per block.
+// Wave-parallel deterministic mode: one 64-lane block per work item, block b taking item
+// d_order[b] (nitems entries), lds_bytes = det_wave_bytes_host(max LEFs, max barriers, burn-in
+// history) of dynamic LDS per block.
 hipError_t launch_det_wave(hipStream_t stream, const DetParams& P, const DetChrom* d_chroms,
-                           const WorkItem* d_items, uint64_t nitems, uint64_t lds_bytes,
-                           uint64_t* d_cell_stats, unsigned long long* d_err);
+                           const WorkItem* d_items, const uint32_t* d_order, uint64_t nitems,
+                           uint64_t lds_bytes, uint64_t* d_cell_stats, unsigned long long* d_err);
 // The same for cells whose arrays live in device memory: d_slab holds nitems slots of `stride`
 // bytes (det_wave_cell_bytes_host), LDS holds det_wave_fixed_bytes_host() per block.
 hipError_t launch_det_wave_slab(hipStream_t stream, const DetParams& P, const DetChrom* d_chroms,
-                                const WorkItem* d_items, uint64_t nitems, uint8_t* d_slab,
-                                uint64_t stride, uint64_t* d_cell_stats,
+                                const WorkItem* d_items, const uint32_t* d_order, uint64_t nitems,
+                                uint8_t* d_slab, uint64_t stride, uint64_t* d_cell_stats,
                                 unsigned long long* d_err);
 // One cell with its per-epoch hashes; d_cell: the cell's arrays in device memory, or null when
 // lds_bytes holds everything.

@@ -516,6 +516,7 @@ struct mgpu_job {
   uint64_t wave_stride = 0;
   uint64_t wave_slots = 0;
   uint8_t* d_wave_slab = nullptr;
+  uint32_t* d_det_order = nullptr;  // work items by expected cost, largest first
   unsigned long long* d_err = nullptr;
   FastJob fast{};
   hipGraph_t graph = nullptr;       // captured run (mgpu_job_capture)
@@ -620,7 +621,7 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
                   "per-epoch move arithmetic is 32-bit); use deterministic mode",
                   (unsigned long long)chroms[c].end, (unsigned long long)kFastMaxEnd);
     }
-    if (nlefs[c] == 0 || nlefs[c] > (1u << 24) || ncells[c] == 0) {
+    if (nlefs[c] == 0 || nlefs[c] > (1u << 24) || ncells[c] == 0 || ncells[c] > 0x7FFFFFFFull) {
       delete job;
       return fail(ctx, MGPU_EINVAL, "chromosome %llu: invalid LEF or cell count",
                   (unsigned long long)c);
@@ -658,8 +659,13 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
     job->items_per_chrom.push_back(ncells[c]);
     job->nitems += ncells[c];
   }
+  if (job->nitems > 0x7FFFFFFFull) {
+    delete job;
+    return fail(ctx, MGPU_EINVAL, "more than 2^31 - 1 cells in one job");
+  }
   const uint64_t off_chroms = pool.add(nchroms * sizeof(DetChrom));
   const uint64_t off_items = pool.add(job->nitems * sizeof(WorkItem));
+  const uint64_t off_order = pool.add(job->nitems * 4);
   const uint64_t inputs_end = pool.total;
   // bands
   for (uint64_t c = 0; c < nchroms; ++c) {
@@ -731,6 +737,7 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
   job->d_err = reinterpret_cast<unsigned long long*>(base + off_err);
   job->d_chroms = reinterpret_cast<DetChrom*>(base + off_chroms);
   job->d_items = reinterpret_cast<WorkItem*>(base + off_items);
+  job->d_det_order = reinterpret_cast<uint32_t*>(base + off_order);
   job->d_cell_stats = reinterpret_cast<uint64_t*>(base + off_stats);
   job->d_bands = reinterpret_cast<uint32_t*>(base + off_bands);
   if (p->track_1d_lef_position) job->d_occ = reinterpret_cast<unsigned long long*>(base + off_occ);
@@ -788,6 +795,22 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
   }
   std::memcpy(stage.data() + off_chroms, job->chroms.data(), nchroms * sizeof(DetChrom));
   std::memcpy(stage.data() + off_items, items.data(), items.size() * sizeof(WorkItem));
+  {
+    // A wave-parallel cell's time grows with its LEFs times its epochs (burn-in plus the epochs
+    // its contacts take at the chromosome's contacts per epoch); blocks are dispatched in order,
+    // so the costliest cells go first and the last ones to start are short.
+    std::vector<double> cost(items.size());
+    for (uint64_t i = 0; i < items.size(); ++i) {
+      const uint32_t c = items[i].chrom;
+      cost[i] = (double)nlefs[c] * ((double)items[i].target / (double)std::max<uint64_t>(1, cpe_of[c]) +
+                                    (double)p->burnin_history_length);
+    }
+    std::vector<uint32_t> order(items.size());
+    for (uint64_t i = 0; i < order.size(); ++i) order[i] = (uint32_t)i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+    std::memcpy(stage.data() + off_order, order.data(), order.size() * 4);
+  }
   e = hipMemcpy(base, stage.data(), stage.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     (void)hipFree(job->d_pool);
@@ -1011,12 +1034,12 @@ hipError_t enqueue_simulation(mgpu_job* job, hipStream_t s, hipEvent_t* ev = nul
     hipError_t e = ev && nev >= 2 ? hipEventRecord(ev[0], s) : hipSuccess;
     if (e == hipSuccess && det_uses_wave(job) && job->wave_stride) {
       for (uint64_t first = 0; e == hipSuccess && first < job->nitems; first += job->wave_slots)
-        e = launch_det_wave_slab(s, P, job->d_chroms, job->d_items + first,
+        e = launch_det_wave_slab(s, P, job->d_chroms, job->d_items, job->d_det_order + first,
                                  std::min(job->wave_slots, job->nitems - first), job->d_wave_slab,
-                                 job->wave_stride, job->d_cell_stats + 4 * first, job->d_err);
+                                 job->wave_stride, job->d_cell_stats, job->d_err);
     } else if (e == hipSuccess && det_uses_wave(job)) {
-      e = launch_det_wave(s, P, job->d_chroms, job->d_items, job->nitems, job->wave_bytes,
-                          job->d_cell_stats, job->d_err);
+      e = launch_det_wave(s, P, job->d_chroms, job->d_items, job->d_det_order, job->nitems,
+                          job->wave_bytes, job->d_cell_stats, job->d_err);
     } else if (e == hipSuccess)
       e = launch_det_cells(s, P, job->d_chroms, job->d_items, job->nitems, job->d_next,
                            job->d_scratch, job->stride, job->use_lds, job->grid,
