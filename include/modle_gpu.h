@@ -338,8 +338,9 @@ typedef struct mgpu_launch_stats {
 int mgpu_job_launch_stats(mgpu_job* job, mgpu_launch_stats* out, uint32_t max, uint32_t* n);
 /* Name of the simulation kernel the job's runs launch (as rocprof lists it, without template
  * arguments): "fast_cells_kernel", "det_wave_kernel" (deterministic mode, one wavefront per
- * cell) or "det_cells_kernel" (deterministic mode on one lane per cell: cells too large for
- * the LDS, optional outputs on, or mgpu_debug_det_kernel). No counterpart in the reference. */
+ * cell, its working set in LDS), "det_wave_slab_kernel" (the same with the cells' arrays in
+ * device memory, for cells too large for the LDS) or "det_cells_kernel" (deterministic mode on
+ * one lane per cell, after mgpu_debug_det_kernel). No counterpart in the reference. */
 int mgpu_job_kernel(mgpu_job* job, char* name, uint64_t cap);
 
 /* ---- optional outputs ----
@@ -467,9 +468,9 @@ int mgpu_debug_phase_times(mgpu_ctx* ctx, int enable, uint64_t* out, uint64_t n)
  * loop's order instead. Results must not change: exposed to check exactly that. */
 int mgpu_debug_fast_serial_fixups(mgpu_job* job, int serial);
 /* Deterministic mode runs each cell on one wavefront (all 64 lanes, the reference's draws in
- * the reference's order) when its working set fits the CU's LDS and no optional output is on;
- * serial != 0 makes later jobs and traces of this context run every cell on the one-lane kernel
- * instead. Both give the reference's bands: exposed to compare them. */
+ * the reference's order); serial != 0 makes later jobs and traces of this context run every
+ * cell on the one-lane kernel instead. Both give the reference's bands and optional outputs:
+ * exposed to compare them. */
 int mgpu_debug_det_kernel(mgpu_ctx* ctx, int serial);
 /* Per-epoch state hash of one cell in deterministic mode (for locating a first divergence). */
 int mgpu_debug_cell_trace(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom* chrom,

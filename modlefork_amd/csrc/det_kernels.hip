@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(64) det_wave_kernel(DetParams P, const DetChro
   const DetChrom ch = chroms[wi.chrom];
   uint64_t st[4] = {0, 0, 0, 0};
   det_wave_cell<DevWave, kPlaceLds>(w, P, ch, lds, nullptr, P.hist_len, wi.seed, wi.target,
-                                    nullptr, 0, st);
+                                    nullptr, 0, st, (uint32_t)it);
   if (w.lane() == 0) {
     if (cell_stats)
       for (int k = 0; k < 4; ++k) cell_stats[4 * it + k] = st[k];
@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(64) det_wave_slab_kernel(DetParams P, const De
   const DetChrom ch = chroms[wi.chrom];
   uint64_t st[4] = {0, 0, 0, 0};
   det_wave_cell<DevWave, kPlaceMem>(w, P, ch, lds, slab + (uint64_t)blockIdx.x * stride, P.hist_len, wi.seed,
-                wi.target, nullptr, 0, st);
+                wi.target, nullptr, 0, st, (uint32_t)it);
   if (w.lane() == 0) {
     if (cell_stats)
       for (int k = 0; k < 4; ++k) cell_stats[4 * it + k] = st[k];
@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(64) det_wave_trace_kernel(DetParams P, DetChro
   extern __shared__ __align__(16) uint8_t lds[];
   const DevWave w;
   uint64_t st[4] = {0, 0, 0, 0};
-  det_wave_cell(w, P, ch, lds, cell_mem, P.hist_len, seed, target, trace, max_trace, st);
+  det_wave_cell(w, P, ch, lds, cell_mem, P.hist_len, seed, target, trace, max_trace, st, 0);
   if (w.lane() == 0)
     for (int k = 0; k < 4; ++k) out[k] = st[k];
 }

@@ -1210,10 +1210,13 @@ __device__ inline void w_burnin(const W& w, const DetParams& P, const DetChrom& 
 // register_contacts_loop / _tad (register_contacts.cpp:129-198): attempts run 64 at a time, at
 // the offsets they would have if every earlier attempt took its full word count (LEF index,
 // two GEV draws with noise, two positions for TAD contacts); the batch is cut at the first
-// attempt that took another count or at the attempt that completes the count.
+// attempt that took another count or at the attempt that completes the count. With `occ`, the
+// same attempts as loop contacts register 1D LEF occupancy instead
+// (register_1d_lef_occupancy, register_contacts.cpp:200-232): each accepted attempt adds one to
+// the occupancy bins of both of its (noisified) unit positions.
 template <class W>
 __device__ inline void w_register(const W& w, const DetParams& P, const DetChrom& ch, WCell& s,
-                                  uint64_t nreg, bool tad) {
+                                  uint64_t nreg, bool tad, bool occ = false) {
   const uint32_t lane = w.lane();
   const uint64_t sp = ch.start + 1, epp = ch.end - 1;
   const double spd = (double)sp, epd = (double)epp;
@@ -1262,7 +1265,14 @@ __device__ inline void w_register(const W& w, const DetParams& P, const DetChrom
       last = (uint32_t)__builtin_ctzll(m);
       valid = last == 63 ? ~0ull : ((2ull << last) - 1);
     }
-    if (ok && ((valid >> lane) & 1ull)) band_add(ch, (q1 - sp) / P.bin_size, (q2 - sp) / P.bin_size);
+    if (ok && ((valid >> lane) & 1ull)) {
+      if (occ) {
+        atomicAdd(&ch.occupancy[(q1 - sp) / P.bin_size], 1ULL);
+        atomicAdd(&ch.occupancy[(q2 - sp) / P.bin_size], 1ULL);
+      } else {
+        band_add(ch, (q1 - sp) / P.bin_size, (q2 - sp) / P.bin_size);
+      }
+    }
     const uint64_t nsucc = (uint64_t)__builtin_popcountll(succ & valid);
     nreg -= nsucc;
     if (last != 0xFFFFFFFFu) {
@@ -1302,7 +1312,65 @@ __device__ inline void w_sample_contacts(const W& w, const DetParams& P, const D
   }
   w_register(w, P, ch, s, nloop, false);
   w_register(w, P, ch, s, n - nloop, true);
+  if (ch.occupancy) w_register(w, P, ch, s, n, false, true);
   s.contacts += n;
+}
+
+// dump_stats (simulation.cpp:1166-1224) as det_log_state records it: stalls are collisions that
+// occurred, the typed counts sum both sides, and the average loop size is the sequential double
+// sum over the active LEFs divided by their count. Loop sizes are integers below 2^32, so while
+// their total stays below 2^53 every partial sum is exact in any order and the wave's integer
+// sum gives the same double; a larger total is summed on lane 0 in LEF order.
+template <class W>
+__device__ inline void w_log_state(const W& w, const DetParams& P, const DetChrom& ch, WCell& s,
+                                   uint32_t item) {
+  const uint32_t lane = w.lane();
+  uint64_t ab = 0, sr = 0, sf = 0, sb = 0, lb = 0, pr = 0, se = 0, len = 0;
+  for (uint64_t i = lane; i < ch.nbar; i += kWL) ab += s.bst[i] == 1;
+  for (uint64_t i = lane; i < s.nact; i += kWL) {
+    const uint64_t rc = s.rc[i], fc = s.fc[i];
+    const bool a = c_any(rc), b = c_any(fc);
+    sr += a;
+    sf += b;
+    sb += a && b;
+    lb += (uint64_t)c_is(rc, kEvLefBar) + (uint64_t)c_is(fc, kEvLefBar);
+    pr += (uint64_t)c_is(rc, kEvPrimary) + (uint64_t)c_is(fc, kEvPrimary);
+    se += (uint64_t)c_is(rc, kEvSecondary) + (uint64_t)c_is(fc, kEvSecondary);
+    len += s.fwd[i] - s.rev[i];
+  }
+  ab = w_sum(w, s.xs, ab);
+  sr = w_sum(w, s.xs, sr);
+  sf = w_sum(w, s.xs, sf);
+  sb = w_sum(w, s.xs, sb);
+  lb = w_sum(w, s.xs, lb);
+  pr = w_sum(w, s.xs, pr);
+  se = w_sum(w, s.xs, se);
+  len = w_sum(w, s.xs, len);
+  if (lane == 0) {
+    double acc = (double)len;
+    if (len >= (1ull << 53)) {
+      acc = 0.0;
+      for (uint64_t i = 0; i < s.nact; ++i) acc = acc + (double)(s.fwd[i] - s.rev[i]);
+    }
+    const unsigned long long slot = atomicAdd(P.log.count, 1ULL);
+    if (slot < P.log.cap) {
+      StateRecord r{};
+      r.item = item;
+      r.epoch = (uint32_t)s.epoch;
+      r.burnin = s.burnin_done ? 0u : 1u;
+      r.nact = (uint32_t)s.nact;
+      r.active_barriers = (uint32_t)ab;
+      r.stalls_rev = (uint32_t)sr;
+      r.stalls_fwd = (uint32_t)sf;
+      r.stalls_both = (uint32_t)sb;
+      r.lef_bar = (uint32_t)lb;
+      r.primary = (uint32_t)pr;
+      r.secondary = (uint32_t)se;
+      r.avg_loop_size = s.nact == 0 ? 0.0 : acc / (double)s.nact;
+      P.log.rec[slot] = r;
+    }
+  }
+  w.sync();
 }
 
 template <class W>
@@ -1331,7 +1399,7 @@ __device__ inline void det_wave_cell(const W& w, const DetParams& P, const DetCh
                                      uint8_t* base, uint8_t* cell_mem, uint64_t hist,
                                      uint64_t seed, uint64_t target,
                                      uint64_t* trace, uint64_t max_trace, uint64_t* stats,
-                                     uint64_t* dbg = nullptr) {
+                                     uint32_t item, uint64_t* dbg = nullptr) {
   const uint32_t lane = w.lane();
   WCell s;
   det_wave_carve<kPlace>(s, base, cell_mem, ch.nlefs, ch.nbar, hist);
@@ -1443,6 +1511,7 @@ __device__ inline void det_wave_cell(const W& w, const DetParams& P, const DetCh
       s.fwd[i] += s.fm[i];
     }
     w.sync();
+    if (P.log.rec) w_log_state(w, P, ch, s, item);
     w_release(w, P, ch, s);
     if (trace && s.epoch < max_trace) {
       if (lane == 0) {

@@ -1016,11 +1016,10 @@ hipError_t enqueue_clears(mgpu_job* job, hipStream_t s) {
   return e;
 }
 
-// Whether a deterministic job's cells run on one wavefront each (det_wave.h): no optional output
-// (1D occupancy, the state log) may be on and the context must not ask for the one-lane kernel.
+// Whether a deterministic job's cells run on one wavefront each (det_wave.h), as they do unless
+// the context asks for the one-lane kernel.
 bool det_wave_fits(const mgpu_job* job) {
-  return job->mode == MGPU_MODE_DETERMINISTIC && job->wave_bytes != 0 && !job->d_occ &&
-         !job->log.rec && !job->ctx->det_serial;
+  return job->mode == MGPU_MODE_DETERMINISTIC && job->wave_bytes != 0 && !job->ctx->det_serial;
 }
 
 bool det_uses_wave(const mgpu_job* job) { return det_wave_fits(job); }

@@ -86,6 +86,9 @@ struct Sched {
 Sched* g_sched = nullptr;
 int g_lane_order = 0;
 int g_separate_cell = 0;
+// optional outputs of the next emulated cells (null: off)
+unsigned long long* g_occupancy = nullptr;
+mgpu::StateLog g_log{nullptr, nullptr, 0};
 uint64_t g_last_syncs = 0;
 
 struct EmuWave {
@@ -130,7 +133,7 @@ void lane_entry(int ln) {
   mgpu::det_wave_cell(w, g_job->P, g_job->ch, g_job->lds, g_job->cell, g_job->P.hist_len,
                       g_job->seed,
                       g_job->target, g_job->trace, g_job->max_trace,
-                      ln == 0 ? g_job->stats : nullptr, g_job->dbg);
+                      ln == 0 ? g_job->stats : nullptr, 0, g_job->dbg);
   Sched& s = *g_sched;
   s.done[ln] = true;
   for (int k = 0; k < kLanes; ++k)
@@ -151,6 +154,17 @@ void emu_set_lane_order(int order) { g_lane_order = order; }
 // 1: the cell's arrays in an allocation of their own (the layout of the device-memory
 // placement), 0: after the fixed part of the LDS image.
 void emu_set_separate_cell(int on) { g_separate_cell = on; }
+
+// Optional outputs of the cells emulated next: 1D LEF occupancy bins (u64, added to) and the
+// state log (records of mgpu::StateRecord, `count` advanced per record, records at or past
+// `cap` not written). Null pointers turn an output off.
+void emu_set_outputs(unsigned long long* occupancy, mgpu::StateRecord* rec,
+                     unsigned long long* count, uint64_t cap) {
+  g_occupancy = occupancy;
+  g_log.rec = rec;
+  g_log.count = count;
+  g_log.cap = cap;
+}
 
 // Wave synchronisations lane 0 met in the last emulated cell (ballots count two).
 uint64_t emu_last_syncs() { return g_last_syncs; }
@@ -198,6 +212,7 @@ int emu_det_wave_cell(const rc_config* c, const rc_chrom* chrom, const rc_barrie
   P.min_burnin_epochs = c->min_burnin_epochs;
   P.max_burnin_epochs = c->max_burnin_epochs;
   P.target_contact_density = c->target_contact_density;
+  P.log = g_log;
 
   const uint64_t cpe = rc_contacts_per_epoch(c, nlefs);
   std::vector<double> qn(cpe + 1);
@@ -222,7 +237,7 @@ int emu_det_wave_cell(const rc_config* c, const rc_chrom* chrom, const rc_barrie
   ch.target_epochs = target_epochs;
   ch.bind_rate = (double)nlefs / (double)c->burnin_target_epochs_for_lef_activation;
   ch.qn = qn.data();
-  ch.occupancy = nullptr;
+  ch.occupancy = g_occupancy;
 
   // the cell's arrays in a separate allocation when the caller asks (as the device-memory
   // placement of large cells does), else after the fixed part

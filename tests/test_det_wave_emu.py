@@ -40,6 +40,8 @@ def emu():
         _lib.emu_det_wave_bytes.argtypes = [C.c_uint64] * 3
         _lib.emu_set_separate_cell.restype = None
         _lib.emu_set_separate_cell.argtypes = [C.c_int]
+        _lib.emu_set_outputs.restype = None
+        _lib.emu_set_outputs.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]
     return _lib
 
 
@@ -160,6 +162,58 @@ def test_cell_arrays_in_separate_memory_match_the_oracle():
         compare_cells(setup_case(2_000_000, 40, 13, 4), range(2))
     finally:
         emu().emu_set_separate_cell(0)
+
+
+# mgpu::StateRecord as the kernel writes it
+EMU_RECORD = np.dtype([("item", np.uint32), ("epoch", np.uint32), ("burnin", np.uint32),
+                       ("nact", np.uint32), ("active_barriers", np.uint32),
+                       ("stalls_rev", np.uint32), ("stalls_fwd", np.uint32),
+                       ("stalls_both", np.uint32), ("lef_bar", np.uint32), ("primary", np.uint32),
+                       ("secondary", np.uint32), ("reserved", np.uint32),
+                       ("avg_loop_size", np.float64)])
+
+
+def test_occupancy_and_state_log_match_the_oracle():
+    """With 1D LEF occupancy and the state log on (register_1d_lef_occupancy,
+    register_contacts.cpp:200-232; dump_stats, simulation.cpp:1166-1224): the occupancy
+    attempts draw after each epoch's contacts, so the band, the occupancy track and every
+    per-epoch record must equal the oracle's run with the same outputs on."""
+    rc, ch, bars, nl, t, te, nrows, ncols = setup_case(2_000_000, 40, 17, 4)
+    cells = 2
+    size = 2_000_000
+    nbins = (size + 5000 - 1) // 5000
+    occ = np.zeros(nbins, np.uint64)
+    oband, omissed, ost, orecs = R.simulate_cells(rc, ch, bars, nl, 0, cells, t, te, nrows, ncols,
+                                                  2, occupancy=occ, state_log_cap=100_000)
+    docc = np.zeros(nbins, np.uint64)
+    band = np.zeros(nrows * ncols + 1, np.uint32)
+    cap = 20_000
+    for cell in range(cells):
+        rec = np.zeros(cap, EMU_RECORD)
+        count = np.zeros(1, np.uint64)
+        emu().emu_set_outputs(docc.ctypes.data, rec.ctypes.data, count.ctypes.data, cap)
+        try:
+            b, missed, st, _ = run_emu(rc, ch, bars, nl, cell, int(t[cell]), te, nrows, ncols,
+                                       max_trace=1)
+        finally:
+            emu().emu_set_outputs(None, None, None, 0)
+        band += b
+        assert tuple(int(x) for x in st) == tuple(int(x) for x in ost[cell])
+        n = int(count[0])
+        assert n == int(st[0]) and n <= cap
+        o = orecs[orecs["cell_id"] == cell]
+        d = rec[:n]
+        assert len(o) == n
+        assert np.array_equal(d["epoch"], o["epoch"].astype(np.uint32))
+        assert np.array_equal(d["burnin"], o["burnin"].astype(np.uint32))
+        assert np.array_equal(d["nact"], o["num_active_lefs"])
+        for f in ("active_barriers", "stalls_rev", "stalls_fwd", "stalls_both", "lef_bar",
+                  "primary", "secondary"):
+            assert np.array_equal(d[f].astype(np.uint64), o[f]), f
+        assert np.array_equal(d["avg_loop_size"], o["avg_loop_size"])
+    assert np.array_equal(docc, occ), int(np.count_nonzero(docc != occ))
+    assert int(docc.sum()) == 2 * int(ost[:, 2].sum())
+    assert np.array_equal(band, oband)
 
 
 def test_every_exact_fallback_path_was_taken():

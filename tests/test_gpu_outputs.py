@@ -70,11 +70,20 @@ def device_outputs(ctx, wl, mode):
     return out[name][0], out[name][1], stats, sim.lef_1d_occupancy[name], sim.state_log, sim
 
 
-def test_deterministic_outputs_bit_exact(ctx):
+@pytest.fixture(params=["wave", "lane"])
+def det_kernel(ctx, request):
+    """Deterministic mode on the wave-parallel kernel or the one-lane kernel
+    (mgpu_debug_det_kernel)."""
+    ctx.check(L.lib().mgpu_debug_det_kernel(ctx.ptr, 0 if request.param == "wave" else 1))
+    yield request.param
+    ctx.check(L.lib().mgpu_debug_det_kernel(ctx.ptr, 0))
+
+
+def test_deterministic_outputs_bit_exact(ctx, det_kernel):
     wl = tiny()
-    say("outputs det: oracle")
+    say(f"outputs det ({det_kernel} kernel): oracle")
     band, missed, st, occ, recs = oracle_outputs(wl)
-    say("outputs det: device")
+    say(f"outputs det ({det_kernel} kernel): device")
     dband, dmissed, stats, docc, drecs, sim = device_outputs(ctx, wl, L.MODE_DETERMINISTIC)
     # tracking consumes draws after the contacts: the band must still match the oracle's run
     # with tracking on
