@@ -207,9 +207,11 @@ def deterministic_leg(ctx, nthreads, no_cpu, genome_cpu=None, phases=False):
         for _ in range(runs):
             st = job.run()
         wall = (time.perf_counter() - t0) / runs
+        words = int(job.cell_stats()[:, 3].sum())
         job.close()
         rec = {"cells": int(st.cells), "cells_per_sec": st.cells / wall,
                "contacts_per_sec": st.contacts / wall, "epochs_per_sec": st.epochs / wall,
+               "draws_per_sec": words / wall, "draws_per_cell_epoch": words / st.epochs,
                "kernel_ms": float(st.kernel_ms), "kernel": kernel}
         log(f"deterministic {wl.name}: {st.cells} cells in {wall * 1e3:.1f} ms "
             f"({st.cells / wall:.1f} cells/s, {kernel})")
@@ -233,6 +235,7 @@ def deterministic_leg(ctx, nthreads, no_cpu, genome_cpu=None, phases=False):
     t0 = time.perf_counter()
     st = job.run()
     wall = time.perf_counter() - t0
+    words = int(job.cell_stats()[:, 3].sum())
     busy = None
     if phases:
         ctx.phase_times(True)
@@ -254,6 +257,7 @@ def deterministic_leg(ctx, nthreads, no_cpu, genome_cpu=None, phases=False):
     rec = {"workload": "genome-24chroms-5kb, first 64 cells per chromosome",
            "cells": int(st.cells), "cells_per_sec": st.cells / wall,
            "contacts_per_sec": st.contacts / wall, "epochs_per_sec": st.epochs / wall,
+           "draws_per_sec": words / wall, "draws_per_cell_epoch": words / st.epochs,
            "kernel_ms": float(st.kernel_ms), "kernel": kernel}
     log(f"deterministic genome sample: {st.cells} cells in {wall * 1e3:.1f} ms "
         f"({st.cells / wall:.1f} cells/s, {kernel})")
@@ -458,6 +462,9 @@ def main():
                     help="after the timed steps, run one more step with the epoch-phase timers on "
                          "and print each phase's share of the time to stderr (fast mode, or the "
                          "wave-parallel deterministic kernel)")
+    ap.add_argument("--write-cool", default="", metavar="PATH",
+                    help="also write the bands of one run to PATH as .cool and time it (outside "
+                         "the timed steps; the file is removed afterwards)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (reference for HBM counters)")
     args = ap.parse_args()
@@ -488,9 +495,14 @@ def main():
     wl = build_workload(args.workload, args.ncells)
     mode = L.MODE_FAST if args.mode == "fast" else L.MODE_DETERMINISTIC
     sim = S.Simulation(wl.options, S.Context(0))
+    # genome import: barrier records to sorted device-ready tables, the task split (the
+    # reference's import_genome and run_simulate planning), reported beside the timed steps
+    tp = time.perf_counter()
     names, chroms, bars, plans = sim.prepare(wl)
+    prepare_ms = 1000.0 * (time.perf_counter() - tp)
     log(f"workload {wl.name}: {len(chroms)} chromosomes, "
-        f"{sum(len(p.targets) for p in plans)} cells, mode {args.mode}")
+        f"{sum(len(p.targets) for p in plans)} cells, mode {args.mode} "
+        f"(import and planning {prepare_ms:.1f} ms)")
     job = S.Job(sim.ctx, sim.params, chroms, bars, plans, mode)
     autotuned = None
     if args.threads:
@@ -545,15 +557,32 @@ def main():
     job.run()
     t2 = time.perf_counter()
     band_bytes = 0
+    results = {}
     for c in range(len(chroms)):
-        band, _ = job.band(c)
+        band, missed = job.band(c)
         band_bytes += band.nbytes
+        if args.write_cool:
+            results[names[c]] = (band, missed, plans[c])
     t3 = time.perf_counter()
     host_bands = {"cells_per_s": cells / (t3 - t1), "readback_ms": 1000.0 * (t3 - t2),
                   "band_bytes": band_bytes,
                   "readback_gb_per_s": band_bytes / (t3 - t2) / 1e9}
     log(f"with the bands read back to host memory: {host_bands['cells_per_s']:.1f} cells/s "
         f"({host_bands['readback_ms']:.1f} ms for {band_bytes / 1e9:.2f} GB)")
+    cool = None
+    if args.write_cool:
+        # the output step (Simulation::write_contacts_to_disk) on the bands just read back
+        from modlefork_amd import cooler as CL
+
+        t4 = time.perf_counter()
+        CL.write_cool(args.write_cool, wl.chroms, results, int(sim.params.bin_size),
+                      metadata={"workload": wl.name})
+        cool_ms = 1000.0 * (time.perf_counter() - t4)
+        cool = {"ms": cool_ms, "file_bytes": os.path.getsize(args.write_cool),
+                "band_bytes": band_bytes}
+        os.remove(args.write_cool)
+        log(f".cool write: {cool_ms:.1f} ms, {cool['file_bytes'] / 1e9:.2f} GB file")
+        del results
 
     # roofline of the simulation kernel. Fast mode launches it once per working-set size tier,
     # each launch on a stream of its own, so the launches of one run overlap on the chip (a
@@ -628,6 +657,8 @@ def main():
                    "launches": launches, "autotuned": autotuned},
         "contacts_per_sec": contacts * args.steps / wall,
         "with_host_readback": host_bands,
+        "genome_import_ms": prepare_ms,
+        "cool_write": cool,
         "epochs_per_sec": epochs * args.steps / wall,
         "kernel_ms": float(np.mean(kernel_ms)),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -342,6 +342,13 @@ int mgpu_job_launch_stats(mgpu_job* job, mgpu_launch_stats* out, uint32_t max, u
  * device memory, for cells too large for the LDS) or "det_cells_kernel" (deterministic mode on
  * one lane per cell, after mgpu_debug_det_kernel). No counterpart in the reference. */
 int mgpu_job_kernel(mgpu_job* job, char* name, uint64_t cap);
+/* Per-cell figures of the last run, four per work item in job order (chromosomes in the order
+ * given, their cells in order): epochs, burn-in epochs, contacts registered, and in
+ * deterministic mode the engine words the cell consumed (Xoshiro256++ outputs, as
+ * Simulation::State's generator advanced; 0 in fast mode). max: entries out holds (4 per
+ * cell); *n: the job's cells. The reference keeps these per task in its state and logs
+ * (simulation.cpp:1056-1152); no entry point of its own. */
+int mgpu_job_cell_stats(mgpu_job* job, uint64_t* out, uint64_t max, uint64_t* n);
 
 /* ---- optional outputs ----
  * 1D LEF occupancy (Chromosome::lef_1d_occupancy, genome.cpp:236-245, filled by

@@ -1617,6 +1617,17 @@ int mgpu_debug_phase_times(mgpu_ctx* ctx, int enable, uint64_t* out, uint64_t n)
   return MGPU_OK;
 }
 
+int mgpu_job_cell_stats(mgpu_job* job, uint64_t* out, uint64_t max, uint64_t* n) {
+  if (!job || !n) return fail(job ? job->ctx : nullptr, MGPU_EINVAL, "null argument");
+  mgpu_ctx* ctx = job->ctx;
+  *n = job->nitems;
+  if (!out || max == 0) return MGPU_OK;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const uint64_t k = std::min<uint64_t>(max, job->nitems * 4);
+  HIPCHK(ctx, hipMemcpy(out, job->d_cell_stats, k * 8, hipMemcpyDeviceToHost));
+  return MGPU_OK;
+}
+
 int mgpu_job_kernel(mgpu_job* job, char* name, uint64_t cap) {
   if (!job || !name || cap == 0) return fail(job ? job->ctx : nullptr, MGPU_EINVAL, "null argument");
   const char* k = job->mode == MGPU_MODE_FAST ? "fast_cells_kernel"

@@ -329,6 +329,16 @@ class Job:
         self.ctx.check(rc)
         return np.zeros(0, STATE_DTYPE), 0
 
+    def cell_stats(self) -> np.ndarray:
+        """[cells, 4] u64 of the last run, in job order: epochs, burn-in epochs, contacts, and
+        the engine words consumed (deterministic mode; 0 in fast mode)."""
+        n = C.c_uint64(0)
+        self.ctx.check(L.lib().mgpu_job_cell_stats(self.ptr, None, 0, C.byref(n)))
+        out = np.zeros((n.value, 4), np.uint64)
+        self.ctx.check(L.lib().mgpu_job_cell_stats(
+            self.ptr, out.ctypes.data_as(C.POINTER(C.c_uint64)), out.size, C.byref(n)))
+        return out
+
     def kernel(self) -> str:
         """Name of the simulation kernel this job launches (as rocprof lists it)."""
         buf = C.create_string_buffer(64)

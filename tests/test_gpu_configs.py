@@ -190,9 +190,12 @@ def test_genome_chromosome_deterministic_bit_exact(ctx, which, ncells, kernel):
     assert job.kernel() == kernel
     st = job.run()
     band, missed = job.band(0)
+    cs = job.cell_stats()
     job.close()
     say(f"genome deterministic ({name}): {st.kernel_ms:.1f} ms, {st.epochs} epochs")
     assert int(st.epochs) == int(ost[:, 0].sum())
+    # per cell: epochs, burn-in epochs, contacts and engine words consumed
+    assert np.array_equal(cs, ost), (cs, ost)
     assert missed == omissed
     assert np.array_equal(band, oband), int(np.count_nonzero(band != oband))
 
