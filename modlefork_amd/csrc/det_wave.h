@@ -415,7 +415,7 @@ __device__ inline uint64_t w_count_less(const uint32_t* arr, uint64_t len, const
 }
 
 // Orders rk[0, n) as rank_lefs does. keep: original order; kl, nl, ns: scratch of n entries;
-// tmp: n entries for the serial sorters.
+// tmp: n entries for the serial sorters, or the slot marks of the merge.
 template <class W>
 __device__ inline void w_rank_side(const W& w, WCell& s, uint32_t* rk, const uint64_t* pos,
                                    bool asc, uint64_t n, bool partially_sorted, uint32_t* keep,
@@ -429,17 +429,32 @@ __device__ inline void w_rank_side(const W& w, WCell& s, uint32_t* rk, const uin
   const uint64_t nn = w_compact(w, keep, n, nl, [&](uint32_t v) { return s.ep[v] == epoch; });
   bool serial = nn > kWMaxNewCount;
   if (!serial) {
-    // new LEFs: rank by counting (they share the binding epoch, so equal positions tie)
+    // new LEFs: rank by counting. They share the binding epoch, so the key is the position
+    // (below 2^32: chromosome ends are) and equal positions tie. Each lane holds the positions
+    // of one chunk of 64 new LEFs in a register and the others read them with readlane.
     bool tie = false;
-    for (uint64_t b = lane; b < nn; b += kWL) {
-      const uint32_t x = nl[b];
+    for (uint64_t q = 0; q < nn; q += kWL) {
+      const uint64_t b = q + lane;
+      const bool mine = b < nn;
+      const uint32_t x = mine ? nl[b] : 0u;
+      const uint32_t px = mine ? (uint32_t)pos[x] : 0u;
       uint64_t c = 0;
-      for (uint64_t j = 0; j < nn; ++j) {
-        const uint32_t y = nl[j];
-        c += key.less(y, x);
-        tie |= j != b && key.tie(y, x);
+      bool tq = false;  // a tie of this lane's LEF in this chunk
+      for (uint64_t r = 0; r < nn; r += kWL) {
+        const uint64_t j = r + lane;
+        const uint32_t py = j < nn ? (uint32_t)pos[nl[j]] : 0u;
+        const uint32_t m = (uint32_t)(nn - r < kWL ? nn - r : kWL);
+        for (uint32_t t = 0; t < m; ++t) {
+          const uint32_t y = (uint32_t)w.bcast(py, t);
+          c += y < px;
+          tq |= y == px && r + t != b;
+        }
       }
-      ns[c] = x;
+      // ties found in earlier chunks stay: a lane without a LEF in this chunk adds nothing
+      if (mine) {
+        ns[c] = x;
+        tie |= tq;
+      }
     }
     serial = w.ballot(tie) != 0;
   }
@@ -476,13 +491,26 @@ __device__ inline void w_rank_side(const W& w, WCell& s, uint32_t* rk, const uin
     }
   }
   if (!serial) {
-    for (uint64_t a = lane; a < nk; a += kWL) {
-      const uint32_t x = kl[a];
-      rk[a + w_count_less(ns, nn, key, x)] = x;
-    }
+    // merge: each new LEF's slot is its rank plus the kept LEFs before it (new and kept LEFs
+    // never tie: their binding epochs differ); the kept LEFs fill the other slots in order
+    uint32_t* mark = tmp;
+    for (uint64_t k = lane; k < n; k += kWL) mark[k] = 0;
+    w.sync();
     for (uint64_t b = lane; b < nn; b += kWL) {
       const uint32_t x = ns[b];
-      rk[b + w_count_less(kl, nk, key, x)] = x;
+      const uint64_t slot = b + w_count_less(kl, nk, key, x);
+      rk[slot] = x;
+      mark[slot] = 1;
+    }
+    w.sync();
+    const uint64_t lt = lanes_below(lane);
+    uint64_t before = 0;  // new LEFs' slots below this chunk
+    for (uint64_t g = 0; g < n; g += kWL) {
+      const uint64_t k = g + lane;
+      const bool isnew = k < n && mark[k] != 0;
+      const uint64_t m = w.ballot(isnew);
+      if (k < n && !isnew) rk[k] = kl[k - before - (uint64_t)__builtin_popcountll(m & lt)];
+      before += (uint64_t)__builtin_popcountll(m);
     }
     w.sync();
   }

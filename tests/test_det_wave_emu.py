@@ -129,6 +129,19 @@ def test_crowded_chromosome_with_units_stacked_at_its_ends():
     compare_cells(case, range(3))
 
 
+def test_many_new_lefs_with_tied_positions():
+    """More than 64 LEFs bound in one epoch (several chunks of the new-LEF ranking) with equal
+    binding positions among them, which must send the rank order to the serial sorters: 1 200
+    LEFs on 60 kb with a 10 kb processivity rebind about 190 LEFs per epoch on 60 000
+    positions."""
+    before = int(DBG[DBG_NAMES.index("rank_serial")])
+    case = setup_case(60_000, 10, 21, 2, dict(number_of_lefs_per_mbp=20_000,
+                                              avg_lef_processivity=10_000),
+                      bin_size=1000, width=20_000)
+    compare_cells(case, range(1))
+    assert int(DBG[DBG_NAMES.index("rank_serial")]) > before
+
+
 def test_noise_free_tad_contacts_and_skipped_burnin():
     case = setup_case(2_000_000, 40, 5, 4, dict(tad_to_loop_contact_ratio=2.0, noisify_contacts=0,
                                                 skip_burnin=True))
