@@ -579,15 +579,23 @@ __device__ inline bool w_adjust_side(const W& w, const DetChrom& ch, WCell& s, u
   int64_t* t1 = reinterpret_cast<int64_t*>(s.x1);  // after it
   const uint32_t* rk = rev ? s.rr : s.fr;
   const int64_t start = (int64_t)ch.start, endm1 = (int64_t)ch.end - 1;
+  // per rank: unbound flags, then whether the walk statically skips pair (k - 1, k) (an
+  // unbound LEF, or the not-yet-adjusted side's unit crossing its end), so the serial block
+  // loops below read one byte per rank instead of going through the rank order
+  uint8_t* ub = reinterpret_cast<uint8_t*>(s.x2);
+  uint8_t* br = ub + n;
   for (uint64_t k = lane; k < n; k += kWL) {
     const uint32_t i = rk[k];
+    ub[k] = s.ep[i] == kUnbound ? 1 : 0;
     t0[k] = rev ? (int64_t)s.rev[i] - (int64_t)s.rm[i] : (int64_t)s.fwd[i] + (int64_t)s.fm[i];
   }
   w.sync();
-  auto brk = [&](uint64_t k) -> bool {  // pair (k - 1, k), statically skipped
-    if (s.ep[rk[k - 1]] == kUnbound || s.ep[rk[k]] == kUnbound) return true;
-    return rev ? t0[k - 1] <= start : t0[k] > endm1;
-  };
+  for (uint64_t k = lane; k < n; k += kWL)
+    br[k] = k > 0 && (ub[k - 1] != 0 || ub[k] != 0 || (rev ? t0[k - 1] <= start : t0[k] > endm1))
+                ? 1
+                : 0;
+  w.sync();
+  auto brk = [&](uint64_t k) -> bool { return br[k] != 0; };  // pair (k - 1, k), k >= 1
   auto comb = [&](int64_t a, int64_t b) -> int64_t {
     return rev ? (a < b ? a : b) : (a > b ? a : b);
   };
@@ -910,17 +918,28 @@ __device__ inline void w_primary(const W& w, const DetParams& P, const DetChrom&
   uint32_t* prt = s.y1;
   const uint64_t i2 = n - (n3 > 0 ? n3 - 1 : 0);
   const uint64_t np = i2 > 0 ? i2 - 1 : 0;
-  for (uint64_t p = lane; p < np; p += kWL) {
-    uint64_t lo = n5, hi = n;  // first rev rank beyond posF[p]
-    while (lo < hi) {
-      const uint64_t mid = (lo + hi) >> 1;
-      if (posR[mid] <= posF[p]) lo = mid + 1;
-      else hi = mid;
+  // each lane takes a contiguous run of fwd ranks; posF is sorted, so the first rev rank
+  // (from n5) beyond posF[p] only grows along the run and is carried from p to p + 1 (a few
+  // ranks walked, a wider gap bisected)
+  const uint64_t bp = (np + kWL - 1) / kWL;
+  const uint64_t p0 = (uint64_t)lane * bp < np ? (uint64_t)lane * bp : np;
+  const uint64_t p1 = p0 + bp < np ? p0 + bp : np;
+  uint64_t lo = n5;
+  for (uint64_t p = p0; p < p1; ++p) {
+    const uint64_t fp = posF[p];
+    for (int step = 0; step < 8 && lo < n && posR[lo] <= fp; ++step) ++lo;
+    if (lo < n && posR[lo] <= fp) {
+      uint64_t hi = n;
+      while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (posR[mid] <= fp) lo = mid + 1;
+        else hi = mid;
+      }
     }
     uint32_t part = ~0u;
     if (lo < n && posF[p + 1] >= posR[lo]) {
       const uint32_t ridx = s.rr[lo], fidx = s.fr[p];
-      if (posR[lo] - posF[p] < s.rm[ridx] + s.fm[fidx]) part = (uint32_t)lo;
+      if (posR[lo] - fp < s.rm[ridx] + s.fm[fidx]) part = (uint32_t)lo;
     }
     prt[p] = part;
   }
@@ -1199,26 +1218,49 @@ __device__ inline void w_burnin(const W& w, const DetParams& P, const DetChrom& 
       for (uint64_t i = lane; i < n; i += kWL) part += s.fwd[i] - s.rev[i];
       const uint64_t total = w_sum(w, s.xs, part);
       const double mean = (double)total / (double)n;
-      double* sq = reinterpret_cast<double*>(s.x0);
-      for (uint64_t i = lane; i < n; i += kWL) {
-        const double d = (double)(s.fwd[i] - s.rev[i]) - mean;
-        sq[i] = d * d;
-      }
-      w.sync();
+      // the squared deviations summed in LEF order, as the reference's serial double sum: each
+      // lane holds one chunk's square in a register, and every lane adds them in order
       double ssd = 0.0;
-      if (lane == 0)
-        for (uint64_t i = 0; i < n; ++i) ssd = ssd + sq[i];
-      s.xs[0] = 0;
-      w.sync();
+      for (uint64_t g = 0; g < n; g += kWL) {
+        const uint64_t i = g + lane;
+        const double d = i < n ? (double)(s.fwd[i] - s.rev[i]) - mean : 0.0;
+        const double q = d * d;
+        uint64_t qb;
+        __builtin_memcpy(&qb, &q, 8);
+        const uint32_t m = (uint32_t)(n - g < kWL ? n - g : kWL);
+        for (uint32_t t = 0; t < m; ++t) {
+          const uint64_t vb = w.bcast(qb, t);
+          double v;
+          __builtin_memcpy(&v, &vb, 8);
+          ssd = ssd + v;
+        }
+      }
+      const uint64_t cap = P.hist_len;
+      if (s.hlen == cap) {
+        // both histories move down one slot, all lanes: a block of values is read, then
+        // written one slot lower (a block's writes never reach the next block's reads)
+        for (uint64_t b = 1; b < cap; b += 4 * kWL) {
+          double va[4], vc[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const uint64_t i = b + (uint64_t)u * kWL + lane;
+            va[u] = i < cap ? s.avg[i] : 0.0;
+            vc[u] = i < cap ? s.cfx[i] : 0.0;
+          }
+          w.sync();
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const uint64_t i = b + (uint64_t)u * kWL + lane;
+            if (i < cap) {
+              s.avg[i - 1] = va[u];
+              s.cfx[i - 1] = vc[u];
+            }
+          }
+          w.sync();
+        }
+      }
       if (lane == 0) {
         const double sd = sqrt(ssd / (double)n);
-        const uint64_t cap = P.hist_len;
-        if (s.hlen == cap) {
-          for (uint64_t i = 1; i < cap; ++i) {
-            s.avg[i - 1] = s.avg[i];
-            s.cfx[i - 1] = s.cfx[i];
-          }
-        }
         const uint64_t h = s.hlen == cap ? cap - 1 : s.hlen;
         s.avg[h] = mean;
         s.cfx[h] = sd / mean;
