@@ -808,11 +808,36 @@ __device__ inline void w_boundaries(const W& w, const DetChrom& ch, WCell& s, ui
   n3 = w_sum(w, s.xs, c3) + (fend ? 1 : 0);
 }
 
+// Position buckets over a sorted position array: bucket b covers [start + (b << shift),
+// start + ((b + 1) << shift)) and first[b] is the first rank whose position lies in bucket b or
+// later (positions at or past the chromosome end count as bucket nbk); first[nbk] closes it.
+struct WBuckets {
+  uint64_t start, end, nbk;
+  uint32_t shift;
+  __device__ uint64_t of(uint64_t p) const { return p >= end ? nbk : (p - start) >> shift; }
+};
+
+template <class W>
+__device__ inline void w_build_buckets(const W& w, const WBuckets& B, const uint64_t* pos,
+                                       uint64_t n, uint32_t* first) {
+  const uint32_t lane = w.lane();
+  for (uint64_t k = lane; k < n; k += kWL) {
+    const uint64_t bk = B.of(pos[k]);
+    const uint64_t b0 = k == 0 ? 0 : B.of(pos[k - 1]) + 1;
+    for (uint64_t b = b0; b <= bk; ++b) first[b] = (uint32_t)k;
+  }
+  const uint64_t last = n == 0 ? 0 : B.of(pos[n - 1]) + 1;
+  for (uint64_t b = (n == 0 ? 0 : last) + lane; b <= B.nbk; b += kWL) first[b] = (uint32_t)n;
+  w.sync();
+}
+
 // LEF-BAR (det_lef_bar, simulation_detect_collisions.cpp:128-252). The rev walk meets the
 // active barriers in ascending order, each with the first rev unit beyond it (from rank
 // max(n5, 1) - 1 on) and stops at the first barrier with no such unit; a unit keeps the last
 // barrier whose trial succeeds. The fwd walk meets them in descending order with the last fwd
-// unit before them (from rank n - max(n3, 1)).
+// unit before them (from rank n - max(n3, 1)). Each barrier's search bisects only the ranks of
+// its position bucket (about one LEF per bucket), found from a per-epoch bucket index of the
+// sorted rev and fwd positions.
 template <class W>
 __device__ inline void w_lef_bar(const W& w, const DetParams& P, const DetChrom& ch, WCell& s,
                                  uint64_t n, uint64_t n5, uint64_t n3) {
@@ -821,6 +846,20 @@ __device__ inline void w_lef_bar(const W& w, const DetParams& P, const DetChrom&
   const uint64_t* posR = s.x0;
   const uint64_t* posF = s.x1;
   uint32_t* win = s.y0;
+  // bucket index: a power of two of buckets below n, both sides' first ranks in x2 (2n u32)
+  WBuckets B{ch.start, ch.end, 0, 0};
+  uint32_t* firstR = reinterpret_cast<uint32_t*>(s.x2);
+  uint32_t* firstF = firstR + n;
+  const bool bucketed = n >= 4;
+  if (bucketed) {
+    uint64_t nbk = 1;
+    while (nbk * 2 < n) nbk *= 2;  // nbk + 1 <= n entries per side
+    B.nbk = nbk;
+    const uint64_t range = ch.end - ch.start;
+    while (((range - 1) >> B.shift) >= nbk) ++B.shift;
+    w_build_buckets(w, B, posR, n, firstR);
+    w_build_buckets(w, B, posF, n, firstF);
+  }
   for (int side = 0; side < 2; ++side) {
     const bool rev = side == 0;
     const uint8_t major = rev ? kDirRev : kDirFwd;
@@ -832,6 +871,11 @@ __device__ inline void w_lef_bar(const W& w, const DetParams& P, const DetChrom&
         const uint64_t bpos = ch.bar_pos[i];
         if (rev) {
           uint64_t lo = j0, hi = n;  // first rank with posR > bpos
+          if (bucketed) {
+            const uint64_t b = B.of(bpos);
+            lo = firstR[b] > j0 ? firstR[b] : j0;
+            hi = firstR[b + 1] > lo ? firstR[b + 1] : lo;
+          }
           while (lo < hi) {
             const uint64_t mid = (lo + hi) >> 1;
             if (posR[mid] <= bpos) lo = mid + 1;
@@ -843,6 +887,11 @@ __device__ inline void w_lef_bar(const W& w, const DetParams& P, const DetChrom&
           }
         } else {
           uint64_t lo = 0, hi = j0 + 1;  // ranks in [0, j0] with posF < bpos
+          if (bucketed) {
+            const uint64_t b = B.of(bpos);
+            lo = firstF[b] < j0 + 1 ? firstF[b] : j0 + 1;
+            hi = firstF[b + 1] < j0 + 1 ? firstF[b + 1] : j0 + 1;
+          }
           while (lo < hi) {
             const uint64_t mid = (lo + hi) >> 1;
             if (posF[mid] < bpos) lo = mid + 1;
