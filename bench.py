@@ -254,6 +254,19 @@ def deterministic_leg(ctx, nthreads, no_cpu, genome_cpu=None, phases=False):
             if v:
                 log(f"  {L.DET_PHASE_NAMES[k]:28s} {100.0 * v / ph.sum():5.1f} %")
     job.close()
+    if phases:
+        # the chain that bounds the sample: the largest chromosome's 64 cells on their own
+        big = max(range(len(plans)), key=lambda i: sub[i].nlefs)
+        jb = S.Job(ctx, sim.params, [chroms[big]], [bars[big]], [sub[big]], L.MODE_DETERMINISTIC)
+        jb.run()
+        sb = jb.run()
+        jb.close()
+        busy["largest_chromosome_alone"] = {"chromosome": names[big], "lefs": int(sub[big].nlefs),
+                                            "cells": int(sb.cells),
+                                            "kernel_ms": float(sb.kernel_ms),
+                                            "epochs_per_cell": float(sb.epochs) / float(sb.cells)}
+        log(f"deterministic genome sample: {names[big]} ({sub[big].nlefs} LEFs) alone, "
+            f"{sb.cells} cells: {sb.kernel_ms:.1f} ms")
     rec = {"workload": "genome-24chroms-5kb, first 64 cells per chromosome",
            "cells": int(st.cells), "cells_per_sec": st.cells / wall,
            "contacts_per_sec": st.contacts / wall, "epochs_per_sec": st.epochs / wall,
