@@ -324,8 +324,9 @@ void mgpu_job_destroy(mgpu_job* job);
 /* Per-launch figures of the last run that was not replayed from a graph (no counterpart in the
  * reference: they time the device launches). Fast mode launches once per working-set size tier,
  * each tier on a stream of its own, so the launches of a run overlap (a tier's workgroups start
- * on the CUs the tier before it leaves); deterministic mode launches once. ms runs from the
- * moment a launch's stream reaches it to its end. *n receives the number of launches; up to max
+ * on the CUs the tier before it leaves); deterministic mode reports one launch (when its cells'
+ * arrays need more device-memory slots than the slab holds, the launches over the same slots are
+ * timed together). ms runs from the moment a launch's stream reaches it to its end. *n receives the number of launches; up to max
  * are written. */
 typedef struct mgpu_launch_stats {
   uint32_t threads, grid; /* workgroup size and workgroups */
@@ -394,7 +395,10 @@ int mgpu_job_read_state_log(mgpu_job* job, mgpu_state_record* out, uint64_t cap,
  * global_working_set: 1 keeps every cell's working set in device memory, in one launch (0 or the
  * chosen threads_per_cell: 512 lanes by default). Bands, missed counts and statistics do not
  * depend on the tuning. Get only: launches (the launches the next run makes, at most 5), their
- * sizes in tier_threads and grids in tier_grid, and grid (the largest of them). */
+ * sizes in tier_threads and grids in tier_grid, and grid (the largest of them).
+ * Deterministic mode (get only): 64 threads per cell, grid = cells per launch, launches > 1
+ * when the cells' arrays sit in device memory (global_working_set 1) and more cells than the
+ * slab's slots run, one launch after another over the same slots. */
 typedef struct mgpu_tuning {
   uint32_t threads_per_cell;
   uint32_t grid;
@@ -479,6 +483,10 @@ int mgpu_debug_fast_serial_fixups(mgpu_job* job, int serial);
  * cell on the one-lane kernel instead. Both give the reference's bands and optional outputs:
  * exposed to compare them. */
 int mgpu_debug_det_kernel(mgpu_ctx* ctx, int serial);
+/* Caps the device-memory cell slots of later deterministic jobs whose cells exceed the LDS
+ * (0: as many as the 8 GiB slab holds). A job with more such cells than slots runs as several
+ * launches over the same slots; the cap lets a test make that happen with a few cells. */
+int mgpu_debug_det_slab_slots(mgpu_ctx* ctx, uint64_t slots);
 /* Per-epoch state hash of one cell in deterministic mode (for locating a first divergence). */
 int mgpu_debug_cell_trace(mgpu_ctx* ctx, const mgpu_params* p, const mgpu_chrom* chrom,
                           const mgpu_barriers* barriers, uint64_t nlefs, uint64_t cell_id,

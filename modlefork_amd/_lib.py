@@ -309,6 +309,7 @@ SIGNATURES = [
     ("mgpu_debug_fast_serial_fixups", C.c_int, [C.c_void_p, C.c_int]),
     ("mgpu_debug_det_kernel", C.c_int, [C.c_void_p, C.c_int]),
     ("mgpu_job_kernel", C.c_int, [C.c_void_p, C.c_char_p, C.c_uint64]),
+    ("mgpu_debug_det_slab_slots", C.c_int, [C.c_void_p, C.c_uint64]),
     ("mgpu_job_cell_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_uint64,
                                       C.POINTER(C.c_uint64)]),
     ("mgpu_context_stream", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int)]),

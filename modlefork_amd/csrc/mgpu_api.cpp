@@ -29,6 +29,7 @@ struct mgpu_ctx {
   int num_cus = 0;
   int phase_timers = 0;
   int det_serial = 0;  // deterministic mode: 1 runs every cell on the one-lane kernel
+  uint64_t det_slab_slots = 0;  // deterministic mode: cap on device-memory cell slots (0: none)
   std::string noalloc_err;  // why mgpu_simulate_chrom_noalloc cannot run on this device, if so
 };
 
@@ -714,6 +715,7 @@ int job_create(mgpu_ctx* ctx, const mgpu_params* params, bool per_chrom, uint64_
       job->wave_stride = det_wave_cell_bytes_host(max_lefs, max_bar, p->burnin_history_length);
       job->wave_slots = std::max<uint64_t>(1, std::min<uint64_t>(job->nitems,
                                                                kWaveSlabBytes / job->wave_stride));
+      if (ctx->det_slab_slots) job->wave_slots = std::min(job->wave_slots, ctx->det_slab_slots);
     }
   }
   // the one-lane kernel's device-memory working sets and the wave kernel's slab are used by
@@ -1191,9 +1193,17 @@ int mgpu_job_get_tuning(mgpu_job* job, mgpu_tuning* t) {
     t->grid = fast_job_grid(&job->fast);
     t->global_working_set = (uint32_t)fast_job_global_storage(&job->fast);
     t->launches = fast_job_tiers(&job->fast, t->tier_threads, t->tier_grid, nullptr, 5);
+  } else if (det_uses_wave(job)) {
+    // one block per cell; with the cells' arrays in the slab, launches of at most wave_slots
+    t->threads_per_cell = 64;
+    t->grid = (uint32_t)(job->wave_stride ? job->wave_slots : job->nitems);
+    t->global_working_set = job->wave_stride ? 1u : 0u;
+    t->launches = (uint32_t)(job->wave_stride ? ceil_div(job->nitems, job->wave_slots) : 1);
   } else {
     t->threads_per_cell = 64;
     t->grid = job->grid;
+    t->global_working_set = job->use_lds ? 0u : 1u;
+    t->launches = 1;
   }
   return MGPU_OK;
 }
@@ -1635,6 +1645,12 @@ int mgpu_job_kernel(mgpu_job* job, char* name, uint64_t cap) {
                   : det_uses_wave(job)         ? "det_wave_kernel"
                                                : "det_cells_kernel";
   snprintf(name, cap, "%s", k);
+  return MGPU_OK;
+}
+
+int mgpu_debug_det_slab_slots(mgpu_ctx* ctx, uint64_t slots) {
+  if (!ctx) return fail(ctx, MGPU_EINVAL, "null context");
+  ctx->det_slab_slots = slots;
   return MGPU_OK;
 }
 

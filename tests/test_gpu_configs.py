@@ -161,15 +161,17 @@ def test_genome_chromosome_statistics_match_oracle(ctx, which):
     assert not failures, failures
 
 
-@pytest.mark.parametrize("which,ncells,kernel", [("smallest", 16, "det_wave_kernel"),
-                                                 ("largest", 8, "det_wave_slab_kernel")])
-def test_genome_chromosome_deterministic_bit_exact(ctx, which, ncells, kernel):
+@pytest.mark.parametrize("which,ncells,kernel,slots", [("smallest", 16, "det_wave_kernel", 0),
+                                                       ("largest", 8, "det_wave_slab_kernel", 0),
+                                                       ("largest", 8, "det_wave_slab_kernel", 3)])
+def test_genome_chromosome_deterministic_bit_exact(ctx, which, ncells, kernel, slots):
     """Deterministic mode at genome scale, against the oracle bit for bit (band, missed updates
     and epoch counts), with the full workload's per-cell targets: 16 cells of the genome
     workload's smallest chromosome (42 Mbp, 841 LEFs, clustered barriers with Beta(8,2)
     occupancy; one wavefront per cell, its 127 KB working set in LDS) and 8 cells of its largest
     (241 Mbp, 4 827 LEFs; the cell's arrays, about 0.5 MB, in a device-memory slot, its
-    engine-word ring in LDS)."""
+    engine-word ring in LDS), also with the slab capped at 3 slots, so the 8 cells run as three
+    launches over the same slots (mgpu_debug_det_slab_slots)."""
     wl, name = _genome_chrom(which)
     c = OC.transform(wl.options)
     rc = OC.to_rc(c)
@@ -186,8 +188,14 @@ def test_genome_chromosome_deterministic_bit_exact(ctx, which, ncells, kernel):
                                            p.nrows, p.ncols, 16)
     say(f"genome deterministic ({name}): device")
     sub = S.ChromPlan(p.nlefs, p.nrows, p.ncols, t.copy(), p.target_epochs)
-    job = S.Job(ctx, sim.params, [chroms[i]], [dbars[i]], [sub], L.MODE_DETERMINISTIC)
+    ctx.check(L.lib().mgpu_debug_det_slab_slots(ctx.ptr, slots))
+    try:
+        job = S.Job(ctx, sim.params, [chroms[i]], [dbars[i]], [sub], L.MODE_DETERMINISTIC)
+    finally:
+        ctx.check(L.lib().mgpu_debug_det_slab_slots(ctx.ptr, 0))
     assert job.kernel() == kernel
+    if slots:
+        assert job.tuning().grid == slots
     st = job.run()
     band, missed = job.band(0)
     cs = job.cell_stats()
