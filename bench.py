@@ -283,6 +283,27 @@ def deterministic_leg(ctx, nthreads, no_cpu, genome_cpu=None, phases=False):
     return out
 
 
+def training_data_leg(ctx, nwin: int, density: float = 10.0):
+    """The surrogate's training data as the training configuration states it: nwin isolated
+    windows of 2 Mbp (simulated with their 500 kb of context on each side, 3 Mbp, 5 kb bins),
+    0-60 uniform barriers, occupancy U(0.3, 0.95), LEF density U(5, 40)/Mbp, processivity
+    U(100, 600) kb, 256 cells per window, fast mode, contact density `density`; chunks of 2 048
+    windows per job, bands read back and cut to the 100-diagonal window targets."""
+    from modlefork_amd import windows as WN
+
+    specs = WN.sample_specs(nwin, seed=3)
+    t0 = time.perf_counter()
+    x, y, kernel_s = WN.simulate_windows(ctx, specs, ncells=256, contact_density=density,
+                                         chunk=2048)
+    wall = time.perf_counter() - t0
+    cells = nwin * 256
+    log(f"training data: {nwin} windows x 256 cells in {wall:.1f} s ({nwin / wall:.1f} "
+        f"windows/s, kernel {kernel_s:.1f} s)")
+    return {"windows": nwin, "cells_per_window": 256, "contact_density": density,
+            "wall_s": wall, "kernel_s": kernel_s, "windows_per_sec": nwin / wall,
+            "cells_per_sec": cells / wall, "inputs": list(x.shape), "targets": list(y.shape)}
+
+
 def perturbate_inputs(wl, params):
     """ChromInput list of a perturbate workload (barriers in interval-tree order, features)."""
     from modlefork_amd import perturbate as P
@@ -475,6 +496,10 @@ def main():
                     help="after the timed steps, run one more step with the epoch-phase timers on "
                          "and print each phase's share of the time to stderr (fast mode, or the "
                          "wave-parallel deterministic kernel)")
+    ap.add_argument("--training-data", type=int, default=0, metavar="WINDOWS",
+                    help="also simulate the surrogate's training windows as its configuration "
+                         "states them (isolated 2 Mbp windows, 256 cells each; 20000 for the "
+                         "whole set)")
     ap.add_argument("--write-cool", default="", metavar="PATH",
                     help="also write the bands of one run to PATH as .cool and time it (outside "
                          "the timed steps; the file is removed afterwards)")
@@ -693,6 +718,8 @@ def main():
     if not args.no_deterministic and args.workload == "genome":
         line["deterministic"] = deterministic_leg(sim.ctx, nthreads, args.no_cpu, cpu,
                                                   phases=args.phases)
+    if args.training_data:
+        line["training_data"] = training_data_leg(sim.ctx, args.training_data)
     if not args.no_sweep:
         log(f"surrogate sweep: {args.sweep_points} points")
         line["surrogate"] = surrogate_leg(sim.ctx, args.sweep_points)

@@ -349,6 +349,26 @@ def test_full_sweep_judges_and_accepts_the_surrogate(ctx):
     say(f"raw 5 kb correlation over all 99 off-diagonals: {np.nanmean(res.raw_5kb):.4f}")
 
 
+def test_training_windows_as_configured(ctx):
+    """The surrogate's training data as its configuration states it: isolated 2 Mbp windows
+    (simulated with their 500 kb of context on each side), 0-60 uniform barriers, occupancy
+    U(0.3, 0.95), LEF density U(5, 40)/Mbp, processivity U(100, 600) kb, 256 cells per window,
+    fast mode. 512 of the 20 000 windows here (bench.py --training-data 20000 runs the whole set,
+    98.5 s on one MI355X): input tracks carry each window's barriers, every window registers
+    contacts and contact frequency falls with distance."""
+    from modlefork_amd import windows as WN
+
+    specs = WN.sample_specs(512, seed=3)
+    x, y, kernel_s = WN.simulate_windows(ctx, specs, ncells=256, contact_density=10.0, chunk=2048)
+    say(f"training windows: 512 x 256 cells, kernel {kernel_s:.2f} s")
+    assert x.shape == (512, WN.IN_BINS, WN.IN_CH) and y.shape == (512, WN.NDIAG, WN.OUT_BINS)
+    for k in (0, 100, 511):
+        assert int(x[k, :, 0].sum()) == len(specs[k].records)
+    assert np.all(y.sum(axis=(1, 2)) > 0)
+    per_diag = y.mean(axis=(0, 2))
+    assert per_diag[1] > per_diag[20] > per_diag[90] > 0
+
+
 def test_attenuation_correction_is_calibrated(ctx):
     """The judge's correction for the simulation's own noise, checked with a prediction whose
     agreement with the expected band is known: a simulation of the same point with 16 times the
