@@ -208,6 +208,46 @@ def test_genome_chromosome_deterministic_bit_exact(ctx, which, ncells, kernel, s
     assert np.array_equal(band, oband), int(np.count_nonzero(band != oband))
 
 
+def test_slab_cells_optional_outputs_bit_exact(ctx):
+    """1D LEF occupancy and the state log from cells whose arrays sit in device memory: 4 cells
+    of the genome workload's largest chromosome (4 827 LEFs) with both outputs on, against the
+    oracle's run with the same outputs, bit for bit (band, occupancy, every record)."""
+    wl, name = _genome_chrom("largest")
+    ncells = 4
+    opts = dict(wl.options, track_1d_lef_position=1, log_model_internal_state=1)
+    c = OC.transform(opts)
+    rc = OC.to_rc(c)
+    size = dict(wl.chroms)[name]
+    bars = OC.prepare_barriers(wl.barriers_of(name), 0, size, c)
+    sim = S.Simulation(opts, ctx)
+    names, chroms, dbars, plans = sim.prepare(wl)
+    i = names.index(name)
+    p = plans[i]
+    t = np.ascontiguousarray(p.targets[:ncells], np.uint64)
+    say(f"slab outputs ({name}): oracle, {ncells} cells")
+    ch = R.make_chrom(name, size, 0, size)
+    occ = np.zeros((size + c["bin_size"] - 1) // c["bin_size"], np.uint64)
+    oband, omissed, ost, orecs = R.simulate_cells(rc, ch, bars, p.nlefs, 0, ncells, t,
+                                                  p.target_epochs, p.nrows, p.ncols, 4,
+                                                  occupancy=occ, state_log_cap=20_000)
+    say("slab outputs: device")
+    sub = S.ChromPlan(p.nlefs, p.nrows, p.ncols, t.copy(), p.target_epochs)
+    job = S.Job(ctx, sim.params, [chroms[i]], [dbars[i]], [sub], L.MODE_DETERMINISTIC)
+    assert job.kernel() == "det_wave_slab_kernel"
+    job.set_state_log(20_000)
+    job.run()
+    band, missed = job.band(0)
+    docc = job.occupancy(0)
+    drecs, nrec = job.state_log()
+    job.close()
+    assert missed == omissed and np.array_equal(band, oband)
+    assert np.array_equal(docc[:len(occ)], occ), int(np.count_nonzero(docc[:len(occ)] != occ))
+    assert drecs is not None and nrec == len(orecs) == int(ost[:, 0].sum())
+    for f in ("epoch", "burnin", "num_active_lefs", "active_barriers", "stalls_rev", "stalls_fwd",
+              "stalls_both", "lef_bar", "primary", "secondary", "avg_loop_size"):
+        assert np.array_equal(drecs[f], orecs[f]), f
+
+
 def _decay_is_decreasing(band, nrows, ncols, slack=1.05):
     p, _ = CS.log_binned_decay(band, nrows, ncols)
     return bool(np.all(p[1:] <= p[:-1] * slack)), p
