@@ -13,8 +13,10 @@ Fast mode runs one persistent launch per size tier at the tiers' default workgro
 tier: on the genome workload it keeps 1024 and 512 lanes for the two large tiers and finds 256
 and 512 lanes within noise for the small one, so the default run is the tuned one; the rocprof
 counters under profiles/ are keyed by the launch shape they were taken on. The timed steps run
-eagerly, so HIP events around each launch give the per-launch roofline in the timed region
-(mgpu_job_capture's graph replays carry no per-launch events).
+as replays of a HIP graph of the run (mgpu_job_capture: the band clears and every launch,
+captured once after the warm-up). Graph replays carry no per-launch events, so one eager run
+after the timed steps gives each launch's duration for the roofline (`eager_run` reports its
+rate beside `value`).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload genome|small|stress]
                     [--mode fast|det] [--cpu-cells K] [--cpu-threads T] [--no-stress]
@@ -563,26 +565,40 @@ def main():
     for i in range(args.warmup):
         st = job.run()
         log(f"warmup {i}: kernel {st.kernel_ms:.1f} ms")
+    # The timed steps replay a HIP graph of the run (its band clears and launches, captured
+    # once after the warm-up); every replay is a whole run of the workload.
+    job.capture()
+    job.run()  # the graph's first replay, untimed
     kernel_ms = []
     t0 = time.perf_counter()
     stats = None
     for i in range(args.steps):
         stats = job.run()  # synchronizes the stream before and after
         kernel_ms.append(stats.kernel_ms)
-        log(f"step {i}: kernel {stats.kernel_ms:.1f} ms, {stats.epochs} epochs")
+        log(f"step {i}: graph replay, {stats.kernel_ms:.1f} ms on the device, "
+            f"{stats.epochs} epochs")
     wall = time.perf_counter() - t0
-    ls = job.launch_stats()  # per-launch times of the last timed run
+    job.release_graph()
+    # one eager run after them: graph replays carry no per-launch events, so the launches'
+    # own durations (the roofline below) come from this run
+    te = time.perf_counter()
+    est = job.run()
+    wall_e = time.perf_counter() - te
+    ls = job.launch_stats()
+    eager = {"ms": 1000.0 * wall_e, "kernel_ms": float(est.kernel_ms),
+             "cells_per_sec": est.cells / wall_e}
+    log(f"eager run: {eager['ms']:.1f} ms ({eager['cells_per_sec']:.1f} cells/s)")
     if args.phases:
         det = mode == L.MODE_DETERMINISTIC
         sim.ctx.phase_times(True)
         job.run()
         ph = sim.ctx.phase_times(False, deterministic=det).astype(np.float64)
-        names = L.DET_PHASE_NAMES if det else L.PHASE_NAMES
+        pnames = L.DET_PHASE_NAMES if det else L.PHASE_NAMES
         log(f"epoch phases (share of cell time, "
             f"{'one wavefront' if det else f'{shape} threads'} per cell):")
         for k, v in enumerate(ph):
             if v:
-                log(f"  {names[k]:28s} {100.0 * v / ph.sum():5.1f} %  ({v / 1e8:.3f} s summed)")
+                log(f"  {pnames[k]:28s} {100.0 * v / ph.sum():5.1f} %  ({v / 1e8:.3f} s summed)")
     ms_per_step = 1000.0 * wall / args.steps
     cells = int(stats.cells)
     contacts = int(stats.contacts)
@@ -629,7 +645,7 @@ def main():
     # launch's start to the last one's end, HIP events on the job's stream), with each launch's
     # own bytes and duration beside it
     all_bytes = sum(launch_bytes(x) for x in ls)
-    span_ms = float(stats.kernel_ms)
+    span_ms = float(est.kernel_ms)  # the eager run: launches timed by their own events
     achieved = all_bytes / (span_ms / 1000.0) / 1e9
     per_launch = [{"threads": int(x.threads), "grid": int(x.grid), "cells": int(x.cells),
                    "contacts": int(x.contacts), "ms": float(x.ms),
@@ -695,6 +711,7 @@ def main():
                    "launches": launches, "autotuned": autotuned},
         "contacts_per_sec": contacts * args.steps / wall,
         "with_host_readback": host_bands,
+        "eager_run": eager,
         "genome_import_ms": prepare_ms,
         "cool_write": cool,
         "epochs_per_sec": epochs * args.steps / wall,
