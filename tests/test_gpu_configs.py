@@ -297,6 +297,39 @@ def test_stress_statistics_match_oracle(ctx):
     assert not failures, failures
 
 
+def test_stress_cell_deterministic_bit_exact(ctx):
+    """Deterministic mode on the stress workload's configuration: one cell of the 250 Mbp
+    chromosome (1 kb bins, 5 000 diagonals, 10 000 LEFs, 12 000 barriers; 4 160 epochs) with the
+    full workload's target, its arrays (1 MB) in a device-memory slot, against the oracle bit for
+    bit: band, missed updates, epochs, burn-in epochs, contacts and the 194 M engine words
+    consumed. About 22 s on the device."""
+    wl = W.stress()
+    name, size = wl.chroms[0]
+    c = OC.transform(wl.options)
+    rc = OC.to_rc(c)
+    bars = OC.prepare_barriers(wl.barriers_of(name), 0, size, c)
+    sim = S.Simulation(wl.options, ctx)
+    names, chroms, dbars, plans = sim.prepare(wl)
+    p = plans[0]
+    t = np.ascontiguousarray(p.targets[:1], np.uint64)
+    say("stress deterministic: oracle, 1 cell")
+    ch = R.make_chrom(name, size, 0, size)
+    oband, omissed, ost = R.simulate_cells(rc, ch, bars, p.nlefs, 0, 1, t, p.target_epochs,
+                                           p.nrows, p.ncols, 1)
+    say("stress deterministic: device")
+    sub = S.ChromPlan(p.nlefs, p.nrows, p.ncols, t.copy(), p.target_epochs)
+    job = S.Job(ctx, sim.params, chroms, dbars, [sub], L.MODE_DETERMINISTIC)
+    assert job.kernel() == "det_wave_slab_kernel"
+    st = job.run()
+    band, missed = job.band(0)
+    cs = job.cell_stats()
+    job.close()
+    say(f"stress deterministic: {st.kernel_ms:.0f} ms, {st.epochs} epochs")
+    assert np.array_equal(cs, ost), (cs, ost)
+    assert missed == omissed
+    assert np.array_equal(band, oband), int(np.count_nonzero(band != oband))
+
+
 def test_stress_workload_full_size_conserves_contacts(ctx):
     """All 2 048 cells of the stress workload: the 5 GB band plus its out-of-band count equals
     the planned 1.25 billion contacts, none dropped, and contact frequency falls with distance."""
