@@ -423,10 +423,26 @@ __device__ inline void w_rank_side(const W& w, WCell& s, uint32_t* rk, const uin
   const uint32_t lane = w.lane();
   const WKey key{pos, s.ep, asc};
   const uint64_t epoch = s.epoch;
-  for (uint64_t k = lane; k < n; k += kWL) keep[k] = rk[k];
-  w.sync();
-  const uint64_t nk = w_compact(w, keep, n, kl, [&](uint32_t v) { return s.ep[v] != epoch; });
-  const uint64_t nn = w_compact(w, keep, n, nl, [&](uint32_t v) { return s.ep[v] == epoch; });
+  // kept and new (bound this epoch) LEFs in rank order, in one pass; rk itself stays as it was
+  // until the merge, so the serial sorters below start from it
+  uint64_t nk = 0, nn = 0;
+  {
+    const uint64_t lt = lanes_below(lane);
+    for (uint64_t g = 0; g < n; g += kWL) {
+      const uint64_t k = g + lane;
+      const bool has = k < n;
+      const uint32_t v = has ? rk[k] : 0u;
+      const bool isnew = has && s.ep[v] == epoch;
+      const uint64_t mn = w.ballot(isnew);
+      const uint64_t mk = w.ballot(has && !isnew);
+      if (isnew) nl[nn + (uint64_t)__builtin_popcountll(mn & lt)] = v;
+      else if (has) kl[nk + (uint64_t)__builtin_popcountll(mk & lt)] = v;
+      nn += (uint64_t)__builtin_popcountll(mn);
+      nk += (uint64_t)__builtin_popcountll(mk);
+    }
+    w.sync();
+  }
+  (void)keep;
   bool serial = nn > kWMaxNewCount;
   if (!serial) {
     // new LEFs: rank by counting. They share the binding epoch, so the key is the position
@@ -517,7 +533,6 @@ __device__ inline void w_rank_side(const W& w, WCell& s, uint32_t* rk, const uin
   if (serial) {
     wdbg(s.dbg, lane, kDbgRankSerial);
     if (lane == 0) {
-      for (uint64_t k = 0; k < n; ++k) rk[k] = keep[k];
       if (partially_sorted) split_sort(rk, (uint32_t)n, tmp, KeyLess{pos});
       else pdq_sort(rk, (uint32_t)n, KeyLess{pos});
       fix_ties(rk, n, pos, s.ep, asc);
