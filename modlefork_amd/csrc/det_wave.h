@@ -414,12 +414,12 @@ __device__ inline uint64_t w_count_less(const uint32_t* arr, uint64_t len, const
   return lo;
 }
 
-// Orders rk[0, n) as rank_lefs does. keep: original order; kl, nl, ns: scratch of n entries;
-// tmp: n entries for the serial sorters, or the slot marks of the merge.
+// Orders rk[0, n) as rank_lefs does. kl, nl, ns: scratch of n entries; tmp: n entries for the
+// serial sorters, or the slot marks of the merge.
 template <class W>
 __device__ inline void w_rank_side(const W& w, WCell& s, uint32_t* rk, const uint64_t* pos,
-                                   bool asc, uint64_t n, bool partially_sorted, uint32_t* keep,
-                                   uint32_t* kl, uint32_t* nl, uint32_t* ns, uint32_t* tmp) {
+                                   bool asc, uint64_t n, bool partially_sorted, uint32_t* kl,
+                                   uint32_t* nl, uint32_t* ns, uint32_t* tmp) {
   const uint32_t lane = w.lane();
   const WKey key{pos, s.ep, asc};
   const uint64_t epoch = s.epoch;
@@ -442,7 +442,6 @@ __device__ inline void w_rank_side(const W& w, WCell& s, uint32_t* rk, const uin
     }
     w.sync();
   }
-  (void)keep;
   bool serial = nn > kWMaxNewCount;
   if (!serial) {
     // new LEFs: rank by counting. They share the binding epoch, so the key is the position
@@ -546,8 +545,8 @@ __device__ inline void w_rank(const W& w, WCell& s, uint64_t n, bool partially_s
   uint32_t* a = reinterpret_cast<uint32_t*>(s.x0);
   uint32_t* b = reinterpret_cast<uint32_t*>(s.x1);
   uint32_t* tmp = reinterpret_cast<uint32_t*>(s.rm);
-  w_rank_side(w, s, s.rr, s.rev, true, n, partially_sorted, a, s.y0, s.y1, a + n, tmp);
-  w_rank_side(w, s, s.fr, s.fwd, false, n, partially_sorted, b, s.y0, s.y1, b + n, tmp);
+  w_rank_side(w, s, s.rr, s.rev, true, n, partially_sorted, s.y0, s.y1, a, tmp);
+  w_rank_side(w, s, s.fr, s.fwd, false, n, partially_sorted, s.y0, s.y1, b, tmp);
 }
 
 // ---------------------------------------------------------------------------------------------
