@@ -191,7 +191,8 @@ def deterministic_leg(ctx, nthreads, no_cpu, genome_cpu=None, phases=False):
     in device-memory slots, engine-word rings in LDS), one timed run after a warm-up, against
     the CPU rate the genome workload's own baseline measured (genome_cpu). With `phases`, one
     more genome-sample run with the per-phase timers gives the summed wavefront time, and so how
-    many wavefronts were busy on average over the kernel."""
+    many wavefronts were busy on average over the kernel. Last, the whole genome workload
+    (12 288 cells) once, with its contacts checked per chromosome."""
     from modlefork_amd import _lib as L
     from modlefork_amd import simulation as S
     from modlefork_amd import workloads as W
@@ -282,6 +283,29 @@ def deterministic_leg(ctx, nthreads, no_cpu, genome_cpu=None, phases=False):
     if busy:
         rec["phases"] = busy
     out["genome_sample"] = rec
+
+    # the whole genome workload (24 x 512 cells), one run (the kernel has just run above), with
+    # every chromosome's contacts checked: band sum + missed == its cells' targets
+    job = S.Job(ctx, sim.params, chroms, bars, plans, L.MODE_DETERMINISTIC)
+    t0 = time.perf_counter()
+    st = job.run()
+    wall = time.perf_counter() - t0
+    conserved = True
+    for i in range(len(plans)):
+        band, missed = job.band(i)
+        conserved &= int(band.sum(dtype=np.uint64)) + missed == int(plans[i].targets.sum())
+    full = {"workload": "genome-24chroms-5kb-512cells", "cells": int(st.cells),
+            "cells_per_sec": st.cells / wall, "contacts_per_sec": st.contacts / wall,
+            "epochs_per_sec": st.epochs / wall, "kernel_ms": float(st.kernel_ms),
+            "kernel": job.kernel(), "launches": int(job.tuning().launches),
+            "contacts_conserved": bool(conserved), "dropped": int(st.dropped)}
+    job.close()
+    log(f"deterministic genome workload: {st.cells} cells in {wall * 1e3:.1f} ms "
+        f"({st.cells / wall:.1f} cells/s, contacts conserved: {conserved})")
+    if genome_cpu:
+        full["cpu_cells_per_sec"] = genome_cpu["value"]
+        full["speedup_vs_cpu"] = full["cells_per_sec"] / genome_cpu["value"]
+    out["genome_full"] = full
     return out
 
 
