@@ -61,9 +61,22 @@ def build_workload(name, ncells):
 def launch_bytes(ls) -> int:
     """Algorithmic bytes of one simulation launch: 8 B per registered contact (a read and a
     write of one int32 pixel), 21 B per barrier-table entry each cell reads (position, direction
-    and transition probabilities) and 32 B of engine state per cell. Working sets stay on chip
-    (LDS) in the measured configurations, so no LEF state spills."""
+    and transition probabilities) and 32 B of engine state per cell. The fast mode's working sets
+    stay on chip (LDS) in the measured configurations, so no LEF state spills."""
     return 8 * int(ls.contacts) + 21 * int(ls.barriers) + 32 * int(ls.cells)
+
+
+def spill_bytes(cell_stats, plans, bars) -> int:
+    """The spill term of the algorithmic bytes for cells whose carried state lives in device
+    memory: per cell-epoch a read and a write of 20 B per LEF (positions, binding epoch, ranks)
+    and 1 B per barrier (its state). cell_stats: [cells, 4] in job order (epochs first)."""
+    total, k = 0, 0
+    for p, b in zip(plans, bars):
+        n = len(p.targets)
+        ep = int(cell_stats[k:k + n, 0].sum())
+        total += ep * 2 * (20 * int(p.nlefs) + len(b.pos))
+        k += n
+    return total
 
 
 def cpu_model() -> str:
@@ -290,6 +303,9 @@ def deterministic_leg(ctx, nthreads, no_cpu, genome_cpu=None, phases=False):
     t0 = time.perf_counter()
     st = job.run()
     wall = time.perf_counter() - t0
+    spill = spill_bytes(job.cell_stats(), plans, bars)
+    ls = job.launch_stats()
+    alg = sum(launch_bytes(x) for x in ls) + spill
     conserved = True
     for i in range(len(plans)):
         band, missed = job.band(i)
@@ -298,7 +314,11 @@ def deterministic_leg(ctx, nthreads, no_cpu, genome_cpu=None, phases=False):
             "cells_per_sec": st.cells / wall, "contacts_per_sec": st.contacts / wall,
             "epochs_per_sec": st.epochs / wall, "kernel_ms": float(st.kernel_ms),
             "kernel": job.kernel(), "launches": int(job.tuning().launches),
-            "contacts_conserved": bool(conserved), "dropped": int(st.dropped)}
+            "contacts_conserved": bool(conserved), "dropped": int(st.dropped),
+            "roofline": {"bound": "hbm", "algorithmic_bytes": alg, "spill_bytes": spill,
+                         "achieved": alg / (st.kernel_ms / 1000.0) / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s",
+                         "frac": alg / (st.kernel_ms / 1000.0) / 1e9 / HBM_PEAK_GBS}}
     job.close()
     log(f"deterministic genome workload: {st.cells} cells in {wall * 1e3:.1f} ms "
         f"({st.cells / wall:.1f} cells/s, contacts conserved: {conserved})")
@@ -669,6 +689,8 @@ def main():
     # launch's start to the last one's end, HIP events on the job's stream), with each launch's
     # own bytes and duration beside it
     all_bytes = sum(launch_bytes(x) for x in ls)
+    if job.kernel() == "det_wave_slab_kernel":  # carried state in device memory
+        all_bytes += spill_bytes(job.cell_stats(), plans, bars)
     span_ms = float(est.kernel_ms)  # the eager run: launches timed by their own events
     achieved = all_bytes / (span_ms / 1000.0) / 1e9
     per_launch = [{"threads": int(x.threads), "grid": int(x.grid), "cells": int(x.cells),
